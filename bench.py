@@ -1,0 +1,244 @@
+"""Benchmark of HeatFlow's time step on one MI355X.
+
+Metric: timesteps/s of the FP32-inner / FP64-refined solver on the 512^3 log-normal-conductivity
+Cattaneo workload (20 steps, Gaussian heating, Dirichlet faces), next to the plain FP64 path at
+the same tolerance and at the reference's own tolerance (1e-32), with the max/rms relative
+temperature error of each against the FP64 run at the reference tolerance; the forward+adjoint
+gradient time at 128^3 / 200 steps; and the reference Fortran timed on the host.
+
+A "step" is one time step (evolution::simulate, src/heatflow/mod_evolve.f90:45-177) of the
+whole grid.  W untimed warm-up steps run first, then the state is reset and the K timed steps
+run from the initial condition; the timed region is bracketed by torch.cuda.synchronize().
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def timed_run(sim, steps: int, warmup: int):
+    """warm-up, reset, then `steps` timed steps with kernel timing on; returns (seconds, stats,
+    kernel times, final state)."""
+    sim.initial_evolve()
+    if warmup:
+        sim.run(1, warmup)
+    sim.initial_evolve()
+    sim.reset_stats()
+    sim.set_timing(True)
+    sync()
+    t0 = time.perf_counter()
+    sim.run(1, steps)
+    sync()
+    dt = time.perf_counter() - t0
+    kt = sim.kernel_times()
+    sim.set_timing(False)
+    st = sim.stats()
+    tp, _ = sim.get_state()
+    return dt, st, kt, tp
+
+
+def kernel_summary(kt: dict, n_cells: int) -> dict:
+    out = {}
+    for name, (ms, launches, by) in kt.items():
+        out[name] = dict(ms_total=round(ms, 4), launches=launches,
+                         us_per_launch=round(1000.0 * ms / launches, 3),
+                         bytes_per_cell=round(by / launches / n_cells, 2),
+                         gbs=round(by / (ms * 1e-3) / 1e9, 1))
+    return out
+
+
+def roofline(kt: dict) -> dict:
+    name = max(kt, key=lambda k: kt[k][0])
+    ms, launches, by = kt[name]
+    per_launch_bytes = by / launches
+    avg_s = ms * 1e-3 / launches
+    ach = per_launch_bytes / avg_s / 1e9
+    return dict(kernel=name, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                algorithmic_bytes_per_launch=per_launch_bytes,
+                avg_launch_us=round(avg_s * 1e6, 2))
+
+
+def rel_errors(a, ref):
+    e = np.abs(a - ref) / np.abs(ref)
+    return float(e.max()), float(np.sqrt(np.mean(e * e)))
+
+
+def cpu_baseline(n_full: int, tol: float, sample_n: int, sample_steps: int) -> dict:
+    """The reference Fortran path (oracle/_ref, amdflang -O3, serial) on the same physics at a
+    reduced grid; scaled to timesteps/s at n_full^3 by cell count (iterations per step do not
+    grow with N for this mass-dominated operator).  Falls back to the C restatement."""
+    import oracle
+    from heatflow_amd import workloads
+    s, _ = workloads.c3(sample_n)
+    n = s.params.n
+    tp = np.full(n, 300.0)
+    kind = "reference"
+    iters = []
+    if oracle.ref_available():
+        r = oracle.RefSystem(s)
+        r.set_state(tp, tp)
+        t0 = time.perf_counter()
+        for it in range(1, sample_steps + 1):
+            iters.append(r.simulate_qextra(it, s.cells.qdens_extra, tol))
+        dt = time.perf_counter() - t0
+    else:
+        kind = "port"
+        o = oracle.OracleSystem(s)
+        t0 = time.perf_counter()
+        rc, _, _, tot = o.run(sample_steps, tol=tol)
+        dt = time.perf_counter() - t0
+        iters = [tot]
+    cell_steps = n * sample_steps / dt
+    return dict(value=round(cell_steps / float(n_full) ** 3, 6), unit="timesteps/s", cores=1,
+                kind=kind,
+                sample=(f"{'reference Fortran modules (amdflang -O3, serial COO BiCG)' if kind == 'reference' else 'C restatement'}"
+                        f" on the same workload at {sample_n}^3, {sample_steps} steps, tol {tol:g}: "
+                        f"{dt:.2f} s, {sum(iters)} solver iterations; scaled by cell count to "
+                        f"{n_full}^3"),
+                sample_seconds=round(dt, 3))
+
+
+def grad_bench(n: int, steps: int, ckpt: int) -> dict:
+    from heatflow_amd import Simulation, workloads
+    out = {}
+    for icat, label in ((1, "cattaneo"), (0, "fourier")):
+        s, m = workloads.c4(n, icattaneo=icat)
+        sim = Simulation(s, tol=1e-32)
+        try:
+            sim.initial_evolve()
+            targets = np.full(len(m["probes"]), 300.05)
+            sync()
+            t0 = time.perf_counter()
+            loss, gk, gp = sim.grad(1, steps, ckpt, m["probes"], targets)
+            sync()
+            out[label] = dict(seconds=round(time.perf_counter() - t0, 3), loss=loss,
+                              grad_kappa_norm=float(np.linalg.norm(gk)),
+                              grad_power_norm=float(np.linalg.norm(gp[m["heaters"]])))
+        finally:
+            sim.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-ref", action="store_true", help="skip the FP64 1e-32 runs")
+    ap.add_argument("--only", choices=["fp32_ir", "fp64"], default=None,
+                    help="time one mode only (profiling)")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--grad-n", type=int, default=128)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU z-slab decomposition is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    from heatflow_amd import Simulation, workloads
+
+    t_gen = time.perf_counter()
+    s, meta = workloads.c3(args.n)
+    log(f"[bench] workload c3 {args.n}^3 generated in {time.perf_counter() - t_gen:.1f} s")
+    N = s.params.n
+    K, W = args.steps, args.warmup
+
+    results = {}
+    modes = [("fp32_ir", args.tol), ("fp64", args.tol)]
+    if args.only:
+        modes = [(args.only, args.tol)]
+    elif not args.no_ref:
+        modes.append(("fp64", 1e-32))
+    states = {}
+    for prec, tol in modes:
+        key = f"{prec}@{tol:g}"
+        t_setup = time.perf_counter()
+        sim = Simulation(s, precision=prec, tol=tol)
+        try:
+            setup = time.perf_counter() - t_setup
+            dt, st, kt, tp = timed_run(sim, K, W)
+        finally:
+            sim.close()
+        states[key] = tp
+        iters = st.inner_iters if prec == "fp32_ir" else st.outer_iters
+        results[key] = dict(
+            timesteps_per_s=K / dt, ms_per_step=1000.0 * dt / K,
+            cg_iters=int(iters), refine_sweeps=int(st.outer_iters) if prec == "fp32_ir" else None,
+            cg_iters_per_s=iters / dt, setup_s=round(setup, 2),
+            kernels=kernel_summary(kt, N), roofline=roofline(kt), last_err=st.last_err)
+        log(f"[bench] {key}: {K / dt:.2f} steps/s, {1000 * dt / K:.2f} ms/step, "
+            f"{iters} CG iterations, roofline {results[key]['roofline']}")
+
+    ref_key = "fp64@1e-32"
+    if ref_key in states:
+        for key in results:
+            if key != ref_key:
+                mx, rms = rel_errors(states[key], states[ref_key])
+                results[key]["err_vs_fp64_reftol"] = dict(max=mx, rms=rms)
+    del states
+
+    head = results.get(f"fp32_ir@{args.tol:g}") or next(iter(results.values()))
+    fp64 = results.get(f"fp64@{args.tol:g}")
+    grad = None if (args.no_grad or args.only) else grad_bench(args.grad_n, 200, 20)
+    cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, args.tol, args.cpu_n,
+                                                                args.cpu_steps)
+    line = {
+        "metric": "timesteps/s at 512^3 (FP32-refined vs FP64, 1e-10 rel); fwd+adjoint grad time",
+        "value": round(head["timesteps_per_s"], 4),
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-refined" if (args.only or "fp32_ir") == "fp32_ir" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"c3: {args.n}^3 Cattaneo, log-normal kappa exp(N(ln50,1)) seed 3, "
+                               f"Gaussian heating sigma=32 cells peak 1e12 W/m^3, Dirichlet 300 K "
+                               f"on 6 faces, dt=5e-11, {K} steps, CG tol {args.tol:g}",
+                   "cells": N},
+        "fp32_ir_over_fp64": (round(head["timesteps_per_s"] / fp64["timesteps_per_s"], 3)
+                              if fp64 and head is not fp64 else None),
+        "modes": results,
+        "roofline": head["roofline"],
+        "cpu_baseline": cpu,
+        "grad_128": grad,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,318 @@
+// hf_adjoint.hip -- reverse-time adjoint of the HeatFlow time step (no reference counterpart).
+//
+// Forward step n (src/heatflow/mod_evolve.f90:112-170):  H T^{n+1} = S^n,
+//   S^n = -a T^n - Qdens - B + c (T^{n-1} - 2 T^n),  a = rho c_p / dt,  c = tau' rho c_p
+// (c = 0 for Fourier, a = c = 0 for steady state).  H is symmetric (mod_setup.f90:138-145), so
+// the adjoint solve is the forward solver on the same operator:
+//   lambda^{n+1} = H^{-1} gbar^{n+1},
+//   gbar^n     += -(a + 2c) lambda^{n+1},   gbar^{n-1} += c lambda^{n+1},
+//   dL/dkappa_I += sum_faces dG/dkappa_I (lambda_I - lambda_J)(T_I - T_J)
+//                + sum_bfaces dGb/dkappa_I lambda_I (T_I - [T_bath surviving in B_I]),
+//   dL/dP_I     += -lambda_I / V_I          (P_I enters as Qdens_I += P_I / V_I),
+// with T = T^{n+1}.  G is the harmonic-mean face conductance of hmatrixfunc
+// (src/heatflow/mod_hmatrix.f90:288-314, 357-385) for per-axis uniform lengths:
+//   G = 2 k_I k_J / ((k_I + k_J) L^2) [x BR if k_I != k_J],  dG/dk_I = 2 k_J^2 / ((k_I+k_J)^2 L^2).
+// The BR switch is treated as locally constant (it is discontinuous at k_I == k_J).
+//
+// Memory: (T^n, T^{n-1}) checkpoints every K steps plus the K states of the segment being
+// reversed, recomputed from its checkpoint.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "hf_ctx.hpp"
+
+namespace hf {
+
+struct AdjointState {
+    size_t N = 0;
+    int64_t nseg_cap = 0, k_cap = 0;
+    double *ckpt = nullptr;   // nseg x 2 x N
+    double *seg = nullptr;    // K x N
+    double *g_hi = nullptr, *g_lo = nullptr, *g_new = nullptr, *lam = nullptr;
+    double *gk = nullptr, *gp = nullptr;
+    double *probe_val = nullptr;  // device, nprobe
+    int64_t *probe_idx = nullptr;
+    double *probe_w = nullptr, *probe_t = nullptr;
+    int64_t nprobe_cap = 0;
+};
+
+void adjoint_free(AdjointState *a) {
+    if (!a) return;
+    hipFree(a->ckpt); hipFree(a->seg);
+    hipFree(a->g_hi); hipFree(a->g_lo); hipFree(a->g_new); hipFree(a->lam);
+    hipFree(a->gk); hipFree(a->gp);
+    hipFree(a->probe_val); hipFree(a->probe_idx); hipFree(a->probe_w); hipFree(a->probe_t);
+    delete a;
+}
+
+struct AdjParams {
+    double inverse_time;
+    double kb[6], tb[6], br;
+    int32_t isteady, icattaneo;
+    int32_t bface[6];   // face is a domain boundary that enters the diagonal (non-periodic)
+    int32_t cg[6];
+};
+
+__device__ __forceinline__ double dG_int(double ki, double kj, double L, double br) {
+    double s = ki + kj;
+    double v = 2.0 * kj * kj / (s * s * L * L);
+    return ki != kj ? v * br : v;
+}
+
+__device__ __forceinline__ double dG_bnd(double kb, double k, double L, double br) {
+    double s = kb + k;
+    double v = 2.0 * kb * kb / (s * s * L * L);
+    return k != kb ? v * br : v;
+}
+
+// One reverse step: gradient contributions of step n (T = T^{n+1}, lam = lambda^{n+1}) and the
+// adjoint history update.  g_lo (partial gbar^n) += -(a+2c) lam; g_new = c lam (gbar^{n-1}).
+__global__ void __launch_bounds__(kBlock)
+k_adj_step(Dims g, AdjParams q, const double *__restrict__ T, const double *__restrict__ lam,
+           const double *__restrict__ kappa, const double *__restrict__ lx,
+           const double *__restrict__ ly, const double *__restrict__ lz,
+           const double *__restrict__ rhoc, const double *__restrict__ cvec,
+           const double *__restrict__ vol, double *__restrict__ g_lo,
+           double *__restrict__ g_new, double *__restrict__ gk, double *__restrict__ gp) {
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        Nbr n = neighbours(g, I);
+        const double li = lam[I], ti = T[I], ki = kappa[I];
+        double acc = 0.0;
+        const bool has[6] = {n.hxm, n.hxp, n.hym, n.hyp, n.hzm, n.hzp};
+        const uint32_t nb[6] = {n.xm, n.xp, n.ym, n.yp, n.zm, n.zp};
+        const double Ls[3] = {lx[I], ly[I], lz[I]};
+        const uint32_t pos[3] = {n.x, n.y, n.z};
+        const uint32_t len[3] = {g.nx, g.ny, g.nz};
+        // which boundary faces still contribute to B_I after CG assignments (mod_boundary.f90:72-150)
+        int survive = 0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int ax = f >> 1;
+            const bool at_face = pos[ax] == ((f & 1) ? len[ax] - 1 : 0u);
+            if (!q.bface[f] || !at_face) continue;
+            if (q.cg[f]) survive = 0;
+            else survive |= (1 << f);
+        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int ax = f >> 1;
+            if (has[f]) {
+                const uint32_t J = nb[f];
+                acc += dG_int(ki, kappa[J], Ls[ax], q.br) * (li - lam[J]) * (ti - T[J]);
+            } else {
+                const bool at_face = pos[ax] == ((f & 1) ? len[ax] - 1 : 0u);
+                if (q.bface[f] && at_face) {
+                    const double tb = (survive >> f) & 1 ? q.tb[f] : 0.0;
+                    acc += dG_bnd(q.kb[f], ki, Ls[ax], q.br) * li * (ti - tb);
+                }
+            }
+        }
+        gk[I] += acc;
+        gp[I] += -li / vol[I];
+        double a = 0.0, cc = 0.0;
+        if (q.isteady == 0) {
+            a = q.inverse_time * rhoc[I];
+            cc = q.icattaneo == 1 ? cvec[I] : 0.0;
+        }
+        g_lo[I] += -(a + 2.0 * cc) * li;
+        g_new[I] = cc * li;
+    }
+}
+
+// loss and the seed gbar^{N} at the probes; one thread, fixed order (duplicates accumulate)
+__global__ void k_adj_seed(int64_t np, const int64_t *__restrict__ idx,
+                           const double *__restrict__ w, const double *__restrict__ t,
+                           const double *__restrict__ T, double *__restrict__ g,
+                           double *__restrict__ out_loss) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double L = 0.0;
+    for (int64_t k = 0; k < np; ++k) {
+        const double diff = T[idx[k]] - t[k];
+        L += w[k] * diff * diff;
+        g[idx[k]] += 2.0 * w[k] * diff;
+    }
+    *out_loss = L;
+}
+
+static int ensure(hf_ctx *c, double **p, size_t n) {
+    if (*p) return HF_OK;
+    if (hipMalloc(p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return ctx_set_err(c, HF_ERR_ALLOC, "hf_grad: device allocation failed");
+    }
+    return HF_OK;
+}
+
+#define ATRY(x)                        \
+    do {                               \
+        int rc_ = (x);                 \
+        if (rc_ != HF_OK) return rc_;  \
+    } while (0)
+#define AHIP(c, call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ctx_set_err(c, HF_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// restore (T^n, T^{n-1}) into the context state and recompute the TINY flag
+static int restore_state(hf_ctx *c, const double *tp, const double *tpp) {
+    const size_t B8 = (size_t)c->g.N * 8;
+    AHIP(c, hipMemcpyAsync(c->T[0], tp, B8, hipMemcpyDeviceToDevice, c->st));
+    AHIP(c, hipMemcpyAsync(c->T[1], tpp, B8, hipMemcpyDeviceToDevice, c->st));
+    launch_step_begin(c->sc, c->st);
+    launch_tiny_from_state(c->g, c->T[0], c->T[1], c->sc, c->st);
+    AHIP(c, hipGetLastError());
+    return HF_OK;
+}
+
+int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, int64_t nprobe,
+                 const int64_t *probe_idx, const double *target, const double *weight,
+                 double *loss, double *dl_dkappa, double *dl_dpower) {
+    if (!c->assembled || !c->has_state)
+        return ctx_set_err(c, HF_ERR_STATE, "hf_grad: needs hf_assemble and hf_set_state first");
+    if (c->p.precision != HF_PREC_FP64)
+        return ctx_set_err(c, HF_ERR_STATE, "hf_grad: needs an FP64 context");
+    if (nsteps < 1 || ckpt_every < 1 || nprobe < 1 || !probe_idx || !target)
+        return ctx_set_err(c, HF_ERR_ARG, "hf_grad: bad arguments");
+    const hf_params &p = c->p;
+    for (int f = 0; f < 6; ++f)
+        if (p.cg[f] && p.icattaneo < 0) return ctx_set_err(c, HF_ERR_ARG, "hf_grad: bad CG");
+    const size_t N = c->g.N, B8 = N * 8;
+    // per-axis uniform lengths (the reference's system.in gives Lx/nx etc., mod_inputs.f90:619-622)
+    {
+        std::vector<double> h(N);
+        const double *arrs[3] = {c->lx, c->ly, c->lz};
+        for (int a = 0; a < 3; ++a) {
+            AHIP(c, hipMemcpy(h.data(), arrs[a], B8, hipMemcpyDeviceToHost));
+            for (size_t i = 1; i < N; ++i)
+                if (h[i] != h[0])
+                    return ctx_set_err(c, HF_ERR_ARG, "hf_grad: cell lengths must be uniform per axis");
+        }
+    }
+    for (int64_t k = 0; k < nprobe; ++k)
+        if (probe_idx[k] < 0 || probe_idx[k] >= (int64_t)N)
+            return ctx_set_err(c, HF_ERR_ARG, "hf_grad: probe index out of range");
+
+    if (!c->adj) c->adj = new AdjointState();
+    AdjointState &A = *c->adj;
+    const int64_t K = ckpt_every;
+    const int64_t nseg = (nsteps + K - 1) / K;
+    if (A.N != N || A.nseg_cap < nseg || A.k_cap < K) {
+        hipFree(A.ckpt); hipFree(A.seg);
+        A.ckpt = A.seg = nullptr;
+        A.nseg_cap = nseg;
+        A.k_cap = K;
+    }
+    if (A.N != N) {
+        hipFree(A.g_hi); hipFree(A.g_lo); hipFree(A.g_new); hipFree(A.lam);
+        hipFree(A.gk); hipFree(A.gp);
+        A.g_hi = A.g_lo = A.g_new = A.lam = A.gk = A.gp = nullptr;
+        A.N = N;
+    }
+    ATRY(ensure(c, &A.ckpt, (size_t)2 * N * nseg));
+    ATRY(ensure(c, &A.seg, (size_t)N * K));
+    ATRY(ensure(c, &A.g_hi, N)); ATRY(ensure(c, &A.g_lo, N)); ATRY(ensure(c, &A.g_new, N));
+    ATRY(ensure(c, &A.lam, N)); ATRY(ensure(c, &A.gk, N)); ATRY(ensure(c, &A.gp, N));
+    if (A.nprobe_cap < nprobe) {
+        hipFree(A.probe_val); hipFree(A.probe_idx); hipFree(A.probe_w); hipFree(A.probe_t);
+        A.probe_val = A.probe_w = A.probe_t = nullptr;
+        A.probe_idx = nullptr;
+        ATRY(ensure(c, &A.probe_val, 1));
+        ATRY(ensure(c, &A.probe_w, nprobe));
+        ATRY(ensure(c, &A.probe_t, nprobe));
+        if (hipMalloc(&A.probe_idx, nprobe * sizeof(int64_t)) != hipSuccess)
+            return ctx_set_err(c, HF_ERR_ALLOC, "hf_grad: probe allocation failed");
+        A.nprobe_cap = nprobe;
+    }
+    std::vector<double> w(nprobe, 1.0);
+    if (weight) std::copy(weight, weight + nprobe, w.begin());
+    AHIP(c, hipMemcpyAsync(A.probe_idx, probe_idx, nprobe * 8, hipMemcpyDefault, c->st));
+    AHIP(c, hipMemcpyAsync(A.probe_t, target, nprobe * 8, hipMemcpyDefault, c->st));
+    AHIP(c, hipMemcpyAsync(A.probe_w, w.data(), nprobe * 8, hipMemcpyHostToDevice, c->st));
+
+    // ---- forward with checkpoints -------------------------------------------------------------
+    for (int64_t s = 0; s < nseg; ++s) {
+        AHIP(c, hipMemcpyAsync(A.ckpt + (size_t)2 * N * s, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
+        AHIP(c, hipMemcpyAsync(A.ckpt + (size_t)2 * N * s + N, c->T[1], B8, hipMemcpyDeviceToDevice, c->st));
+        const int64_t e = std::min(nsteps, (s + 1) * K);
+        for (int64_t n = s * K; n < e; ++n) ATRY(ctx_step(c, itime0 + n, n + 1 == nsteps));
+    }
+    // final state, kept to restore the context afterwards (T^{N} in g_new, T^{N-1} in lam)
+    AHIP(c, hipMemcpyAsync(A.g_new, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
+    AHIP(c, hipMemcpyAsync(A.lam, c->T[1], B8, hipMemcpyDeviceToDevice, c->st));
+    // seed
+    AHIP(c, hipMemsetAsync(A.g_hi, 0, B8, c->st));
+    hipLaunchKernelGGL(k_adj_seed, dim3(1), dim3(64), 0, c->st, nprobe, A.probe_idx, A.probe_w,
+                       A.probe_t, c->T[0], A.g_hi, A.probe_val);
+    AHIP(c, hipGetLastError());
+    double L = 0.0;
+    AHIP(c, hipMemcpyAsync(&L, A.probe_val, 8, hipMemcpyDeviceToHost, c->st));
+    AHIP(c, hipStreamSynchronize(c->st));
+    // stash the final state in the checkpoint area's spare? keep it on the host side instead
+    std::vector<double> fin_tp(N), fin_tpp(N);
+    AHIP(c, hipMemcpy(fin_tp.data(), A.g_new, B8, hipMemcpyDeviceToHost));
+    AHIP(c, hipMemcpy(fin_tpp.data(), A.lam, B8, hipMemcpyDeviceToHost));
+
+    AHIP(c, hipMemsetAsync(A.g_lo, 0, B8, c->st));
+    AHIP(c, hipMemsetAsync(A.lam, 0, B8, c->st));
+    AHIP(c, hipMemsetAsync(A.gk, 0, B8, c->st));
+    AHIP(c, hipMemsetAsync(A.gp, 0, B8, c->st));
+
+    AdjParams q{};
+    q.inverse_time = 1.0 / p.time_step;
+    for (int f = 0; f < 6; ++f) {
+        q.kb[f] = p.kappa_bound[f];
+        q.tb[f] = p.t_bath[f];
+        q.cg[f] = p.cg[f] != 0;
+        const int ax = f >> 1;
+        q.bface[f] = !p.periodic[ax];
+    }
+    q.br = p.br;
+    q.isteady = p.isteady;
+    q.icattaneo = p.icattaneo;
+
+    // ---- reverse sweep ------------------------------------------------------------------------
+    for (int64_t s = nseg - 1; s >= 0; --s) {
+        const int64_t b = s * K, e = std::min(nsteps, (s + 1) * K);
+        ATRY(restore_state(c, A.ckpt + (size_t)2 * N * s, A.ckpt + (size_t)2 * N * s + N));
+        for (int64_t n = b; n < e; ++n) {
+            ATRY(ctx_step(c, itime0 + n, false));
+            AHIP(c, hipMemcpyAsync(A.seg + (size_t)N * (n - b), c->T[0], B8,
+                                   hipMemcpyDeviceToDevice, c->st));
+        }
+        for (int64_t n = e - 1; n >= b; --n) {
+            int64_t it = 0;
+            // lambda^{n+1} = H^{-1} gbar^{n+1}, warm-started from lambda^{n+2}
+            ATRY(ctx_solve_dev(c, A.g_hi, A.lam, p.tol, p.itmax, &it));
+            const double *Tn1 = A.seg + (size_t)N * (n - b);
+            hipLaunchKernelGGL(k_adj_step, dim3(c->g.nblk), dim3(kBlock), 0, c->st, c->g, q, Tn1,
+                               A.lam, c->kappa, c->lx, c->ly, c->lz, c->rhoc, c->cvec, c->vol,
+                               A.g_lo, A.g_new, A.gk, A.gp);
+            AHIP(c, hipGetLastError());
+            // shift: gbar^n (complete) becomes the next right-hand side
+            double *old_hi = A.g_hi;
+            A.g_hi = A.g_lo;
+            A.g_lo = A.g_new;
+            A.g_new = old_hi;
+        }
+    }
+    if (dl_dkappa) AHIP(c, hipMemcpyAsync(dl_dkappa, A.gk, B8, hipMemcpyDefault, c->st));
+    if (dl_dpower) AHIP(c, hipMemcpyAsync(dl_dpower, A.gp, B8, hipMemcpyDefault, c->st));
+    AHIP(c, hipStreamSynchronize(c->st));
+    // leave the context at the end of the forward run
+    AHIP(c, hipMemcpy(A.g_new, fin_tp.data(), B8, hipMemcpyHostToDevice));
+    AHIP(c, hipMemcpy(A.g_lo, fin_tpp.data(), B8, hipMemcpyHostToDevice));
+    ATRY(restore_state(c, A.g_new, A.g_lo));
+    AHIP(c, hipStreamSynchronize(c->st));
+    if (loss) *loss = L;
+    return HF_OK;
+}
+
+}  // namespace hf

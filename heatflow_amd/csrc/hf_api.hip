@@ -1,0 +1,848 @@
+// hf_api.hip -- host side of the C-ABI declared in include/heatflow_hip.h.
+//
+// One hf_ctx owns one HIP stream and all device buffers of one grid.  The time step mirrors
+// evolution::simulate (src/heatflow/mod_evolve.f90:45-177): heater (when time-dependent) ->
+// fused S / x0 / residual -> CG on the Jacobi-scaled operator with device-resident scalars ->
+// x = x0 + s*y -> history rotation by pointer swap.  The host synchronises once per batch of
+// CG iterations (the batch length follows the previous step's iteration count) to read the
+// convergence flag; no per-iteration host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/heatflow_hip.h"
+#include "hf_device.hpp"
+#include "hf_kernels.hpp"
+#include "hf_ctx.hpp"
+
+using namespace hf;
+
+static thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------------------------
+
+static int set_err(hf_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_err(c, HF_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+static int dalloc(hf_ctx *c, T **ptr, size_t n) {
+    if (*ptr) return HF_OK;
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(c, HF_ERR_ALLOC, "hipMalloc of " + std::to_string(n * sizeof(T)) +
+                                            " bytes failed: " + hipGetErrorString(e));
+    }
+    c->allocs.push_back(v);
+    *ptr = (T *)v;
+    return HF_OK;
+}
+
+#define TRY(x)                       \
+    do {                             \
+        int rc_ = (x);               \
+        if (rc_ != HF_OK) return rc_; \
+    } while (0)
+
+static int check_launch(hf_ctx *c, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, HF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return HF_OK;
+}
+
+// ---- kernel timing -----------------------------------------------------------------------
+
+// Opens a timed launch: records the start event on the context stream; returns the slot or -1.
+static long tbeg(hf_ctx *c, int id, double bytes) {
+    if (!c->timing) return -1;
+    if (c->tused == c->tpool.size()) {
+        hf_ctx::Timed t{};
+        if (hipEventCreate(&t.a) != hipSuccess || hipEventCreate(&t.b) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1;
+        }
+        c->tpool.push_back(t);
+    }
+    hf_ctx::Timed &t = c->tpool[c->tused];
+    t.id = id;
+    t.bytes = bytes;
+    t.valid = true;
+    (void)hipEventRecord(t.a, c->st);
+    return (long)c->tused++;
+}
+
+static void tend(hf_ctx *c, long slot) {
+    if (slot >= 0) (void)hipEventRecord(c->tpool[slot].b, c->st);
+}
+
+static void make_dims(Dims &g, const hf_params &p) {
+    g.nx = (uint32_t)p.nx; g.ny = (uint32_t)p.ny; g.nz = (uint32_t)p.nz;
+    g.nxny = g.nx * g.ny;
+    g.N = g.nxny * g.nz;
+    g.per[0] = p.periodic[0] && p.nx >= 3;
+    g.per[1] = p.periodic[1] && p.ny >= 3;
+    g.per[2] = p.periodic[2] && p.nz >= 3;
+    g.dnx.init(g.nx);
+    g.dny.init(g.ny);
+    // launch layout: up to `cap` blocks of kBlock threads, contiguous chunks of cells
+    uint32_t cap = 8192;
+    if (const char *e = getenv("HF_NBLK")) cap = std::max(8u, (uint32_t)atoi(e) / 8u * 8u);
+    uint64_t want = ((uint64_t)g.N + kBlock - 1) / kBlock;
+    uint64_t nb = std::min<uint64_t>(want, cap);
+    uint64_t chunk = ((uint64_t)g.N + nb - 1) / nb;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    nb = ((uint64_t)g.N + chunk - 1) / chunk;
+    nb = (nb + 7) / 8 * 8;
+    g.chunk = (uint32_t)chunk;
+    g.nblk = (uint32_t)nb;
+}
+
+// ---------------------------------------------------------------------------------------------
+
+extern "C" void hf_params_default(hf_params *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->nx = p->ny = p->nz = 1;
+    p->time_step = 1.0;      // mod_inputs.f90:226
+    p->icattaneo = 1;        // :229
+    p->isteady = 0;
+    p->br = 1.0;             // :242
+    p->freq = 1.0;           // :227
+    p->tol = 1.0e-32;        // mod_evolve.f90:143
+    p->itmax = 50000;        // :144
+    p->precision = HF_PREC_FP64;
+    p->inner_tol = 1.0e-6;
+    p->inner_itmax = 200;
+}
+
+extern "C" const char *hf_version(void) { return "heatflow_hip 0.1 (gfx950)"; }
+
+extern "C" const char *hf_last_error(const hf_ctx *c) {
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+static bool allocs_base(hf_ctx *c, int *rc) {
+    const size_t N = c->g.N;
+    int r = HF_OK;
+    if ((r = dalloc(c, &c->kappa, N)) || (r = dalloc(c, &c->rhoc, N)) ||
+        (r = dalloc(c, &c->tau_s, N)) || (r = dalloc(c, &c->cvec, N)) ||
+        (r = dalloc(c, &c->ells, N)) || (r = dalloc(c, &c->lx, N)) || (r = dalloc(c, &c->ly, N)) ||
+        (r = dalloc(c, &c->lz, N)) || (r = dalloc(c, &c->vol, N)) ||
+        (r = dalloc(c, &c->iheater, N)) || (r = dalloc(c, &c->d, N)) ||
+        (r = dalloc(c, &c->cx, N)) || (r = dalloc(c, &c->cy, N)) || (r = dalloc(c, &c->cz, N)) ||
+        (r = dalloc(c, &c->bvec, N)) || (r = dalloc(c, &c->sv, N)) || (r = dalloc(c, &c->qd, N)) ||
+        (r = dalloc(c, &c->T[0], N)) || (r = dalloc(c, &c->T[1], N)) ||
+        (r = dalloc(c, &c->sc, 1)) || (r = dalloc(c, &c->sci, 1)) ||
+        (r = dalloc(c, &c->partials, (size_t)4 * c->g.nblk))) {
+        *rc = r;
+        return false;
+    }
+    return true;
+}
+
+static int alloc_mode(hf_ctx *c) {
+    const size_t N = c->g.N;
+    if (c->p.precision == HF_PREC_FP32_IR) {
+        TRY(dalloc(c, &c->fx, N)); TRY(dalloc(c, &c->fy, N)); TRY(dalloc(c, &c->fz, N));
+        TRY(dalloc(c, &c->r32, N)); TRY(dalloc(c, &c->p32[0], N)); TRY(dalloc(c, &c->p32[1], N));
+        TRY(dalloc(c, &c->q32, N)); TRY(dalloc(c, &c->y32, N));
+        TRY(dalloc(c, &c->S, N)); TRY(dalloc(c, &c->T[2], N));
+    } else {
+        TRY(dalloc(c, &c->hx, N)); TRY(dalloc(c, &c->hy, N)); TRY(dalloc(c, &c->hz, N));
+        TRY(dalloc(c, &c->r64, N)); TRY(dalloc(c, &c->p64[0], N)); TRY(dalloc(c, &c->p64[1], N));
+        TRY(dalloc(c, &c->q64, N)); TRY(dalloc(c, &c->y64, N));
+    }
+    return HF_OK;
+}
+
+extern "C" int hf_create(hf_ctx **out, const hf_params *p) {
+    if (!out || !p) return set_err(nullptr, HF_ERR_ARG, "hf_create: null argument");
+    *out = nullptr;
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1)
+        return set_err(nullptr, HF_ERR_ARG, "hf_create: nx, ny, nz must be >= 1");
+    const long double n = (long double)p->nx * p->ny * p->nz;
+    if (n > 2147483647.0L) return set_err(nullptr, HF_ERR_ARG, "hf_create: more than 2^31-1 cells");
+    if (p->nx == 1 && n > 1)
+        return set_err(nullptr, HF_ERR_ARG,
+                       "hf_create: nx == 1 with ny*nz > 1 is not supported (the reference's "
+                       "sparse_Hmatrix emits duplicate offsets and overruns its COO length)");
+    for (int a = 0; a < 3; ++a) {
+        int64_t len = a == 0 ? p->nx : (a == 1 ? p->ny : p->nz);
+        if (p->periodic[a] && len < 3)
+            return set_err(nullptr, HF_ERR_ARG,
+                           "hf_create: periodic axes need length >= 3 (length 1 zeroes the "
+                           "diagonal in hmatrixfunc's wrap test, length 2 duplicates COO offsets)");
+    }
+    if (!(p->time_step > 0.0) && p->isteady == 0)
+        return set_err(nullptr, HF_ERR_ARG, "hf_create: time_step must be > 0");
+    if (p->precision != HF_PREC_FP64 && p->precision != HF_PREC_FP32_IR)
+        return set_err(nullptr, HF_ERR_ARG, "hf_create: unknown precision mode");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return set_err(nullptr, HF_ERR_NO_DEVICE, "hf_create: no HIP device");
+    }
+    hf_ctx *c = new hf_ctx();
+    c->p = *p;
+    hipGetDevice(&c->dev);
+    make_dims(c->g, c->p);
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        std::string m = "hf_create: hipStreamCreate failed";
+        delete c;
+        return set_err(nullptr, HF_ERR_HIP, m);
+    }
+    int rc = HF_OK;
+    if (!allocs_base(c, &rc) || (rc = alloc_mode(c)) != HF_OK ||
+        hipHostMalloc((void **)&c->h_sc, sizeof(Scal), 0) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_sci, sizeof(Scal), 0) != hipSuccess) {
+        std::string m = c->err.empty() ? "hf_create: allocation failed" : c->err;
+        hf_destroy(c);
+        return set_err(nullptr, rc ? rc : HF_ERR_ALLOC, m);
+    }
+    hipMemsetAsync(c->sc, 0, sizeof(Scal), c->st);
+    hipMemsetAsync(c->sci, 0, sizeof(Scal), c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+        hf_destroy(c);
+        return set_err(nullptr, HF_ERR_HIP, "hf_create: stream sync failed");
+    }
+    // static assembly parameters
+    AsmParams &ap = c->ap;
+    ap.inverse_time = 1.0 / p->time_step;
+    for (int f = 0; f < 6; ++f) {
+        ap.kb[f] = p->kappa_bound[f];
+        ap.tb[f] = p->t_bath[f];
+        ap.cg[f] = p->cg[f] != 0;
+        ap.cg_pd[f] = 0.0;
+    }
+    ap.br = p->br;
+    ap.isteady = p->isteady;
+    ap.icattaneo = p->icattaneo;
+    for (int a = 0; a < 3; ++a) ap.periodic[a] = p->periodic[a] != 0;
+    *out = c;
+    return HF_OK;
+}
+
+extern "C" void hf_destroy(hf_ctx *c) {
+    if (!c) return;
+    if (c->st) hipStreamSynchronize(c->st);
+    hf::adjoint_free(c->adj);
+    for (auto &t : c->tpool) {
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    for (void *v : c->allocs) hipFree(v);
+    if (c->h_sc) hipHostFree(c->h_sc);
+    if (c->h_sci) hipHostFree(c->h_sci);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+// derived per-cell fields: lin_rhoc = rho*c_p (mod_setup.f90:64), tau*lin_rhoc (S_catS),
+// em*L1*L2*StefBoltz (mod_heating.f90:121-122)
+__global__ void k_derived(uint32_t N, const double *__restrict__ rho,
+                          const double *__restrict__ hc, const double *__restrict__ tau_s,
+                          const double *__restrict__ em, const double *__restrict__ lx,
+                          const double *__restrict__ ly, double *__restrict__ rhoc,
+                          double *__restrict__ cvec, double *__restrict__ ells) {
+    uint32_t I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    double rc = rho[I] * hc[I];
+    rhoc[I] = rc;
+    cvec[I] = tau_s[I] * rc;
+    ells[I] = em ? em[I] * lx[I] * ly[I] * kStefBoltz : 0.0;
+}
+
+extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho,
+                               const double *heat_capacity, const double *tau_scaled,
+                               const double *em, const double *len_x, const double *len_y,
+                               const double *len_z, const double *volume,
+                               const int64_t *iheater, const double *qdens_extra) {
+    if (!c) return HF_ERR_ARG;
+    if (!kappa || !rho || !heat_capacity || !tau_scaled || !len_x || !len_y || !len_z || !volume)
+        return set_err(c, HF_ERR_ARG, "hf_upload_cells: required array is NULL");
+    const size_t N = c->g.N, B8 = N * sizeof(double);
+    double *rho_d = nullptr, *hc_d = nullptr, *em_d = nullptr;
+    // rho and heat_capacity are staged in the (not yet used) CG work buffers
+    rho_d = c->p.precision == HF_PREC_FP64 ? c->r64 : c->S;
+    hc_d = c->p.precision == HF_PREC_FP64 ? c->q64 : c->T[2];
+    HIPCHK(c, hipMemcpyAsync(c->kappa, kappa, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(rho_d, rho, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(hc_d, heat_capacity, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->tau_s, tau_scaled, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->lx, len_x, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->ly, len_y, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->lz, len_z, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->vol, volume, B8, hipMemcpyDefault, c->st));
+    if (em) {
+        em_d = c->qd;  // staged; qd is rebuilt at assembly
+        HIPCHK(c, hipMemcpyAsync(em_d, em, B8, hipMemcpyDefault, c->st));
+    }
+    hipLaunchKernelGGL(k_derived, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->st,
+                       (uint32_t)N, rho_d, hc_d, c->tau_s, em_d, c->lx, c->ly, c->rhoc, c->cvec,
+                       c->ells);
+    TRY(check_launch(c, "k_derived"));
+    if (qdens_extra) {
+        TRY(dalloc(c, &c->qextra, N));
+        HIPCHK(c, hipMemcpyAsync(c->qextra, qdens_extra, B8, hipMemcpyDefault, c->st));
+    }
+    // heater bookkeeping on the host: heated_volume is the sequential sum of grid%volume
+    // over iheater > 0 in loop order (mod_heating.f90:137-141)
+    std::vector<int64_t> ih(iheater ? N : 0);
+    std::vector<double> vh, emh;
+    if (iheater) HIPCHK(c, hipMemcpy(ih.data(), iheater, N * 8, hipMemcpyDefault));
+    c->any_heater = false;
+    c->dyn_heat = false;
+    c->heated_volume = 0.0;
+    std::vector<int8_t> ih8(N, 0);
+    if (iheater) {
+        vh.resize(N);
+        HIPCHK(c, hipMemcpy(vh.data(), volume, B8, hipMemcpyDefault));
+        bool other_modes = false;
+        for (size_t i = 0; i < N; ++i) {
+            int64_t m = ih[i];
+            ih8[i] = m <= 0 ? 0 : (m <= 6 ? (int8_t)m : 7);
+            if (m > 0) {
+                c->any_heater = true;
+                c->heated_volume = c->heated_volume + vh[i];
+                if (m != 1 && m <= 6) other_modes = true;
+            }
+        }
+        bool any_em = false;
+        if (em && c->any_heater) {
+            emh.resize(N);
+            HIPCHK(c, hipMemcpy(emh.data(), em, B8, hipMemcpyDefault));
+            for (size_t i = 0; i < N; ++i)
+                if (emh[i] != 0.0) { any_em = true; break; }
+        }
+        c->dyn_heat = c->any_heater && (other_modes || any_em);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->iheater, ih8.data(), N, hipMemcpyHostToDevice, c->st));
+    // CG_dir power densities use grid(1,1,1), (1,1,nz), (1,ny,1), (nx,1,1) volumes
+    // (mod_boundary.f90:225-248)
+    {
+        const hf_params &p = c->p;
+        auto vol_at = [&](int64_t x, int64_t y, int64_t z) -> double {
+            double v = 0.0;
+            size_t idx = (size_t)(x + p.nx * (y + p.ny * z));
+            hipMemcpy(&v, volume + idx, sizeof(double), hipMemcpyDefault);
+            return v;
+        };
+        const double nx = (double)p.nx, ny = (double)p.ny, nz = (double)p.nz;
+        if (p.cg[5]) c->ap.cg_pd[5] = p.t_bath_cg / (vol_at(0, 0, p.nz - 1) * nx * ny);
+        if (p.cg[4]) c->ap.cg_pd[4] = p.t_bath_cg / (vol_at(0, 0, 0) * nx * ny);
+        if (p.cg[2]) c->ap.cg_pd[2] = p.t_bath_cg / (vol_at(0, 0, 0) * nx * nz);
+        if (p.cg[3]) c->ap.cg_pd[3] = p.t_bath_cg / (vol_at(0, p.ny - 1, 0) * nx * nz);
+        if (p.cg[0]) c->ap.cg_pd[0] = p.t_bath_cg / (vol_at(0, 0, 0) * ny * nz);
+        if (p.cg[1]) c->ap.cg_pd[1] = p.t_bath_cg / (vol_at(p.nx - 1, 0, 0) * ny * nz);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->uploaded = true;
+    c->assembled = false;
+    return HF_OK;
+}
+
+static HeatParams heat_params(const hf_ctx *c, int64_t itime) {
+    // mode powers of this step (src/heatflow/mod_heating.f90:39-117), host libm
+    const double PI = 3.141592653589793;
+    const hf_params &p = c->p;
+    HeatParams hp{};
+    const double POWER = p.power_in;
+    const double time = p.time_step * (double)itime;
+    const double time_pulse = (double)p.heated_steps * p.time_step;
+    hp.qmode[0] = 0.0;
+    hp.qmode[1] = POWER;
+    hp.qmode[2] = (time <= time_pulse) ? POWER : 0.0;
+    {
+        double x = time * 2.0 * PI * p.freq;
+        double x2 = p.time_step * 2.0 * PI * p.freq;
+        hp.qmode[3] = POWER * 0.5 * ((x2) - sin(x + x2) * cos(x + x2) + sin(x) * cos(x)) / x2;
+    }
+    hp.qmode[4] = POWER * pow(sin(time * 2.0 * PI * p.freq), 2.0);
+    hp.qmode[5] = POWER * pow(sin(time * 2.0 * PI * p.freq), 2.0);
+    hp.q5b = POWER * 2.0 * PI * p.freq;
+    hp.q5c = sin(2.0 * time * 2.0 * PI * p.freq);
+    hp.qmode[6] = (itime == 1) ? POWER : 0.0;
+    hp.dt2 = pow(p.time_step, 2.0);
+    hp.tb4 = pow(p.t_bath_rad, 4.0);
+    hp.heated_volume = c->heated_volume;
+    hp.any_heater = c->any_heater ? 1 : 0;
+    return hp;
+}
+
+static int sync_scal(hf_ctx *c, Scal *dev, Scal *host) {
+    HIPCHK(c, hipMemcpyAsync(host, dev, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return HF_OK;
+}
+
+extern "C" int hf_assemble(hf_ctx *c) {
+    if (!c) return HF_ERR_ARG;
+    if (!c->uploaded) return set_err(c, HF_ERR_STATE, "hf_assemble: cells not uploaded");
+    launch_assemble(c->g, c->ap, c->kappa, c->lx, c->ly, c->lz, c->rhoc, c->tau_s, c->d, c->cx,
+                    c->cy, c->cz, c->bvec, c->sc, c->st);
+    TRY(check_launch(c, "k_assemble"));
+    launch_scale(c->g, c->d, c->cx, c->cy, c->cz, c->sv, c->hx, c->hy, c->hz, c->fx, c->fy, c->fz,
+                 c->st);
+    TRY(check_launch(c, "k_scale"));
+    if (!c->dyn_heat) {
+        // time-invariant Qdens (constant heaters, no radiative term): built once
+        HeatParams hp = heat_params(c, 1);
+        hp.tb4 = 0.0;
+        launch_qdens(c->g, hp, c->iheater, c->ells, c->tau_s, c->kappa /* unused: ells == 0 */,
+                     c->qextra, c->qd, c->sc, c->st);
+        TRY(check_launch(c, "k_qdens"));
+    }
+    TRY(sync_scal(c, c->sc, c->h_sc));
+    c->setup_flags = c->h_sc->flags & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS);
+    c->assembled = true;
+    return HF_OK;
+}
+
+extern "C" int hf_export_coo(hf_ctx *c, int64_t *n, int64_t *len, int64_t *count, double *val,
+                             int64_t *irow, int64_t *jcol) {
+    if (!c) return HF_ERR_ARG;
+    if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_export_coo: not assembled");
+    const hf_params &p = c->p;
+    int64_t L = 7 * p.nx * p.ny * p.nz - 2 * (p.nx * p.ny + p.ny * p.nz + p.nz * p.nx);
+    if (p.periodic[0]) L += 2 * p.ny * p.nz;
+    if (p.periodic[1]) L += 2 * p.nz * p.nx;
+    if (p.periodic[2]) L += 2 * p.nx * p.ny;
+    const size_t N = c->g.N;
+    const size_t nb = (N + kBlock - 1) / kBlock;
+    int64_t *cnt = nullptr, *off = nullptr, *bsum = nullptr, *total = nullptr;
+    double *dval = nullptr;
+    int64_t *dirow = nullptr, *djcol = nullptr;
+    auto cleanup = [&]() {
+        hipFree(cnt); hipFree(off); hipFree(bsum); hipFree(total);
+        hipFree(dval); hipFree(dirow); hipFree(djcol);
+    };
+    if (hipMalloc(&cnt, N * 8) != hipSuccess || hipMalloc(&off, N * 8) != hipSuccess ||
+        hipMalloc(&bsum, nb * 8) != hipSuccess || hipMalloc(&total, 8) != hipSuccess) {
+        cleanup();
+        return set_err(c, HF_ERR_ALLOC, "hf_export_coo: scratch allocation failed");
+    }
+    const bool want = val && irow && jcol;
+    const int64_t cap = std::max<int64_t>(L, (int64_t)(7 * N));
+    if (want && (hipMalloc(&dval, cap * 8) != hipSuccess || hipMalloc(&dirow, cap * 8) != hipSuccess ||
+                 hipMalloc(&djcol, cap * 8) != hipSuccess)) {
+        cleanup();
+        return set_err(c, HF_ERR_ALLOC, "hf_export_coo: output allocation failed");
+    }
+    int64_t cnt_host = 0;
+    launch_coo(c->g, c->d, c->cx, c->cy, c->cz, cnt, off, bsum, total, want ? dval : nullptr,
+               dirow, djcol, L, &cnt_host, c->st);
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess && want) {
+        // the reference never writes past len; an over-full export keeps the first len entries
+        int64_t ncopy = L;
+        e = hipMemcpy(val, dval, ncopy * 8, hipMemcpyDefault);
+        if (e == hipSuccess) e = hipMemcpy(irow, dirow, ncopy * 8, hipMemcpyDefault);
+        if (e == hipSuccess) e = hipMemcpy(jcol, djcol, ncopy * 8, hipMemcpyDefault);
+    }
+    cleanup();
+    if (e != hipSuccess) return set_err(c, HF_ERR_HIP, std::string("hf_export_coo: ") + hipGetErrorString(e));
+    if (n) *n = (int64_t)N;
+    if (len) *len = L;
+    if (count) *count = cnt_host;
+    return HF_OK;
+}
+
+extern "C" int hf_set_state(hf_ctx *c, const double *tp, const double *tpp) {
+    if (!c || !tp || !tpp) return set_err(c, HF_ERR_ARG, "hf_set_state: null argument");
+    const size_t B8 = (size_t)c->g.N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->T[0], tp, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->T[1], tpp, B8, hipMemcpyDefault, c->st));
+    launch_step_begin(c->sc, c->st);   // clears stale per-step flags
+    launch_tiny_from_state(c->g, c->T[0], c->T[1], c->sc, c->st);
+    TRY(check_launch(c, "k_tiny_from_state"));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->has_state = true;
+    return HF_OK;
+}
+
+extern "C" int hf_get_state(hf_ctx *c, double *tp, double *tpp) {
+    if (!c) return HF_ERR_ARG;
+    if (!c->has_state) return set_err(c, HF_ERR_STATE, "hf_get_state: no state");
+    const size_t B8 = (size_t)c->g.N * sizeof(double);
+    if (tp) HIPCHK(c, hipMemcpyAsync(tp, c->T[0], B8, hipMemcpyDefault, c->st));
+    if (tpp) HIPCHK(c, hipMemcpyAsync(tpp, c->T[1], B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return HF_OK;
+}
+
+static int flags_to_rc(hf_ctx *c, int32_t fl) {
+    // the reference's guard order: B, Q, Qdens, S_CAT, S, x (mod_evolve.f90:70-158)
+    if (fl & FLAG_NAN_B) return set_err(c, HF_ERR_NAN_B, "fatal error: NAN in B vector");
+    if (fl & FLAG_NAN_Q) return set_err(c, HF_ERR_NAN_Q, "fatal error: NAN in Q vector");
+    if (fl & FLAG_NAN_QDENS) return set_err(c, HF_ERR_NAN_QDENS, "fatal error: NAN in Qdens vector");
+    if (fl & FLAG_NAN_SCAT) return set_err(c, HF_ERR_NAN_SCAT, "fatal error: NAN in S_CAT vector");
+    if (fl & FLAG_NAN_S) return set_err(c, HF_ERR_NAN_S, "fatal error: NAN in S vector");
+    if (fl & FLAG_NAN_X) return set_err(c, HF_ERR_NAN_X, "fatal error: NAN in x tempurature vector");
+    return HF_OK;
+}
+
+// CG on A^ y = r^ until sc->done, in batches of `pred` iterations per host synchronisation.
+// Launches queued after convergence return at once; their timing slots are invalidated.
+template <typename T>
+static int run_cg(hf_ctx *c, Scal *sc, Scal *h, T *r, T **p, T *q, T *y, const T *hx,
+                  const T *hy, const T *hz, int64_t *pred, int64_t *iters, int ida, int idb) {
+    const double N = (double)c->g.N, w = (double)sizeof(T);
+    int cur = 0;
+    int64_t batch = std::max<int64_t>(1, *pred);
+    int64_t queued = 0;
+    std::vector<std::pair<long, int64_t>> slots;
+    for (;;) {
+        for (int64_t b = 0; b < batch; ++b, ++queued) {
+            const bool first = queued == 0;
+            // k_cg_a: read r, p_old (not on the first iteration), 3 couplings; write p, q
+            long sa = tbeg(c, ida, N * w * (first ? 6 : 7));
+            launch_cg_a<T>(c->g, r, p[cur], p[cur ^ 1], q, hx, hy, hz, sc, c->partials, c->st);
+            tend(c, sa);
+            // k_cg_b: read y (not on the first iteration), p, r, q; write y, r
+            long sb = tbeg(c, idb, N * w * (first ? 5 : 6));
+            launch_cg_b<T>(c->g, y, p[cur ^ 1], r, q, sc, c->partials, c->st);
+            tend(c, sb);
+            if (sa >= 0) slots.push_back({sa, queued});
+            if (sb >= 0) slots.push_back({sb, queued});
+            cur ^= 1;
+        }
+        TRY(check_launch(c, "k_cg"));
+        TRY(sync_scal(c, sc, h));
+        if (h->done) break;
+        batch = std::max<int64_t>(2, batch);
+    }
+    for (auto &sl : slots)
+        if (sl.second >= h->iter) c->tpool[sl.first].valid = false;
+    *iters = h->iter;
+    *pred = std::max<int64_t>(1, h->iter + 1);
+    return HF_OK;
+}
+
+static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
+    const hf_params &p = c->p;
+    launch_step_begin(c->sc, c->st);
+    const double N = (double)c->g.N;
+    if (c->dyn_heat) {
+        HeatParams hp = heat_params(c, itime);
+        long sq = tbeg(c, HF_K_QDENS, N * (1 + 8 * 3 + 8 + (c->qextra ? 8 : 0)));
+        launch_qdens(c->g, hp, c->iheater, c->ells, c->tau_s, c->T[0], c->qextra, c->qd, c->sc,
+                     c->st);
+        tend(c, sq);
+    }
+    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
+    // k_rhs_res: read Tp, Tpp, rho c_p, tau' rho c_p, Qdens, B, diag, 3 couplings; write r^
+    long sr = tbeg(c, HF_K_RHS64, N * 8 * (10 + 1 + ((c->debug & 1) ? 1 : 0)));
+    launch_rhs_res_state64(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec, c->d,
+                           c->cx, c->cy, c->cz, c->r64, (c->debug & 1) ? c->S : nullptr, c->sc,
+                           c->partials, c->st);
+    tend(c, sr);
+    TRY(check_launch(c, "k_rhs_res"));
+    int64_t it = 0;
+    TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
+                       &c->pred_iters, &it, HF_K_CG_A64, HF_K_CG_B64));
+    int32_t fl = c->h_sc->flags | c->setup_flags;
+    if (fl & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S))
+        return flags_to_rc(c, fl);
+    const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bnorm2) : 0.0;
+    // k_finalize: read Tp, Tpp, s, y; write T^{n+1}
+    long sf = tbeg(c, HF_K_FINAL, N * 8 * 5);
+    launch_finalize(c->g, true, c->T[0], c->T[1], c->sv, c->y64, c->sc, c->st);
+    tend(c, sf);
+    TRY(check_launch(c, "k_finalize"));
+    std::swap(c->T[0], c->T[1]);  // Temp_pp <- Temp_p, Temp_p <- x (mod_evolve.f90:169-170)
+    c->stats.steps += 1;
+    c->stats.outer_iters += it;
+    c->stats.last_err = e;
+    if (iter) *iter = it;
+    if (err) *err = e;
+    return HF_OK;
+}
+
+static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
+    const hf_params &p = c->p;
+    launch_step_begin(c->sc, c->st);
+    const double N = (double)c->g.N;
+    if (c->dyn_heat) {
+        HeatParams hp = heat_params(c, itime);
+        long sq = tbeg(c, HF_K_QDENS, N * (1 + 8 * 3 + 8 + (c->qextra ? 8 : 0)));
+        launch_qdens(c->g, hp, c->iheater, c->ells, c->tau_s, c->T[0], c->qextra, c->qd, c->sc,
+                     c->st);
+        tend(c, sq);
+    }
+    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
+    // x0 goes to T[2]; S is kept for the refinement residuals.
+    // k_rhs_res: read 10 FP64 arrays; write r^ (FP32), S and x0 (FP64)
+    long sr = tbeg(c, HF_K_RHS32, N * (8 * 10 + 4 + 8 + 8));
+    launch_rhs_res_state32(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec, c->d,
+                           c->cx, c->cy, c->cz, c->r32, c->S, c->T[2], c->sc, c->partials, c->st);
+    tend(c, sr);
+    TRY(check_launch(c, "k_rhs_res"));
+    double *xa = c->T[2], *xb = c->T[1];  // T[1] (Temp_pp) is free once x0 and S exist
+    // T[1] is still needed: the refined x of the last sweep must not overwrite Temp_pp before
+    // the rotation, but Temp_pp is only read by k_rhs_res (done above), so it may host x.
+    IrParams ip{p.itmax, 1};
+    int64_t inner_total = 0, sweeps = 0;
+    for (;;) {
+        launch_inner_begin(c->sc, c->sci, p.inner_tol, p.tol, p.inner_itmax, 0.25, c->st);
+        int cur = 0;
+        int64_t batch = std::max<int64_t>(1, c->pred_inner);
+        std::vector<std::pair<long, int64_t>> slots;
+        for (int64_t b = 0; b < batch; ++b) {
+            long sa = tbeg(c, HF_K_CG_A32, N * 4 * (b == 0 ? 6 : 7));
+            launch_cg_a<float>(c->g, c->r32, c->p32[cur], c->p32[cur ^ 1], c->q32, c->fx, c->fy,
+                               c->fz, c->sci, c->partials, c->st);
+            tend(c, sa);
+            long sb = tbeg(c, HF_K_CG_B32, N * 4 * (b == 0 ? 5 : 6));
+            launch_cg_b<float>(c->g, c->y32, c->p32[cur ^ 1], c->r32, c->q32, c->sci, c->partials,
+                               c->st);
+            tend(c, sb);
+            if (sa >= 0) slots.push_back({sa, b});
+            if (sb >= 0) slots.push_back({sb, b});
+            cur ^= 1;
+        }
+        TRY(check_launch(c, "k_cg32"));
+        // k_ir_res: read x, y (FP32), s, S, diag, 3 couplings, Tp; write x, r^ (FP32)
+        long si = tbeg(c, HF_K_IR_RES, N * (8 * 8 + 4 + 8 + 4));
+        launch_ir_res(c->g, ip, xa, c->y32, c->sv, c->S, c->d, c->cx, c->cy, c->cz, c->T[0], xb,
+                      c->r32, c->sc, c->partials, c->st);
+        tend(c, si);
+        TRY(check_launch(c, "k_ir_res"));
+        std::swap(xa, xb);
+        HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
+        TRY(sync_scal(c, c->sc, c->h_sc));
+        const int64_t in_it = c->h_sci->iter;
+        for (auto &sl : slots)
+            if (sl.second >= in_it) c->tpool[sl.first].valid = false;
+        inner_total += in_it;
+        sweeps += 1;
+        // next prediction: what this sweep needed (+1), or a longer batch if it was cut short
+        c->pred_inner = c->h_sci->done ? std::max<int64_t>(1, in_it + 1) : batch * 2;
+        int32_t fl = c->h_sc->flags | c->setup_flags;
+        if (fl & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S))
+            return flags_to_rc(c, fl);
+        if (c->h_sc->done) break;
+    }
+    // rotation: Temp_pp <- Temp_p, Temp_p <- x; the spare takes the old Temp_pp buffer
+    double *old_tp = c->T[0], *old_tpp = c->T[1];
+    double *spare = (xa == old_tpp) ? c->T[2] : old_tpp;
+    c->T[0] = xa;
+    c->T[1] = old_tp;
+    c->T[2] = spare;
+    const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->outer_rr / c->h_sc->bnorm2) : 0.0;
+    c->stats.steps += 1;
+    c->stats.outer_iters += sweeps;
+    c->stats.inner_iters += inner_total;
+    c->stats.last_err = e;
+    if (iter) *iter = inner_total + sweeps;
+    if (err) *err = e;
+    return HF_OK;
+}
+
+static int step_common(hf_ctx *c, int64_t itime, int64_t *iter, double *err, bool final_sync) {
+    if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_step: not assembled");
+    if (!c->has_state) return set_err(c, HF_ERR_STATE, "hf_step: no state (hf_set_state)");
+    if (c->setup_flags) return flags_to_rc(c, c->setup_flags);
+    int rc = c->p.precision == HF_PREC_FP32_IR ? step_ir(c, itime, iter, err)
+                                               : step_fp64(c, itime, iter, err);
+    if (rc != HF_OK) return rc;
+    if (final_sync) {
+        TRY(sync_scal(c, c->sc, c->h_sc));
+        if (c->h_sc->flags & FLAG_NAN_X) return flags_to_rc(c, FLAG_NAN_X);
+    }
+    return HF_OK;
+}
+
+extern "C" int hf_step(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
+    if (!c) return HF_ERR_ARG;
+    return step_common(c, itime, iter, err, true);
+}
+
+extern "C" int hf_run(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t *iters_total) {
+    if (!c) return HF_ERR_ARG;
+    int64_t tot = 0;
+    for (int64_t k = 0; k < nsteps; ++k) {
+        int64_t it = 0;
+        // NaN in x of step k is seen at the next step's first synchronisation (the flag is
+        // sticky within a step) or at the final check
+        int rc = step_common(c, itime0 + k, &it, nullptr, k + 1 == nsteps);
+        if (rc != HF_OK) return rc;
+        if (c->h_sc->flags & FLAG_NAN_X) return flags_to_rc(c, FLAG_NAN_X);
+        tot += it;
+    }
+    if (iters_total) *iters_total = tot;
+    return HF_OK;
+}
+
+extern "C" int hf_solve(hf_ctx *c, const double *b, double *x, int32_t itol, double tol,
+                        int64_t itmax, int64_t *iter, double *err) {
+    (void)itol;
+    if (!c || !b || !x) return set_err(c, HF_ERR_ARG, "hf_solve: null argument");
+    if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_solve: not assembled");
+    if (c->p.precision != HF_PREC_FP64 || !c->hx)
+        return set_err(c, HF_ERR_STATE, "hf_solve: needs an FP64 context");
+    const size_t N = c->g.N, B8 = N * sizeof(double);
+    double *bd = nullptr, *xd = nullptr;
+    TRY(dalloc(c, &c->S, N));
+    bd = c->S;
+    TRY(dalloc(c, &c->T[2], N));
+    xd = c->T[2];
+    HIPCHK(c, hipMemcpyAsync(bd, b, B8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipMemcpyAsync(xd, x, B8, hipMemcpyDefault, c->st));
+    RhsParams rp{0.0, tol, itmax, 0, 0};
+    launch_rhs_res_solve64(c->g, rp, xd, bd, c->d, c->cx, c->cy, c->cz, c->r64, c->sc,
+                           c->partials, c->st);
+    TRY(check_launch(c, "k_rhs_res(solve)"));
+    int64_t it = 0, pred = std::max<int64_t>(4, c->pred_iters);
+    TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
+                       &pred, &it, HF_K_CG_A64, HF_K_CG_B64));
+    launch_finalize(c->g, false, nullptr, xd, c->sv, c->y64, c->sc, c->st);
+    TRY(check_launch(c, "k_finalize(solve)"));
+    HIPCHK(c, hipMemcpyAsync(x, xd, B8, hipMemcpyDefault, c->st));
+    TRY(sync_scal(c, c->sc, c->h_sc));
+    if (iter) *iter = it;
+    if (err) *err = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bnorm2) : 0.0;
+    return HF_OK;
+}
+
+extern "C" int hf_get_vector(hf_ctx *c, int32_t which, double *out) {
+    if (!c || !out) return HF_ERR_ARG;
+    const double *src = nullptr;
+    switch (which) {
+        case 0: src = c->bvec; break;
+        case 1: src = c->qd; break;
+        case 2: src = c->S; break;
+        case 3: src = c->d; break;
+        case 4: src = c->cx; break;
+        case 5: src = c->cy; break;
+        case 6: src = c->cz; break;
+        default: return set_err(c, HF_ERR_ARG, "hf_get_vector: unknown vector id");
+    }
+    if (!src) return set_err(c, HF_ERR_STATE, "hf_get_vector: vector not available");
+    if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_get_vector: not assembled");
+    HIPCHK(c, hipMemcpyAsync(out, src, (size_t)c->g.N * 8, hipMemcpyDefault, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return HF_OK;
+}
+
+extern "C" int hf_set_debug(hf_ctx *c, int32_t flags) {
+    if (!c) return HF_ERR_ARG;
+    c->debug = flags;
+    if (flags & 1) TRY(dalloc(c, &c->S, c->g.N));
+    return HF_OK;
+}
+
+extern "C" int hf_get_stats(hf_ctx *c, hf_stats *st) {
+    if (!c || !st) return HF_ERR_ARG;
+    *st = c->stats;
+    return HF_OK;
+}
+
+extern "C" int hf_reset_stats(hf_ctx *c) {
+    if (!c) return HF_ERR_ARG;
+    c->stats = hf_stats{};
+    return HF_OK;
+}
+
+extern "C" int hf_set_timing(hf_ctx *c, int32_t on) {
+    if (!c) return HF_ERR_ARG;
+    c->timing = on != 0;
+    c->tused = 0;
+    return HF_OK;
+}
+
+extern "C" int hf_kernel_times(hf_ctx *c, double *ms, int64_t *launches, double *bytes) {
+    if (!c) return HF_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int k = 0; k < HF_K_COUNT; ++k) {
+        if (ms) ms[k] = 0.0;
+        if (launches) launches[k] = 0;
+        if (bytes) bytes[k] = 0.0;
+    }
+    for (size_t i = 0; i < c->tused; ++i) {
+        const hf_ctx::Timed &t = c->tpool[i];
+        if (!t.valid || t.id < 0 || t.id >= HF_K_COUNT) continue;
+        float e = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&e, t.a, t.b));
+        if (ms) ms[t.id] += (double)e;
+        if (launches) launches[t.id] += 1;
+        if (bytes) bytes[t.id] += t.bytes;
+    }
+    c->tused = 0;
+    return HF_OK;
+}
+
+extern "C" int hf_set_solver(hf_ctx *c, double tol, int64_t itmax, int32_t precision,
+                             double inner_tol, int32_t inner_itmax) {
+    if (!c) return HF_ERR_ARG;
+    if (precision != HF_PREC_FP64 && precision != HF_PREC_FP32_IR)
+        return set_err(c, HF_ERR_ARG, "hf_set_solver: unknown precision");
+    const int32_t old = c->p.precision;
+    c->p.tol = tol;
+    c->p.itmax = itmax;
+    c->p.precision = precision;
+    c->p.inner_tol = inner_tol;
+    c->p.inner_itmax = inner_itmax;
+    if (precision != old) {
+        TRY(alloc_mode(c));
+        if (c->assembled) {
+            launch_scale(c->g, c->d, c->cx, c->cy, c->cz, c->sv, c->hx, c->hy, c->hz, c->fx,
+                         c->fy, c->fz, c->st);
+            TRY(check_launch(c, "k_scale"));
+            HIPCHK(c, hipStreamSynchronize(c->st));
+        }
+    }
+    return HF_OK;
+}
+
+// ---- internal entry points shared with the adjoint ------------------------------------------
+
+namespace hf {
+int ctx_step(hf_ctx *c, int64_t itime, bool final_sync) {
+    return step_common(c, itime, nullptr, nullptr, final_sync);
+}
+int ctx_set_err(hf_ctx *c, int code, const std::string &msg) { return set_err(c, code, msg); }
+int ctx_alloc(hf_ctx *c, double **ptr, size_t n) { return dalloc(c, ptr, n); }
+
+int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itmax,
+                  int64_t *iters) {
+    if (c->p.precision != HF_PREC_FP64 || !c->hx)
+        return set_err(c, HF_ERR_STATE, "adjoint solve needs an FP64 context");
+    RhsParams rp{0.0, tol, itmax, 0, 0};
+    launch_rhs_res_solve64(c->g, rp, x, b, c->d, c->cx, c->cy, c->cz, c->r64, c->sc, c->partials,
+                           c->st);
+    TRY(check_launch(c, "k_rhs_res(solve)"));
+    int64_t it = 0;
+    TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
+                       &c->pred_adj, &it, HF_K_CG_A64, HF_K_CG_B64));
+    launch_finalize(c->g, false, nullptr, x, c->sv, c->y64, c->sc, c->st);
+    TRY(check_launch(c, "k_finalize(solve)"));
+    if (iters) *iters = it;
+    return HF_OK;
+}
+}  // namespace hf
+
+extern "C" int hf_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every,
+                       int64_t nprobe, const int64_t *probe_idx, const double *target,
+                       const double *weight, double *loss, double *dl_dkappa,
+                       double *dl_dpower) {
+    if (!c) return HF_ERR_ARG;
+    return hf::adjoint_grad(c, itime0, nsteps, ckpt_every, nprobe, probe_idx, target, weight,
+                            loss, dl_dkappa, dl_dpower);
+}

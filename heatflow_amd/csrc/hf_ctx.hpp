@@ -1,0 +1,88 @@
+// hf_ctx.hpp -- the opaque context behind hf_ctx* and the internal entry points shared by the
+// forward (hf_api.hip) and adjoint (hf_adjoint.hip) translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/heatflow_hip.h"
+#include "hf_device.hpp"
+#include "hf_kernels.hpp"
+
+namespace hf {
+struct AdjointState;
+void adjoint_free(AdjointState *a);
+}  // namespace hf
+
+
+using hf::Dims;
+using hf::AsmParams;
+using hf::Scal;
+
+struct hf_ctx {
+    hf_params p{};
+    Dims g{};
+    AsmParams ap{};
+    hipStream_t st = nullptr;
+    int dev = 0;
+    std::string err;
+    int debug = 0;
+
+    // per-cell inputs and derived fields (device)
+    double *kappa = nullptr, *rhoc = nullptr, *tau_s = nullptr, *cvec = nullptr, *ells = nullptr;
+    double *lx = nullptr, *ly = nullptr, *lz = nullptr, *vol = nullptr, *qextra = nullptr;
+    int8_t *iheater = nullptr;
+    // assembled operator
+    double *d = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *bvec = nullptr;
+    double *sv = nullptr, *hx = nullptr, *hy = nullptr, *hz = nullptr;
+    float *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    double *qd = nullptr;        // Qdens (constant, or of the last step)
+    // state: T[0] = Temp_p, T[1] = Temp_pp, T[2] = spare (mixed-mode refined x)
+    double *T[3] = {nullptr, nullptr, nullptr};
+    double *S = nullptr;
+    // FP64 CG
+    double *r64 = nullptr, *p64[2] = {nullptr, nullptr}, *q64 = nullptr, *y64 = nullptr;
+    // FP32 inner CG
+    float *r32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr, *y32 = nullptr;
+    // reductions / scalars
+    Scal *sc = nullptr, *sci = nullptr;
+    double *partials = nullptr;
+    Scal *h_sc = nullptr;        // pinned mirror
+    Scal *h_sci = nullptr;
+
+    bool uploaded = false, assembled = false, has_state = false;
+    bool any_heater = false, dyn_heat = false;
+    double heated_volume = 0.0;
+    int32_t setup_flags = 0;
+    int64_t pred_iters = 4, pred_inner = 8, pred_adj = 8;
+    hf_stats stats{};
+    std::vector<void *> allocs;
+    hf::AdjointState *adj = nullptr;
+
+    // kernel timing pool (hf_set_timing / hf_kernel_times)
+    struct Timed {
+        int id;
+        hipEvent_t a, b;
+        double bytes;
+        bool valid;
+    };
+    bool timing = false;
+    std::vector<Timed> tpool;
+    size_t tused = 0;
+};
+
+
+namespace hf {
+// one time step without the final host synchronisation (hf_api.hip)
+int ctx_step(hf_ctx *c, int64_t itime, bool final_sync);
+// FP64 solve of H x = b on device vectors; x holds the initial guess (hf_api.hip)
+int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itmax,
+                  int64_t *iters);
+int ctx_set_err(hf_ctx *c, int code, const std::string &msg);
+int ctx_alloc(hf_ctx *c, double **ptr, size_t n);
+// reverse-time gradient (hf_adjoint.hip)
+int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, int64_t nprobe,
+                 const int64_t *probe_idx, const double *target, const double *weight,
+                 double *loss, double *dl_dkappa, double *dl_dpower);
+}  // namespace hf

@@ -1,0 +1,190 @@
+// hf_device.hpp -- shared device-side definitions of the HeatFlow gfx950 library.
+//
+// Layout: one HIP thread handles one cell per loop trip; every array over cells is SoA in the
+// reference's linear order (x fastest, then y, then z; src/heatflow/mod_hmatrix.f90:72-74).
+// A launch uses a fixed grid of `nblk` 256-thread blocks (a multiple of 8); hardware block h
+// is mapped to the logical chunk (h % 8) * (nblk / 8) + h / 8, so each of the 8 XCDs of an
+// MI355X sweeps one contiguous 1/8 of the domain and z-neighbour planes stay in that XCD's L2.
+// All reductions are deterministic: wave-level butterfly, then a fixed-order LDS sum, one
+// partial per block, and the last block to finish sums the partials in block order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hf {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr double kTiny = 1.0e-12;        // constants::TINY, src/heatflow/mod_constants.f90:17
+constexpr double kStefBoltz = 5.670373e-8;  // src/heatflow/mod_constants.f90:16
+
+// floor(n / d) for n < 2^31 by a multiply-shift: m = ceil(2^(32+l) / d), 2^(l-1) < d <= 2^l.
+struct FastDiv {
+    uint64_t m;
+    uint32_t d, s;
+    void init(uint32_t dd) {
+        uint32_t l = 0;
+        while ((1ull << l) < dd) ++l;
+        d = dd;
+        s = 32 + l;
+        m = ((1ull << s) + dd - 1) / dd;
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const {
+        return (uint32_t)(((uint64_t)n * m) >> s);
+    }
+};
+
+struct Dims {
+    uint32_t nx, ny, nz, N, nxny;
+    uint32_t chunk;      // cells per logical block (multiple of kBlock)
+    uint32_t nblk;       // launch grid (multiple of 8)
+    int32_t per[3];      // periodic axes (length >= 3)
+    FastDiv dnx, dny;
+};
+
+// Device-resident solver scalars.  Written only by the last block of a reduction.
+struct Scal {
+    double rr;        // current r.r of the (scaled) CG residual
+    double rr_start;  // r.r at the start of this solve
+    double bnorm2;    // ||b_hat||^2 of the outer system
+    double thresh2;   // stop when rr <= thresh2
+    double pq, alpha, beta;
+    double outer_rr;  // mixed mode: FP64 residual^2 after the last refinement sweep
+    int64_t iter;     // iterations of this solve
+    int64_t itmax;
+    int32_t done;
+    int32_t first;    // next CG iteration is the first one (p = r, y = 0)
+    int32_t flags;    // HF_FLAG_* bits (sticky until the host clears them)
+    int32_t tiny;     // x0 += TINY for the next step (mod_evolve.f90:141)
+    uint32_t counter; // last-block counter of the running reduction
+    int32_t pad;
+};
+
+enum : int32_t {
+    FLAG_NAN_B = 1, FLAG_NAN_Q = 2, FLAG_NAN_QDENS = 4, FLAG_NAN_SCAT = 8, FLAG_NAN_S = 16,
+    FLAG_NAN_X = 32, FLAG_TINY_NEXT = 64, FLAG_BREAKDOWN = 128
+};
+
+__device__ __forceinline__ uint32_t logical_block(uint32_t nblk) {
+    uint32_t h = blockIdx.x;
+    return (h & 7u) * (nblk >> 3) + (h >> 3);
+}
+
+// cell range of this block: [*beg, *end)
+__device__ __forceinline__ void block_range(const Dims &g, uint32_t *beg, uint32_t *end) {
+    uint64_t b = (uint64_t)logical_block(g.nblk) * g.chunk;
+    uint64_t e = b + g.chunk;
+    *beg = (uint32_t)(b < g.N ? b : g.N);
+    *end = (uint32_t)(e < g.N ? e : g.N);
+}
+
+struct Nbr {
+    uint32_t xm, xp, ym, yp, zm, zp;
+    bool hxm, hxp, hym, hyp, hzm, hzp;
+    uint32_t x, y, z;
+};
+
+// Neighbour indices of cell I; h** = false where the face is a domain boundary (or a
+// length-1 periodic axis).  Periodic axes wrap (hmatrixfunc's xp/xm logic, :78-92).
+__device__ __forceinline__ Nbr neighbours(const Dims &g, uint32_t I) {
+    Nbr n;
+    uint32_t t = g.dnx.div(I);
+    n.x = I - t * g.nx;
+    n.z = g.dny.div(t);
+    n.y = t - n.z * g.ny;
+    // x
+    n.hxm = n.x > 0 || g.per[0];
+    n.xm = n.x > 0 ? I - 1 : I + (g.nx - 1);
+    n.hxp = n.x + 1 < g.nx || g.per[0];
+    n.xp = n.x + 1 < g.nx ? I + 1 : I - (g.nx - 1);
+    // y
+    n.hym = n.y > 0 || g.per[1];
+    n.ym = n.y > 0 ? I - g.nx : I + (g.ny - 1) * g.nx;
+    n.hyp = n.y + 1 < g.ny || g.per[1];
+    n.yp = n.y + 1 < g.ny ? I + g.nx : I - (g.ny - 1) * g.nx;
+    // z
+    n.hzm = n.z > 0 || g.per[2];
+    n.zm = n.z > 0 ? I - g.nxny : I + (g.nz - 1) * g.nxny;
+    n.hzp = n.z + 1 < g.nz || g.per[2];
+    n.zp = n.z + 1 < g.nz ? I + g.nxny : I - (g.nz - 1) * g.nxny;
+    return n;
+}
+
+// ---- deterministic reductions ------------------------------------------------------------
+
+template <int K>
+__device__ __forceinline__ void wave_sum(double (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    }
+}
+
+// Block sum; the result is valid in every thread.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*kWaves */) {
+    wave_sum<K>(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[k * kWaves + w] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < kWaves; ++i) s += lds[k * kWaves + i];
+        v[k] = s;
+    }
+}
+
+// Grid-wide deterministic sum of K doubles.  Every block calls it exactly once (all threads).
+// The last block to arrive sums the per-block partials in block order and calls fin(sums)
+// from thread 0; it also resets the counter.  Returns nothing to other blocks.
+template <int K, class Fin>
+__device__ __forceinline__ void grid_sum(double (&v)[K], double *partials, uint32_t *counter,
+                                         Fin fin) {
+    __shared__ double lds[K * kWaves];
+    __shared__ int is_last;
+    block_sum<K>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+        __threadfence();
+        uint32_t t = atomicAdd(counter, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    // fixed assignment of partials to threads, fixed-order tree: deterministic
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += kBlock) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            acc[k] += __builtin_nontemporal_load(&partials[(size_t)k * gridDim.x + i]);
+    }
+    block_sum<K>(acc, lds);
+    if (threadIdx.x == 0) {
+        fin(acc);
+        *counter = 0u;
+        __threadfence();
+    }
+}
+
+// OR-reduce an int flag word of the block into *dst with one atomic per block (order-free).
+__device__ __forceinline__ void block_flags(int32_t f, int32_t *dst) {
+    __shared__ int32_t sf;
+    if (threadIdx.x == 0) sf = 0;
+    __syncthreads();
+    if (f) atomicOr(&sf, f);
+    __syncthreads();
+    if (threadIdx.x == 0 && sf) atomicOr(dst, sf);
+}
+
+}  // namespace hf

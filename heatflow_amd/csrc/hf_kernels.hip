@@ -1,0 +1,725 @@
+// hf_kernels.hip -- gfx950 kernels of the HeatFlow time-step path.
+//
+// Kernels (one thread per cell per loop trip, see hf_device.hpp for the launch layout):
+//   k_assemble     H as a 7-point stencil + boundary vector B        (mod_hmatrix.f90, mod_boundary.f90)
+//   k_scale        Jacobi scaling s = 1/sqrt(-diag), scaled couplings, FP32 copies
+//   k_coo_*        export in the reference COO order                  (mod_setup.f90:89-152)
+//   k_qdens        per-step heater power density                      (mod_heating.f90:27-169)
+//   k_rhs_res      S, x0, r0 = S - H x0 fused, scaled, + norms         (mod_evolve.f90:112-141)
+//   k_cg_a/k_cg_b  CG iteration on the scaled operator A^ = I - C^     (mod_Sparse.f90:404-433)
+//   k_finalize     x = x0 + s*y, NaN guard, TINY flag of the next step (mod_evolve.f90:140-170)
+//   k_ir_res       mixed mode: x += s*y (FP32 y) and the FP64 residual of the refined x
+//
+// All HIP code here is compiled with -ffp-contract=off so every product and sum rounds as
+// written; the assembly therefore reproduces hmatrixfunc's values bit for bit.
+#include "hf_device.hpp"
+#include "hf_kernels.hpp"
+
+namespace hf {
+
+// ---- assembly ------------------------------------------------------------------------------
+
+// calculate_conductivity, interior branch (src/heatflow/mod_hmatrix.f90:288-315): `in` is the
+// neighbour whose coordinate differs, `out` the cell the value is computed for.
+__device__ __forceinline__ double cond_int(double kin, double kout, double lin, double lout,
+                                           double br) {
+    double kab = (lin + lout) * kin * kout / (lin * kout + lout * kin);
+    kab = kab / (lout * lout);
+    if (kin != kout) kab = kab * br;
+    return kab;
+}
+
+// boundry_diag_term (src/heatflow/mod_hmatrix.f90:357-385)
+__device__ __forceinline__ double cond_bnd(double kb, double kappa, double L, double br) {
+    double kab = ((2.0 * kb * kappa) / (kb + kappa)) / (L * L);
+    if (kappa != kb) kab = kab * br;
+    return kab;
+}
+
+// Per-face value of the diagonal conductance of cell I towards one side of one axis:
+// neighbour J (if any), the boundary term otherwise.
+__device__ __forceinline__ double face_cond(bool has_nbr, uint32_t I, uint32_t J,
+                                            const double *__restrict__ kappa,
+                                            const double *__restrict__ L, double kb, double br) {
+    if (has_nbr) return cond_int(kappa[J], kappa[I], L[J], L[I], br);
+    return cond_bnd(kb, kappa[I], L[I], br);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_assemble(Dims g, AsmParams ap, const double *__restrict__ kappa, const double *__restrict__ lx,
+           const double *__restrict__ ly, const double *__restrict__ lz,
+           const double *__restrict__ rhoc, const double *__restrict__ tau_s,
+           double *__restrict__ d, double *__restrict__ cx, double *__restrict__ cy,
+           double *__restrict__ cz, double *__restrict__ bvec, Scal *sc) {
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    int32_t fl = 0;
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        Nbr n = neighbours(g, I);
+        // calculate_alpha (mod_hmatrix.f90:254-267)
+        double alpha = 0.0;
+        if (ap.isteady == 0) {
+            double tau = ap.icattaneo == 0 ? 0.0 : tau_s[I];
+            alpha = (tau * rhoc[I]) + (ap.inverse_time * rhoc[I]);
+        }
+        // A..G: in = neighbour, out = I (mod_hmatrix.f90:133-138)
+        double A = face_cond(n.hxm, I, n.xm, kappa, lx, ap.kb[0], ap.br);
+        double B = face_cond(n.hxp, I, n.xp, kappa, lx, ap.kb[1], ap.br);
+        double D = face_cond(n.hym, I, n.ym, kappa, ly, ap.kb[2], ap.br);
+        double E = face_cond(n.hyp, I, n.yp, kappa, ly, ap.kb[3], ap.br);
+        double F = face_cond(n.hzm, I, n.zm, kappa, lz, ap.kb[4], ap.br);
+        double G = face_cond(n.hzp, I, n.zp, kappa, lz, ap.kb[5], ap.br);
+        d[I] = -(A + B + D + E + F + G) - alpha;  // :144
+        // forward couplings: value of hmatrixfunc(i, j) for the pair, in = lower linear index
+        double c;
+        c = 0.0;
+        if (n.hxp) {
+            uint32_t lo = n.xp > I ? I : n.xp, hi = n.xp > I ? n.xp : I;
+            c = cond_int(kappa[lo], kappa[hi], lx[lo], lx[hi], ap.br);
+            if (fabs(c) < kTiny) c = 0.0;  // dropped from the COO (mod_setup.f90:137)
+        }
+        cx[I] = c;
+        c = 0.0;
+        if (n.hyp) {
+            uint32_t lo = n.yp > I ? I : n.yp, hi = n.yp > I ? n.yp : I;
+            c = cond_int(kappa[lo], kappa[hi], ly[lo], ly[hi], ap.br);
+            if (fabs(c) < kTiny) c = 0.0;
+        }
+        cy[I] = c;
+        c = 0.0;
+        if (n.hzp) {
+            uint32_t lo = n.zp > I ? I : n.zp, hi = n.zp > I ? n.zp : I;
+            c = cond_int(kappa[lo], kappa[hi], lz[lo], lz[hi], ap.br);
+            if (fabs(c) < kTiny) c = 0.0;
+        }
+        cz[I] = c;
+
+        // boundary vector (src/heatflow/mod_boundary.f90:72-150)
+        double kap = kappa[I];
+        double b = 0.0;
+        const uint32_t pos[3] = {n.x, n.y, n.z};
+        const uint32_t len[3] = {g.nx, g.ny, g.nz};
+        const double Ls[3] = {lx[I], ly[I], lz[I]};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (ap.periodic[ax]) continue;
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int f = 2 * ax + side;
+                if (pos[ax] != (side == 0 ? 0u : len[ax] - 1)) continue;
+                if (ap.cg[f]) {
+                    b = ap.cg_pd[f];
+                } else {
+                    double kh = (2.0 * kap * ap.kb[f] / (kap + ap.kb[f])) / (Ls[ax] * Ls[ax]);
+                    if (kap != ap.kb[f]) kh = kh * ap.br;
+                    b = b + (kh) * ap.tb[f];
+                }
+            }
+        }
+        bvec[I] = b;
+        if (isnan(b)) fl |= FLAG_NAN_B;
+    }
+    block_flags(fl, &sc->flags);
+}
+
+// s = 1/sqrt(-diag) and the couplings of A^ = S(-H)S = I - C^, C^_IJ = s_I c_IJ s_J (FP64 and
+// FP32 copies).  The scaled coupling of a pair is stored once, at the same index as c.
+__global__ void __launch_bounds__(kBlock)
+k_scale(Dims g, const double *__restrict__ d, const double *__restrict__ cx,
+        const double *__restrict__ cy, const double *__restrict__ cz, double *__restrict__ sv,
+        double *__restrict__ hx, double *__restrict__ hy, double *__restrict__ hz,
+        float *__restrict__ fx, float *__restrict__ fy, float *__restrict__ fz) {
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        Nbr n = neighbours(g, I);
+        double si = 1.0 / sqrt(-d[I]);
+        sv[I] = si;
+        double vx = n.hxp ? si * cx[I] * (1.0 / sqrt(-d[n.xp])) : 0.0;
+        double vy = n.hyp ? si * cy[I] * (1.0 / sqrt(-d[n.yp])) : 0.0;
+        double vz = n.hzp ? si * cz[I] * (1.0 / sqrt(-d[n.zp])) : 0.0;
+        if (hx) { hx[I] = vx; hy[I] = vy; hz[I] = vz; }
+        if (fx) { fx[I] = (float)vx; fy[I] = (float)vy; fz[I] = (float)vz; }
+    }
+}
+
+// ---- COO export (reference order, mod_setup.f90:121-148) ------------------------------------
+
+__device__ __forceinline__ bool emitted(double c) { return !(fabs(c) < kTiny); }
+
+// emitted couplings of column j in addit order: x+1, x wrap, y+nx, y wrap, z+nxny, z wrap
+__device__ __forceinline__ int coo_slots(const Dims &g, const Nbr &n, uint32_t j,
+                                         const double *cx, const double *cy, const double *cz,
+                                         uint32_t *rows, double *vals) {
+    int k = 0;
+    if (n.x + 1 < g.nx && emitted(cx[j])) { rows[k] = j + 1; vals[k] = cx[j]; ++k; }
+    if (g.per[0] && n.x == 0) {
+        uint32_t i = j + g.nx - 1;
+        if (emitted(cx[i])) { rows[k] = i; vals[k] = cx[i]; ++k; }
+    }
+    if (g.ny > 1 && n.y + 1 < g.ny && emitted(cy[j])) { rows[k] = j + g.nx; vals[k] = cy[j]; ++k; }
+    if (g.per[1] && n.y == 0) {
+        uint32_t i = j + (g.ny - 1) * g.nx;
+        if (emitted(cy[i])) { rows[k] = i; vals[k] = cy[i]; ++k; }
+    }
+    if (g.nz > 1 && n.z + 1 < g.nz && emitted(cz[j])) { rows[k] = j + g.nxny; vals[k] = cz[j]; ++k; }
+    if (g.per[2] && n.z == 0) {
+        uint32_t i = j + (g.nz - 1) * g.nxny;
+        if (emitted(cz[i])) { rows[k] = i; vals[k] = cz[i]; ++k; }
+    }
+    return k;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_coo_count(Dims g, const double *__restrict__ cx, const double *__restrict__ cy,
+            const double *__restrict__ cz, int64_t *__restrict__ cnt) {
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    for (uint32_t j = beg + threadIdx.x; j < end; j += kBlock) {
+        Nbr n = neighbours(g, j);
+        uint32_t rows[6];
+        double vals[6];
+        int k = coo_slots(g, n, j, cx, cy, cz, rows, vals);
+        cnt[j] = 1 + 2 * k;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_coo_fill(Dims g, const double *__restrict__ d, const double *__restrict__ cx,
+           const double *__restrict__ cy, const double *__restrict__ cz,
+           const int64_t *__restrict__ off, double *__restrict__ val, int64_t *__restrict__ irow,
+           int64_t *__restrict__ jcol) {
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    for (uint32_t j = beg + threadIdx.x; j < end; j += kBlock) {
+        Nbr n = neighbours(g, j);
+        uint32_t rows[6];
+        double vals[6];
+        int k = coo_slots(g, n, j, cx, cy, cz, rows, vals);
+        int64_t o = off[j];
+        val[o] = d[j]; irow[o] = (int64_t)j + 1; jcol[o] = (int64_t)j + 1; ++o;
+        for (int s = 0; s < k; ++s) {
+            val[o] = vals[s]; irow[o] = (int64_t)rows[s] + 1; jcol[o] = (int64_t)j + 1; ++o;
+            val[o] = vals[s]; irow[o] = (int64_t)j + 1; jcol[o] = (int64_t)rows[s] + 1; ++o;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_coo_tail(int64_t count, int64_t len, double *__restrict__ val, int64_t *__restrict__ irow,
+           int64_t *__restrict__ jcol) {
+    for (int64_t k = count + (int64_t)blockIdx.x * kBlock + threadIdx.x; k < len;
+         k += (int64_t)gridDim.x * kBlock) {
+        val[k] = 0.0; irow[k] = -2; jcol[k] = -1;
+    }
+}
+
+// exclusive scan of int64 counts in three passes (block sums, one-block scan, apply)
+__global__ void __launch_bounds__(kBlock)
+k_scan_blocksum(const int64_t *__restrict__ in, int64_t n, int64_t *__restrict__ bsum) {
+    __shared__ int64_t lds[kBlock];
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    lds[threadIdx.x] = i < n ? in[i] : 0;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = lds[0];
+}
+
+__global__ void __launch_bounds__(1024)
+k_scan_single(int64_t *__restrict__ bsum, int64_t nb, int64_t *__restrict__ total) {
+    __shared__ int64_t lds[1024];
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t b0 = threadIdx.x * per;
+    int64_t s = 0;
+    for (int64_t k = 0; k < per && b0 + k < nb; ++k) s += bsum[b0 + k];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    // inclusive Hillis-Steele scan over the 1024 thread sums
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = threadIdx.x >= (unsigned)off ? lds[threadIdx.x - off] : 0;
+        __syncthreads();
+        lds[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int64_t run = threadIdx.x == 0 ? 0 : lds[threadIdx.x - 1];
+    for (int64_t k = 0; k < per && b0 + k < nb; ++k) {
+        int64_t v = bsum[b0 + k];
+        bsum[b0 + k] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) *total = lds[1023];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_scan_apply(const int64_t *__restrict__ in, int64_t n, const int64_t *__restrict__ bsum,
+             int64_t *__restrict__ out) {
+    __shared__ int64_t lds[kBlock];
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    int64_t v = i < n ? in[i] : 0;
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        int64_t t = threadIdx.x >= (unsigned)off ? lds[threadIdx.x - off] : 0;
+        __syncthreads();
+        lds[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) out[i] = bsum[blockIdx.x] + lds[threadIdx.x] - v;
+}
+
+// ---- right-hand side -----------------------------------------------------------------------
+
+// Heater power density Qdens of one step (src/heatflow/mod_heating.f90:59-167) plus the
+// superset per-cell density.  qmode[m] is the mode-m power of this step computed on the host;
+// mode 5 adds (q5b * tau) * q5c with tau = tau_s * dt^2 (:56, :104-105).
+__device__ __forceinline__ double heater_q(const HeatParams &hp, int8_t mode, double tau_s,
+                                           double ells, double tp) {
+    double q;
+    switch (mode) {
+        case 0: q = 0.0; break;
+        case 5: q = hp.qmode[5] + hp.q5b * (tau_s * hp.dt2) * hp.q5c; break;
+        default: q = (mode >= 1 && mode <= 6) ? hp.qmode[mode] : 0.0; break;
+    }
+    return q - ells * (pow(tp, 4.0) - hp.tb4);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_qdens(Dims g, HeatParams hp, const int8_t *__restrict__ iheater,
+        const double *__restrict__ ells, const double *__restrict__ tau_s,
+        const double *__restrict__ tp, const double *__restrict__ qextra,
+        double *__restrict__ qd, Scal *sc) {
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    int32_t fl = 0;
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        double q = 0.0, qdv = 0.0;
+        if (hp.any_heater) {
+            q = heater_q(hp, iheater[I], tau_s[I], ells[I], tp[I]);
+            qdv = hp.heated_volume > 0.0 ? q / hp.heated_volume : 0.0;
+            if (isnan(q)) fl |= FLAG_NAN_Q;
+            if (isnan(qdv)) fl |= FLAG_NAN_QDENS;
+        }
+        if (qextra) qdv = qdv + qextra[I];
+        qd[I] = qdv;
+    }
+    block_flags(fl, &sc->flags);
+}
+
+template <typename OutT, bool FromState, bool StoreS, bool StoreX>
+__global__ void __launch_bounds__(kBlock)
+k_rhs_res(Dims g, RhsParams rp, const double *__restrict__ tp, const double *__restrict__ tpp,
+          const double *__restrict__ xin, const double *__restrict__ bin,
+          const double *__restrict__ rhoc, const double *__restrict__ cvec,
+          const double *__restrict__ qd, const double *__restrict__ bvec,
+          const double *__restrict__ d, const double *__restrict__ cx,
+          const double *__restrict__ cy, const double *__restrict__ cz,
+          OutT *__restrict__ rhat, double *__restrict__ sout, double *__restrict__ xout,
+          Scal *sc, double *partials) {
+    const bool tiny = sc->tiny != 0;
+    auto X = [&](uint32_t J) -> double {
+        if (FromState) {
+            double x0 = tp[J] + (tp[J] - tpp[J]);  // mod_evolve.f90:140
+            if (tiny) x0 = x0 + kTiny;             // :141
+            return x0;
+        } else {
+            return xin[J];
+        }
+    };
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    double acc[2] = {0.0, 0.0};
+    int32_t fl = 0;
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        Nbr n = neighbours(g, I);
+        double S;
+        if (FromState) {
+            // mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36
+            if (rp.isteady == 0) {
+                S = -(rp.inverse_time * tp[I] * rhoc[I]) - qd[I] - bvec[I];
+                if (rp.icattaneo == 1) {
+                    double scat = (tpp[I] - 2.0 * tp[I]) * cvec[I];
+                    if (isnan(scat)) fl |= FLAG_NAN_SCAT;
+                    S = S + scat;
+                }
+            } else {
+                S = -qd[I] - bvec[I];
+            }
+            if (isnan(S)) fl |= FLAG_NAN_S;
+        } else {
+            S = bin[I];
+        }
+        // H x in ascending column order (the serial SRSax order, mod_Sparse.f90:306-314)
+        double hxv = 0.0;
+        if (n.hzm && n.zm < I) hxv = hxv + cz[n.zm] * X(n.zm);
+        if (n.hym && n.ym < I) hxv = hxv + cy[n.ym] * X(n.ym);
+        if (n.hxm && n.xm < I) hxv = hxv + cx[n.xm] * X(n.xm);
+        const double xi = X(I);
+        hxv = hxv + d[I] * xi;
+        if (n.hxp && n.xp > I) hxv = hxv + cx[I] * X(n.xp);
+        if (n.hyp && n.yp > I) hxv = hxv + cy[I] * X(n.yp);
+        if (n.hzp && n.zp > I) hxv = hxv + cz[I] * X(n.zp);
+        // periodic wrap couplings (neighbour on the other side of the domain)
+        if (n.hxm && n.xm > I) hxv = hxv + cx[n.xm] * X(n.xm);
+        if (n.hxp && n.xp < I) hxv = hxv + cx[I] * X(n.xp);
+        if (n.hym && n.ym > I) hxv = hxv + cy[n.ym] * X(n.ym);
+        if (n.hyp && n.yp < I) hxv = hxv + cy[I] * X(n.yp);
+        if (n.hzm && n.zm > I) hxv = hxv + cz[n.zm] * X(n.zm);
+        if (n.hzp && n.zp < I) hxv = hxv + cz[I] * X(n.zp);
+        const double r = S - hxv;
+        const double s = 1.0 / sqrt(-d[I]);
+        const double rh = -s * r;
+        rhat[I] = (OutT)rh;
+        if (StoreS) sout[I] = S;
+        if (StoreX) xout[I] = xi;
+        const double bh = s * S;
+        const double rq = (double)(OutT)rh;   // r.r of the stored (possibly FP32) residual
+        acc[0] += bh * bh;
+        acc[1] += rq * rq;
+    }
+    block_flags(fl, &sc->flags);
+    const double tol2 = rp.tol * rp.tol;
+    const int64_t itmax = rp.itmax;
+    grid_sum<2>(acc, partials, &sc->counter, [=](double *v) {
+        sc->bnorm2 = v[0];
+        sc->rr = v[1];
+        sc->rr_start = v[1];
+        sc->outer_rr = v[1];
+        sc->thresh2 = tol2 * v[0];
+        sc->iter = 0;
+        sc->itmax = itmax;
+        sc->first = 1;
+        sc->pq = 0.0;
+        sc->alpha = 0.0;
+        sc->beta = 0.0;
+        // An exactly zero residual needs no iteration (the reference would divide 0/0).
+        sc->done = (v[1] == 0.0) || isnan(v[1]) || itmax <= 0;
+        if (isnan(v[1])) atomicOr(&sc->flags, (int32_t)FLAG_BREAKDOWN);
+    });
+}
+
+// ---- CG on A^ = I - C^ (unit diagonal; the Jacobi-preconditioned CG of linbcg in scaled form)
+
+// p = r + beta * p_old (p = r on the first iteration), q = A^ p, partial p.q.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_a(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
+       T *__restrict__ q, const T *__restrict__ hx, const T *__restrict__ hy,
+       const T *__restrict__ hz, Scal *sc, double *partials) {
+    if (sc->done) return;
+    const bool first = sc->first != 0;
+    const T beta = (T)sc->beta;
+    auto P = [&](uint32_t J) -> T { return first ? r[J] : (T)(r[J] + beta * pold[J]); };
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    double acc[1] = {0.0};
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        Nbr n = neighbours(g, I);
+        const T pi = P(I);
+        T s = (T)0;
+        if (n.hzm) s = s + hz[n.zm] * P(n.zm);
+        if (n.hym) s = s + hy[n.ym] * P(n.ym);
+        if (n.hxm) s = s + hx[n.xm] * P(n.xm);
+        if (n.hxp) s = s + hx[I] * P(n.xp);
+        if (n.hyp) s = s + hy[I] * P(n.yp);
+        if (n.hzp) s = s + hz[I] * P(n.zp);
+        const T qi = pi - s;
+        pnew[I] = pi;
+        q[I] = qi;
+        acc[0] += (double)pi * (double)qi;
+    }
+    grid_sum<1>(acc, partials, &sc->counter, [=](double *v) {
+        sc->pq = v[0];
+        sc->alpha = sc->rr / v[0];
+        if (!(v[0] > 0.0)) {  // A^ is SPD: p.q <= 0 or NaN is a breakdown
+            sc->done = 1;
+            sc->alpha = 0.0;
+            atomicOr(&sc->flags, (int32_t)FLAG_BREAKDOWN);
+        }
+    });
+}
+
+// y += alpha p, r -= alpha q, partial r.r; convergence test on ||r^|| / ||b^||.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_b(Dims g, T *__restrict__ y, const T *__restrict__ p, T *__restrict__ r,
+       const T *__restrict__ q, Scal *sc, double *partials) {
+    if (sc->done) return;
+    const bool first = sc->first != 0;
+    const T alpha = (T)sc->alpha;
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    double acc[1] = {0.0};
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        const T yi = first ? alpha * p[I] : y[I] + alpha * p[I];
+        const T ri = r[I] - alpha * q[I];
+        y[I] = yi;
+        r[I] = ri;
+        acc[0] += (double)ri * (double)ri;
+    }
+    grid_sum<1>(acc, partials, &sc->counter, [=](double *v) {
+        const double rr = v[0];
+        sc->iter += 1;
+        sc->beta = rr / sc->rr;
+        sc->rr = rr;
+        sc->first = 0;
+        if (rr <= sc->thresh2 || sc->iter >= sc->itmax || isnan(rr)) sc->done = 1;
+        if (isnan(rr)) atomicOr(&sc->flags, (int32_t)FLAG_BREAKDOWN);
+    });
+}
+
+// x = x0 + s*y written over T^{n-1} (in place per cell), NaN guard of x (mod_evolve.f90:153),
+// and the TINY test of the next step's initial guess (:141).  When FromState is false the
+// explicit initial guess x is updated instead (linbcg-compatible solve).
+template <bool FromState>
+__global__ void __launch_bounds__(kBlock)
+k_finalize(Dims g, const double *__restrict__ tp, double *__restrict__ tpp_io,
+           const double *__restrict__ sv, const double *__restrict__ y, Scal *sc) {
+    const bool tiny = sc->tiny != 0;
+    const bool have_y = sc->iter > 0;
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    int32_t fl = 0;
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        double x0;
+        if (FromState) {
+            x0 = tp[I] + (tp[I] - tpp_io[I]);
+            if (tiny) x0 = x0 + kTiny;
+        } else {
+            x0 = tpp_io[I];
+        }
+        const double x = have_y ? x0 + sv[I] * y[I] : x0;
+        tpp_io[I] = x;
+        if (isnan(x)) fl |= FLAG_NAN_X;
+        if (FromState && (x + (x - tp[I])) - x < kTiny) fl |= FLAG_TINY_NEXT;
+    }
+    block_flags(fl, &sc->flags);
+}
+
+// Mixed mode refinement: x_new = x + s*y (y FP32) formed on the fly for the stencil, the FP64
+// residual r^ = -s (S - H x_new) stored as FP32 for the next inner solve, ||r^||^2, and the
+// NaN / next-step TINY tests on x_new.
+__global__ void __launch_bounds__(kBlock)
+k_ir_res(Dims g, IrParams ip, const double *__restrict__ x, const float *__restrict__ y,
+         const double *__restrict__ sv, const double *__restrict__ S,
+         const double *__restrict__ d, const double *__restrict__ cx,
+         const double *__restrict__ cy, const double *__restrict__ cz,
+         const double *__restrict__ tp, double *__restrict__ xout, float *__restrict__ rhat,
+         Scal *sc, double *partials) {
+    auto X = [&](uint32_t J) -> double { return x[J] + sv[J] * (double)y[J]; };
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    double acc[1] = {0.0};
+    int32_t fl = 0;
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        Nbr n = neighbours(g, I);
+        double hxv = 0.0;
+        if (n.hzm && n.zm < I) hxv = hxv + cz[n.zm] * X(n.zm);
+        if (n.hym && n.ym < I) hxv = hxv + cy[n.ym] * X(n.ym);
+        if (n.hxm && n.xm < I) hxv = hxv + cx[n.xm] * X(n.xm);
+        const double xi = X(I);
+        hxv = hxv + d[I] * xi;
+        if (n.hxp && n.xp > I) hxv = hxv + cx[I] * X(n.xp);
+        if (n.hyp && n.yp > I) hxv = hxv + cy[I] * X(n.yp);
+        if (n.hzp && n.zp > I) hxv = hxv + cz[I] * X(n.zp);
+        if (n.hxm && n.xm > I) hxv = hxv + cx[n.xm] * X(n.xm);
+        if (n.hxp && n.xp < I) hxv = hxv + cx[I] * X(n.xp);
+        if (n.hym && n.ym > I) hxv = hxv + cy[n.ym] * X(n.ym);
+        if (n.hyp && n.yp < I) hxv = hxv + cy[I] * X(n.yp);
+        if (n.hzm && n.zm > I) hxv = hxv + cz[n.zm] * X(n.zm);
+        if (n.hzp && n.zp < I) hxv = hxv + cz[I] * X(n.zp);
+        const double rh = -sv[I] * (S[I] - hxv);
+        const float rf = (float)rh;
+        rhat[I] = rf;
+        xout[I] = xi;
+        acc[0] += (double)rf * (double)rf;
+        if (isnan(xi)) fl |= FLAG_NAN_X;
+        if (ip.from_state && (xi + (xi - tp[I])) - xi < kTiny) fl |= FLAG_TINY_NEXT;
+    }
+    block_flags(fl, &sc->flags);
+    const int64_t sweeps_max = ip.itmax;
+    grid_sum<1>(acc, partials, &sc->counter, [=](double *v) {
+        sc->outer_rr = v[0];
+        sc->iter += 1;   // refinement sweeps
+        sc->done = (v[0] <= sc->thresh2) || sc->iter >= sweeps_max || isnan(v[0]);
+        if (isnan(v[0])) atomicOr(&sc->flags, (int32_t)FLAG_BREAKDOWN);
+    });
+}
+
+// Starts an inner FP32 solve from the outer scalars: thresholds relative to this sweep's
+// residual, floored by the outer target (no point solving further than the outer test needs).
+__global__ void k_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
+                              int64_t inner_itmax, double safety) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double rr = outer->outer_rr;
+    double t_rel = inner_tol * inner_tol * rr;
+    double t_abs = safety * safety * outer_tol * outer_tol * outer->bnorm2;
+    inner->rr = rr;
+    inner->rr_start = rr;
+    inner->bnorm2 = outer->bnorm2;
+    inner->thresh2 = t_rel > t_abs ? t_rel : t_abs;
+    inner->iter = 0;
+    inner->itmax = inner_itmax;
+    inner->first = 1;
+    inner->beta = 0.0;
+    inner->alpha = 0.0;
+    inner->done = (rr == 0.0) || isnan(rr) || inner_itmax <= 0;
+    inner->counter = 0u;
+}
+
+// Clears the per-step flag bits, keeping TINY_NEXT -> tiny for this step.
+__global__ void k_step_begin(Scal *sc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    sc->tiny = (sc->flags & FLAG_TINY_NEXT) ? 1 : 0;
+    sc->flags = sc->flags & FLAG_NAN_B;
+    sc->counter = 0u;
+}
+
+// TINY rule from an arbitrary state (first step after hf_set_state)
+__global__ void __launch_bounds__(kBlock)
+k_tiny_from_state(Dims g, const double *__restrict__ tp, const double *__restrict__ tpp,
+                  Scal *sc) {
+    uint32_t beg, end;
+    block_range(g, &beg, &end);
+    int32_t fl = 0;
+    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+        double x = tp[I] + (tp[I] - tpp[I]);
+        if (x - tp[I] < kTiny) fl |= FLAG_TINY_NEXT;
+    }
+    block_flags(fl, &sc->flags);
+}
+
+// ---- host-side launchers -----------------------------------------------------------------
+
+#define HF_LAUNCH(kern, g, st, ...) \
+    hipLaunchKernelGGL(kern, dim3((g).nblk), dim3(kBlock), 0, st, __VA_ARGS__)
+
+void launch_assemble(const Dims &g, const AsmParams &ap, const double *kappa, const double *lx,
+                     const double *ly, const double *lz, const double *rhoc,
+                     const double *tau_s, double *d, double *cx, double *cy, double *cz,
+                     double *bvec, Scal *sc, hipStream_t st) {
+    HF_LAUNCH(k_assemble, g, st, g, ap, kappa, lx, ly, lz, rhoc, tau_s, d, cx, cy, cz, bvec, sc);
+}
+
+void launch_scale(const Dims &g, const double *d, const double *cx, const double *cy,
+                  const double *cz, double *sv, double *hx, double *hy, double *hz, float *fx,
+                  float *fy, float *fz, hipStream_t st) {
+    HF_LAUNCH(k_scale, g, st, g, d, cx, cy, cz, sv, hx, hy, hz, fx, fy, fz);
+}
+
+void launch_coo(const Dims &g, const double *d, const double *cx, const double *cy,
+                const double *cz, int64_t *cnt, int64_t *off, int64_t *bsum, int64_t *total,
+                double *val, int64_t *irow, int64_t *jcol, int64_t len, int64_t *count_host,
+                hipStream_t st) {
+    HF_LAUNCH(k_coo_count, g, st, g, cx, cy, cz, cnt);
+    const int64_t n = g.N;
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_scan_blocksum, dim3((unsigned)nb), dim3(kBlock), 0, st, cnt, n, bsum);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kBlock), 0, st, cnt, n, bsum, off);
+    (void)hipMemcpyAsync(count_host, total, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    (void)hipStreamSynchronize(st);
+    if (val) {
+        HF_LAUNCH(k_coo_fill, g, st, g, d, cx, cy, cz, off, val, irow, jcol);
+        if (*count_host < len) {
+            int64_t rem = len - *count_host;
+            unsigned nbt = (unsigned)((rem + kBlock - 1) / kBlock);
+            if (nbt > 65536u) nbt = 65536u;
+            hipLaunchKernelGGL(k_coo_tail, dim3(nbt), dim3(kBlock), 0, st, *count_host, len, val,
+                               irow, jcol);
+        }
+    }
+}
+
+void launch_qdens(const Dims &g, const HeatParams &hp, const int8_t *iheater, const double *ells,
+                  const double *tau_s, const double *tp, const double *qextra, double *qd,
+                  Scal *sc, hipStream_t st) {
+    HF_LAUNCH(k_qdens, g, st, g, hp, iheater, ells, tau_s, tp, qextra, qd, sc);
+}
+
+void launch_rhs_res_state64(const Dims &g, const RhsParams &rp, const double *tp,
+                            const double *tpp, const double *rhoc, const double *cvec,
+                            const double *qd, const double *bvec, const double *d,
+                            const double *cx, const double *cy, const double *cz, double *rhat,
+                            double *sout, Scal *sc, double *partials, hipStream_t st) {
+    if (sout)
+        HF_LAUNCH((k_rhs_res<double, true, true, false>), g, st, g, rp, tp, tpp, nullptr, nullptr,
+                  rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, sout, nullptr, sc, partials);
+    else
+        HF_LAUNCH((k_rhs_res<double, true, false, false>), g, st, g, rp, tp, tpp, nullptr, nullptr,
+                  rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, nullptr, nullptr, sc, partials);
+}
+
+void launch_rhs_res_state32(const Dims &g, const RhsParams &rp, const double *tp,
+                            const double *tpp, const double *rhoc, const double *cvec,
+                            const double *qd, const double *bvec, const double *d,
+                            const double *cx, const double *cy, const double *cz, float *rhat,
+                            double *sout, double *xout, Scal *sc, double *partials,
+                            hipStream_t st) {
+    HF_LAUNCH((k_rhs_res<float, true, true, true>), g, st, g, rp, tp, tpp, nullptr, nullptr, rhoc,
+              cvec, qd, bvec, d, cx, cy, cz, rhat, sout, xout, sc, partials);
+}
+
+void launch_rhs_res_solve64(const Dims &g, const RhsParams &rp, const double *x,
+                            const double *b, const double *d, const double *cx,
+                            const double *cy, const double *cz, double *rhat, Scal *sc,
+                            double *partials, hipStream_t st) {
+    HF_LAUNCH((k_rhs_res<double, false, false, false>), g, st, g, rp, nullptr, nullptr, x, b,
+              nullptr, nullptr, nullptr, nullptr, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
+              partials);
+}
+
+template <typename T>
+void launch_cg_a(const Dims &g, const T *r, const T *pold, T *pnew, T *q, const T *hx,
+                 const T *hy, const T *hz, Scal *sc, double *partials, hipStream_t st) {
+    HF_LAUNCH(k_cg_a<T>, g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+}
+template <typename T>
+void launch_cg_b(const Dims &g, T *y, const T *p, T *r, const T *q, Scal *sc, double *partials,
+                 hipStream_t st) {
+    HF_LAUNCH(k_cg_b<T>, g, st, g, y, p, r, q, sc, partials);
+}
+template void launch_cg_a<double>(const Dims &, const double *, const double *, double *,
+                                  double *, const double *, const double *, const double *,
+                                  Scal *, double *, hipStream_t);
+template void launch_cg_a<float>(const Dims &, const float *, const float *, float *, float *,
+                                 const float *, const float *, const float *, Scal *, double *,
+                                 hipStream_t);
+template void launch_cg_b<double>(const Dims &, double *, const double *, double *,
+                                  const double *, Scal *, double *, hipStream_t);
+template void launch_cg_b<float>(const Dims &, float *, const float *, float *, const float *,
+                                 Scal *, double *, hipStream_t);
+
+void launch_finalize(const Dims &g, bool from_state, const double *tp, double *tpp_io,
+                     const double *sv, const double *y, Scal *sc, hipStream_t st) {
+    if (from_state)
+        HF_LAUNCH(k_finalize<true>, g, st, g, tp, tpp_io, sv, y, sc);
+    else
+        HF_LAUNCH(k_finalize<false>, g, st, g, tp, tpp_io, sv, y, sc);
+}
+
+void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const float *y,
+                   const double *sv, const double *S, const double *d, const double *cx,
+                   const double *cy, const double *cz, const double *tp, double *xout,
+                   float *rhat, Scal *sc, double *partials, hipStream_t st) {
+    HF_LAUNCH(k_ir_res, g, st, g, ip, x, y, sv, S, d, cx, cy, cz, tp, xout, rhat, sc, partials);
+}
+
+void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
+                        int64_t inner_itmax, double safety, hipStream_t st) {
+    hipLaunchKernelGGL(k_inner_begin, dim3(1), dim3(64), 0, st, outer, inner, inner_tol,
+                       outer_tol, inner_itmax, safety);
+}
+
+void launch_step_begin(Scal *sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, st, sc);
+}
+
+void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,
+                            hipStream_t st) {
+    HF_LAUNCH(k_tiny_from_state, g, st, g, tp, tpp, sc);
+}
+
+}  // namespace hf

@@ -1,0 +1,89 @@
+// hf_kernels.hpp -- parameter blocks and host launchers of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hf_device.hpp"
+
+namespace hf {
+
+struct AsmParams {
+    double inverse_time;       // 1/time_step (mod_setup.f90:45)
+    double kb[6];              // kappaBound x1, Nx, y1, Ny, z1, Nz
+    double tb[6];              // T_Bath x1, x2, y1, y2, z1, z2
+    double cg_pd[6];           // CG face power densities (mod_boundary.f90:43-66)
+    double br;
+    int32_t isteady, icattaneo;
+    int32_t periodic[3];       // periodic axes (no boundary terms; length >= 3)
+    int32_t cg[6];
+};
+
+struct HeatParams {
+    double qmode[7];           // mode power of this step, index = iheater (mod_heating.f90:59-117)
+    double q5b, q5c;           // mode 5: + (q5b * tau) * q5c
+    double dt2;                // time_step**2 (mod_heating.f90:56)
+    double tb4;                // T_Bath**4
+    double heated_volume;
+    int32_t any_heater;
+};
+
+struct RhsParams {
+    double inverse_time;
+    double tol;
+    int64_t itmax;
+    int32_t isteady, icattaneo;
+};
+
+struct IrParams {
+    int64_t itmax;             // refinement sweep cap
+    int32_t from_state;
+};
+
+void launch_assemble(const Dims &g, const AsmParams &ap, const double *kappa, const double *lx,
+                     const double *ly, const double *lz, const double *rhoc,
+                     const double *tau_s, double *d, double *cx, double *cy, double *cz,
+                     double *bvec, Scal *sc, hipStream_t st);
+void launch_scale(const Dims &g, const double *d, const double *cx, const double *cy,
+                  const double *cz, double *sv, double *hx, double *hy, double *hz, float *fx,
+                  float *fy, float *fz, hipStream_t st);
+void launch_coo(const Dims &g, const double *d, const double *cx, const double *cy,
+                const double *cz, int64_t *cnt, int64_t *off, int64_t *bsum, int64_t *total,
+                double *val, int64_t *irow, int64_t *jcol, int64_t len, int64_t *count_host,
+                hipStream_t st);
+void launch_qdens(const Dims &g, const HeatParams &hp, const int8_t *iheater, const double *ells,
+                  const double *tau_s, const double *tp, const double *qextra, double *qd,
+                  Scal *sc, hipStream_t st);
+void launch_rhs_res_state64(const Dims &g, const RhsParams &rp, const double *tp,
+                            const double *tpp, const double *rhoc, const double *cvec,
+                            const double *qd, const double *bvec, const double *d,
+                            const double *cx, const double *cy, const double *cz, double *rhat,
+                            double *sout, Scal *sc, double *partials, hipStream_t st);
+void launch_rhs_res_state32(const Dims &g, const RhsParams &rp, const double *tp,
+                            const double *tpp, const double *rhoc, const double *cvec,
+                            const double *qd, const double *bvec, const double *d,
+                            const double *cx, const double *cy, const double *cz, float *rhat,
+                            double *sout, double *xout, Scal *sc, double *partials,
+                            hipStream_t st);
+void launch_rhs_res_solve64(const Dims &g, const RhsParams &rp, const double *x,
+                            const double *b, const double *d, const double *cx,
+                            const double *cy, const double *cz, double *rhat, Scal *sc,
+                            double *partials, hipStream_t st);
+template <typename T>
+void launch_cg_a(const Dims &g, const T *r, const T *pold, T *pnew, T *q, const T *hx,
+                 const T *hy, const T *hz, Scal *sc, double *partials, hipStream_t st);
+template <typename T>
+void launch_cg_b(const Dims &g, T *y, const T *p, T *r, const T *q, Scal *sc, double *partials,
+                 hipStream_t st);
+void launch_finalize(const Dims &g, bool from_state, const double *tp, double *tpp_io,
+                     const double *sv, const double *y, Scal *sc, hipStream_t st);
+void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const float *y,
+                   const double *sv, const double *S, const double *d, const double *cx,
+                   const double *cy, const double *cz, const double *tp, double *xout,
+                   float *rhat, Scal *sc, double *partials, hipStream_t st);
+void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
+                        int64_t inner_itmax, double safety, hipStream_t st);
+void launch_step_begin(Scal *sc, hipStream_t st);
+void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,
+                            hipStream_t st);
+
+}  // namespace hf

@@ -1,0 +1,209 @@
+"""Host-side description of a HeatFlow system: the `inputs` module state the time step reads.
+
+`Params` holds the scalar keys of param.in that the path uses (src/heatflow/mod_inputs.f90:
+204-360, defaults :213-270, the check_param fix-ups :366-470) and `Cells` the per-cell fields of
+grid(:,:,:) (src/heatflow/mod_constructions.f90:25-31) as structure-of-arrays in the
+reference's linear order (x fastest).  `read_inputs` reads the reference's own input files
+(param.in, mat.in, system.in) for file materials (index <= 99).
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+import re
+from typing import Optional, Sequence
+
+import numpy as np
+
+FACES = ("x1", "xN", "y1", "yN", "z1", "zN")
+
+
+@dataclasses.dataclass
+class Params:
+    nx: int
+    ny: int
+    nz: int
+    time_step: float = 1.0          # mod_inputs.f90:226
+    icattaneo: int = 1              # :229
+    isteady: int = 0
+    periodic: Sequence[bool] = (False, False, False)
+    kappa_bound: Sequence[float] = (0.0,) * 6   # kappaBoundx1, Nx, y1, Ny, z1, Nz
+    t_bath: Sequence[float] = (0.0,) * 6        # T_Bathx1, x2, y1, y2, z1, z2
+    br: float = 1.0                 # BR
+    cg: Sequence[bool] = (False,) * 6           # CG_dir: -x, +x, -y, +y, -z, +z
+    t_bath_cg: float = 0.0
+    power_in: float = 0.0
+    freq: float = 1.0
+    heated_steps: int = 0           # heattime
+    t_bath_rad: float = 0.0         # T_Bath (radiative term)
+    t_system: float = 300.0         # T_System (initial_evolve)
+    ntime: int = 10
+
+    @property
+    def n(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def inverse_time(self) -> float:
+        return 1.0 / self.time_step
+
+
+@dataclasses.dataclass
+class Cells:
+    kappa: np.ndarray
+    rho: np.ndarray
+    heat_capacity: np.ndarray
+    tau: np.ndarray                 # physical relaxation time (s)
+    em: np.ndarray
+    len_x: np.ndarray
+    len_y: np.ndarray
+    len_z: np.ndarray
+    iheater: np.ndarray             # int64
+    qdens_extra: Optional[np.ndarray] = None   # superset: W/m^3 added to Qdens
+
+    @property
+    def volume(self) -> np.ndarray:
+        # grid%volume = Length(1)*Length(2)*Length(3) (mod_inputs.f90:623)
+        return self.len_x * self.len_y * self.len_z
+
+    def tau_scaled(self, time_step: float) -> np.ndarray:
+        # grid%tau = tau*inverse_time*inverse_time (mod_setup.f90:62)
+        it = 1.0 / time_step
+        return self.tau * it * it
+
+    def lin_rhoc(self) -> np.ndarray:
+        return self.rho * self.heat_capacity
+
+
+@dataclasses.dataclass
+class System:
+    params: Params
+    cells: Cells
+
+    @staticmethod
+    def uniform(params: Params, lengths, kappa, rho, heat_capacity, tau=0.0, em=0.0,
+                iheater=None, qdens_extra=None) -> "System":
+        """Builds a system with per-axis uniform cell lengths (Lx/nx, Ly/ny, Lz/nz as
+        read_system does, mod_inputs.f90:619-622); scalars broadcast to every cell."""
+        n = params.n
+        Lx, Ly, Lz = lengths
+
+        def full(v, dtype=np.float64):
+            a = np.asarray(v, dtype=dtype)
+            return np.broadcast_to(a, (n,)).copy() if a.ndim == 0 else a.reshape(n).astype(dtype)
+
+        cells = Cells(
+            kappa=full(kappa), rho=full(rho), heat_capacity=full(heat_capacity),
+            tau=full(tau), em=full(em),
+            len_x=full(Lx / float(params.nx)), len_y=full(Ly / float(params.ny)),
+            len_z=full(Lz / float(params.nz)),
+            iheater=full(0 if iheater is None else iheater, np.int64),
+            qdens_extra=None if qdens_extra is None else full(qdens_extra),
+        )
+        return System(params, cells)
+
+
+# ---- reference input files -----------------------------------------------------------------
+
+def _kv_lines(path: str):
+    with open(path) as f:
+        for raw in f:
+            line = raw.split("!", 1)[0]
+            if not line.strip() or "=" not in line:
+                continue
+            key, val = line[: line.index("=")], line[line.rindex("=") + 1:]
+            yield key.strip(), val.strip()
+
+
+def _logical(v: str) -> bool:
+    # assignL: 'T'/'t' anywhere -> .TRUE., then 'F'/'f' anywhere -> .FALSE.
+    out = False
+    if "T" in v or "t" in v:
+        out = True
+    if "F" in v or "f" in v:
+        out = False
+    return out
+
+
+def read_inputs(dirname: str) -> System:
+    """Reads <dirname>/param.in, mat.in and system.in like read_all_files
+    (src/heatflow/mod_inputs.f90:116-193) for file materials, then applies
+    set_global_variables' per-cell material assignment (mod_setup.f90:53-68)."""
+    # system.in (read_system, :602-649)
+    with open(os.path.join(dirname, "system.in")) as f:
+        lines = [ln for ln in f.read().splitlines()]
+    pos = 0
+
+    def nextline():
+        nonlocal pos
+        ln = lines[pos]
+        pos += 1
+        return ln
+
+    nx, ny, nz = (int(v) for v in nextline().split()[:3])
+    Lx, Ly, Lz = (float(v.replace("d", "e").replace("D", "e")) for v in nextline().split()[:3])
+    mat = np.zeros((nz, ny, nx), np.int64)
+    heat = np.zeros((nz, ny, nx), np.int64)
+    for iz in range(nz):
+        nextline()  # separator line before each z block
+        for iy in range(ny):
+            toks = nextline().split()[:nx]
+            for ix, t in enumerate(toks):
+                m, h = t.split(":")
+                mat[iz, iy, ix] = int(m)
+                heat[iz, iy, ix] = int(h)
+
+    # param.in (read_param, :204-360)
+    kv = {}
+    for k, v in _kv_lines(os.path.join(dirname, "param.in")):
+        kv[k] = v
+    D = lambda k, d: float(kv[k].replace("d", "e").replace("D", "e")) if k in kv else d
+    I = lambda k, d: int(kv[k]) if k in kv else d
+    t_bath = D("T_Bath", 0.0)
+    baths_keys = ("T_Bathx1", "T_Bathx2", "T_Bathy1", "T_Bathy2", "T_Bathz1", "T_Bathz2")
+    baths = [D(k, t_bath) for k in baths_keys]
+    tcg = D("T_BathCG", 0.0)
+    # check_param (:436-460): if any face bath is missing all six take T_Bath
+    if not ("T_BathCG" in kv and tcg > 0) and any(k not in kv for k in baths_keys):
+        baths = [t_bath] * 6
+    kb_keys = ("kappaBoundx1", "kappaBoundNx", "kappaBoundy1", "kappaBoundNy", "kappaBoundz1",
+               "kappaBoundNz")
+    kb = [D(k, 0.0) for k in kb_keys]
+    periodic_s = kv.get("Periodic", "")
+    periodic = tuple(c in periodic_s or c.upper() in periodic_s for c in "xyz")
+    if not all(periodic) and "kappaBound" in kv and any(k not in kv for k in kb_keys):
+        kb = [D("kappaBound", 0.0)] * 6   # :412-426
+    cg_s = kv.get("CG_dir", "")
+    cg = tuple(tok in cg_s for tok in ("-x", "+x", "-y", "+y", "-z", "+z"))
+    params = Params(
+        nx=nx, ny=ny, nz=nz, time_step=D("time_step", 1.0), icattaneo=I("icattaneo", 1),
+        isteady=I("isteady", 0), periodic=periodic, kappa_bound=tuple(kb), t_bath=tuple(baths),
+        br=D("BR", 1.0), cg=cg, t_bath_cg=tcg, power_in=D("power_in", 0.0),
+        freq=D("freq", 1.0), heated_steps=I("heattime", 0), t_bath_rad=t_bath,
+        t_system=D("T_System", 300.0), ntime=I("ntime", 10))
+
+    # mat.in (read_mat, :654-721): blocks "index / key = value ... / 0"
+    mats = {}
+    cur = None
+    with open(os.path.join(dirname, "mat.in")) as f:
+        for raw in f:
+            line = raw.split("!", 1)[0].strip()
+            if not line:
+                continue
+            if re.fullmatch(r"[+-]?\d+", line.split()[0]) and "=" not in line:
+                idx = int(line.split()[0])
+                if idx != 0:
+                    cur = mats.setdefault(idx, {})
+                continue
+            if "=" in line and cur is not None:
+                k, v = line.split("=", 1)
+                cur[k.strip()] = float(v.strip().replace("d", "e").replace("D", "e"))
+    n = nx * ny * nz
+    flat_mat = mat.reshape(n)
+    get = lambda key: np.array([mats[int(m)][key] for m in flat_mat], np.float64)
+    cells = Cells(
+        kappa=get("kappa"), rho=get("rho"), heat_capacity=get("heat_capacity"), tau=get("tau"),
+        em=get("em"),
+        len_x=np.full(n, Lx / float(nx)), len_y=np.full(n, Ly / float(ny)),
+        len_z=np.full(n, Lz / float(nz)), iheater=heat.reshape(n).copy())
+    return System(params, cells)

@@ -1,0 +1,204 @@
+/*
+ * heatflow_hip.h -- C-ABI of the MI355X (gfx950) HeatFlow time-step library.
+ *
+ * The library replaces the per-timestep hot path of HeatFlow's Fortran program:
+ *   - H-matrix assembly        setup::sparse_Hmatrix  (src/heatflow/mod_setup.f90:89-152)
+ *                              hmatrixmod::hmatrixfunc (src/heatflow/mod_hmatrix.f90:62-248)
+ *   - right-hand side          boundary_vector::boundary (src/heatflow/mod_boundary.f90:24-152),
+ *                              Heating::heater (src/heatflow/mod_heating.f90:27-169),
+ *                              cattaneo::S_catS (src/heatflow/mod_cattaneo.f90:23-42),
+ *                              S build and x0 (src/heatflow/mod_evolve.f90:112-141)
+ *   - linear solve             solver::linbcg (src/heatflow/mod_Sparse.f90:372-434)
+ *   - the time step itself     evolution::simulate (src/heatflow/mod_evolve.f90:45-177)
+ *
+ * Plain C types only.  Every array pointer may be a host pointer or a device pointer
+ * (the library copies with hipMemcpyDefault).  Arrays over cells have N = nx*ny*nz entries
+ * in the reference's linear order: x fastest, then y, then z
+ * (src/heatflow/mod_hmatrix.f90:72-74).  All entry points return 0 on success or an HF_ERR_*
+ * code; hf_last_error() gives a message.  One context = one HIP stream on the current device;
+ * calls on one context must not overlap.
+ */
+#ifndef HEATFLOW_HIP_H
+#define HEATFLOW_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes.  HF_ERR_NAN_* map one-to-one onto the six fatal NaN guards of simulate()
+ * (src/heatflow/mod_evolve.f90:70-73, 82-85, 86-89, 102-105, 124-127, 153-158); a Fortran
+ * caller prints the reference message and stops with status 1. */
+enum {
+    HF_OK = 0,
+    HF_ERR_NAN_B = 1,
+    HF_ERR_NAN_Q = 2,
+    HF_ERR_NAN_QDENS = 3,
+    HF_ERR_NAN_SCAT = 4,
+    HF_ERR_NAN_S = 5,
+    HF_ERR_NAN_X = 6,
+    HF_ERR_ARG = 10,        /* invalid argument or unsupported geometry               */
+    HF_ERR_ALLOC = 11,      /* device allocation failed                               */
+    HF_ERR_HIP = 12,        /* HIP runtime error                                      */
+    HF_ERR_STATE = 13,      /* call order: e.g. hf_step before hf_assemble            */
+    HF_ERR_NO_DEVICE = 14   /* no HIP device                                          */
+};
+
+/* Solver precision modes */
+enum {
+    HF_PREC_FP64 = 0,       /* FP64 throughout (the drop-in)                          */
+    HF_PREC_FP32_IR = 1     /* FP32 inner CG (FP64 dot accumulation) inside FP64
+                               iterative refinement; history, RHS and residual in FP64 */
+};
+
+/* Scalar run parameters: the members of module `inputs` that the path reads
+ * (src/heatflow/mod_inputs.f90:80-107, keys parsed at :283-327), plus solver controls.
+ * The face order of the 6-vectors is x1, xN, y1, yN, z1, zN. */
+typedef struct hf_params {
+    int64_t nx, ny, nz;
+    double time_step;          /* time_step                                             */
+    int32_t icattaneo;         /* icattaneo (0 Fourier, 1 Cattaneo)                     */
+    int32_t isteady;           /* isteady                                               */
+    int32_t periodic[3];       /* Periodicx/y/z                                         */
+    int32_t cg[6];             /* CG_x_m, CG_x_p, CG_y_m, CG_y_p, CG_z_m, CG_z_p        */
+    double kappa_bound[6];     /* kappaBoundx1, kappaBoundNx, ..y1, ..Ny, ..z1, ..Nz    */
+    double t_bath[6];          /* T_Bathx1, T_Bathx2, T_Bathy1, T_Bathy2, T_Bathz1, z2 */
+    double br;                 /* BR                                                    */
+    double t_bath_cg;          /* T_BathCG                                              */
+    double power_in;           /* power_in                                              */
+    double freq;               /* freq                                                  */
+    double t_bath_rad;         /* T_Bath (radiative term, mod_heating.f90:121-123)      */
+    int64_t heated_steps;      /* heattime                                              */
+    /* solver controls (the reference hard-codes itol=1, tol=1e-32, itmax=50000 at
+     * src/heatflow/mod_evolve.f90:142-144) */
+    double tol;                /* relative residual tolerance of the (outer) solve      */
+    int64_t itmax;             /* iteration cap of the (outer) solve                    */
+    int32_t precision;         /* HF_PREC_*                                             */
+    int32_t inner_itmax;       /* FP32 inner CG iteration cap per refinement sweep      */
+    double inner_tol;          /* FP32 inner CG relative tolerance per sweep            */
+} hf_params;
+
+typedef struct hf_ctx hf_ctx;
+
+/* Fills *p with the reference defaults (src/heatflow/mod_inputs.f90:213-270) and the
+ * reference's hard-coded solver controls. */
+void hf_params_default(hf_params *p);
+
+/* Creates a context for one grid; allocates device memory for N cells.
+ * Replaces the allocations of set_global_variables (src/heatflow/mod_setup.f90:38-45).
+ * Geometry the reference itself mishandles is rejected with HF_ERR_ARG: nx == 1 with N > 1
+ * (duplicate COO offsets overflow `len`, mod_setup.f90:96-112) and periodic axes of length 2. */
+int hf_create(hf_ctx **ctx, const hf_params *p);
+
+/* Uploads the per-cell material fields of grid(:,:,:) (src/heatflow/mod_constructions.f90:25-31)
+ * as structure-of-arrays, after set_global_variables has filled them
+ * (src/heatflow/mod_setup.f90:53-68): tau_scaled is grid%tau = tau/dt^2.  len_x/y/z are
+ * grid%Length(1:3) (mod_inputs.f90:619-622), volume is grid%volume.  iheater may be NULL
+ * (no heaters), em may be NULL (zero).  qdens_extra (may be NULL) is a superset input with no
+ * reference counterpart: a per-cell volumetric power density in W/m^3 added to Qdens. */
+int hf_upload_cells(hf_ctx *ctx, const double *kappa, const double *rho,
+                    const double *heat_capacity, const double *tau_scaled, const double *em,
+                    const double *len_x, const double *len_y, const double *len_z,
+                    const double *volume, const int64_t *iheater, const double *qdens_extra);
+
+/* Assembles H on the device as a 7-point stencil (diagonal + 3 forward couplings) and the
+ * boundary vector B.  Replaces setup::sparse_Hmatrix (src/heatflow/mod_setup.f90:89-152).
+ * Values follow hmatrixfunc's operand order (src/heatflow/mod_hmatrix.f90:62-391). */
+int hf_assemble(hf_ctx *ctx);
+
+/* Exports the assembled H in the reference's COO layout, globe_data::ra
+ * (src/heatflow/mod_global.f90:14, type sprs2_dp src/heatflow/mod_SPtype.f90:56-61):
+ * 1-based (val, irow, jcol) in sparse_Hmatrix's emission order, `len` from the reference
+ * formula (mod_setup.f90:96-99), entries past `count` hold the sentinel (0, -2, -1)
+ * (mod_setup.f90:104-106).  With val/irow/jcol NULL only n/len/count are returned.
+ * Arrays must hold `len` entries. */
+int hf_export_coo(hf_ctx *ctx, int64_t *n, int64_t *len, int64_t *count, double *val,
+                  int64_t *irow, int64_t *jcol);
+
+/* Sets / reads globe_data::Temp_p and Temp_pp (src/heatflow/mod_global.f90:10), i.e. T^n and
+ * T^{n-1}.  hf_set_state replaces initial_evolve's assignment
+ * (src/heatflow/mod_init_evolve.f90:34-51). */
+int hf_set_state(hf_ctx *ctx, const double *temp_p, const double *temp_pp);
+int hf_get_state(hf_ctx *ctx, double *temp_p, double *temp_pp);
+
+/* One time step: evolution::simulate(itime) (src/heatflow/mod_evolve.f90:45-177).  Builds S,
+ * the initial guess, solves H x = S to `tol`, rotates the history.  *iter gets the solver's
+ * iteration count (outer FP64 + inner FP32 iterations in mixed mode) and *err its final
+ * relative residual (both may be NULL).  NaN guards return HF_ERR_NAN_*. */
+int hf_step(hf_ctx *ctx, int64_t itime, int64_t *iter, double *err);
+
+/* nsteps consecutive time steps itime0 .. itime0+nsteps-1 without host copies of the state
+ * (the reference driver loop, src/heatflow.f90:67-90).  *iters_total may be NULL. */
+int hf_run(hf_ctx *ctx, int64_t itime0, int64_t nsteps, int64_t *iters_total);
+
+/* linbcg-compatible solve of H x = b on the assembled H (src/heatflow/mod_Sparse.f90:372-434):
+ * x is the initial guess on entry and the solution on exit.  itol = 1 or 2 picks the
+ * reference's error norm label; the device solver stops on the Jacobi-scaled relative
+ * residual.  *iter and *err as in the reference (unused by its caller). */
+int hf_solve(hf_ctx *ctx, const double *b, double *x, int32_t itol, double tol, int64_t itmax,
+             int64_t *iter, double *err);
+
+/* Reads intermediate vectors of the last hf_step for parity checks:
+ * which = 0: B (boundary vector, mod_boundary.f90:24-152)
+ *         1: Qdens of the last step (mod_heating.f90:164-167)
+ *         2: S of the last step (mod_evolve.f90:112-119)
+ *         3: diagonal of H, 4/5/6: forward couplings in x/y/z (0 where absent) */
+int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
+
+/* Debug switches (no reference counterpart): bit 0 keeps S of every FP64 step for
+ * hf_get_vector(2) (costs one extra N-vector write per step). */
+int hf_set_debug(hf_ctx *ctx, int32_t flags);
+
+/* Timing / introspection for the benchmark: achieved counters since the last reset. */
+typedef struct hf_stats {
+    int64_t steps;            /* time steps taken                                      */
+    int64_t outer_iters;      /* FP64 CG iterations (or refinement sweeps in mixed mode) */
+    int64_t inner_iters;      /* FP32 inner CG iterations (mixed mode)                 */
+    double last_err;          /* final relative residual of the last solve             */
+} hf_stats;
+int hf_get_stats(hf_ctx *ctx, hf_stats *st);
+int hf_reset_stats(hf_ctx *ctx);
+
+/* Kernel timing (no reference counterpart; for the benchmark).  While enabled, each kernel
+ * launch of a time step is bracketed by a pair of HIP events on the context's stream.
+ * hf_kernel_times fills, per kernel id (HF_K_*), the summed device time in ms, the number of
+ * launches that did work (CG launches queued past convergence exit at once and are not
+ * counted) and the algorithmic bytes those launches move (compulsory reads + writes of the
+ * arrays each kernel touches, every array element counted once).  Reading resets the pool. */
+enum {
+    HF_K_QDENS = 0, HF_K_RHS64, HF_K_RHS32, HF_K_CG_A64, HF_K_CG_B64, HF_K_CG_A32, HF_K_CG_B32,
+    HF_K_FINAL, HF_K_IR_RES, HF_K_COUNT
+};
+int hf_set_timing(hf_ctx *ctx, int32_t on);
+int hf_kernel_times(hf_ctx *ctx, double *ms, int64_t *launches, double *bytes);
+
+/* Changes the solver controls of an existing context (tol, itmax, precision, inner_*). */
+int hf_set_solver(hf_ctx *ctx, double tol, int64_t itmax, int32_t precision, double inner_tol,
+                  int32_t inner_itmax);
+
+/* ---- adjoint (no reference counterpart: reverse-time gradient of a probe loss) ----------
+ * Forward-with-checkpoints + reverse sweep over the same context.  The loss is
+ * L = sum_k w_k * (T^{n_end}[probe_k] - target_k)^2 at the final step.  Gradients are
+ * returned per cell with respect to kappa (dL/dkappa, length N) and with respect to the
+ * per-cell power P_I that enters as Qdens_I += P_I / volume_I (dL/dP, length N).
+ * Checkpoints of (T^n, T^{n-1}) are kept every `ckpt_every` steps on the device and each
+ * segment is recomputed during the reverse sweep.  Requires per-axis uniform cell lengths
+ * and no CG_dir faces.  *loss gets L. */
+int hf_grad(hf_ctx *ctx, int64_t itime0, int64_t nsteps, int64_t ckpt_every, int64_t nprobe,
+            const int64_t *probe_idx, const double *target, const double *weight,
+            double *loss, double *dl_dkappa, double *dl_dpower);
+
+/* Last error message of this context (or of the last failed hf_create when ctx is NULL). */
+const char *hf_last_error(const hf_ctx *ctx);
+
+/* Releases the context and its device memory. */
+void hf_destroy(hf_ctx *ctx);
+
+/* Library version string. */
+const char *hf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HEATFLOW_HIP_H */

@@ -1,0 +1,266 @@
+"""Oracle for HeatFlow's time-step path -- TEST INFRASTRUCTURE ONLY.
+
+Two CPU implementations, used only by tests/, __graft_entry__.smoke() and the cpu_baseline leg
+of bench.py, and only as the checker (never as the thing measured or shipped):
+
+  C     liboracle.so: hf_oracle.c, a serial restatement of the reference's algorithm
+        (mod_hmatrix, mod_setup, mod_boundary, mod_heating, mod_cattaneo, mod_evolve,
+        mod_Sparse), each function citing the reference lines it follows.
+  Ref   _ref/libhfref.so: the reference's own Fortran modules compiled from the HeatFlow
+        sources by oracle/Makefile (amdflang -O3), driven through ref_harness.f90.  Built only
+        where the HeatFlow checkout exists; the built file travels with the tree.
+
+Parity pins (see tests/test_oracle.py): the C oracle is checked against the Ref build on random
+inputs (COO bit-exact, B/Q/S_cat bit-exact, temperatures), against the reference's golden
+test/data/TempDis.dat (10 Fourier steps, 25x25x1), against the HeatFlow manual's closed-form
+3-cell cases, and against dense numpy solves and conservation invariants.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libhfref.so")
+REF_EXE = os.path.join(HERE, "_ref", "heatflow_ref")
+REF_SRC = "/root/reference"
+
+
+def build(ref: bool = True) -> None:
+    """Builds liboracle.so, and the reference build when the HeatFlow sources are present."""
+    targets = ["oracle"]
+    if ref and os.path.isdir(os.path.join(REF_SRC, "src", "heatflow")):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE, *targets], check=True)
+
+
+class _Sys(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+        ("time_step", C.c_double),
+        ("icattaneo", C.c_int64), ("isteady", C.c_int64),
+        ("periodic", C.c_int32 * 3),
+        ("kappa_bound", C.c_double * 6), ("t_bath", C.c_double * 6),
+        ("br", C.c_double),
+        ("cg", C.c_int32 * 6),
+        ("t_bath_cg", C.c_double), ("power_in", C.c_double), ("freq", C.c_double),
+        ("heated_steps", C.c_int64), ("t_bath_rad", C.c_double),
+        ("kappa", C.c_void_p), ("rho", C.c_void_p), ("heat_capacity", C.c_void_p),
+        ("tau", C.c_void_p), ("em", C.c_void_p),
+        ("len_x", C.c_void_p), ("len_y", C.c_void_p), ("len_z", C.c_void_p),
+        ("volume", C.c_void_p), ("iheater", C.c_void_p), ("qdens_extra", C.c_void_p),
+    ]
+
+
+class _Solver(C.Structure):
+    _fields_ = [("itol", C.c_int64), ("tol", C.c_double), ("itmax", C.c_int64)]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+_olib = None
+
+
+def olib() -> C.CDLL:
+    global _olib
+    if _olib is None:
+        if not os.path.exists(ORACLE_SO):
+            build(ref=False)
+        _olib = C.CDLL(ORACLE_SO)
+        _olib.hfo_hmatrixfunc.restype = C.c_double
+        _olib.hfo_coo_len.restype = C.c_int64
+        _olib.hfo_sparse_hmatrix.restype = C.c_int64
+        _olib.hfo_heater.restype = C.c_double
+    return _olib
+
+
+class OracleSystem:
+    """A System bound to the C oracle (keeps the arrays alive)."""
+
+    def __init__(self, system, qdens_extra: Optional[np.ndarray] = None):
+        p, c = system.params, system.cells
+        self.n = p.n
+        f = lambda a: np.ascontiguousarray(a, np.float64)
+        self.arrs = dict(
+            kappa=f(c.kappa), rho=f(c.rho), heat_capacity=f(c.heat_capacity),
+            tau=f(c.tau_scaled(p.time_step)), em=f(c.em), len_x=f(c.len_x), len_y=f(c.len_y),
+            len_z=f(c.len_z), volume=f(c.volume),
+            iheater=np.ascontiguousarray(c.iheater, np.int64))
+        qe = qdens_extra if qdens_extra is not None else c.qdens_extra
+        self.arrs["qdens_extra"] = None if qe is None else f(qe)
+        s = _Sys()
+        s.nx, s.ny, s.nz = p.nx, p.ny, p.nz
+        s.time_step = p.time_step
+        s.icattaneo, s.isteady = p.icattaneo, p.isteady
+        for a in range(3):
+            s.periodic[a] = int(bool(p.periodic[a]))
+        for k in range(6):
+            s.kappa_bound[k] = p.kappa_bound[k]
+            s.t_bath[k] = p.t_bath[k]
+            s.cg[k] = int(bool(p.cg[k]))
+        s.br, s.t_bath_cg, s.power_in, s.freq = p.br, p.t_bath_cg, p.power_in, p.freq
+        s.heated_steps, s.t_bath_rad = p.heated_steps, p.t_bath_rad
+        for k, v in self.arrs.items():
+            setattr(s, k, _p(v))
+        self.s = s
+        self.params = p
+
+    def hmatrixfunc(self, i: int, j: int) -> float:
+        return olib().hfo_hmatrixfunc(C.byref(self.s), C.c_int64(i), C.c_int64(j))
+
+    def coo(self):
+        L = olib().hfo_coo_len(C.byref(self.s))
+        val = np.empty(L)
+        irow = np.empty(L, np.int64)
+        jcol = np.empty(L, np.int64)
+        cnt = C.c_int64()
+        olib().hfo_sparse_hmatrix(C.byref(self.s), _p(val), _p(irow), _p(jcol), C.byref(cnt))
+        return self.n, L, cnt.value, val, irow, jcol
+
+    def boundary(self) -> np.ndarray:
+        B = np.empty(self.n)
+        olib().hfo_boundary(C.byref(self.s), _p(B))
+        return B
+
+    def heater(self, itime: int, temp_p: np.ndarray):
+        Q = np.empty(self.n)
+        Qd = np.empty(self.n)
+        tp = np.ascontiguousarray(temp_p, np.float64)
+        hv = olib().hfo_heater(C.byref(self.s), C.c_int64(itime), _p(tp), _p(Q), _p(Qd))
+        return Q, Qd, hv
+
+    def scat(self, temp_p, temp_pp) -> np.ndarray:
+        out = np.empty(self.n)
+        a = np.ascontiguousarray(temp_p, np.float64)
+        b = np.ascontiguousarray(temp_pp, np.float64)
+        olib().hfo_scat(C.byref(self.s), _p(a), _p(b), _p(out))
+        return out
+
+    def run(self, ntime: int, temp_p=None, temp_pp=None, t_system: Optional[float] = None,
+            tol: float = 1e-32, itmax: int = 50000, itol: int = 1):
+        """Runs ntime steps (simulate(1..ntime)); returns (rc, T_p, T_pp, total_iters)."""
+        init = temp_p is None
+        tp = np.empty(self.n) if init else np.array(temp_p, np.float64)
+        tpp = np.empty(self.n) if init else np.array(temp_pp if temp_pp is not None else temp_p,
+                                                    np.float64)
+        ts = self.params.t_system if t_system is None else t_system
+        sv = _Solver(itol, tol, itmax)
+        tot = C.c_int64()
+        rc = olib().hfo_run(C.byref(self.s), C.byref(sv), C.c_int64(ntime), C.c_double(ts),
+                            C.c_int(1 if init else 0), _p(tp), _p(tpp), C.byref(tot))
+        return rc, tp, tpp, tot.value
+
+    def simulate_steps(self, itimes, temp_p, temp_pp, tol: float = 1e-32, itmax: int = 50000):
+        """simulate(itime) for each itime in order, from an explicit state; returns
+        (rc, T_p, T_pp, iters per step)."""
+        n, L, cnt, val, irow, jcol = self.coo()
+        tp = np.array(temp_p, np.float64)
+        tpp = np.array(temp_pp, np.float64)
+        sv = _Solver(1, tol, itmax)
+        iters = []
+        for it in itimes:
+            k = C.c_int64()
+            rc = olib().hfo_simulate(C.byref(self.s), C.byref(sv), C.c_int64(L), _p(val),
+                                     _p(irow), _p(jcol), C.c_int64(it), _p(tp), _p(tpp),
+                                     C.byref(k), None)
+            iters.append(k.value)
+            if rc != 0:
+                return rc, tp, tpp, iters
+        return 0, tp, tpp, iters
+
+
+# ---- the reference build (Fortran modules + ref_harness.f90) ------------------------------
+
+_rlib = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def rlib() -> C.CDLL:
+    global _rlib
+    if _rlib is None:
+        if not os.path.exists(REF_SO):
+            raise RuntimeError("reference build missing: run `make -C oracle ref`")
+        _rlib = C.CDLL(REF_SO)
+    return _rlib
+
+
+class RefSystem:
+    """The reference modules set up with a System (one at a time: module globals)."""
+
+    def __init__(self, system):
+        p, c = system.params, system.cells
+        self.n = p.n
+        self.params = p
+        f = lambda a: np.ascontiguousarray(a, np.float64)
+        ip = np.array([p.nx, p.ny, p.nz, p.icattaneo, p.isteady, *[int(bool(v)) for v in p.periodic],
+                       *[int(bool(v)) for v in p.cg], p.heated_steps], np.int64)
+        dp = np.array([p.time_step, *p.kappa_bound, *p.t_bath, p.br, p.t_bath_cg, p.power_in,
+                       p.freq, p.t_bath_rad], np.float64)
+        self.keep = [ip, dp, f(c.kappa), f(c.rho), f(c.heat_capacity), f(c.tau), f(c.em),
+                     f(c.len_x), f(c.len_y), f(c.len_z),
+                     np.ascontiguousarray(c.iheater, np.int64)]
+        rlib().hfr_setup(*[_p(a) for a in self.keep])
+
+    def coo(self):
+        n, L = C.c_int64(), C.c_int64()
+        rlib().hfr_coo_sizes(C.byref(n), C.byref(L))
+        val = np.empty(L.value)
+        irow = np.empty(L.value, np.int64)
+        jcol = np.empty(L.value, np.int64)
+        rlib().hfr_coo(_p(val), _p(irow), _p(jcol))
+        return n.value, L.value, val, irow, jcol
+
+    def set_state(self, tp, tpp):
+        a = np.ascontiguousarray(tp, np.float64)
+        b = np.ascontiguousarray(tpp, np.float64)
+        rlib().hfr_set_state(_p(a), _p(b))
+
+    def get_state(self):
+        a = np.empty(self.n)
+        b = np.empty(self.n)
+        rlib().hfr_get_state(_p(a), _p(b))
+        return a, b
+
+    def simulate(self, itime: int):
+        rlib().hfr_simulate(C.c_int64(itime))
+
+    def simulate_qextra(self, itime: int, qextra, tol: float = 1e-32) -> int:
+        q = np.ascontiguousarray(qextra, np.float64)
+        it = C.c_int64()
+        rlib().hfr_simulate_qextra(C.c_int64(itime), _p(q), C.c_double(tol), C.byref(it))
+        return it.value
+
+    def boundary(self):
+        b = np.empty(self.n)
+        rlib().hfr_boundary(_p(b))
+        return b
+
+    def heater(self, itime: int):
+        q = np.empty(self.n)
+        qd = np.empty(self.n)
+        hv = C.c_double()
+        rlib().hfr_heater(C.c_int64(itime), _p(q), _p(qd), C.byref(hv))
+        return q, qd, hv.value
+
+    def scat(self):
+        s = np.empty(self.n)
+        rlib().hfr_scat(_p(s))
+        return s
+
+    def linbcg(self, b, x, tol=1e-32, itmax=50000, itol=1):
+        bb = np.ascontiguousarray(b, np.float64)
+        xx = np.array(x, np.float64)
+        it, err = C.c_int64(), C.c_double()
+        rlib().hfr_linbcg(_p(bb), _p(xx), C.c_int64(itol), C.c_double(tol), C.c_int64(itmax),
+                          C.byref(it), C.byref(err))
+        return xx, it.value, err.value

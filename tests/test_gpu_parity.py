@@ -1,0 +1,263 @@
+"""GPU parity: the gfx950 path through the C-ABI against the oracle (which is pinned to the
+reference build and the reference's golden vectors in test_oracle.py)."""
+import numpy as np
+import pytest
+
+import oracle
+from heatflow_amd import HeatFlowError, Simulation, workloads
+from heatflow_amd.system import Params, System, read_inputs
+
+from _cases import GEOMETRIES, manual_cases, random_state, random_system, read_tempdis
+
+pytestmark = pytest.mark.gpu
+
+# FP64 path at the reference's tolerance (1e-32 on the recursive residual): temperatures agree
+# with the oracle to round-off; the bar of the port is 1e-10 relative.
+RTOL_FP64 = 1e-12
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.int64)
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+# ---- assembly: bit-exact sparsity, indexing and values -------------------------------------
+
+@pytest.mark.parametrize("gi", range(len(GEOMETRIES)))
+def test_coo_export_bit_exact(gi):
+    s = random_system(300 + gi, **GEOMETRIES[gi])
+    o = oracle.OracleSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    with Simulation(s) as sim:
+        gn, gL, gcnt, gval, girow, gjcol = sim.export_coo()
+    assert (gn, gL, gcnt) == (n, L, cnt)
+    assert np.array_equal(girow, irow) and np.array_equal(gjcol, jcol)
+    assert np.array_equal(bits(gval), bits(val))
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_coo_export_c1_grid(n):
+    s, _ = workloads.c1(n)
+    o = oracle.OracleSystem(s)
+    ref = o.coo()
+    with Simulation(s) as sim:
+        got = sim.export_coo()
+    assert got[:3] == ref[:3]
+    for a, b in zip(got[3:], ref[3:]):
+        assert np.array_equal(bits(a) if a.dtype == np.float64 else a,
+                              bits(b) if b.dtype == np.float64 else b)
+
+
+def test_coo_export_two_phase_tiny_dropped():
+    """Two-phase voxels with near-zero conductivity cells: couplings under TINY are dropped,
+    count < len and the tail keeps the (0, -2, -1) sentinel."""
+    s = random_system(9, 8, 7, 6, tiny_cells=40)
+    o = oracle.OracleSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    assert cnt < L
+    with Simulation(s) as sim:
+        gn, gL, gcnt, gval, girow, gjcol = sim.export_coo()
+    assert gcnt == cnt and gL == L
+    assert np.all(girow[cnt:] == -2) and np.all(gjcol[cnt:] == -1) and np.all(gval[cnt:] == 0)
+    assert np.array_equal(girow, irow) and np.array_equal(bits(gval), bits(val))
+
+
+@pytest.mark.parametrize("gi", range(len(GEOMETRIES)))
+def test_boundary_and_diag_bit_exact(gi):
+    s = random_system(400 + gi, **GEOMETRIES[gi])
+    o = oracle.OracleSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    diag = np.zeros(n)
+    m = (irow == jcol) & (irow > 0)
+    diag[irow[m] - 1] = val[m]
+    with Simulation(s) as sim:
+        assert np.array_equal(bits(sim.vector("B")), bits(o.boundary()))
+        assert np.array_equal(bits(sim.vector("diag")), bits(diag))
+
+
+# ---- right-hand side -------------------------------------------------------------------------
+
+@pytest.mark.parametrize("gi", [0, 1, 3, 4, 9])
+def test_rhs_S_matches_oracle(gi):
+    """S of one step (mod_evolve.f90:112-119): bit-exact for constant heaters; the radiative
+    T^4 term (device pow vs host libm) is held to 1 ulp-scale relative error."""
+    s = random_system(500 + gi, **GEOMETRIES[gi])
+    o = oracle.OracleSystem(s)
+    p, c = s.params, s.cells
+    tp, tpp = random_state(gi, p.n)
+    q, qd, hv = o.heater(2, tp)
+    if s.cells.qdens_extra is not None:
+        qd = qd + s.cells.qdens_extra
+    B = o.boundary()
+    rc_ = c.rho * c.heat_capacity
+    S = -(p.inverse_time * tp * rc_) - qd - B
+    if p.icattaneo == 1:
+        S = S + o.scat(tp, tpp)
+    with Simulation(s) as sim:
+        sim.set_debug(1)
+        sim.set_state(tp, tpp)
+        sim.simulate(2)
+        gS = sim.vector("S")
+        gQ = sim.vector("Qdens")
+    if GEOMETRIES[gi].get("em", 0.0) == 0.0:
+        assert np.array_equal(bits(gS), bits(S))
+        assert np.array_equal(bits(gQ), bits(qd))
+    else:
+        np.testing.assert_allclose(gS, S, rtol=1e-15)
+        np.testing.assert_allclose(gQ, qd, rtol=1e-13, atol=1e-3)
+
+
+# ---- whole steps against the oracle ------------------------------------------------------------
+
+def test_golden_tempdis_on_gpu(golden_dir):
+    s = read_inputs(golden_dir)
+    gold = read_tempdis(f"{golden_dir}/TempDis.dat")
+    _, otp, _, _ = oracle.OracleSystem(s).run(s.params.ntime)
+    with Simulation(s) as sim:
+        sim.initial_evolve()
+        sim.run(1, s.params.ntime)
+        tp, tpp = sim.get_state()
+    assert np.abs(tp - gold).max() <= 5.0e-7 + 1e-12
+    assert relerr(tp, otp) <= RTOL_FP64
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_manual_cases_on_gpu(idx):
+    case, s = manual_cases()[idx]
+    with Simulation(s) as sim:
+        n, L, cnt, val, irow, jcol = sim.export_coo()
+        H = np.zeros((n, n))
+        m = irow > 0
+        np.add.at(H, (irow[m] - 1, jcol[m] - 1), val[m])
+        np.testing.assert_allclose(H, case["H"], rtol=0, atol=1e-14)
+        sim.initial_evolve(200.0)
+        sim.simulate(1)
+        tp, _ = sim.get_state()
+    np.testing.assert_allclose(tp, case["T"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("gi", range(len(GEOMETRIES)))
+def test_steps_match_oracle(gi):
+    s = random_system(600 + gi, **GEOMETRIES[gi])
+    tp0, tpp0 = random_state(gi, s.params.n)
+    rc, otp, otpp, oit = oracle.OracleSystem(s).simulate_steps([1, 2, 3, 4], tp0, tpp0)
+    assert rc == 0
+    with Simulation(s) as sim:
+        sim.set_state(tp0, tpp0)
+        its = [sim.simulate(k)[0] for k in (1, 2, 3, 4)]
+        tp, tpp = sim.get_state()
+    assert relerr(tp, otp) <= RTOL_FP64 and relerr(tpp, otpp) <= RTOL_FP64
+    assert all(i > 0 for i in its)
+
+
+def test_c1_hundred_steps():
+    """16^3 Cattaneo, one centre heater, 100 steps at the reference tolerance."""
+    s, meta = workloads.c1(16)
+    rc, otp, otpp, _ = oracle.OracleSystem(s).run(100)
+    assert rc == 0
+    with Simulation(s) as sim:
+        sim.initial_evolve()
+        sim.run(1, 100)
+        tp, tpp = sim.get_state()
+    assert relerr(tp, otp) <= RTOL_FP64
+    assert tp.max() > 300.0 + 1e-3        # the heater did heat
+
+
+def test_c1_tol_1e10_within_gate():
+    """At the benchmark tolerance 1e-10 (relative residual) the temperatures stay within the
+    1e-10 relative gate of the reference-tolerance oracle."""
+    s, meta = workloads.c1(16)
+    rc, otp, _, _ = oracle.OracleSystem(s).run(100)
+    with Simulation(s, tol=1e-10) as sim:
+        sim.initial_evolve()
+        sim.run(1, 100)
+        tp, _ = sim.get_state()
+    assert relerr(tp, otp) <= 1e-10
+
+
+def test_two_phase_c2_small():
+    s, meta = workloads.c2(24, nheat=6)
+    rc, otp, _, _ = oracle.OracleSystem(s).run(10)
+    assert rc == 0
+    with Simulation(s) as sim:
+        sim.initial_evolve()
+        sim.run(1, 10)
+        tp, _ = sim.get_state()
+    assert relerr(tp, otp) <= RTOL_FP64
+
+
+def test_linbcg_dense():
+    s = random_system(21, 5, 4, 4)
+    o = oracle.OracleSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    H = np.zeros((n, n))
+    m = irow > 0
+    np.add.at(H, (irow[m] - 1, jcol[m] - 1), val[m])
+    b = np.random.default_rng(0).uniform(-1e20, 1e20, n)
+    with Simulation(s) as sim:
+        x, it, err = sim.linbcg(b, np.zeros(n))
+    xd = np.linalg.solve(H, b)
+    np.testing.assert_allclose(x, xd, rtol=1e-12, atol=1e-12 * np.abs(xd).max())
+    assert it > 0
+
+
+# ---- failure handling (the reference's fatal NaN guards) -----------------------------------
+
+def _nan_system(kind):
+    p = Params(nx=4, ny=3, nz=3, time_step=1e-10, icattaneo=1, kappa_bound=(150.0,) * 6,
+               t_bath=(300.0,) * 6, power_in=1e-4)
+    kap = np.full(36, 150.0)
+    tau = np.full(36, 1e-12)
+    ih = np.zeros(36, np.int64)
+    ih[5] = 1
+    if kind == "B":           # kappa = kappa_b = 0 on a boundary face: 0/0 in B
+        p.kappa_bound = (0.0, 150.0, 150.0, 150.0, 150.0, 150.0)
+        kap[0] = 0.0
+    elif kind == "Q":         # NaN heater power
+        p.power_in = float("nan")
+    elif kind == "SCAT":      # NaN relaxation time: S_CAT is NaN first
+        tau[7] = float("nan")
+    return System.uniform(p, (4e-6, 3e-6, 3e-6), kappa=kap, rho=2330.0, heat_capacity=700.0,
+                          tau=tau, iheater=ih)
+
+
+@pytest.mark.parametrize("kind,code", [("B", 1), ("Q", 2), ("SCAT", 4)])
+def test_nan_guards(kind, code):
+    s = _nan_system(kind)
+    with Simulation(s) as sim:
+        sim.initial_evolve(300.0)
+        with pytest.raises(HeatFlowError) as ei:
+            sim.simulate(1)
+    assert ei.value.code == code
+
+
+def test_state_errors():
+    s = random_system(1, 4, 4, 4)
+    with Simulation(s) as sim:
+        with pytest.raises(HeatFlowError) as ei:
+            sim.simulate(1)                       # no state yet
+        assert ei.value.code == 13
+
+
+# ---- mixed precision ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [24, 40])
+def test_fp32_ir_matches_fp64(n):
+    """FP32 inner CG inside FP64 refinement at the 1e-10 gate against the FP64 path run to the
+    reference tolerance; log-normal kappa with Gaussian heating (the 512^3 workload, scaled)."""
+    s, meta = workloads.c3(n)
+    with Simulation(s, tol=1e-32) as ref:
+        ref.initial_evolve()
+        ref.run(1, 20)
+        t64, _ = ref.get_state()
+    with Simulation(s, precision="fp32_ir", tol=1e-12) as sim:
+        sim.initial_evolve()
+        sim.run(1, 20)
+        t32, _ = sim.get_state()
+        st = sim.stats()
+    err = np.abs(t32 - t64) / np.abs(t64)
+    assert err.max() <= 1e-10
+    assert st.inner_iters > 0 and st.outer_iters >= 20

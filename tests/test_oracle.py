@@ -1,0 +1,206 @@
+"""The oracle pinned against the reference: golden vectors, the manual's closed forms, the
+reference build itself, and invariants the mathematics fixes.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle
+from heatflow_amd.system import read_inputs
+
+from _cases import GEOMETRIES, manual_cases, random_state, random_system, read_tempdis
+
+
+def dense_from_coo(n, irow, jcol, val):
+    H = np.zeros((n, n))
+    ok = irow > 0
+    np.add.at(H, (irow[ok] - 1, jcol[ok] - 1), val[ok])
+    return H
+
+
+# ---- golden vector: the reference's own TempDis.dat ----------------------------------------
+
+def test_golden_tempdis(golden_dir):
+    s = read_inputs(golden_dir)
+    assert (s.params.nx, s.params.ny, s.params.nz, s.params.ntime) == (25, 25, 1, 10)
+    o = oracle.OracleSystem(s)
+    rc, tp, tpp, iters = o.run(s.params.ntime)
+    gold = read_tempdis(f"{golden_dir}/TempDis.dat")
+    assert rc == 0 and gold.shape == tp.shape
+    # ES16.8 keeps 9 significant digits: the print floor is 5e-7 K at 280 K
+    assert np.abs(tp - gold).max() <= 5.0e-7 + 1e-12
+    # a wrong step count is far off (9 or 11 steps differ by > 1e-3 K)
+    rc9, tp9, _, _ = o.run(9)
+    assert np.abs(tp9 - gold).max() > 1e-4
+
+
+# ---- the manual's chapter-6 cases (closed forms) ---------------------------------------------
+
+@pytest.mark.parametrize("idx", range(4))
+def test_manual_cases(idx):
+    case, s = manual_cases()[idx]
+    o = oracle.OracleSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    H = dense_from_coo(n, irow, jcol, val)
+    np.testing.assert_allclose(H, np.array(case["H"]), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(o.boundary(), case["B"], rtol=0, atol=1e-12)
+    rc, tp, tpp, _ = o.run(1, t_system=200.0)
+    assert rc == 0
+    np.testing.assert_allclose(tp, case["T"], rtol=1e-12)
+
+
+# ---- the oracle against the reference build ---------------------------------------------------
+
+needs_ref = pytest.mark.skipif(not oracle.ref_available(),
+                               reason="reference build oracle/_ref/libhfref.so not present")
+
+
+@needs_ref
+@pytest.mark.parametrize("gi", range(len(GEOMETRIES)))
+def test_oracle_matches_reference_build(gi):
+    g = dict(GEOMETRIES[gi])
+    s = random_system(100 + gi, **g)
+    o = oracle.OracleSystem(s)
+    r = oracle.RefSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    rn, rL, rv, ri, rj = r.coo()
+    assert L == rL
+    assert np.array_equal(irow, ri) and np.array_equal(jcol, rj)
+    assert np.array_equal(val.view(np.int64), rv.view(np.int64))      # bit-exact values
+    assert np.array_equal(o.boundary(), r.boundary())
+    tp, tpp = random_state(gi, n)
+    r.set_state(tp, tpp)
+    q, qd, hv = o.heater(2, tp)
+    rq, rqd, rhv = r.heater(2)
+    assert np.array_equal(q, rq) and np.array_equal(qd, rqd) and hv == rhv
+    assert np.array_equal(o.scat(tp, tpp), r.scat())
+    if s.cells.qdens_extra is None:
+        rc, otp, otpp, _ = o.simulate_steps([1, 2, 3], tp, tpp)
+        r.set_state(tp, tpp)
+        for it in (1, 2, 3):
+            r.simulate(it)
+        rtp, rtpp = r.get_state()
+        assert rc == 0
+        assert np.array_equal(otp, rtp) and np.array_equal(otpp, rtpp)
+
+
+@needs_ref
+def test_oracle_matches_reference_qextra():
+    s = random_system(7, 5, 4, 3, qextra=True)
+    o = oracle.OracleSystem(s)
+    r = oracle.RefSystem(s)
+    tp, tpp = random_state(7, s.params.n)
+    rc, otp, otpp, _ = o.simulate_steps([1, 2], tp, tpp)
+    r.set_state(tp, tpp)
+    for it in (1, 2):
+        r.simulate_qextra(it, s.cells.qdens_extra)
+    rtp, _ = r.get_state()
+    assert rc == 0 and np.array_equal(otp, rtp)
+
+
+# ---- invariants ----------------------------------------------------------------------------
+
+def _gb(kb, k, L, br):
+    v = (2.0 * kb * k / (kb + k)) / (L * L)
+    return np.where(k != kb, v * br, v)
+
+
+@pytest.mark.parametrize("gi", [0, 2, 7])
+def test_h_symmetric_and_row_sums(gi):
+    """H is symmetric; each row sums to -alpha - (bath conductances of its boundary faces)."""
+    s = random_system(200 + gi, **GEOMETRIES[gi])
+    p, c = s.params, s.cells
+    o = oracle.OracleSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    assert cnt <= L and np.all(irow[cnt:] == -2) and np.all(jcol[cnt:] == -1)
+    H = dense_from_coo(n, irow, jcol, val)
+    assert np.array_equal(H, H.T)
+    tau = c.tau_scaled(p.time_step) if p.icattaneo else 0.0
+    rc = c.rho * c.heat_capacity
+    alpha = tau * rc + rc / p.time_step
+    x = np.arange(n) % p.nx
+    y = (np.arange(n) // p.nx) % p.ny
+    z = np.arange(n) // (p.nx * p.ny)
+    gsum = np.zeros(n)
+    for ax, (pos, m, Lv) in enumerate(((x, p.nx, c.len_x), (y, p.ny, c.len_y), (z, p.nz, c.len_z))):
+        for side in (0, 1):
+            at = pos == (0 if side == 0 else m - 1)
+            gsum += np.where(at, _gb(p.kappa_bound[2 * ax + side], c.kappa, Lv, p.br), 0.0)
+    # dropped (|H| < TINY) couplings still enter the diagonal: add them back
+    dropped = np.zeros(n)
+    for i in range(n):
+        for j in range(n):
+            if i != j and H[i, j] == 0.0:
+                hij = o.hmatrixfunc(i + 1, j + 1)
+                if hij != 0.0:
+                    dropped[i] += hij
+    rows = H.sum(axis=1) + dropped
+    np.testing.assert_allclose(rows, -alpha - gsum, rtol=1e-12, atol=1e-9 * np.abs(alpha).max())
+
+
+def test_coo_against_brute_force_hmatrixfunc():
+    """Every non-zero hmatrixfunc(i, j), i >= j, over all pairs of a 3x3x3 periodic-x grid is in
+    the COO with the same bits and nothing else is; the upper triangle is its mirror
+    (mod_setup.f90:138-145), equal to hmatrixfunc(j, i) up to the rounding of the reversed
+    harmonic-mean operand order."""
+    s = random_system(5, 3, 3, 3, periodic=(True, False, False))
+    o = oracle.OracleSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    H = dense_from_coo(n, irow, jcol, val)
+    assert np.array_equal(H, H.T)
+    for i in range(1, n + 1):
+        for j in range(1, i + 1):
+            h = o.hmatrixfunc(i, j)
+            if abs(h) >= 1e-12:
+                assert H[i - 1, j - 1] == h
+            else:
+                assert H[i - 1, j - 1] == 0.0
+            hu = o.hmatrixfunc(j, i)
+            assert abs(hu - H[j - 1, i - 1]) <= 4 * np.spacing(abs(hu) + 1e-300)
+
+
+def test_uniform_fixed_point():
+    """Uniform material at the bath temperature, no heating: T stays at the bath value."""
+    from heatflow_amd.system import Params, System
+    p = Params(nx=6, ny=5, nz=4, time_step=1e-10, icattaneo=1, kappa_bound=(150.0,) * 6,
+               t_bath=(300.0,) * 6, t_system=300.0)
+    s = System.uniform(p, (6e-6, 5e-6, 4e-6), kappa=150.0, rho=2330.0, heat_capacity=700.0,
+                       tau=1e-12)
+    rc, tp, tpp, _ = oracle.OracleSystem(s).run(5)
+    assert rc == 0
+    np.testing.assert_allclose(tp, 300.0, rtol=1e-13)
+
+
+@pytest.mark.parametrize("icat", [0, 1])
+def test_energy_balance_insulated(icat):
+    """All faces insulating: summing H T^{n+1} = S^n over cells, the couplings cancel and
+    sum[a (T^{n+1}-T^n) + c (T^{n+1} - 2T^n + T^{n-1})] = sum Qdens exactly (to solver
+    tolerance), a = rho c_p / dt, c = tau rho c_p / dt^2."""
+    s = random_system(11, 5, 4, 3, icattaneo=icat, nheat=2)
+    p = s.params
+    p.kappa_bound = (0.0,) * 6
+    o = oracle.OracleSystem(s)
+    tp0, tpp0 = random_state(3, p.n)
+    rc, tp1, _, _ = o.simulate_steps([1], tp0, tpp0)
+    assert rc == 0
+    c = s.cells
+    rc_ = c.rho * c.heat_capacity
+    a = rc_ / p.time_step
+    cc = c.tau_scaled(p.time_step) * rc_ if icat else 0.0 * rc_
+    q, qd, hv = o.heater(1, tp0)
+    lhs = np.sum(a * (tp1 - tp0) + cc * (tp1 - 2 * tp0 + tpp0))
+    assert abs(lhs - qd.sum()) <= 1e-9 * max(abs(qd.sum()), np.abs(a * tp0).sum() * 1e-6)
+
+
+def test_linbcg_matches_dense_solve():
+    s = random_system(21, 5, 4, 4)
+    o = oracle.OracleSystem(s)
+    n, L, cnt, val, irow, jcol = o.coo()
+    H = dense_from_coo(n, irow, jcol, val)
+    b = np.random.default_rng(0).uniform(-1e20, 1e20, n)
+    x = np.zeros(n)
+    import ctypes as C
+    it, err = C.c_int64(), C.c_double()
+    oracle.olib().hfo_linbcg(C.c_int64(n), C.c_int64(L), oracle._p(val), oracle._p(irow),
+                             oracle._p(jcol), oracle._p(b), oracle._p(x), C.c_int64(1),
+                             C.c_double(1e-32), C.c_int64(50000), C.byref(it), C.byref(err))
+    xd = np.linalg.solve(H, b)
+    np.testing.assert_allclose(x, xd, rtol=1e-12, atol=1e-12 * np.abs(xd).max())

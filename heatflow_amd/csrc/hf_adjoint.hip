@@ -37,6 +37,10 @@ struct AdjointState {
     int64_t *probe_idx = nullptr;
     double *probe_w = nullptr, *probe_t = nullptr;
     int64_t nprobe_cap = 0;
+    double *fin = nullptr;    // end state of the forward run (T^N, T^{N-1})
+    double *seed = nullptr;   // hf_grad's dL/dT^N
+    int64_t itime0 = 1, nsteps = 0, K = 1;
+    bool valid = false;
 };
 
 void adjoint_free(AdjointState *a) {
@@ -45,7 +49,12 @@ void adjoint_free(AdjointState *a) {
     hipFree(a->g_hi); hipFree(a->g_lo); hipFree(a->g_new); hipFree(a->lam);
     hipFree(a->gk); hipFree(a->gp);
     hipFree(a->probe_val); hipFree(a->probe_idx); hipFree(a->probe_w); hipFree(a->probe_t);
+    hipFree(a->fin); hipFree(a->seed);
     delete a;
+}
+
+void adjoint_invalidate(AdjointState *a) {
+    if (a) a->valid = false;
 }
 
 struct AdjParams {
@@ -172,34 +181,30 @@ static int restore_state(hf_ctx *c, const double *tp, const double *tpp) {
     return HF_OK;
 }
 
-int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, int64_t nprobe,
-                 const int64_t *probe_idx, const double *target, const double *weight,
-                 double *loss, double *dl_dkappa, double *dl_dpower) {
+static int validate(hf_ctx *c) {
     if (!c->assembled || !c->has_state)
-        return ctx_set_err(c, HF_ERR_STATE, "hf_grad: needs hf_assemble and hf_set_state first");
+        return ctx_set_err(c, HF_ERR_STATE, "adjoint: needs hf_assemble and hf_set_state first");
     if (c->p.precision != HF_PREC_FP64)
-        return ctx_set_err(c, HF_ERR_STATE, "hf_grad: needs an FP64 context");
-    if (nsteps < 1 || ckpt_every < 1 || nprobe < 1 || !probe_idx || !target)
-        return ctx_set_err(c, HF_ERR_ARG, "hf_grad: bad arguments");
-    const hf_params &p = c->p;
-    for (int f = 0; f < 6; ++f)
-        if (p.cg[f] && p.icattaneo < 0) return ctx_set_err(c, HF_ERR_ARG, "hf_grad: bad CG");
+        return ctx_set_err(c, HF_ERR_STATE, "adjoint: needs an FP64 context");
     const size_t N = c->g.N, B8 = N * 8;
     // per-axis uniform lengths (the reference's system.in gives Lx/nx etc., mod_inputs.f90:619-622)
-    {
-        std::vector<double> h(N);
-        const double *arrs[3] = {c->lx, c->ly, c->lz};
-        for (int a = 0; a < 3; ++a) {
-            AHIP(c, hipMemcpy(h.data(), arrs[a], B8, hipMemcpyDeviceToHost));
-            for (size_t i = 1; i < N; ++i)
-                if (h[i] != h[0])
-                    return ctx_set_err(c, HF_ERR_ARG, "hf_grad: cell lengths must be uniform per axis");
-        }
+    std::vector<double> h(N);
+    const double *arrs[3] = {c->lx, c->ly, c->lz};
+    for (int a = 0; a < 3; ++a) {
+        AHIP(c, hipMemcpy(h.data(), arrs[a], B8, hipMemcpyDeviceToHost));
+        for (size_t i = 1; i < N; ++i)
+            if (h[i] != h[0])
+                return ctx_set_err(c, HF_ERR_ARG, "adjoint: cell lengths must be uniform per axis");
     }
-    for (int64_t k = 0; k < nprobe; ++k)
-        if (probe_idx[k] < 0 || probe_idx[k] >= (int64_t)N)
-            return ctx_set_err(c, HF_ERR_ARG, "hf_grad: probe index out of range");
+    return HF_OK;
+}
 
+// Forward run of nsteps from the current state, keeping (T^n, T^{n-1}) every ckpt_every steps.
+int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every) {
+    ATRY(validate(c));
+    if (nsteps < 1 || ckpt_every < 1)
+        return ctx_set_err(c, HF_ERR_ARG, "hf_forward_checkpointed: bad step counts");
+    const size_t N = c->g.N, B8 = N * 8;
     if (!c->adj) c->adj = new AdjointState();
     AdjointState &A = *c->adj;
     const int64_t K = ckpt_every;
@@ -207,59 +212,46 @@ int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, 
     if (A.N != N || A.nseg_cap < nseg || A.k_cap < K) {
         hipFree(A.ckpt); hipFree(A.seg);
         A.ckpt = A.seg = nullptr;
-        A.nseg_cap = nseg;
-        A.k_cap = K;
+        A.nseg_cap = std::max(nseg, A.N == N ? A.nseg_cap : 0);
+        A.k_cap = std::max(K, A.N == N ? A.k_cap : 0);
     }
     if (A.N != N) {
         hipFree(A.g_hi); hipFree(A.g_lo); hipFree(A.g_new); hipFree(A.lam);
-        hipFree(A.gk); hipFree(A.gp);
-        A.g_hi = A.g_lo = A.g_new = A.lam = A.gk = A.gp = nullptr;
+        hipFree(A.gk); hipFree(A.gp); hipFree(A.fin);
+        A.g_hi = A.g_lo = A.g_new = A.lam = A.gk = A.gp = A.fin = nullptr;
         A.N = N;
     }
-    ATRY(ensure(c, &A.ckpt, (size_t)2 * N * nseg));
-    ATRY(ensure(c, &A.seg, (size_t)N * K));
-    ATRY(ensure(c, &A.g_hi, N)); ATRY(ensure(c, &A.g_lo, N)); ATRY(ensure(c, &A.g_new, N));
-    ATRY(ensure(c, &A.lam, N)); ATRY(ensure(c, &A.gk, N)); ATRY(ensure(c, &A.gp, N));
-    if (A.nprobe_cap < nprobe) {
-        hipFree(A.probe_val); hipFree(A.probe_idx); hipFree(A.probe_w); hipFree(A.probe_t);
-        A.probe_val = A.probe_w = A.probe_t = nullptr;
-        A.probe_idx = nullptr;
-        ATRY(ensure(c, &A.probe_val, 1));
-        ATRY(ensure(c, &A.probe_w, nprobe));
-        ATRY(ensure(c, &A.probe_t, nprobe));
-        if (hipMalloc(&A.probe_idx, nprobe * sizeof(int64_t)) != hipSuccess)
-            return ctx_set_err(c, HF_ERR_ALLOC, "hf_grad: probe allocation failed");
-        A.nprobe_cap = nprobe;
-    }
-    std::vector<double> w(nprobe, 1.0);
-    if (weight) std::copy(weight, weight + nprobe, w.begin());
-    AHIP(c, hipMemcpyAsync(A.probe_idx, probe_idx, nprobe * 8, hipMemcpyDefault, c->st));
-    AHIP(c, hipMemcpyAsync(A.probe_t, target, nprobe * 8, hipMemcpyDefault, c->st));
-    AHIP(c, hipMemcpyAsync(A.probe_w, w.data(), nprobe * 8, hipMemcpyHostToDevice, c->st));
-
-    // ---- forward with checkpoints -------------------------------------------------------------
+    ATRY(ensure(c, &A.ckpt, (size_t)2 * N * A.nseg_cap));
+    ATRY(ensure(c, &A.seg, (size_t)N * A.k_cap));
+    ATRY(ensure(c, &A.fin, (size_t)2 * N));
     for (int64_t s = 0; s < nseg; ++s) {
         AHIP(c, hipMemcpyAsync(A.ckpt + (size_t)2 * N * s, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
         AHIP(c, hipMemcpyAsync(A.ckpt + (size_t)2 * N * s + N, c->T[1], B8, hipMemcpyDeviceToDevice, c->st));
         const int64_t e = std::min(nsteps, (s + 1) * K);
         for (int64_t n = s * K; n < e; ++n) ATRY(ctx_step(c, itime0 + n, n + 1 == nsteps));
     }
-    // final state, kept to restore the context afterwards (T^{N} in g_new, T^{N-1} in lam)
-    AHIP(c, hipMemcpyAsync(A.g_new, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
-    AHIP(c, hipMemcpyAsync(A.lam, c->T[1], B8, hipMemcpyDeviceToDevice, c->st));
-    // seed
-    AHIP(c, hipMemsetAsync(A.g_hi, 0, B8, c->st));
-    hipLaunchKernelGGL(k_adj_seed, dim3(1), dim3(64), 0, c->st, nprobe, A.probe_idx, A.probe_w,
-                       A.probe_t, c->T[0], A.g_hi, A.probe_val);
-    AHIP(c, hipGetLastError());
-    double L = 0.0;
-    AHIP(c, hipMemcpyAsync(&L, A.probe_val, 8, hipMemcpyDeviceToHost, c->st));
+    AHIP(c, hipMemcpyAsync(A.fin, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
+    AHIP(c, hipMemcpyAsync(A.fin + N, c->T[1], B8, hipMemcpyDeviceToDevice, c->st));
     AHIP(c, hipStreamSynchronize(c->st));
-    // stash the final state in the checkpoint area's spare? keep it on the host side instead
-    std::vector<double> fin_tp(N), fin_tpp(N);
-    AHIP(c, hipMemcpy(fin_tp.data(), A.g_new, B8, hipMemcpyDeviceToHost));
-    AHIP(c, hipMemcpy(fin_tpp.data(), A.lam, B8, hipMemcpyDeviceToHost));
+    A.itime0 = itime0;
+    A.nsteps = nsteps;
+    A.K = K;
+    A.valid = true;
+    return HF_OK;
+}
 
+// Reverse sweep of the last adjoint_forward, seeded with gbar^{N} = dL/dT^{N} (length N, host
+// or device).  Leaves the context at the end state of the forward run.
+int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *dl_dpower) {
+    if (!c->adj || !c->adj->valid)
+        return ctx_set_err(c, HF_ERR_STATE, "hf_adjoint: no checkpointed forward run");
+    if (!seed) return ctx_set_err(c, HF_ERR_ARG, "hf_adjoint: null seed");
+    AdjointState &A = *c->adj;
+    const hf_params &p = c->p;
+    const size_t N = c->g.N, B8 = N * 8;
+    ATRY(ensure(c, &A.g_hi, N)); ATRY(ensure(c, &A.g_lo, N)); ATRY(ensure(c, &A.g_new, N));
+    ATRY(ensure(c, &A.lam, N)); ATRY(ensure(c, &A.gk, N)); ATRY(ensure(c, &A.gp, N));
+    AHIP(c, hipMemcpyAsync(A.g_hi, seed, B8, hipMemcpyDefault, c->st));
     AHIP(c, hipMemsetAsync(A.g_lo, 0, B8, c->st));
     AHIP(c, hipMemsetAsync(A.lam, 0, B8, c->st));
     AHIP(c, hipMemsetAsync(A.gk, 0, B8, c->st));
@@ -271,14 +263,14 @@ int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, 
         q.kb[f] = p.kappa_bound[f];
         q.tb[f] = p.t_bath[f];
         q.cg[f] = p.cg[f] != 0;
-        const int ax = f >> 1;
-        q.bface[f] = !p.periodic[ax];
+        q.bface[f] = !p.periodic[f >> 1];
     }
     q.br = p.br;
     q.isteady = p.isteady;
     q.icattaneo = p.icattaneo;
 
-    // ---- reverse sweep ------------------------------------------------------------------------
+    const int64_t K = A.K, nsteps = A.nsteps, itime0 = A.itime0;
+    const int64_t nseg = (nsteps + K - 1) / K;
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t b = s * K, e = std::min(nsteps, (s + 1) * K);
         ATRY(restore_state(c, A.ckpt + (size_t)2 * N * s, A.ckpt + (size_t)2 * N * s + N));
@@ -305,12 +297,47 @@ int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, 
     }
     if (dl_dkappa) AHIP(c, hipMemcpyAsync(dl_dkappa, A.gk, B8, hipMemcpyDefault, c->st));
     if (dl_dpower) AHIP(c, hipMemcpyAsync(dl_dpower, A.gp, B8, hipMemcpyDefault, c->st));
+    ATRY(restore_state(c, A.fin, A.fin + N));
     AHIP(c, hipStreamSynchronize(c->st));
-    // leave the context at the end of the forward run
-    AHIP(c, hipMemcpy(A.g_new, fin_tp.data(), B8, hipMemcpyHostToDevice));
-    AHIP(c, hipMemcpy(A.g_lo, fin_tpp.data(), B8, hipMemcpyHostToDevice));
-    ATRY(restore_state(c, A.g_new, A.g_lo));
+    return HF_OK;
+}
+
+int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, int64_t nprobe,
+                 const int64_t *probe_idx, const double *target, const double *weight,
+                 double *loss, double *dl_dkappa, double *dl_dpower) {
+    if (nprobe < 1 || !probe_idx || !target)
+        return ctx_set_err(c, HF_ERR_ARG, "hf_grad: bad probe arguments");
+    const size_t N = c->g.N, B8 = N * 8;
+    for (int64_t k = 0; k < nprobe; ++k)
+        if (probe_idx[k] < 0 || probe_idx[k] >= (int64_t)N)
+            return ctx_set_err(c, HF_ERR_ARG, "hf_grad: probe index out of range");
+    ATRY(adjoint_forward(c, itime0, nsteps, ckpt_every));
+    AdjointState &A = *c->adj;
+    if (A.nprobe_cap < nprobe) {
+        hipFree(A.probe_val); hipFree(A.probe_idx); hipFree(A.probe_w); hipFree(A.probe_t);
+        A.probe_val = A.probe_w = A.probe_t = nullptr;
+        A.probe_idx = nullptr;
+        ATRY(ensure(c, &A.probe_val, 1));
+        ATRY(ensure(c, &A.probe_w, nprobe));
+        ATRY(ensure(c, &A.probe_t, nprobe));
+        if (hipMalloc(&A.probe_idx, nprobe * sizeof(int64_t)) != hipSuccess)
+            return ctx_set_err(c, HF_ERR_ALLOC, "hf_grad: probe allocation failed");
+        A.nprobe_cap = nprobe;
+    }
+    ATRY(ensure(c, &A.seed, N));
+    std::vector<double> w(nprobe, 1.0);
+    if (weight) std::copy(weight, weight + nprobe, w.begin());
+    AHIP(c, hipMemcpyAsync(A.probe_idx, probe_idx, nprobe * 8, hipMemcpyDefault, c->st));
+    AHIP(c, hipMemcpyAsync(A.probe_t, target, nprobe * 8, hipMemcpyDefault, c->st));
+    AHIP(c, hipMemcpyAsync(A.probe_w, w.data(), nprobe * 8, hipMemcpyHostToDevice, c->st));
+    AHIP(c, hipMemsetAsync(A.seed, 0, B8, c->st));
+    hipLaunchKernelGGL(k_adj_seed, dim3(1), dim3(64), 0, c->st, nprobe, A.probe_idx, A.probe_w,
+                       A.probe_t, c->T[0], A.seed, A.probe_val);
+    AHIP(c, hipGetLastError());
+    double L = 0.0;
+    AHIP(c, hipMemcpyAsync(&L, A.probe_val, 8, hipMemcpyDeviceToHost, c->st));
     AHIP(c, hipStreamSynchronize(c->st));
+    ATRY(adjoint_backward(c, A.seed, dl_dkappa, dl_dpower));
     if (loss) *loss = L;
     return HF_OK;
 }

@@ -515,10 +515,12 @@ static int run_cg(hf_ctx *c, Scal *sc, Scal *h, T *r, T **p, T *q, T *y, const T
             long sa = tbeg(c, ida, N * w * (first ? 6 : 7));
             launch_cg_a<T>(c->g, r, p[cur], p[cur ^ 1], q, hx, hy, hz, sc, c->partials, c->st);
             tend(c, sa);
+            launch_fin_a(c->g, c->partials, sc, c->st);
             // k_cg_b: read y (not on the first iteration), p, r, q; write y, r
             long sb = tbeg(c, idb, N * w * (first ? 5 : 6));
             launch_cg_b<T>(c->g, y, p[cur ^ 1], r, q, sc, c->partials, c->st);
             tend(c, sb);
+            launch_fin_b(c->g, c->partials, sc, c->st);
             if (sa >= 0) slots.push_back({sa, queued});
             if (sb >= 0) slots.push_back({sb, queued});
             cur ^= 1;
@@ -553,6 +555,7 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
                            c->cx, c->cy, c->cz, c->r64, (c->debug & 1) ? c->S : nullptr, c->sc,
                            c->partials, c->st);
     tend(c, sr);
+    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
     TRY(check_launch(c, "k_rhs_res"));
     int64_t it = 0;
     TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
@@ -593,6 +596,7 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     launch_rhs_res_state32(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec, c->d,
                            c->cx, c->cy, c->cz, c->r32, c->S, c->T[2], c->sc, c->partials, c->st);
     tend(c, sr);
+    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
     TRY(check_launch(c, "k_rhs_res"));
     double *xa = c->T[2], *xb = c->T[1];  // T[1] (Temp_pp) is free once x0 and S exist
     // T[1] is still needed: the refined x of the last sweep must not overwrite Temp_pp before
@@ -609,10 +613,12 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
             launch_cg_a<float>(c->g, c->r32, c->p32[cur], c->p32[cur ^ 1], c->q32, c->fx, c->fy,
                                c->fz, c->sci, c->partials, c->st);
             tend(c, sa);
+            launch_fin_a(c->g, c->partials, c->sci, c->st);
             long sb = tbeg(c, HF_K_CG_B32, N * 4 * (b == 0 ? 5 : 6));
             launch_cg_b<float>(c->g, c->y32, c->p32[cur ^ 1], c->r32, c->q32, c->sci, c->partials,
                                c->st);
             tend(c, sb);
+            launch_fin_b(c->g, c->partials, c->sci, c->st);
             if (sa >= 0) slots.push_back({sa, b});
             if (sb >= 0) slots.push_back({sb, b});
             cur ^= 1;
@@ -623,6 +629,7 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         launch_ir_res(c->g, ip, xa, c->y32, c->sv, c->S, c->d, c->cx, c->cy, c->cz, c->T[0], xb,
                       c->r32, c->sc, c->partials, c->st);
         tend(c, si);
+        launch_fin_ir(c->g, c->partials, c->sc, ip.itmax, c->st);
         TRY(check_launch(c, "k_ir_res"));
         std::swap(xa, xb);
         HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
@@ -708,6 +715,7 @@ extern "C" int hf_solve(hf_ctx *c, const double *b, double *x, int32_t itol, dou
     RhsParams rp{0.0, tol, itmax, 0, 0};
     launch_rhs_res_solve64(c->g, rp, xd, bd, c->d, c->cx, c->cy, c->cz, c->r64, c->sc,
                            c->partials, c->st);
+    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
     TRY(check_launch(c, "k_rhs_res(solve)"));
     int64_t it = 0, pred = std::max<int64_t>(4, c->pred_iters);
     TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
@@ -827,6 +835,7 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
     RhsParams rp{0.0, tol, itmax, 0, 0};
     launch_rhs_res_solve64(c->g, rp, x, b, c->d, c->cx, c->cy, c->cz, c->r64, c->sc, c->partials,
                            c->st);
+    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
     TRY(check_launch(c, "k_rhs_res(solve)"));
     int64_t it = 0;
     TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
@@ -837,6 +846,33 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
     return HF_OK;
 }
 }  // namespace hf
+
+extern "C" int hf_forward_checkpointed(hf_ctx *c, int64_t itime0, int64_t nsteps,
+                                       int64_t ckpt_every) {
+    if (!c) return HF_ERR_ARG;
+    return hf::adjoint_forward(c, itime0, nsteps, ckpt_every);
+}
+
+extern "C" int hf_adjoint(hf_ctx *c, const double *dl_dt_final, double *dl_dkappa,
+                          double *dl_dpower) {
+    if (!c) return HF_ERR_ARG;
+    return hf::adjoint_backward(c, dl_dt_final, dl_dkappa, dl_dpower);
+}
+
+extern "C" int hf_set_fields(hf_ctx *c, const double *kappa, const double *qdens_extra) {
+    if (!c) return HF_ERR_ARG;
+    if (!c->uploaded) return set_err(c, HF_ERR_STATE, "hf_set_fields: cells not uploaded");
+    const size_t B8 = (size_t)c->g.N * 8;
+    if (kappa) HIPCHK(c, hipMemcpyAsync(c->kappa, kappa, B8, hipMemcpyDefault, c->st));
+    if (qdens_extra) {
+        TRY(dalloc(c, &c->qextra, c->g.N));
+        HIPCHK(c, hipMemcpyAsync(c->qextra, qdens_extra, B8, hipMemcpyDefault, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->assembled = false;
+    hf::adjoint_invalidate(c->adj);
+    return HF_OK;
+}
 
 extern "C" int hf_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every,
                        int64_t nprobe, const int64_t *probe_idx, const double *target,

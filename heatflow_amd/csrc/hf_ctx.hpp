@@ -13,6 +13,7 @@
 namespace hf {
 struct AdjointState;
 void adjoint_free(AdjointState *a);
+void adjoint_invalidate(AdjointState *a);
 }  // namespace hf
 
 
@@ -82,6 +83,8 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
 int ctx_set_err(hf_ctx *c, int code, const std::string &msg);
 int ctx_alloc(hf_ctx *c, double **ptr, size_t n);
 // reverse-time gradient (hf_adjoint.hip)
+int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every);
+int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *dl_dpower);
 int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, int64_t nprobe,
                  const int64_t *probe_idx, const double *target, const double *weight,
                  double *loss, double *dl_dkappa, double *dl_dpower);

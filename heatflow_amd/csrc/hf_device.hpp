@@ -6,7 +6,7 @@
 // is mapped to the logical chunk (h % 8) * (nblk / 8) + h / 8, so each of the 8 XCDs of an
 // MI355X sweeps one contiguous 1/8 of the domain and z-neighbour planes stay in that XCD's L2.
 // All reductions are deterministic: wave-level butterfly, then a fixed-order LDS sum, one
-// partial per block, and the last block to finish sums the partials in block order.
+// partial per block, and a one-block kernel sums the partials in block order.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -42,7 +42,7 @@ struct Dims {
     FastDiv dnx, dny;
 };
 
-// Device-resident solver scalars.  Written only by the last block of a reduction.
+// Device-resident solver scalars.  Written only by the one-block finishing kernels.
 struct Scal {
     double rr;        // current r.r of the (scaled) CG residual
     double rr_start;  // r.r at the start of this solve
@@ -56,8 +56,7 @@ struct Scal {
     int32_t first;    // next CG iteration is the first one (p = r, y = 0)
     int32_t flags;    // HF_FLAG_* bits (sticky until the host clears them)
     int32_t tiny;     // x0 += TINY for the next step (mod_evolve.f90:141)
-    uint32_t counter; // last-block counter of the running reduction
-    int32_t pad;
+    int32_t pad[2];
 };
 
 enum : int32_t {
@@ -141,40 +140,35 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*kWave
     }
 }
 
-// Grid-wide deterministic sum of K doubles.  Every block calls it exactly once (all threads).
-// The last block to arrive sums the per-block partials in block order and calls fin(sums)
-// from thread 0; it also resets the counter.  Returns nothing to other blocks.
-template <int K, class Fin>
-__device__ __forceinline__ void grid_sum(double (&v)[K], double *partials, uint32_t *counter,
-                                         Fin fin) {
+// Per-block partial sums: every block writes its K block sums to partials[k * gridDim.x + b]
+// with plain stores.  A one-block finishing kernel launched next on the same stream (kernel
+// boundaries make the stores visible across the XCDs' L2s) sums them in block order and updates
+// the solver scalars -- no device-scope fences or contended atomics inside the wide kernel.
+template <int K>
+__device__ __forceinline__ void block_partials(double (&v)[K], double *partials) {
     __shared__ double lds[K * kWaves];
-    __shared__ int is_last;
     block_sum<K>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
-        __threadfence();
-        uint32_t t = atomicAdd(counter, 1u);
-        is_last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+// Fixed-order sum of nb partials per component, for a one-block kernel of kBlock threads.
+template <int K>
+__device__ __forceinline__ void sum_partials(const double *partials, uint32_t nb,
+                                             double (&out)[K]) {
+    __shared__ double lds[K * kWaves];
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    // fixed assignment of partials to threads, fixed-order tree: deterministic
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            acc[k] += __builtin_nontemporal_load(&partials[(size_t)k * gridDim.x + i]);
+        for (int k = 0; k < K; ++k) acc[k] += partials[(size_t)k * nb + i];
     }
     block_sum<K>(acc, lds);
-    if (threadIdx.x == 0) {
-        fin(acc);
-        *counter = 0u;
-        __threadfence();
-    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = acc[k];
 }
 
 // OR-reduce an int flag word of the block into *dst with one atomic per block (order-free).

@@ -380,24 +380,29 @@ k_rhs_res(Dims g, RhsParams rp, const double *__restrict__ tp, const double *__r
         acc[1] += rq * rq;
     }
     block_flags(fl, &sc->flags);
-    const double tol2 = rp.tol * rp.tol;
-    const int64_t itmax = rp.itmax;
-    grid_sum<2>(acc, partials, &sc->counter, [=](double *v) {
-        sc->bnorm2 = v[0];
-        sc->rr = v[1];
-        sc->rr_start = v[1];
-        sc->outer_rr = v[1];
-        sc->thresh2 = tol2 * v[0];
-        sc->iter = 0;
-        sc->itmax = itmax;
-        sc->first = 1;
-        sc->pq = 0.0;
-        sc->alpha = 0.0;
-        sc->beta = 0.0;
-        // An exactly zero residual needs no iteration (the reference would divide 0/0).
-        sc->done = (v[1] == 0.0) || isnan(v[1]) || itmax <= 0;
-        if (isnan(v[1])) atomicOr(&sc->flags, (int32_t)FLAG_BREAKDOWN);
-    });
+    block_partials<2>(acc, partials);
+}
+
+// Start of a solve: ||b^||^2, r.r, thresholds (after k_rhs_res).
+__global__ void __launch_bounds__(kBlock)
+k_fin_start(uint32_t nb, const double *partials, Scal *sc, double tol, int64_t itmax) {
+    double v[2];
+    sum_partials<2>(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    sc->bnorm2 = v[0];
+    sc->rr = v[1];
+    sc->rr_start = v[1];
+    sc->outer_rr = v[1];
+    sc->thresh2 = tol * tol * v[0];
+    sc->iter = 0;
+    sc->itmax = itmax;
+    sc->first = 1;
+    sc->pq = 0.0;
+    sc->alpha = 0.0;
+    sc->beta = 0.0;
+    // An exactly zero residual needs no iteration (the reference would divide 0/0).
+    sc->done = (v[1] == 0.0) || isnan(v[1]) || itmax <= 0;
+    if (isnan(v[1])) sc->flags |= FLAG_BREAKDOWN;
 }
 
 // ---- CG on A^ = I - C^ (unit diagonal; the Jacobi-preconditioned CG of linbcg in scaled form)
@@ -430,15 +435,23 @@ k_cg_a(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restric
         q[I] = qi;
         acc[0] += (double)pi * (double)qi;
     }
-    grid_sum<1>(acc, partials, &sc->counter, [=](double *v) {
-        sc->pq = v[0];
-        sc->alpha = sc->rr / v[0];
-        if (!(v[0] > 0.0)) {  // A^ is SPD: p.q <= 0 or NaN is a breakdown
-            sc->done = 1;
-            sc->alpha = 0.0;
-            atomicOr(&sc->flags, (int32_t)FLAG_BREAKDOWN);
-        }
-    });
+    block_partials<1>(acc, partials);
+}
+
+// alpha = r.r / p.q (after k_cg_a)
+__global__ void __launch_bounds__(kBlock)
+k_fin_a(uint32_t nb, const double *partials, Scal *sc) {
+    if (sc->done) return;
+    double v[1];
+    sum_partials<1>(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    sc->pq = v[0];
+    sc->alpha = sc->rr / v[0];
+    if (!(v[0] > 0.0)) {  // A^ is SPD: p.q <= 0 or NaN is a breakdown
+        sc->done = 1;
+        sc->alpha = 0.0;
+        sc->flags |= FLAG_BREAKDOWN;
+    }
 }
 
 // y += alpha p, r -= alpha q, partial r.r; convergence test on ||r^|| / ||b^||.
@@ -459,15 +472,23 @@ k_cg_b(Dims g, T *__restrict__ y, const T *__restrict__ p, T *__restrict__ r,
         r[I] = ri;
         acc[0] += (double)ri * (double)ri;
     }
-    grid_sum<1>(acc, partials, &sc->counter, [=](double *v) {
-        const double rr = v[0];
-        sc->iter += 1;
-        sc->beta = rr / sc->rr;
-        sc->rr = rr;
-        sc->first = 0;
-        if (rr <= sc->thresh2 || sc->iter >= sc->itmax || isnan(rr)) sc->done = 1;
-        if (isnan(rr)) atomicOr(&sc->flags, (int32_t)FLAG_BREAKDOWN);
-    });
+    block_partials<1>(acc, partials);
+}
+
+// beta, convergence test on ||r^|| / ||b^|| (after k_cg_b)
+__global__ void __launch_bounds__(kBlock)
+k_fin_b(uint32_t nb, const double *partials, Scal *sc) {
+    if (sc->done) return;
+    double v[1];
+    sum_partials<1>(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    const double rr = v[0];
+    sc->iter += 1;
+    sc->beta = rr / sc->rr;
+    sc->rr = rr;
+    sc->first = 0;
+    if (rr <= sc->thresh2 || sc->iter >= sc->itmax || isnan(rr)) sc->done = 1;
+    if (isnan(rr)) sc->flags |= FLAG_BREAKDOWN;
 }
 
 // x = x0 + s*y written over T^{n-1} (in place per cell), NaN guard of x (mod_evolve.f90:153),
@@ -539,13 +560,19 @@ k_ir_res(Dims g, IrParams ip, const double *__restrict__ x, const float *__restr
         if (ip.from_state && (xi + (xi - tp[I])) - xi < kTiny) fl |= FLAG_TINY_NEXT;
     }
     block_flags(fl, &sc->flags);
-    const int64_t sweeps_max = ip.itmax;
-    grid_sum<1>(acc, partials, &sc->counter, [=](double *v) {
-        sc->outer_rr = v[0];
-        sc->iter += 1;   // refinement sweeps
-        sc->done = (v[0] <= sc->thresh2) || sc->iter >= sweeps_max || isnan(v[0]);
-        if (isnan(v[0])) atomicOr(&sc->flags, (int32_t)FLAG_BREAKDOWN);
-    });
+    block_partials<1>(acc, partials);
+}
+
+// outer residual of a refinement sweep (after k_ir_res)
+__global__ void __launch_bounds__(kBlock)
+k_fin_ir(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max) {
+    double v[1];
+    sum_partials<1>(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    sc->outer_rr = v[0];
+    sc->iter += 1;   // refinement sweeps
+    sc->done = (v[0] <= sc->thresh2) || sc->iter >= sweeps_max || isnan(v[0]);
+    if (isnan(v[0])) sc->flags |= FLAG_BREAKDOWN;
 }
 
 // Starts an inner FP32 solve from the outer scalars: thresholds relative to this sweep's
@@ -566,7 +593,6 @@ __global__ void k_inner_begin(Scal *outer, Scal *inner, double inner_tol, double
     inner->beta = 0.0;
     inner->alpha = 0.0;
     inner->done = (rr == 0.0) || isnan(rr) || inner_itmax <= 0;
-    inner->counter = 0u;
 }
 
 // Clears the per-step flag bits, keeping TINY_NEXT -> tiny for this step.
@@ -574,7 +600,6 @@ __global__ void k_step_begin(Scal *sc) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     sc->tiny = (sc->flags & FLAG_TINY_NEXT) ? 1 : 0;
     sc->flags = sc->flags & FLAG_NAN_B;
-    sc->counter = 0u;
 }
 
 // TINY rule from an arbitrary state (first step after hf_set_state)
@@ -638,6 +663,9 @@ void launch_qdens(const Dims &g, const HeatParams &hp, const int8_t *iheater, co
                   Scal *sc, hipStream_t st) {
     HF_LAUNCH(k_qdens, g, st, g, hp, iheater, ells, tau_s, tp, qextra, qd, sc);
 }
+
+#define HF_FIN(kern, g, st, ...) \
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kBlock), 0, st, (g).nblk, __VA_ARGS__)
 
 void launch_rhs_res_state64(const Dims &g, const RhsParams &rp, const double *tp,
                             const double *tpp, const double *rhoc, const double *cvec,
@@ -705,6 +733,21 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
                    const double *cy, const double *cz, const double *tp, double *xout,
                    float *rhat, Scal *sc, double *partials, hipStream_t st) {
     HF_LAUNCH(k_ir_res, g, st, g, ip, x, y, sv, S, d, cx, cy, cz, tp, xout, rhat, sc, partials);
+}
+
+void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double tol, int64_t itmax,
+                      hipStream_t st) {
+    HF_FIN(k_fin_start, g, st, partials, sc, tol, itmax);
+}
+void launch_fin_a(const Dims &g, const double *partials, Scal *sc, hipStream_t st) {
+    HF_FIN(k_fin_a, g, st, partials, sc);
+}
+void launch_fin_b(const Dims &g, const double *partials, Scal *sc, hipStream_t st) {
+    HF_FIN(k_fin_b, g, st, partials, sc);
+}
+void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
+                   hipStream_t st) {
+    HF_FIN(k_fin_ir, g, st, partials, sc, sweeps_max);
 }
 
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,

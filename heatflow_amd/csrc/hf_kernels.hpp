@@ -80,6 +80,13 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
                    const double *sv, const double *S, const double *d, const double *cx,
                    const double *cy, const double *cz, const double *tp, double *xout,
                    float *rhat, Scal *sc, double *partials, hipStream_t st);
+// one-block finishing kernels: sum the per-block partials in block order, update the scalars
+void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double tol, int64_t itmax,
+                      hipStream_t st);
+void launch_fin_a(const Dims &g, const double *partials, Scal *sc, hipStream_t st);
+void launch_fin_b(const Dims &g, const double *partials, Scal *sc, hipStream_t st);
+void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
+                   hipStream_t st);
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
                         int64_t inner_itmax, double safety, hipStream_t st);
 void launch_step_begin(Scal *sc, hipStream_t st);

@@ -32,6 +32,24 @@ def _f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _anyptr(a):
+    """Pointer to a contiguous float64 array: numpy (host) or torch tensor (host or device).
+    The library copies with hipMemcpyDefault, so either works."""
+    if a is None:
+        return None, None
+    if hasattr(a, "data_ptr"):
+        import torch
+        t = a.detach()
+        if t.dtype != torch.float64:
+            t = t.double()
+        t = t.contiguous()
+        if t.is_cuda:
+            torch.cuda.current_stream(t.device).synchronize()
+        return C.c_void_p(t.data_ptr()), t
+    arr = _f64(a)
+    return _ptr(arr), arr
+
+
 def make_params(system: System, precision: str = "fp64", tol: float = 1.0e-32,
                 itmax: int = 50000, inner_tol: float = 1.0e-6, inner_itmax: int = 200) -> HfParams:
     lib = _lib.load()
@@ -195,6 +213,37 @@ class Simulation:
         check(self.lib.hf_kernel_times(self.ctx, _ptr(ms), _ptr(n), _ptr(by)), self.ctx)
         return {name: (float(ms[i]), int(n[i]), float(by[i]))
                 for i, name in enumerate(_lib.KERNELS) if n[i] > 0}
+
+    def set_fields(self, kappa=None, qdens_extra=None) -> None:
+        """Replaces kappa and/or the per-cell power density (numpy or torch); re-assemble next."""
+        pk, keep_k = _anyptr(kappa)
+        pq, keep_q = _anyptr(qdens_extra)
+        check(self.lib.hf_set_fields(self.ctx, pk, pq), self.ctx)
+
+    def forward_checkpointed(self, itime0: int, nsteps: int, ckpt_every: int) -> None:
+        check(self.lib.hf_forward_checkpointed(self.ctx, int(itime0), int(nsteps),
+                                               int(ckpt_every)), self.ctx)
+
+    def adjoint(self, dl_dt_final, out_kappa=None, out_power=None):
+        """Reverse sweep of the last forward_checkpointed run.  Outputs may be numpy arrays or
+        contiguous float64 torch tensors (filled in place); numpy arrays are returned if None."""
+        ps, keep = _anyptr(dl_dt_final)
+        if out_kappa is None:
+            out_kappa = np.empty(self.n)
+        if out_power is None:
+            out_power = np.empty(self.n)
+        pk = C.c_void_p(out_kappa.data_ptr()) if hasattr(out_kappa, "data_ptr") else _ptr(out_kappa)
+        pp = C.c_void_p(out_power.data_ptr()) if hasattr(out_power, "data_ptr") else _ptr(out_power)
+        check(self.lib.hf_adjoint(self.ctx, ps, pk, pp), self.ctx)
+        return out_kappa, out_power
+
+    def state_into(self, temp_p, temp_pp=None) -> None:
+        """Copies T^n (and T^{n-1}) into preallocated float64 torch tensors or numpy arrays."""
+        pa = C.c_void_p(temp_p.data_ptr()) if hasattr(temp_p, "data_ptr") else _ptr(temp_p)
+        pb = None
+        if temp_pp is not None:
+            pb = C.c_void_p(temp_pp.data_ptr()) if hasattr(temp_pp, "data_ptr") else _ptr(temp_pp)
+        check(self.lib.hf_get_state(self.ctx, pa, pb), self.ctx)
 
     def close(self) -> None:
         if getattr(self, "ctx", None):

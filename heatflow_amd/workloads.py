@@ -81,18 +81,41 @@ def c3(n: int = 512, seed: int = 3, sigma: float = 32.0):
     return s, dict(name="c3", steps=20, tol=1.0e-10, seeds=dict(kappa=seed))
 
 
+def near_cells(n: int, centres: np.ndarray, radius: int) -> np.ndarray:
+    """Linear indices of the cells within Chebyshev distance `radius` of any centre (n^3 grid,
+    x fastest), sorted."""
+    out = set()
+    for c in centres:
+        x, y, z = int(c % n), int((c // n) % n), int(c // (n * n))
+        for dz in range(-radius, radius + 1):
+            for dy in range(-radius, radius + 1):
+                for dx in range(-radius, radius + 1):
+                    X, Y, Z = x + dx, y + dy, z + dz
+                    if 0 <= X < n and 0 <= Y < n and 0 <= Z < n:
+                        out.add(X + n * (Y + n * Z))
+    return np.array(sorted(out), np.int64)
+
+
 def c4(n: int = 128, icattaneo: int = 1, nheat: int = 8, nprobe: int = 64, seed_k: int = 4,
        seed_kstar: int = 40, seed_probe: int = 41, seed_heat: int = 42):
     """kappa = 100 (1 + 0.1 U(-1,1)) (trainable), 1 um cells, rho c_p = 1.631e6, BCs as c1,
     `nheat` heaters of 1e-5 W each entering as per-cell power density P/V (trainable),
-    `nprobe` probe cells, targets from a ground-truth kappa* (seed_kstar); dt=1e-10, 200
-    steps, checkpoints every 20.  Returns the system (with kappa) and kappa*, probes, powers."""
+    `nprobe` probe cells drawn from the heaters' neighbourhoods (Chebyshev radius 2: over 200
+    steps of 0.1 ns heat diffuses about one cell, so farther probes would not see the heaters),
+    targets from a ground-truth kappa* (seed_kstar); dt=1e-10, 200 steps, checkpoints every 20.
+    Returns the system (with kappa) and kappa*, probes, powers."""
     N = n ** 3
     L = n * 1.0e-6
     kappa = 100.0 * (1.0 + 0.1 * _rng(seed_k).uniform(-1.0, 1.0, N))
     kstar = 100.0 * (1.0 + 0.1 * _rng(seed_kstar).uniform(-1.0, 1.0, N))
-    probes = _rng(seed_probe).choice(N, size=min(nprobe, N), replace=False).astype(np.int64)
-    heat = _rng(seed_heat).choice(N, size=min(nheat, N), replace=False).astype(np.int64)
+    # heaters away from the domain faces
+    lo, hi = (2, n - 2) if n > 6 else (0, n)
+    side = hi - lo
+    hsel = _rng(seed_heat).choice(side ** 3, size=min(nheat, side ** 3), replace=False)
+    hx, hy, hz = hsel % side + lo, (hsel // side) % side + lo, hsel // (side * side) + lo
+    heat = (hx + n * (hy + n * hz)).astype(np.int64)
+    near = near_cells(n, heat, 2)
+    probes = _rng(seed_probe).choice(near, size=min(nprobe, len(near)), replace=False).astype(np.int64)
     vol = (L / n) ** 3
     power = np.zeros(N)
     power[heat] = 1.0e-5
@@ -102,5 +125,5 @@ def c4(n: int = 128, icattaneo: int = 1, nheat: int = 8, nprobe: int = 64, seed_
     s = System.uniform(p, (L, L, L), kappa=kappa, rho=RHO_A, heat_capacity=CP_A,
                        tau=1.0e-12 if icattaneo else 0.0, qdens_extra=power / vol)
     return s, dict(name="c4", steps=200, ckpt=20, kstar=kstar, probes=probes, heaters=heat,
-                   power=power, volume=vol,
+                   power=power, volume=vol, near=near,
                    seeds=dict(kappa=seed_k, kstar=seed_kstar, probes=seed_probe, heat=seed_heat))

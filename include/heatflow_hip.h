@@ -88,7 +88,9 @@ void hf_params_default(hf_params *p);
 /* Creates a context for one grid; allocates device memory for N cells.
  * Replaces the allocations of set_global_variables (src/heatflow/mod_setup.f90:38-45).
  * Geometry the reference itself mishandles is rejected with HF_ERR_ARG: nx == 1 with N > 1
- * (duplicate COO offsets overflow `len`, mod_setup.f90:96-112) and periodic axes of length 2. */
+ * (duplicate COO offsets overflow `len`, mod_setup.f90:96-112) and periodic axes shorter than 3
+ * (length 1 zeroes the diagonal in hmatrixfunc's wrap test, mod_hmatrix.f90:213-227; length 2
+ * duplicates a COO offset). */
 int hf_create(hf_ctx **ctx, const hf_params *p);
 
 /* Uploads the per-cell material fields of grid(:,:,:) (src/heatflow/mod_constructions.f90:25-31)
@@ -177,17 +179,27 @@ int hf_kernel_times(hf_ctx *ctx, double *ms, int64_t *launches, double *bytes);
 int hf_set_solver(hf_ctx *ctx, double tol, int64_t itmax, int32_t precision, double inner_tol,
                   int32_t inner_itmax);
 
-/* ---- adjoint (no reference counterpart: reverse-time gradient of a probe loss) ----------
- * Forward-with-checkpoints + reverse sweep over the same context.  The loss is
- * L = sum_k w_k * (T^{n_end}[probe_k] - target_k)^2 at the final step.  Gradients are
- * returned per cell with respect to kappa (dL/dkappa, length N) and with respect to the
- * per-cell power P_I that enters as Qdens_I += P_I / volume_I (dL/dP, length N).
- * Checkpoints of (T^n, T^{n-1}) are kept every `ckpt_every` steps on the device and each
- * segment is recomputed during the reverse sweep.  Requires per-axis uniform cell lengths
- * and no CG_dir faces.  *loss gets L. */
+/* ---- adjoint (no reference counterpart: reverse-time gradients) ---------------------------
+ * hf_forward_checkpointed runs nsteps time steps (itime0 .. itime0+nsteps-1) from the current
+ * state, keeping (T^n, T^{n-1}) on the device every `ckpt_every` steps; the context ends at the
+ * final state.  hf_adjoint then runs the reverse sweep (each segment recomputed from its
+ * checkpoint) seeded with dL/dT^{final} (length N), and returns the gradients with respect to
+ * the per-cell conductivity kappa (dl_dkappa, length N) and to a per-cell power P_I that enters
+ * as Qdens_I += P_I / volume_I (dl_dpower, length N).  Requires an FP64 context, per-axis
+ * uniform cell lengths; the BR switch (x BR where neighbouring kappas differ) is treated as
+ * locally constant.  The context is left at the end state of the forward run. */
+int hf_forward_checkpointed(hf_ctx *ctx, int64_t itime0, int64_t nsteps, int64_t ckpt_every);
+int hf_adjoint(hf_ctx *ctx, const double *dl_dt_final, double *dl_dkappa, double *dl_dpower);
+
+/* Convenience: forward + reverse for L = sum_k w_k (T^{final}[probe_k] - target_k)^2
+ * (weight may be NULL for w = 1); *loss gets L. */
 int hf_grad(hf_ctx *ctx, int64_t itime0, int64_t nsteps, int64_t ckpt_every, int64_t nprobe,
             const int64_t *probe_idx, const double *target, const double *weight,
             double *loss, double *dl_dkappa, double *dl_dpower);
+
+/* Replaces the per-cell conductivity and/or the superset per-cell power density (either may be
+ * NULL) of an uploaded context; call hf_assemble afterwards.  Used by the autograd wrapper. */
+int hf_set_fields(hf_ctx *ctx, const double *kappa, const double *qdens_extra);
 
 /* Last error message of this context (or of the last failed hf_create when ctx is NULL). */
 const char *hf_last_error(const hf_ctx *ctx);

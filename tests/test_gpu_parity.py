@@ -166,12 +166,14 @@ def test_c1_hundred_steps():
     assert tp.max() > 300.0 + 1e-3        # the heater did heat
 
 
-def test_c1_tol_1e10_within_gate():
-    """At the benchmark tolerance 1e-10 (relative residual) the temperatures stay within the
-    1e-10 relative gate of the reference-tolerance oracle."""
+def test_c1_tol_1e12_within_gate():
+    """A relative residual of 1e-10 is not enough for a 1e-10 temperature error (local errors
+    around the heater exceed the 2-norm test by ~50x over 100 steps); at the benchmark
+    tolerance 1e-12 the temperatures stay within the 1e-10 relative gate of the
+    reference-tolerance oracle."""
     s, meta = workloads.c1(16)
     rc, otp, _, _ = oracle.OracleSystem(s).run(100)
-    with Simulation(s, tol=1e-10) as sim:
+    with Simulation(s, tol=1e-12) as sim:
         sim.initial_evolve()
         sim.run(1, 100)
         tp, _ = sim.get_state()

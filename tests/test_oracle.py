@@ -204,3 +204,21 @@ def test_linbcg_matches_dense_solve():
                              C.c_double(1e-32), C.c_int64(50000), C.byref(it), C.byref(err))
     xd = np.linalg.solve(H, b)
     np.testing.assert_allclose(x, xd, rtol=1e-12, atol=1e-12 * np.abs(xd).max())
+
+
+# ---- the dense torch restatement used to check the adjoint ----------------------------------
+
+@pytest.mark.parametrize("icat", [0, 1])
+def test_dense_reference_matches_oracle(icat):
+    """tests/dense_ref.py (the autograd reference of the adjoint tests) steps the same physics
+    as the oracle: per-cell power P enters as Qdens = P / V."""
+    import torch
+    from dense_ref import dense_run
+    from heatflow_amd import workloads
+    s, m = workloads.c4(6, icattaneo=icat, nheat=3, nprobe=5)
+    rc, tp, _, _ = oracle.OracleSystem(s).run(7)
+    assert rc == 0
+    k = torch.as_tensor(s.cells.kappa)
+    pw = torch.as_tensor(m["power"])
+    td = dense_run(s, k, pw, 7).numpy()
+    np.testing.assert_allclose(td, tp, rtol=1e-12)

@@ -154,8 +154,10 @@ def main():
     ap.add_argument("--only", choices=["fp32_ir", "fp64"], default=None,
                     help="time one mode only (profiling)")
     ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--grad-n", type=int, default=128)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 2 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -167,6 +169,24 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     from heatflow_amd import Simulation, workloads
+
+    if args.copy_probe:
+        # known traffic: 2 GiB read + 2 GiB written per copy
+        src = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / 5
+        log(f"[bench] copy probe: 2 GiB copy {ms:.3f} ms = {2 * (1 << 31) / (ms * 1e-3) / 1e9:.0f} GB/s")
+        del src, dst
 
     t_gen = time.perf_counter()
     s, meta = workloads.c3(args.n)

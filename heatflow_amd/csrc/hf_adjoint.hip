@@ -86,9 +86,7 @@ k_adj_step(Dims g, AdjParams q, const double *__restrict__ T, const double *__re
            const double *__restrict__ rhoc, const double *__restrict__ cvec,
            const double *__restrict__ vol, double *__restrict__ g_lo,
            double *__restrict__ g_new, double *__restrict__ gk, double *__restrict__ gp) {
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+    HF_FOR_CELLS(g, I) {
         Nbr n = neighbours(g, I);
         const double li = lam[I], ti = T[I], ki = kappa[I];
         double acc = 0.0;

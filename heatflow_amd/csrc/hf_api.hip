@@ -100,17 +100,29 @@ static void make_dims(Dims &g, const hf_params &p) {
     g.per[2] = p.periodic[2] && p.nz >= 3;
     g.dnx.init(g.nx);
     g.dny.init(g.ny);
-    // launch layout: up to `cap` blocks of kBlock threads, contiguous chunks of cells
+    // launch layout: tiles of kBlock cells over a grid of at most `cap` blocks (multiple of 8);
+    // HF_NBLK / HF_LAYOUT override the defaults (tuning sweeps)
     uint32_t cap = 8192;
     if (const char *e = getenv("HF_NBLK")) cap = std::max(8u, (uint32_t)atoi(e) / 8u * 8u);
-    uint64_t want = ((uint64_t)g.N + kBlock - 1) / kBlock;
-    uint64_t nb = std::min<uint64_t>(want, cap);
-    uint64_t chunk = ((uint64_t)g.N + nb - 1) / nb;
-    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
-    nb = ((uint64_t)g.N + chunk - 1) / chunk;
-    nb = (nb + 7) / 8 * 8;
-    g.chunk = (uint32_t)chunk;
-    g.nblk = (uint32_t)nb;
+    g.layout = LAYOUT_CHUNKED | LAYOUT_XCD;
+    if (const char *e = getenv("HF_LAYOUT")) g.layout = atoi(e);
+    g.ntiles = (uint32_t)(((uint64_t)g.N + kBlock - 1) / kBlock);
+    uint64_t nb = std::min<uint64_t>(g.ntiles, cap);
+    g.tpb = (uint32_t)((g.ntiles + nb - 1) / nb);
+    nb = (g.ntiles + g.tpb - 1) / g.tpb;
+    g.nblk = (uint32_t)((nb + 7) / 8 * 8);
+    // z-marching tiles: enough z-chunks for ~`zcap` blocks (HF_ZBLK overrides)
+    uint32_t zcap = 8192;
+    if (const char *e = getenv("HF_ZBLK")) zcap = std::max(8u, (uint32_t)atoi(e));
+    g.ztx = (g.nx + kTileX - 1) / kTileX;
+    g.zty = (g.ny + kTileY - 1) / kTileY;
+    const uint64_t per_plane = (uint64_t)g.ztx * g.zty;
+    uint64_t nzc = std::max<uint64_t>(1, (zcap + per_plane - 1) / per_plane);
+    nzc = std::min<uint64_t>(nzc, g.nz);
+    g.zchunk = (uint32_t)((g.nz + nzc - 1) / nzc);
+    nzc = (g.nz + g.zchunk - 1) / g.zchunk;
+    g.ztiles = (uint32_t)(per_plane * nzc);
+    g.zblk = (g.ztiles + 7) / 8 * 8;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -148,7 +160,7 @@ static bool allocs_base(hf_ctx *c, int *rc) {
         (r = dalloc(c, &c->bvec, N)) || (r = dalloc(c, &c->sv, N)) || (r = dalloc(c, &c->qd, N)) ||
         (r = dalloc(c, &c->T[0], N)) || (r = dalloc(c, &c->T[1], N)) ||
         (r = dalloc(c, &c->sc, 1)) || (r = dalloc(c, &c->sci, 1)) ||
-        (r = dalloc(c, &c->partials, (size_t)4 * c->g.nblk))) {
+        (r = dalloc(c, &c->partials, (size_t)4 * std::max(c->g.nblk, c->g.zblk)))) {
         *rc = r;
         return false;
     }
@@ -601,7 +613,9 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     double *xa = c->T[2], *xb = c->T[1];  // T[1] (Temp_pp) is free once x0 and S exist
     // T[1] is still needed: the refined x of the last sweep must not overwrite Temp_pp before
     // the rotation, but Temp_pp is only read by k_rhs_res (done above), so it may host x.
-    IrParams ip{p.itmax, 1};
+    // refinement sweeps: each reduces the FP64 residual by the inner tolerance, so a handful
+    // suffice; the cap guards tolerances below the FP64 round-off floor
+    IrParams ip{std::min<int64_t>(p.itmax, 64), 1};
     int64_t inner_total = 0, sweeps = 0;
     for (;;) {
         launch_inner_begin(c->sc, c->sci, p.inner_tol, p.tol, p.inner_itmax, 0.25, c->st);

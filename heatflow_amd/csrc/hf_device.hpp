@@ -15,6 +15,8 @@ namespace hf {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
+constexpr int kTileX = 64;               // z-marching tile: 64 x 4 columns per block
+constexpr int kTileY = kBlock / kTileX;
 constexpr double kTiny = 1.0e-12;        // constants::TINY, src/heatflow/mod_constants.f90:17
 constexpr double kStefBoltz = 5.670373e-8;  // src/heatflow/mod_constants.f90:16
 
@@ -36,11 +38,22 @@ struct FastDiv {
 
 struct Dims {
     uint32_t nx, ny, nz, N, nxny;
-    uint32_t chunk;      // cells per logical block (multiple of kBlock)
+    uint32_t ntiles;     // tiles of kBlock consecutive cells
+    uint32_t tpb;        // tiles per block (chunked layout)
     uint32_t nblk;       // launch grid (multiple of 8)
+    int32_t layout;      // LAYOUT_* bits
+    // z-marching stencil kernels: ztx x zty tiles per plane, z-chunks of `zchunk` planes,
+    // ztiles = ztx * zty * (number of chunks), zblk = ztiles rounded up to a multiple of 8
+    uint32_t ztx, zty, zchunk, ztiles, zblk;
     int32_t per[3];      // periodic axes (length >= 3)
     FastDiv dnx, dny;
 };
+
+// Tile-to-block layouts.  CHUNKED: block b owns tiles [b*tpb, (b+1)*tpb); otherwise block b
+// takes tiles b, b+nblk, b+2 nblk, ... (all blocks sweep the grid together).  XCD: hardware
+// block h acts as logical block (h % 8) * (nblk / 8) + h / 8, so the blocks one XCD runs are
+// neighbours in the logical order (shared L2 for their halos).
+enum : int32_t { LAYOUT_CHUNKED = 1, LAYOUT_XCD = 2 };
 
 // Device-resident solver scalars.  Written only by the one-block finishing kernels.
 struct Scal {
@@ -64,18 +77,36 @@ enum : int32_t {
     FLAG_NAN_X = 32, FLAG_TINY_NEXT = 64, FLAG_BREAKDOWN = 128
 };
 
-__device__ __forceinline__ uint32_t logical_block(uint32_t nblk) {
-    uint32_t h = blockIdx.x;
-    return (h & 7u) * (nblk >> 3) + (h >> 3);
+__device__ __forceinline__ uint32_t logical_block(const Dims &g) {
+    const uint32_t h = blockIdx.x;
+    return (g.layout & LAYOUT_XCD) ? (h & 7u) * (g.nblk >> 3) + (h >> 3) : h;
 }
 
-// cell range of this block: [*beg, *end)
-__device__ __forceinline__ void block_range(const Dims &g, uint32_t *beg, uint32_t *end) {
-    uint64_t b = (uint64_t)logical_block(g.nblk) * g.chunk;
-    uint64_t e = b + g.chunk;
-    *beg = (uint32_t)(b < g.N ? b : g.N);
-    *end = (uint32_t)(e < g.N ? e : g.N);
+// Tiles of this block: t = first; t < end; t += step; cell I = t * kBlock + threadIdx.x.
+struct Tiles {
+    uint32_t first, end, step;
+};
+
+__device__ __forceinline__ Tiles block_tiles(const Dims &g) {
+    const uint32_t b = logical_block(g);
+    Tiles t;
+    if (g.layout & LAYOUT_CHUNKED) {
+        const uint64_t f = (uint64_t)b * g.tpb, e = f + g.tpb;
+        t.first = (uint32_t)(f < g.ntiles ? f : g.ntiles);
+        t.end = (uint32_t)(e < g.ntiles ? e : g.ntiles);
+        t.step = 1;
+    } else {
+        t.first = b;
+        t.end = g.ntiles;
+        t.step = g.nblk;
+    }
+    return t;
 }
+
+// Loop over the cells of this block (one cell per thread per trip); I < N inside.
+#define HF_FOR_CELLS(g, I)                                                          \
+    for (Tiles _tl = block_tiles(g); _tl.first < _tl.end; _tl.first += _tl.step)  \
+        for (uint32_t I = _tl.first * kBlock + threadIdx.x; I < (g).N; I = (g).N)
 
 struct Nbr {
     uint32_t xm, xp, ym, yp, zm, zp;

@@ -5,10 +5,10 @@
 //   k_scale        Jacobi scaling s = 1/sqrt(-diag), scaled couplings, FP32 copies
 //   k_coo_*        export in the reference COO order                  (mod_setup.f90:89-152)
 //   k_qdens        per-step heater power density                      (mod_heating.f90:27-169)
-//   k_rhs_res      S, x0, r0 = S - H x0 fused, scaled, + norms         (mod_evolve.f90:112-141)
-//   k_cg_a/k_cg_b  CG iteration on the scaled operator A^ = I - C^     (mod_Sparse.f90:404-433)
+//   k_cg_b         y += alpha p, r -= alpha q, r.r                     (mod_Sparse.f90:412-420)
 //   k_finalize     x = x0 + s*y, NaN guard, TINY flag of the next step (mod_evolve.f90:140-170)
-//   k_ir_res       mixed mode: x += s*y (FP32 y) and the FP64 residual of the refined x
+//   k_fin_*        one-block finishing kernels of the reductions (scalars, convergence)
+// The stencil kernels (fused RHS/residual, CG SpMV, refinement residual) are in hf_stencil.hip.
 //
 // All HIP code here is compiled with -ffp-contract=off so every product and sum rounds as
 // written; the assembly therefore reproduces hmatrixfunc's values bit for bit.
@@ -51,10 +51,8 @@ k_assemble(Dims g, AsmParams ap, const double *__restrict__ kappa, const double 
            const double *__restrict__ rhoc, const double *__restrict__ tau_s,
            double *__restrict__ d, double *__restrict__ cx, double *__restrict__ cy,
            double *__restrict__ cz, double *__restrict__ bvec, Scal *sc) {
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
     int32_t fl = 0;
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+    HF_FOR_CELLS(g, I) {
         Nbr n = neighbours(g, I);
         // calculate_alpha (mod_hmatrix.f90:254-267)
         double alpha = 0.0;
@@ -129,9 +127,7 @@ k_scale(Dims g, const double *__restrict__ d, const double *__restrict__ cx,
         const double *__restrict__ cy, const double *__restrict__ cz, double *__restrict__ sv,
         double *__restrict__ hx, double *__restrict__ hy, double *__restrict__ hz,
         float *__restrict__ fx, float *__restrict__ fy, float *__restrict__ fz) {
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+    HF_FOR_CELLS(g, I) {
         Nbr n = neighbours(g, I);
         double si = 1.0 / sqrt(-d[I]);
         sv[I] = si;
@@ -173,9 +169,7 @@ __device__ __forceinline__ int coo_slots(const Dims &g, const Nbr &n, uint32_t j
 __global__ void __launch_bounds__(kBlock)
 k_coo_count(Dims g, const double *__restrict__ cx, const double *__restrict__ cy,
             const double *__restrict__ cz, int64_t *__restrict__ cnt) {
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
-    for (uint32_t j = beg + threadIdx.x; j < end; j += kBlock) {
+    HF_FOR_CELLS(g, j) {
         Nbr n = neighbours(g, j);
         uint32_t rows[6];
         double vals[6];
@@ -189,9 +183,7 @@ k_coo_fill(Dims g, const double *__restrict__ d, const double *__restrict__ cx,
            const double *__restrict__ cy, const double *__restrict__ cz,
            const int64_t *__restrict__ off, double *__restrict__ val, int64_t *__restrict__ irow,
            int64_t *__restrict__ jcol) {
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
-    for (uint32_t j = beg + threadIdx.x; j < end; j += kBlock) {
+    HF_FOR_CELLS(g, j) {
         Nbr n = neighbours(g, j);
         uint32_t rows[6];
         double vals[6];
@@ -291,10 +283,8 @@ k_qdens(Dims g, HeatParams hp, const int8_t *__restrict__ iheater,
         const double *__restrict__ ells, const double *__restrict__ tau_s,
         const double *__restrict__ tp, const double *__restrict__ qextra,
         double *__restrict__ qd, Scal *sc) {
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
     int32_t fl = 0;
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+    HF_FOR_CELLS(g, I) {
         double q = 0.0, qdv = 0.0;
         if (hp.any_heater) {
             q = heater_q(hp, iheater[I], tau_s[I], ells[I], tp[I]);
@@ -306,81 +296,6 @@ k_qdens(Dims g, HeatParams hp, const int8_t *__restrict__ iheater,
         qd[I] = qdv;
     }
     block_flags(fl, &sc->flags);
-}
-
-template <typename OutT, bool FromState, bool StoreS, bool StoreX>
-__global__ void __launch_bounds__(kBlock)
-k_rhs_res(Dims g, RhsParams rp, const double *__restrict__ tp, const double *__restrict__ tpp,
-          const double *__restrict__ xin, const double *__restrict__ bin,
-          const double *__restrict__ rhoc, const double *__restrict__ cvec,
-          const double *__restrict__ qd, const double *__restrict__ bvec,
-          const double *__restrict__ d, const double *__restrict__ cx,
-          const double *__restrict__ cy, const double *__restrict__ cz,
-          OutT *__restrict__ rhat, double *__restrict__ sout, double *__restrict__ xout,
-          Scal *sc, double *partials) {
-    const bool tiny = sc->tiny != 0;
-    auto X = [&](uint32_t J) -> double {
-        if (FromState) {
-            double x0 = tp[J] + (tp[J] - tpp[J]);  // mod_evolve.f90:140
-            if (tiny) x0 = x0 + kTiny;             // :141
-            return x0;
-        } else {
-            return xin[J];
-        }
-    };
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
-    double acc[2] = {0.0, 0.0};
-    int32_t fl = 0;
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
-        Nbr n = neighbours(g, I);
-        double S;
-        if (FromState) {
-            // mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36
-            if (rp.isteady == 0) {
-                S = -(rp.inverse_time * tp[I] * rhoc[I]) - qd[I] - bvec[I];
-                if (rp.icattaneo == 1) {
-                    double scat = (tpp[I] - 2.0 * tp[I]) * cvec[I];
-                    if (isnan(scat)) fl |= FLAG_NAN_SCAT;
-                    S = S + scat;
-                }
-            } else {
-                S = -qd[I] - bvec[I];
-            }
-            if (isnan(S)) fl |= FLAG_NAN_S;
-        } else {
-            S = bin[I];
-        }
-        // H x in ascending column order (the serial SRSax order, mod_Sparse.f90:306-314)
-        double hxv = 0.0;
-        if (n.hzm && n.zm < I) hxv = hxv + cz[n.zm] * X(n.zm);
-        if (n.hym && n.ym < I) hxv = hxv + cy[n.ym] * X(n.ym);
-        if (n.hxm && n.xm < I) hxv = hxv + cx[n.xm] * X(n.xm);
-        const double xi = X(I);
-        hxv = hxv + d[I] * xi;
-        if (n.hxp && n.xp > I) hxv = hxv + cx[I] * X(n.xp);
-        if (n.hyp && n.yp > I) hxv = hxv + cy[I] * X(n.yp);
-        if (n.hzp && n.zp > I) hxv = hxv + cz[I] * X(n.zp);
-        // periodic wrap couplings (neighbour on the other side of the domain)
-        if (n.hxm && n.xm > I) hxv = hxv + cx[n.xm] * X(n.xm);
-        if (n.hxp && n.xp < I) hxv = hxv + cx[I] * X(n.xp);
-        if (n.hym && n.ym > I) hxv = hxv + cy[n.ym] * X(n.ym);
-        if (n.hyp && n.yp < I) hxv = hxv + cy[I] * X(n.yp);
-        if (n.hzm && n.zm > I) hxv = hxv + cz[n.zm] * X(n.zm);
-        if (n.hzp && n.zp < I) hxv = hxv + cz[I] * X(n.zp);
-        const double r = S - hxv;
-        const double s = 1.0 / sqrt(-d[I]);
-        const double rh = -s * r;
-        rhat[I] = (OutT)rh;
-        if (StoreS) sout[I] = S;
-        if (StoreX) xout[I] = xi;
-        const double bh = s * S;
-        const double rq = (double)(OutT)rh;   // r.r of the stored (possibly FP32) residual
-        acc[0] += bh * bh;
-        acc[1] += rq * rq;
-    }
-    block_flags(fl, &sc->flags);
-    block_partials<2>(acc, partials);
 }
 
 // Start of a solve: ||b^||^2, r.r, thresholds (after k_rhs_res).
@@ -407,37 +322,6 @@ k_fin_start(uint32_t nb, const double *partials, Scal *sc, double tol, int64_t i
 
 // ---- CG on A^ = I - C^ (unit diagonal; the Jacobi-preconditioned CG of linbcg in scaled form)
 
-// p = r + beta * p_old (p = r on the first iteration), q = A^ p, partial p.q.
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_cg_a(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
-       T *__restrict__ q, const T *__restrict__ hx, const T *__restrict__ hy,
-       const T *__restrict__ hz, Scal *sc, double *partials) {
-    if (sc->done) return;
-    const bool first = sc->first != 0;
-    const T beta = (T)sc->beta;
-    auto P = [&](uint32_t J) -> T { return first ? r[J] : (T)(r[J] + beta * pold[J]); };
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
-    double acc[1] = {0.0};
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
-        Nbr n = neighbours(g, I);
-        const T pi = P(I);
-        T s = (T)0;
-        if (n.hzm) s = s + hz[n.zm] * P(n.zm);
-        if (n.hym) s = s + hy[n.ym] * P(n.ym);
-        if (n.hxm) s = s + hx[n.xm] * P(n.xm);
-        if (n.hxp) s = s + hx[I] * P(n.xp);
-        if (n.hyp) s = s + hy[I] * P(n.yp);
-        if (n.hzp) s = s + hz[I] * P(n.zp);
-        const T qi = pi - s;
-        pnew[I] = pi;
-        q[I] = qi;
-        acc[0] += (double)pi * (double)qi;
-    }
-    block_partials<1>(acc, partials);
-}
-
 // alpha = r.r / p.q (after k_cg_a)
 __global__ void __launch_bounds__(kBlock)
 k_fin_a(uint32_t nb, const double *partials, Scal *sc) {
@@ -462,10 +346,8 @@ k_cg_b(Dims g, T *__restrict__ y, const T *__restrict__ p, T *__restrict__ r,
     if (sc->done) return;
     const bool first = sc->first != 0;
     const T alpha = (T)sc->alpha;
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
     double acc[1] = {0.0};
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+    HF_FOR_CELLS(g, I) {
         const T yi = first ? alpha * p[I] : y[I] + alpha * p[I];
         const T ri = r[I] - alpha * q[I];
         y[I] = yi;
@@ -500,10 +382,8 @@ k_finalize(Dims g, const double *__restrict__ tp, double *__restrict__ tpp_io,
            const double *__restrict__ sv, const double *__restrict__ y, Scal *sc) {
     const bool tiny = sc->tiny != 0;
     const bool have_y = sc->iter > 0;
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
     int32_t fl = 0;
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+    HF_FOR_CELLS(g, I) {
         double x0;
         if (FromState) {
             x0 = tp[I] + (tp[I] - tpp_io[I]);
@@ -519,59 +399,19 @@ k_finalize(Dims g, const double *__restrict__ tp, double *__restrict__ tpp_io,
     block_flags(fl, &sc->flags);
 }
 
-// Mixed mode refinement: x_new = x + s*y (y FP32) formed on the fly for the stencil, the FP64
-// residual r^ = -s (S - H x_new) stored as FP32 for the next inner solve, ||r^||^2, and the
-// NaN / next-step TINY tests on x_new.
-__global__ void __launch_bounds__(kBlock)
-k_ir_res(Dims g, IrParams ip, const double *__restrict__ x, const float *__restrict__ y,
-         const double *__restrict__ sv, const double *__restrict__ S,
-         const double *__restrict__ d, const double *__restrict__ cx,
-         const double *__restrict__ cy, const double *__restrict__ cz,
-         const double *__restrict__ tp, double *__restrict__ xout, float *__restrict__ rhat,
-         Scal *sc, double *partials) {
-    auto X = [&](uint32_t J) -> double { return x[J] + sv[J] * (double)y[J]; };
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
-    double acc[1] = {0.0};
-    int32_t fl = 0;
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
-        Nbr n = neighbours(g, I);
-        double hxv = 0.0;
-        if (n.hzm && n.zm < I) hxv = hxv + cz[n.zm] * X(n.zm);
-        if (n.hym && n.ym < I) hxv = hxv + cy[n.ym] * X(n.ym);
-        if (n.hxm && n.xm < I) hxv = hxv + cx[n.xm] * X(n.xm);
-        const double xi = X(I);
-        hxv = hxv + d[I] * xi;
-        if (n.hxp && n.xp > I) hxv = hxv + cx[I] * X(n.xp);
-        if (n.hyp && n.yp > I) hxv = hxv + cy[I] * X(n.yp);
-        if (n.hzp && n.zp > I) hxv = hxv + cz[I] * X(n.zp);
-        if (n.hxm && n.xm > I) hxv = hxv + cx[n.xm] * X(n.xm);
-        if (n.hxp && n.xp < I) hxv = hxv + cx[I] * X(n.xp);
-        if (n.hym && n.ym > I) hxv = hxv + cy[n.ym] * X(n.ym);
-        if (n.hyp && n.yp < I) hxv = hxv + cy[I] * X(n.yp);
-        if (n.hzm && n.zm > I) hxv = hxv + cz[n.zm] * X(n.zm);
-        if (n.hzp && n.zp < I) hxv = hxv + cz[I] * X(n.zp);
-        const double rh = -sv[I] * (S[I] - hxv);
-        const float rf = (float)rh;
-        rhat[I] = rf;
-        xout[I] = xi;
-        acc[0] += (double)rf * (double)rf;
-        if (isnan(xi)) fl |= FLAG_NAN_X;
-        if (ip.from_state && (xi + (xi - tp[I])) - xi < kTiny) fl |= FLAG_TINY_NEXT;
-    }
-    block_flags(fl, &sc->flags);
-    block_partials<1>(acc, partials);
-}
-
 // outer residual of a refinement sweep (after k_ir_res)
 __global__ void __launch_bounds__(kBlock)
 k_fin_ir(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max) {
     double v[1];
     sum_partials<1>(partials, nb, v);
     if (threadIdx.x != 0) return;
+    // refinement stops at the tolerance, at the sweep cap, or when a sweep no longer halves
+    // the FP64 residual (it has reached the FP64 round-off floor of H x)
+    const double prev = sc->outer_rr;
     sc->outer_rr = v[0];
     sc->iter += 1;   // refinement sweeps
-    sc->done = (v[0] <= sc->thresh2) || sc->iter >= sweeps_max || isnan(v[0]);
+    const bool stalled = sc->iter > 1 && v[0] > 0.25 * prev;
+    sc->done = (v[0] <= sc->thresh2) || sc->iter >= sweeps_max || stalled || isnan(v[0]);
     if (isnan(v[0])) sc->flags |= FLAG_BREAKDOWN;
 }
 
@@ -606,10 +446,8 @@ __global__ void k_step_begin(Scal *sc) {
 __global__ void __launch_bounds__(kBlock)
 k_tiny_from_state(Dims g, const double *__restrict__ tp, const double *__restrict__ tpp,
                   Scal *sc) {
-    uint32_t beg, end;
-    block_range(g, &beg, &end);
     int32_t fl = 0;
-    for (uint32_t I = beg + threadIdx.x; I < end; I += kBlock) {
+    HF_FOR_CELLS(g, I) {
         double x = tp[I] + (tp[I] - tpp[I]);
         if (x - tp[I] < kTiny) fl |= FLAG_TINY_NEXT;
     }
@@ -664,57 +502,14 @@ void launch_qdens(const Dims &g, const HeatParams &hp, const int8_t *iheater, co
     HF_LAUNCH(k_qdens, g, st, g, hp, iheater, ells, tau_s, tp, qextra, qd, sc);
 }
 
-#define HF_FIN(kern, g, st, ...) \
-    hipLaunchKernelGGL(kern, dim3(1), dim3(kBlock), 0, st, (g).nblk, __VA_ARGS__)
+#define HF_FIN(kern, nb, st, ...) \
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kBlock), 0, st, nb, __VA_ARGS__)
 
-void launch_rhs_res_state64(const Dims &g, const RhsParams &rp, const double *tp,
-                            const double *tpp, const double *rhoc, const double *cvec,
-                            const double *qd, const double *bvec, const double *d,
-                            const double *cx, const double *cy, const double *cz, double *rhat,
-                            double *sout, Scal *sc, double *partials, hipStream_t st) {
-    if (sout)
-        HF_LAUNCH((k_rhs_res<double, true, true, false>), g, st, g, rp, tp, tpp, nullptr, nullptr,
-                  rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, sout, nullptr, sc, partials);
-    else
-        HF_LAUNCH((k_rhs_res<double, true, false, false>), g, st, g, rp, tp, tpp, nullptr, nullptr,
-                  rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, nullptr, nullptr, sc, partials);
-}
-
-void launch_rhs_res_state32(const Dims &g, const RhsParams &rp, const double *tp,
-                            const double *tpp, const double *rhoc, const double *cvec,
-                            const double *qd, const double *bvec, const double *d,
-                            const double *cx, const double *cy, const double *cz, float *rhat,
-                            double *sout, double *xout, Scal *sc, double *partials,
-                            hipStream_t st) {
-    HF_LAUNCH((k_rhs_res<float, true, true, true>), g, st, g, rp, tp, tpp, nullptr, nullptr, rhoc,
-              cvec, qd, bvec, d, cx, cy, cz, rhat, sout, xout, sc, partials);
-}
-
-void launch_rhs_res_solve64(const Dims &g, const RhsParams &rp, const double *x,
-                            const double *b, const double *d, const double *cx,
-                            const double *cy, const double *cz, double *rhat, Scal *sc,
-                            double *partials, hipStream_t st) {
-    HF_LAUNCH((k_rhs_res<double, false, false, false>), g, st, g, rp, nullptr, nullptr, x, b,
-              nullptr, nullptr, nullptr, nullptr, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
-              partials);
-}
-
-template <typename T>
-void launch_cg_a(const Dims &g, const T *r, const T *pold, T *pnew, T *q, const T *hx,
-                 const T *hy, const T *hz, Scal *sc, double *partials, hipStream_t st) {
-    HF_LAUNCH(k_cg_a<T>, g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
-}
 template <typename T>
 void launch_cg_b(const Dims &g, T *y, const T *p, T *r, const T *q, Scal *sc, double *partials,
                  hipStream_t st) {
     HF_LAUNCH(k_cg_b<T>, g, st, g, y, p, r, q, sc, partials);
 }
-template void launch_cg_a<double>(const Dims &, const double *, const double *, double *,
-                                  double *, const double *, const double *, const double *,
-                                  Scal *, double *, hipStream_t);
-template void launch_cg_a<float>(const Dims &, const float *, const float *, float *, float *,
-                                 const float *, const float *, const float *, Scal *, double *,
-                                 hipStream_t);
 template void launch_cg_b<double>(const Dims &, double *, const double *, double *,
                                   const double *, Scal *, double *, hipStream_t);
 template void launch_cg_b<float>(const Dims &, float *, const float *, float *, const float *,
@@ -728,26 +523,21 @@ void launch_finalize(const Dims &g, bool from_state, const double *tp, double *t
         HF_LAUNCH(k_finalize<false>, g, st, g, tp, tpp_io, sv, y, sc);
 }
 
-void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const float *y,
-                   const double *sv, const double *S, const double *d, const double *cx,
-                   const double *cy, const double *cz, const double *tp, double *xout,
-                   float *rhat, Scal *sc, double *partials, hipStream_t st) {
-    HF_LAUNCH(k_ir_res, g, st, g, ip, x, y, sv, S, d, cx, cy, cz, tp, xout, rhat, sc, partials);
-}
 
+// the stencil kernels (k_rhs_res_z, k_cg_a_z, k_ir_res_z) run g.zblk blocks, the flat ones g.nblk
 void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double tol, int64_t itmax,
                       hipStream_t st) {
-    HF_FIN(k_fin_start, g, st, partials, sc, tol, itmax);
+    HF_FIN(k_fin_start, g.zblk, st, partials, sc, tol, itmax);
 }
 void launch_fin_a(const Dims &g, const double *partials, Scal *sc, hipStream_t st) {
-    HF_FIN(k_fin_a, g, st, partials, sc);
+    HF_FIN(k_fin_a, g.zblk, st, partials, sc);
 }
 void launch_fin_b(const Dims &g, const double *partials, Scal *sc, hipStream_t st) {
-    HF_FIN(k_fin_b, g, st, partials, sc);
+    HF_FIN(k_fin_b, g.nblk, st, partials, sc);
 }
 void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                    hipStream_t st) {
-    HF_FIN(k_fin_ir, g, st, partials, sc, sweeps_max);
+    HF_FIN(k_fin_ir, g.zblk, st, partials, sc, sweeps_max);
 }
 
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,

@@ -73,7 +73,9 @@ typedef struct hf_params {
     /* solver controls (the reference hard-codes itol=1, tol=1e-32, itmax=50000 at
      * src/heatflow/mod_evolve.f90:142-144) */
     double tol;                /* relative residual tolerance of the (outer) solve      */
-    int64_t itmax;             /* iteration cap of the (outer) solve                    */
+    int64_t itmax;             /* iteration cap of the (outer) solve; in mixed mode the
+                                  refinement sweeps are capped at min(itmax, 64) and stop
+                                  when a sweep no longer halves the FP64 residual        */
     int32_t precision;         /* HF_PREC_*                                             */
     int32_t inner_itmax;       /* FP32 inner CG iteration cap per refinement sweep      */
     double inner_tol;          /* FP32 inner CG relative tolerance per sweep            */

@@ -179,6 +179,33 @@ class OracleSystem:
 # ---- the reference build (Fortran modules + ref_harness.f90) ------------------------------
 
 _rlib = None
+# simulate() and linbcg() keep their N-vectors as automatic arrays on the stack
+# (src/heatflow/mod_evolve.f90:47, src/heatflow/mod_Sparse.f90:384): every call into the
+# reference build runs on a thread with a large stack.
+REF_STACK = 2 << 30
+
+
+def _big_stack(fn, *args):
+    import threading
+    out, err = {}, {}
+
+    def target():
+        try:
+            out["v"] = fn(*args)
+        except BaseException as e:  # re-raised in the caller
+            err["e"] = e
+
+    old = threading.stack_size()
+    threading.stack_size(REF_STACK)
+    try:
+        t = threading.Thread(target=target)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    if "e" in err:
+        raise err["e"]
+    return out.get("v")
 
 
 def ref_available() -> bool:
@@ -209,7 +236,7 @@ class RefSystem:
         self.keep = [ip, dp, f(c.kappa), f(c.rho), f(c.heat_capacity), f(c.tau), f(c.em),
                      f(c.len_x), f(c.len_y), f(c.len_z),
                      np.ascontiguousarray(c.iheater, np.int64)]
-        rlib().hfr_setup(*[_p(a) for a in self.keep])
+        _big_stack(rlib().hfr_setup, *[_p(a) for a in self.keep])
 
     def coo(self):
         n, L = C.c_int64(), C.c_int64()
@@ -232,35 +259,36 @@ class RefSystem:
         return a, b
 
     def simulate(self, itime: int):
-        rlib().hfr_simulate(C.c_int64(itime))
+        _big_stack(rlib().hfr_simulate, C.c_int64(itime))
 
     def simulate_qextra(self, itime: int, qextra, tol: float = 1e-32) -> int:
         q = np.ascontiguousarray(qextra, np.float64)
         it = C.c_int64()
-        rlib().hfr_simulate_qextra(C.c_int64(itime), _p(q), C.c_double(tol), C.byref(it))
+        _big_stack(rlib().hfr_simulate_qextra, C.c_int64(itime), _p(q), C.c_double(tol),
+                   C.byref(it))
         return it.value
 
     def boundary(self):
         b = np.empty(self.n)
-        rlib().hfr_boundary(_p(b))
+        _big_stack(rlib().hfr_boundary, _p(b))
         return b
 
     def heater(self, itime: int):
         q = np.empty(self.n)
         qd = np.empty(self.n)
         hv = C.c_double()
-        rlib().hfr_heater(C.c_int64(itime), _p(q), _p(qd), C.byref(hv))
+        _big_stack(rlib().hfr_heater, C.c_int64(itime), _p(q), _p(qd), C.byref(hv))
         return q, qd, hv.value
 
     def scat(self):
         s = np.empty(self.n)
-        rlib().hfr_scat(_p(s))
+        _big_stack(rlib().hfr_scat, _p(s))
         return s
 
     def linbcg(self, b, x, tol=1e-32, itmax=50000, itol=1):
         bb = np.ascontiguousarray(b, np.float64)
         xx = np.array(x, np.float64)
         it, err = C.c_int64(), C.c_double()
-        rlib().hfr_linbcg(_p(bb), _p(xx), C.c_int64(itol), C.c_double(tol), C.c_int64(itmax),
-                          C.byref(it), C.byref(err))
+        _big_stack(rlib().hfr_linbcg, _p(bb), _p(xx), C.c_int64(itol), C.c_double(tol),
+                   C.c_int64(itmax), C.byref(it), C.byref(err))
         return xx, it.value, err.value

@@ -26,7 +26,7 @@ def test_adjoint_matches_dense_autograd(icat):
     # targets from the ground-truth kappa* (dense forward)
     with torch.no_grad():
         tstar = dense_run(s, torch.as_tensor(m["kstar"]), torch.as_tensor(m["power"]), steps)
-    targets = tstar.numpy()[m["probes"]]
+    targets = tstar.numpy()[m["probes"]] - 0.01   # a well-conditioned loss: T - t ~ 1e-2
     k = torch.tensor(s.cells.kappa, requires_grad=True)
     pw = torch.tensor(m["power"], requires_grad=True)
     T = dense_run(s, k, pw, steps)
@@ -35,8 +35,9 @@ def test_adjoint_matches_dense_autograd(icat):
     with Simulation(s, tol=1e-32) as sim:
         sim.initial_evolve()
         L, gk, gp = sim.grad(1, steps, ckpt, m["probes"], targets)
-    assert loss.item() > 1e-6
-    assert abs(L - loss.item()) <= 1e-9 * abs(loss.item())
+    assert loss.item() > 1e-4
+    # T - t ~ 1e-2: the loss carries the temperatures' round-off (~1e-14 relative) x 1e4
+    assert abs(L - loss.item()) <= 1e-7 * abs(loss.item())
     gk_ref = k.grad.numpy()
     gp_ref = pw.grad.numpy()
     scale = np.abs(gk_ref).max()
@@ -73,8 +74,11 @@ def test_autograd_function_on_device():
 
 @pytest.mark.parametrize("icat", [0, 1])
 def test_adjoint_central_fd(icat):
-    """Central differences (relative step 1e-6) of the GPU forward on 16 seeded cells and on
-    the heater powers, 32^3, 40 steps."""
+    """Central differences (relative step 1e-6) of the GPU forward, 32^3, 40 steps, on 16
+    seeded cells next to the heaters and on 4 heater powers.  A central difference of L carries
+    the forward's round-off: |dL_noise| <= 2 sum|T - t| * dT with dT ~ 1e-13 * 300 K (the
+    FP64 solve converges to ~1e-16 relative; 1e-13 leaves margin), so the bar for each cell is
+    1e-6 relative plus that floor divided by 2h."""
     s, m = workloads.c4(32, icattaneo=icat, nprobe=64)
     steps, ckpt = 40, 10
     probes = m["probes"]
@@ -83,12 +87,11 @@ def test_adjoint_central_fd(icat):
         sim.initial_evolve()
         sim.run(1, steps)
         T0, _ = sim.get_state()
-        targets = T0[probes] + np.random.default_rng(5).normal(0, 1e-2, len(probes))
+        targets = T0[probes] - 0.01
         sim.initial_evolve()
         L, gk, gp = sim.grad(1, steps, ckpt, probes, targets)
-        # FD cells: seeded, from the heaters' neighbourhoods where the gradient is not negligible
-        rng = np.random.default_rng(43)
-        cells = rng.choice(m["near"], 16, replace=False)
+        near1 = workloads.near_cells(32, m["heaters"], 1)
+        cells = np.random.default_rng(43).choice(near1, 16, replace=False)
         kap = s.cells.kappa.copy()
         vol = s.cells.volume
 
@@ -98,27 +101,24 @@ def test_adjoint_central_fd(icat):
             sim.initial_evolve()
             sim.run(1, steps)
             T, _ = sim.get_state()
-            return _loss_np(T, probes, targets)
+            return _loss_np(T, probes, targets), T
 
-        fd = []
+        L0, T0b = loss_at(kap, m["power"])
+        assert abs(L0 - L) <= 1e-12 * L
+        noise = 2.0 * np.sum(np.abs(T0b[probes] - targets)) * 1e-13 * 300.0
         for c in cells:
             h = 1e-6 * kap[c]
             kp, km = kap.copy(), kap.copy()
             kp[c] += h
             km[c] -= h
-            fd.append((loss_at(kp, m["power"]) - loss_at(km, m["power"])) / (2 * h))
-        fd = np.array(fd)
-        ad = gk[cells]
-        scale = np.abs(gk).max()
-        err = np.abs(fd - ad)
-        assert np.all(err <= 1e-4 * np.abs(ad) + 1e-6 * scale), (fd, ad)
-        # heater powers
+            fd = (loss_at(kp, m["power"])[0] - loss_at(km, m["power"])[0]) / (2 * h)
+            assert abs(fd - gk[c]) <= 1e-6 * abs(gk[c]) + noise / (2 * h), (c, fd, gk[c])
         for hc in m["heaters"][:4]:
             hp = 1e-6 * m["power"][hc]
             pp, pm = m["power"].copy(), m["power"].copy()
             pp[hc] += hp
             pm[hc] -= hp
-            fdp = (loss_at(kap, pp) - loss_at(kap, pm)) / (2 * hp)
-            assert abs(fdp - gp[hc]) <= 1e-5 * abs(gp[hc]), (fdp, gp[hc])
+            fdp = (loss_at(kap, pp)[0] - loss_at(kap, pm)[0]) / (2 * hp)
+            assert abs(fdp - gp[hc]) <= 1e-6 * abs(gp[hc]) + noise / (2 * hp), (hc, fdp, gp[hc])
     finally:
         sim.close()

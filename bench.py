@@ -80,6 +80,17 @@ def roofline(kt: dict) -> dict:
                 avg_launch_us=round(avg_s * 1e6, 2))
 
 
+def measured_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` at n^3 from the committed rocprofv3 counter summary
+    (FETCH_SIZE / WRITE_SIZE passes, scaled by a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_c3-{n}_fp32-ir.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d["bytes_per_launch"] if d.get("kernel") == kernel and d.get("n") == n else None
+
+
 def rel_errors(a, ref):
     e = np.abs(a - ref) / np.abs(ref)
     return float(e.max()), float(np.sqrt(np.mean(e * e)))
@@ -233,6 +244,7 @@ def main():
     grad = None if (args.no_grad or args.only) else grad_bench(args.grad_n, 200, 20)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, args.tol, args.cpu_n,
                                                                 args.cpu_steps)
+    head["roofline"]["traffic"] = measured_traffic(head["roofline"]["kernel"], args.n)
     line = {
         "metric": "timesteps/s at 512^3 (FP32-refined vs FP64, 1e-10 rel); fwd+adjoint grad time",
         "value": round(head["timesteps_per_s"], 4),

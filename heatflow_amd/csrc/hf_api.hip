@@ -111,11 +111,24 @@ static void make_dims(Dims &g, const hf_params &p) {
     g.tpb = (uint32_t)((g.ntiles + nb - 1) / nb);
     nb = (g.ntiles + g.tpb - 1) / g.tpb;
     g.nblk = (uint32_t)((nb + 7) / 8 * 8);
-    // z-marching tiles: enough z-chunks for ~`zcap` blocks (HF_ZBLK overrides)
-    uint32_t zcap = 8192;
+    // z-marching tiles: enough z-chunks for ~`zcap` blocks (HF_ZBLK overrides); 128 x 4 tiles
+    // with ~8M threads per launch measured best at 512^3 (64 planes per chunk)
+    uint32_t zcap = 16384;
     if (const char *e = getenv("HF_ZBLK")) zcap = std::max(8u, (uint32_t)atoi(e));
-    g.ztx = (g.nx + kTileX - 1) / kTileX;
-    g.zty = (g.ny + kTileY - 1) / kTileY;
+    // tile shape: tile_x (power of two) by tile_y threads; HF_TILE="tx,ty" overrides
+    uint32_t tx = 128, ty = 4;
+    if (const char *e = getenv("HF_TILE")) {
+        unsigned a = 0, b = 0;
+        if (sscanf(e, "%u,%u", &a, &b) == 2 && a >= 1 && b >= 1 && a * b <= 1024 && a * b >= 64 &&
+            (a & (a - 1)) == 0)
+            tx = a, ty = b;
+    }
+    g.tile_x = tx;
+    g.tile_y = ty;
+    g.tile_xs = 0;
+    while ((1u << g.tile_xs) < tx) ++g.tile_xs;
+    g.ztx = (g.nx + tx - 1) / tx;
+    g.zty = (g.ny + ty - 1) / ty;
     const uint64_t per_plane = (uint64_t)g.ztx * g.zty;
     uint64_t nzc = std::max<uint64_t>(1, (zcap + per_plane - 1) / per_plane);
     nzc = std::min<uint64_t>(nzc, g.nz);

@@ -15,8 +15,7 @@ namespace hf {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kTileX = 64;               // z-marching tile: 64 x 4 columns per block
-constexpr int kTileY = kBlock / kTileX;
+constexpr int kMaxWaves = 16;            // largest block: 1024 threads (z-marching kernels)
 constexpr double kTiny = 1.0e-12;        // constants::TINY, src/heatflow/mod_constants.f90:17
 constexpr double kStefBoltz = 5.670373e-8;  // src/heatflow/mod_constants.f90:16
 
@@ -42,8 +41,10 @@ struct Dims {
     uint32_t tpb;        // tiles per block (chunked layout)
     uint32_t nblk;       // launch grid (multiple of 8)
     int32_t layout;      // LAYOUT_* bits
-    // z-marching stencil kernels: ztx x zty tiles per plane, z-chunks of `zchunk` planes,
-    // ztiles = ztx * zty * (number of chunks), zblk = ztiles rounded up to a multiple of 8
+    // z-marching stencil kernels: blocks of tile_x * tile_y threads (tile_x = 1 << tile_xs),
+    // ztx x zty tiles per plane, z-chunks of `zchunk` planes, ztiles = ztx * zty * chunks,
+    // zblk = ztiles rounded up to a multiple of 8
+    uint32_t tile_x, tile_y, tile_xs;
     uint32_t ztx, zty, zchunk, ztiles, zblk;
     int32_t per[3];      // periodic axes (length >= 3)
     FastDiv dnx, dny;
@@ -140,6 +141,18 @@ __device__ __forceinline__ Nbr neighbours(const Dims &g, uint32_t I) {
     return n;
 }
 
+// Streaming accesses: arrays a kernel touches only at its own cell are loaded and stored with
+// the non-temporal hint, so they do not displace the stencil operands (whose neighbours are
+// re-read) from L2.
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ void stnt(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
 // ---- deterministic reductions ------------------------------------------------------------
 
 template <int K>
@@ -151,22 +164,22 @@ __device__ __forceinline__ void wave_sum(double (&v)[K]) {
     }
 }
 
-// Block sum; the result is valid in every thread.
+// Block sum over blockDim.x threads (<= 1024); the result is valid in every thread.
 template <int K>
-__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*kWaves */) {
+__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*kMaxWaves */) {
     wave_sum<K>(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
     __syncthreads();
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) lds[k * kWaves + w] = v[k];
+        for (int k = 0; k < K; ++k) lds[k * kMaxWaves + w] = v[k];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double s = 0.0;
-#pragma unroll
-        for (int i = 0; i < kWaves; ++i) s += lds[k * kWaves + i];
+        for (int i = 0; i < nw; ++i) s += lds[k * kMaxWaves + i];
         v[k] = s;
     }
 }
@@ -177,7 +190,7 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K*kWave
 // the solver scalars -- no device-scope fences or contended atomics inside the wide kernel.
 template <int K>
 __device__ __forceinline__ void block_partials(double (&v)[K], double *partials) {
-    __shared__ double lds[K * kWaves];
+    __shared__ double lds[K * kMaxWaves];
     block_sum<K>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -189,7 +202,7 @@ __device__ __forceinline__ void block_partials(double (&v)[K], double *partials)
 template <int K>
 __device__ __forceinline__ void sum_partials(const double *partials, uint32_t nb,
                                              double (&out)[K]) {
-    __shared__ double lds[K * kWaves];
+    __shared__ double lds[K * kMaxWaves];
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;

@@ -348,10 +348,11 @@ k_cg_b(Dims g, T *__restrict__ y, const T *__restrict__ p, T *__restrict__ r,
     const T alpha = (T)sc->alpha;
     double acc[1] = {0.0};
     HF_FOR_CELLS(g, I) {
-        const T yi = first ? alpha * p[I] : y[I] + alpha * p[I];
-        const T ri = r[I] - alpha * q[I];
-        y[I] = yi;
-        r[I] = ri;
+        const T pi = ldnt(&p[I]);
+        const T yi = first ? alpha * pi : ldnt(&y[I]) + alpha * pi;
+        const T ri = ldnt(&r[I]) - alpha * ldnt(&q[I]);
+        stnt(&y[I], yi);
+        stnt(&r[I], ri);
         acc[0] += (double)ri * (double)ri;
     }
     block_partials<1>(acc, partials);
@@ -384,17 +385,18 @@ k_finalize(Dims g, const double *__restrict__ tp, double *__restrict__ tpp_io,
     const bool have_y = sc->iter > 0;
     int32_t fl = 0;
     HF_FOR_CELLS(g, I) {
-        double x0;
+        double x0, tpi = 0.0;
         if (FromState) {
-            x0 = tp[I] + (tp[I] - tpp_io[I]);
+            tpi = ldnt(&tp[I]);
+            x0 = tpi + (tpi - ldnt(&tpp_io[I]));
             if (tiny) x0 = x0 + kTiny;
         } else {
-            x0 = tpp_io[I];
+            x0 = ldnt(&tpp_io[I]);
         }
-        const double x = have_y ? x0 + sv[I] * y[I] : x0;
-        tpp_io[I] = x;
+        const double x = have_y ? x0 + ldnt(&sv[I]) * ldnt(&y[I]) : x0;
+        stnt(&tpp_io[I], x);
         if (isnan(x)) fl |= FLAG_NAN_X;
-        if (FromState && (x + (x - tp[I])) - x < kTiny) fl |= FLAG_TINY_NEXT;
+        if (FromState && (x + (x - tpi)) - x < kTiny) fl |= FLAG_TINY_NEXT;
     }
     block_flags(fl, &sc->flags);
 }

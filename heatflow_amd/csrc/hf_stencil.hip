@@ -1,6 +1,6 @@
 // hf_stencil.hip -- the 7-point stencil kernels of the time step, z-marching.
 //
-// A block is a 64 x 4 (x by y) tile of columns; each thread walks its column through a chunk
+// A block is a tile_x x tile_y (x by y) tile of columns; each thread walks its column through a chunk
 // of z-planes and keeps the stencil operand of planes z-1, z, z+1 -- and the z coupling of
 // plane z-1 -- in registers, so the z neighbours are never re-read; the x and y neighbours of
 // plane z come from the same tile rows (L1) or the adjacent tile (same XCD, L2).  Tiles are
@@ -35,8 +35,8 @@ __device__ __forceinline__ ZCol zcolumn(const Dims &g) {
     const uint32_t zc = lt / per_plane;
     const uint32_t rem = lt - zc * per_plane;
     const uint32_t by = rem / g.ztx, bx = rem - by * g.ztx;
-    c.x = bx * kTileX + (threadIdx.x % kTileX);
-    c.y = by * kTileY + (threadIdx.x / kTileX);
+    c.x = bx * g.tile_x + (threadIdx.x & (g.tile_x - 1));
+    c.y = by * g.tile_y + (threadIdx.x >> g.tile_xs);
     c.z0 = zc * g.zchunk;
     c.z1 = c.z0 + g.zchunk < g.nz ? c.z0 + g.zchunk : g.nz;
     c.active = lt < g.ztiles && c.x < g.nx && c.y < g.ny && c.z0 < g.nz;
@@ -60,28 +60,26 @@ __device__ __forceinline__ ZCol zcolumn(const Dims &g) {
 // the z neighbours' operands and the z- coupling carried in registers.
 template <class XF>
 __device__ __forceinline__ double hx_sum(const Dims &g, const ZCol &c, uint32_t I, uint32_t z,
-                                         const double *__restrict__ d,
-                                         const double *__restrict__ cx,
-                                         const double *__restrict__ cy,
-                                         const double *__restrict__ cz, double xi, double xz_m,
-                                         double xz_p, double cz_m, XF X) {
+                                         double di, const double *__restrict__ cx,
+                                         const double *__restrict__ cy, double czc, double xi,
+                                         double xz_m, double xz_p, double cz_m, XF X) {
     const bool hzm = z > 0 || g.per[2], hzp = z + 1 < g.nz || g.per[2];
     const bool wzm = z == 0, wzp = z + 1 == g.nz;
     double h = 0.0;
     if (hzm && !wzm) h = h + cz_m * xz_m;
     if (c.hym && !c.wym) h = h + cy[I + c.oym] * X(I + c.oym);
     if (c.hxm && !c.wxm) h = h + cx[I + c.oxm] * X(I + c.oxm);
-    h = h + d[I] * xi;
+    h = h + di * xi;
     if (c.hxp && !c.wxp) h = h + cx[I] * X(I + c.oxp);
     if (c.hyp && !c.wyp) h = h + cy[I] * X(I + c.oyp);
-    if (hzp && !wzp) h = h + cz[I] * xz_p;
+    if (hzp && !wzp) h = h + czc * xz_p;
     // periodic wrap couplings
     if (c.hxm && c.wxm) h = h + cx[I + c.oxm] * X(I + c.oxm);
     if (c.hxp && c.wxp) h = h + cx[I] * X(I + c.oxp);
     if (c.hym && c.wym) h = h + cy[I + c.oym] * X(I + c.oym);
     if (c.hyp && c.wyp) h = h + cy[I] * X(I + c.oyp);
     if (hzm && wzm) h = h + cz_m * xz_m;
-    if (hzp && wzp) h = h + cz[I] * xz_p;
+    if (hzp && wzp) h = h + czc * xz_p;
     return h;
 }
 
@@ -94,7 +92,7 @@ __device__ __forceinline__ uint32_t zp_index(const Dims &g, uint32_t col, uint32
 }
 
 template <typename OutT, bool FromState, bool StoreS, bool StoreX>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(1024)
 k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *__restrict__ tpp,
             const double *__restrict__ xin, const double *__restrict__ bin,
             const double *__restrict__ rhoc, const double *__restrict__ cvec,
@@ -134,27 +132,28 @@ k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *_
             if (FromState) {
                 // mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36
                 if (rp.isteady == 0) {
-                    S = -(rp.inverse_time * tp[I] * rhoc[I]) - qd[I] - bvec[I];
+                    S = -(rp.inverse_time * tp[I] * ldnt(&rhoc[I])) - ldnt(&qd[I]) - ldnt(&bvec[I]);
                     if (rp.icattaneo == 1) {
-                        double scat = (tpp[I] - 2.0 * tp[I]) * cvec[I];
+                        double scat = (tpp[I] - 2.0 * tp[I]) * ldnt(&cvec[I]);
                         if (isnan(scat)) fl |= FLAG_NAN_SCAT;
                         S = S + scat;
                     }
                 } else {
-                    S = -qd[I] - bvec[I];
+                    S = -ldnt(&qd[I]) - ldnt(&bvec[I]);
                 }
                 if (isnan(S)) fl |= FLAG_NAN_S;
             } else {
-                S = bin[I];
+                S = ldnt(&bin[I]);
             }
-            const double hxv = hx_sum(g, c, I, z, d, cx, cy, cz, xi, xz_m, xz_p, cz_m, X);
-            const double czc = cz[I];
+            const double di = ldnt(&d[I]);
+            const double czc = ldnt(&cz[I]);
+            const double hxv = hx_sum(g, c, I, z, di, cx, cy, czc, xi, xz_m, xz_p, cz_m, X);
             const double r = S - hxv;
-            const double s = 1.0 / sqrt(-d[I]);
+            const double s = 1.0 / sqrt(-di);
             const double rh = -s * r;
-            rhat[I] = (OutT)rh;
-            if (StoreS) sout[I] = S;
-            if (StoreX) xout[I] = xi;
+            stnt(&rhat[I], (OutT)rh);
+            if (StoreS) stnt(&sout[I], S);
+            if (StoreX) stnt(&xout[I], xi);
             const double bh = s * S;
             const double rq = (double)(OutT)rh;   // r.r of the stored (possibly FP32) residual
             acc[0] += bh * bh;
@@ -170,7 +169,7 @@ k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *_
 
 // p = r + beta * p_old (p = r on the first iteration), q = A^ p, partial p.q.
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(1024)
 k_cg_a_z(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
          T *__restrict__ q, const T *__restrict__ hx, const T *__restrict__ hy,
          const T *__restrict__ hz, Scal *sc, double *partials) {
@@ -193,7 +192,7 @@ k_cg_a_z(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restr
         for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
             const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
             const T pz_p = hzp ? P(zp_index(g, c.col, z)) : (T)0;
-            const T hzc = hz[I];
+            const T hzc = ldnt(&hz[I]);
             T s = (T)0;
             if (hzm) s = s + hz_m * pz_m;
             if (c.hym) s = s + hy[I + c.oym] * P(I + c.oym);
@@ -202,8 +201,8 @@ k_cg_a_z(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restr
             if (c.hyp) s = s + hy[I] * P(I + c.oyp);
             if (hzp) s = s + hzc * pz_p;
             const T qi = pi - s;
-            pnew[I] = pi;
-            q[I] = qi;
+            stnt(&pnew[I], pi);
+            stnt(&q[I], qi);
             acc[0] += (double)pi * (double)qi;
             pz_m = pi;
             pi = pz_p;
@@ -215,7 +214,7 @@ k_cg_a_z(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restr
 
 // Mixed-precision refinement: x' = x + s y (y FP32) formed on the fly, r^ = -s (S - H x')
 // stored as FP32 for the next inner solve, ||r^||^2, NaN and next-step TINY tests on x'.
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(1024)
 k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__restrict__ y,
            const double *__restrict__ sv, const double *__restrict__ S,
            const double *__restrict__ d, const double *__restrict__ cx,
@@ -239,15 +238,15 @@ k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__res
         for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
             const bool hzp = z + 1 < g.nz || per_z;
             const double xz_p = hzp ? X(zp_index(g, c.col, z)) : 0.0;
-            const double hxv = hx_sum(g, c, I, z, d, cx, cy, cz, xi, xz_m, xz_p, cz_m, X);
-            const double czc = cz[I];
-            const double rh = -sv[I] * (S[I] - hxv);
+            const double czc = ldnt(&cz[I]);
+            const double hxv = hx_sum(g, c, I, z, ldnt(&d[I]), cx, cy, czc, xi, xz_m, xz_p, cz_m, X);
+            const double rh = -sv[I] * (ldnt(&S[I]) - hxv);
             const float rf = (float)rh;
-            rhat[I] = rf;
-            xout[I] = xi;
+            stnt(&rhat[I], rf);
+            stnt(&xout[I], xi);
             acc[0] += (double)rf * (double)rf;
             if (isnan(xi)) fl |= FLAG_NAN_X;
-            if (ip.from_state && (xi + (xi - tp[I])) - xi < kTiny) fl |= FLAG_TINY_NEXT;
+            if (ip.from_state && (xi + (xi - ldnt(&tp[I]))) - xi < kTiny) fl |= FLAG_TINY_NEXT;
             xz_m = xi;
             xi = xz_p;
             cz_m = czc;
@@ -260,7 +259,7 @@ k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__res
 // ---- launchers -----------------------------------------------------------------------------
 
 #define HF_ZLAUNCH(kern, g, st, ...) \
-    hipLaunchKernelGGL(kern, dim3((g).zblk), dim3(kBlock), 0, st, __VA_ARGS__)
+    hipLaunchKernelGGL(kern, dim3((g).zblk), dim3((g).tile_x * (g).tile_y), 0, st, __VA_ARGS__)
 
 void launch_rhs_res_state64(const Dims &g, const RhsParams &rp, const double *tp,
                             const double *tpp, const double *rhoc, const double *cvec,

@@ -1,0 +1,110 @@
+"""The drop-in: HeatFlow's own Fortran program with module `evolution` replaced by
+heatflow_amd/fortran/hf_evolution.f90 (the time step on the GPU through the C-ABI), run on the
+reference's input files, against the golden TempDis.dat and against the reference program built
+from its sources (oracle/_ref/heatflow_ref) on a Cattaneo case with a heater."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from _cases import read_tempdis
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DROPIN = os.path.join(ROOT, "heatflow_amd", "fortran", "_build", "heatflow_hip")
+REFEXE = os.path.join(ROOT, "oracle", "_ref", "heatflow_ref")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(DROPIN),
+                                 reason="drop-in program not built (needs the HeatFlow sources)")]
+
+# output switches that avoid the reference driver's data_write log unit (a non-SAVEd local,
+# src/heatflow/mod_output.f90:56, 91-111) -- they only affect logging, not the time step
+QUIET = {"IVERB": "0", "_WriteToTxt": ".False.", "_Test_Run": ".True.",
+         "_Percentage_Completion": ".False."}
+
+
+def write_param(src_lines, path, overrides):
+    seen = set()
+    out = []
+    for ln in src_lines:
+        key = ln.split("!", 1)[0].split("=", 1)[0].strip()
+        if "=" in ln and key in overrides:
+            out.append(f"    {key} = {overrides[key]}")
+            seen.add(key)
+        else:
+            out.append(ln.rstrip("\n"))
+    for k, v in overrides.items():
+        if k not in seen:
+            out.append(f"    {k} = {v}")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def run_program(exe, workdir):
+    os.makedirs(os.path.join(workdir, "outputs"), exist_ok=True)
+    p = subprocess.run([exe], cwd=workdir, capture_output=True, text=True, timeout=240)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    return read_tempdis(os.path.join(workdir, "outputs", "TempDis.dat"))
+
+
+def test_dropin_golden_tempdis(tmp_path):
+    wd = tmp_path / "run"
+    (wd / "inputs").mkdir(parents=True)
+    shutil.copy(os.path.join(GOLDEN, "mat.in"), wd / "inputs")
+    shutil.copy(os.path.join(GOLDEN, "system.in"), wd / "inputs")
+    with open(os.path.join(GOLDEN, "param.in")) as f:
+        write_param(f.readlines(), wd / "inputs" / "param.in", QUIET)
+    T = run_program(DROPIN, str(wd))
+    gold = read_tempdis(os.path.join(GOLDEN, "TempDis.dat"))
+    assert T.shape == gold.shape
+    assert np.abs(T - gold).max() <= 5e-7 + 1e-12
+
+
+def write_c1_inputs(d, n=16, ntime=100):
+    """16^3 Cattaneo, kappa=150, rho=2330, c_p=700, tau=1e-12, 1 um cells, x faces 300 K
+    (kappaBound 150), y/z insulating, one heater 1e-4 W at the centre, dt=1e-10 s -- in the
+    reference's input format (param.in / mat.in / system.in)."""
+    os.makedirs(d, exist_ok=True)
+    params = {
+        "IVERB": "0", "_WriteToTxt": ".False.", "_Test_Run": ".True.",
+        "_Percentage_Completion": ".False.", "_RunName": "c1", "ntime": str(ntime),
+        "write_every": "10", "heattime": "0", "time_step": "1.0e-10", "freq": "0",
+        "iboundary": "1", "icattaneo": "1", "isteady": "0", "T_System": "300", "T_Bath": "300",
+        "T_Bathx1": "300", "T_Bathx2": "300", "T_Bathy1": "300", "T_Bathy2": "300",
+        "T_Bathz1": "300", "T_Bathz2": "300", "power_in": "1.0e-4",
+        "kappaBoundx1": "150", "kappaBoundNx": "150", "kappaBoundy1": "0", "kappaBoundNy": "0",
+        "kappaBoundz1": "0", "kappaBoundNz": "0", "TempDepProp": "0", "Periodic": "''",
+        "start_ix": "1", "end_ix": str(n), "start_iy": "1", "end_iy": str(n), "start_iz": "1",
+        "end_iz": str(n), "BR": "1", "T_BathCG": "0", "CG_dir": "none", "kappaBound": "0",
+        "_Check_Sparse_Full": ".False.", "_Check_Stability": ".False.",
+        "_InputTempDis": ".False.", "_FullRestart": ".False.", "T_BathCC": ".False.",
+    }
+    with open(os.path.join(d, "param.in"), "w") as f:
+        for k, v in params.items():
+            f.write(f"    {k} = {v}\n")
+    with open(os.path.join(d, "mat.in"), "w") as f:
+        f.write("1\nheat_capacity = 700\nh_conv = 0\nkappa = 150\nkappa3D = 150\n"
+                "rho = 2330\nsound_speed = 8430\ntau = 1e-12\nem = 0\n0\n")
+    L = n * 1e-6
+    c = n // 2
+    with open(os.path.join(d, "system.in"), "w") as f:
+        f.write(f"{n} {n} {n}\n{L} {L} {L}\n")
+        for iz in range(n):
+            f.write("\n")
+            for iy in range(n):
+                toks = ["1:1" if (ix, iy, iz) == (c, c, c) else "1:0" for ix in range(n)]
+                f.write(" ".join(toks) + "\n")
+
+
+@pytest.mark.skipif(not os.path.exists(REFEXE), reason="reference program not built")
+def test_dropin_matches_reference_program_c1(tmp_path):
+    for name in ("ref", "hip"):
+        write_c1_inputs(str(tmp_path / name / "inputs"))
+    Tref = run_program(REFEXE, str(tmp_path / "ref"))
+    Thip = run_program(DROPIN, str(tmp_path / "hip"))
+    assert Tref.max() > 300.001          # the heater heated
+    # both print ES16.8 (9 significant digits): agreement to the print resolution
+    assert np.abs(Thip - Tref).max() <= 1e-6

@@ -131,24 +131,193 @@ def cpu_baseline(n_full: int, tol: float, sample_n: int, sample_steps: int) -> d
                 sample_seconds=round(dt, 3))
 
 
-def grad_bench(n: int, steps: int, ckpt: int) -> dict:
+def grad_bench(n: int, steps: int, ckpt: int, fd_cells: int = 16) -> dict:
+    """Forward+adjoint gradient of the 128^3 / 200-step workload (Fourier and Cattaneo, FP64):
+    loss = sum over the probes of (T - T*)^2 at the last step, T* from the same run with the
+    seeded ground-truth conductivity kappa*.  Timed: one hf_grad call (forward with checkpoints
+    every `ckpt` steps, reverse sweep with recomputation) on the 300 K system.
+
+    Checked against central finite differences (relative step 1e-6) of the GPU forward:
+      - kappa on the `fd_cells` cells with the largest |dL/dkappa|;
+      - kappa along a seeded direction, kappa (1 + eps u), u ~ U(-1, 1), against
+        sum_i g_i kappa_i u_i;
+      - 4 heater powers.
+    T - T* is ~1e-5 K while FP64 resolves T ~ 300 K only to 6e-14 K, so a difference of the
+    loss at 300 K is round-off bound at ~1e-3 relative.  The differences are therefore taken on
+    the same system with temperatures measured from the bath (baths and initial state 0 K
+    instead of 300 K; the operator is linear, so the field is the same minus 300 K), where the
+    loss is resolved to ~1e-17 K.  The 300 K single-cell differences are reported beside them.
+    """
     from heatflow_amd import Simulation, workloads
     out = {}
     for icat, label in ((1, "cattaneo"), (0, "fourier")):
         s, m = workloads.c4(n, icattaneo=icat)
-        sim = Simulation(s, tol=1e-32)
+        s0, _ = workloads.c4(n, icattaneo=icat, t_ref=0.0)
+        kap = s.cells.kappa.copy()
+        vol = s.cells.volume
+        probes = m["probes"]
+        sims = []
+
+        def final_t(sim, kv, pv):
+            sim.set_fields(kv, pv / vol)
+            sim.sparse_Hmatrix()
+            sim.initial_evolve()
+            sim.run(1, steps)
+            return sim.get_state()[0]
+
+        try:
+            sim = Simulation(s, tol=1e-32)
+            sims.append(sim)
+            targets = final_t(sim, m["kstar"], m["power"])[probes]
+            sim.set_fields(kap, m["power"] / vol)
+            sim.sparse_Hmatrix()
+            sim.initial_evolve()
+            sync()
+            t0 = time.perf_counter()
+            loss, gk, gp = sim.grad(1, steps, ckpt, probes, targets)
+            sync()
+            secs = time.perf_counter() - t0
+            res = dict(seconds=round(secs, 3), loss=loss, targets="T(kappa*) at the probes",
+                       grad_kappa_norm=float(np.linalg.norm(gk)),
+                       grad_power_norm=float(np.linalg.norm(gp[m["heaters"]])))
+            if fd_cells:
+                sim0 = Simulation(s0, tol=1e-32)
+                sims.append(sim0)
+                targets0 = final_t(sim0, m["kstar"], m["power"])[probes]
+
+                def loss_at(sm, tg, kv, pv):
+                    return float(np.sum((final_t(sm, kv, pv)[probes] - tg) ** 2))
+
+                def cd(sm, tg, kp, km, pp, pm, h):
+                    return (loss_at(sm, tg, kp, pp) - loss_at(sm, tg, km, pm)) / (2 * h)
+
+                def stats(errs):
+                    return dict(max_rel_err=float(max(errs)),
+                                median_rel_err=float(np.median(errs)))
+
+                cells = np.argsort(-np.abs(gk))[:fd_cells]
+                e0, e300 = [], []
+                for c in cells:
+                    h = 1e-6 * kap[c]
+                    kp, km = kap.copy(), kap.copy()
+                    kp[c] += h
+                    km[c] -= h
+                    pw = m["power"]
+                    e0.append(abs(cd(sim0, targets0, kp, km, pw, pw, h) - gk[c]) / abs(gk[c]))
+                    e300.append(abs(cd(sim, targets, kp, km, pw, pw, h) - gk[c]) / abs(gk[c]))
+                res["fd_kappa_cells"] = {"cells": len(cells), "choice": "largest |dL/dkappa|",
+                                         "rel_step": 1e-6, "bath_origin": stats(e0),
+                                         "at_300K": stats(e300)}
+                u = np.random.default_rng(44).uniform(-1.0, 1.0, kap.size)
+                dirder = float(np.dot(gk, kap * u))
+                eps = 1e-6
+                fd = cd(sim0, targets0, kap * (1 + eps * u), kap * (1 - eps * u),
+                        m["power"], m["power"], eps)
+                res["fd_kappa_directional"] = {"rel_step": eps, "adjoint": dirder, "fd": fd,
+                                               "rel_err": abs(fd - dirder) / abs(dirder)}
+                ep = []
+                for hc in m["heaters"][:4]:
+                    hp = 1e-6 * m["power"][hc]
+                    pp, pm = m["power"].copy(), m["power"].copy()
+                    pp[hc] += hp
+                    pm[hc] -= hp
+                    ep.append(abs(cd(sim0, targets0, kap, kap, pp, pm, hp) - gp[hc]) / abs(gp[hc]))
+                res["fd_power"] = dict(heaters=len(ep), rel_step=1e-6, **stats(ep))
+            out[label] = res
+            log(f"[bench] grad {n}^3 {label}: {res}")
+        finally:
+            for sm in sims:
+                sm.close()
+    return out
+
+
+def grad_c5_bench(n: int, steps: int) -> dict:
+    """The 384^3 layered workload: FP32-inner / FP64-refined forward and adjoint solves with
+    sqrt(steps) checkpoints, against the same gradient in FP64, both stopping at a scaled
+    residual of 1e-13; device memory of each run."""
+    import torch
+    from heatflow_amd import Simulation, workloads
+    s, m = workloads.c5(n)
+    ckpt = int(np.ceil(np.sqrt(steps)))
+    targets = np.full(len(m["probes"]), 300.0)
+    res = {}
+    grads = {}
+    for prec, tol in (("fp32_ir", 1e-13), ("fp64", 1e-13)):
+        torch.cuda.synchronize()
+        free0, total = torch.cuda.mem_get_info()
+        sim = Simulation(s, precision=prec, tol=tol)
         try:
             sim.initial_evolve()
-            targets = np.full(len(m["probes"]), 300.05)
             sync()
             t0 = time.perf_counter()
             loss, gk, gp = sim.grad(1, steps, ckpt, m["probes"], targets)
             sync()
-            out[label] = dict(seconds=round(time.perf_counter() - t0, 3), loss=loss,
-                              grad_kappa_norm=float(np.linalg.norm(gk)),
-                              grad_power_norm=float(np.linalg.norm(gp[m["heaters"]])))
+            secs = time.perf_counter() - t0
+            free1, _ = torch.cuda.mem_get_info()
         finally:
             sim.close()
+        grads[prec] = (gk, gp[m["heaters"]])
+        res[prec] = dict(tol=tol, seconds=round(secs, 3), loss=loss,
+                         device_bytes=int(free0 - free1))
+        log(f"[bench] grad c5 {n}^3 {prec}: {res[prec]}")
+    gk32, gp32 = grads["fp32_ir"]
+    gk64, gp64 = grads["fp64"]
+    res["grad_kappa_err_vs_fp64"] = float(np.abs(gk32 - gk64).max() / np.abs(gk64).max())
+    res["grad_power_err_vs_fp64"] = float(np.abs(gp32 - gp64).max() / np.abs(gp64).max())
+    res.update(cells=s.params.n, steps=steps, checkpoint_every=ckpt,
+               probes=len(m["probes"]), inclusion_fraction=round(m["inclusion_fraction"], 4))
+    return res
+
+
+def tol_sweep(n: int, steps: int) -> dict:
+    """Accuracy against solver tolerance on the layered 384^3 workload: the forward (`steps`
+    steps) and the forward+adjoint gradient in FP64 and FP32-IR at scaled-residual tolerances
+    1e-12 .. 1e-15, each against FP64 at the reference tolerance 1e-32."""
+    from heatflow_amd import Simulation, workloads
+    s, m = workloads.c5(n)
+    ckpt = int(np.ceil(np.sqrt(steps)))
+    targets = np.full(len(m["probes"]), 300.0)
+    h = m["heaters"]
+
+    def fwd(prec, tol):
+        with Simulation(s, precision=prec, tol=tol) as sim:
+            sim.initial_evolve()
+            sync()
+            t0 = time.perf_counter()
+            it = sim.run(1, steps)
+            sync()
+            dt = time.perf_counter() - t0
+            return sim.get_state()[0], dt, it
+
+    def grad(prec, tol):
+        with Simulation(s, precision=prec, tol=tol) as sim:
+            sim.initial_evolve()
+            sync()
+            t0 = time.perf_counter()
+            L, gk, gp = sim.grad(1, steps, ckpt, m["probes"], targets)
+            sync()
+            return L, gk, gp[h], time.perf_counter() - t0
+
+    ref, dt, it = fwd("fp64", 1e-32)
+    out = dict(cells=s.params.n, steps=steps, checkpoint_every=ckpt,
+               forward={"fp64@1e-32": dict(seconds=round(dt, 3), iterations=it)}, gradient={})
+    for prec in ("fp64", "fp32_ir"):
+        for tol in (1e-12, 1e-13, 1e-14, 1e-15):
+            tp, dt, it = fwd(prec, tol)
+            mx, rms = rel_errors(tp, ref)
+            out["forward"][f"{prec}@{tol:g}"] = dict(seconds=round(dt, 3), iterations=it,
+                                                     max_rel_err=mx, rms_rel_err=rms)
+            log(f"[sweep] forward {prec}@{tol:g}: {out['forward'][f'{prec}@{tol:g}']}")
+    Lr, gkr, gpr, dt = grad("fp64", 1e-32)
+    out["gradient"]["fp64@1e-32"] = dict(seconds=round(dt, 3), loss=Lr)
+    for prec in ("fp64", "fp32_ir"):
+        for tol in (1e-13, 1e-14, 1e-15):
+            L, gk, gp, dt = grad(prec, tol)
+            out["gradient"][f"{prec}@{tol:g}"] = dict(
+                seconds=round(dt, 3), loss_rel_err=abs(L - Lr) / abs(Lr),
+                grad_kappa_err=float(np.abs(gk - gkr).max() / np.abs(gkr).max()),
+                grad_power_err=float(np.abs(gp - gpr).max() / np.abs(gpr).max()))
+            log(f"[sweep] gradient {prec}@{tol:g}: {out['gradient'][f'{prec}@{tol:g}']}")
     return out
 
 
@@ -167,6 +336,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--grad-n", type=int, default=128)
+    ap.add_argument("--c5-n", type=int, default=384)
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--tol-sweep", action="store_true",
+                    help="only run the 384^3 accuracy-vs-tolerance sweep and print it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 2 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -180,6 +353,10 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     from heatflow_amd import Simulation, workloads
+
+    if args.tol_sweep:
+        print(json.dumps({"tol_sweep": tol_sweep(args.c5_n, 500)}), flush=True)
+        return
 
     if args.copy_probe:
         # known traffic: 2 GiB read + 2 GiB written per copy
@@ -241,7 +418,8 @@ def main():
 
     head = results.get(f"fp32_ir@{args.tol:g}") or next(iter(results.values()))
     fp64 = results.get(f"fp64@{args.tol:g}")
-    grad = None if (args.no_grad or args.only) else grad_bench(args.grad_n, 200, 20)
+    grad = None if (args.no_grad or args.only) else grad_bench(args.grad_n, 200, 20, fd_cells=12)
+    grad5 = None if (args.no_grad or args.no_c5 or args.only) else grad_c5_bench(args.c5_n, 500)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, args.tol, args.cpu_n,
                                                                 args.cpu_steps)
     head["roofline"]["traffic"] = measured_traffic(head["roofline"]["kernel"], args.n)
@@ -268,6 +446,7 @@ def main():
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "grad_128": grad,
+        "grad_c5_384": grad5,
     }
     print(json.dumps(line), flush=True)
 

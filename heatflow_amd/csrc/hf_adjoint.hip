@@ -182,8 +182,6 @@ static int restore_state(hf_ctx *c, const double *tp, const double *tpp) {
 static int validate(hf_ctx *c) {
     if (!c->assembled || !c->has_state)
         return ctx_set_err(c, HF_ERR_STATE, "adjoint: needs hf_assemble and hf_set_state first");
-    if (c->p.precision != HF_PREC_FP64)
-        return ctx_set_err(c, HF_ERR_STATE, "adjoint: needs an FP64 context");
     const size_t N = c->g.N, B8 = N * 8;
     // per-axis uniform lengths (the reference's system.in gives Lx/nx etc., mod_inputs.f90:619-622)
     std::vector<double> h(N);

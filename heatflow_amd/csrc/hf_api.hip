@@ -857,8 +857,47 @@ int ctx_alloc(hf_ctx *c, double **ptr, size_t n) { return dalloc(c, ptr, n); }
 
 int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itmax,
                   int64_t *iters) {
+    if (c->p.precision == HF_PREC_FP32_IR) {
+        // FP32 inner CG inside FP64 refinement, x refined in place (ping-pong with T[2])
+        RhsParams rp{0.0, tol, itmax, 0, 0};
+        launch_rhs_res_solve32(c->g, rp, x, b, c->d, c->cx, c->cy, c->cz, c->r32, c->sc,
+                               c->partials, c->st);
+        launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
+        TRY(check_launch(c, "k_rhs_res(solve32)"));
+        IrParams ip{std::min<int64_t>(itmax, 64), 0};
+        double *xa = x, *xb = c->T[2];
+        int64_t inner_total = 0;
+        for (;;) {
+            launch_inner_begin(c->sc, c->sci, c->p.inner_tol, tol, c->p.inner_itmax, 0.25, c->st);
+            int cur = 0;
+            const int64_t batch = std::max<int64_t>(1, c->pred_adj);
+            for (int64_t k = 0; k < batch; ++k) {
+                launch_cg_a<float>(c->g, c->r32, c->p32[cur], c->p32[cur ^ 1], c->q32, c->fx,
+                                   c->fy, c->fz, c->sci, c->partials, c->st);
+                launch_fin_a(c->g, c->partials, c->sci, c->st);
+                launch_cg_b<float>(c->g, c->y32, c->p32[cur ^ 1], c->r32, c->q32, c->sci,
+                                   c->partials, c->st);
+                launch_fin_b(c->g, c->partials, c->sci, c->st);
+                cur ^= 1;
+            }
+            launch_ir_res(c->g, ip, xa, c->y32, c->sv, b, c->d, c->cx, c->cy, c->cz, nullptr, xb,
+                          c->r32, c->sc, c->partials, c->st);
+            launch_fin_ir(c->g, c->partials, c->sc, ip.itmax, c->st);
+            TRY(check_launch(c, "k_ir_res(solve)"));
+            std::swap(xa, xb);
+            HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
+            TRY(sync_scal(c, c->sc, c->h_sc));
+            inner_total += c->h_sci->iter;
+            c->pred_adj = c->h_sci->done ? std::max<int64_t>(1, c->h_sci->iter + 1) : batch * 2;
+            if (c->h_sc->done) break;
+        }
+        if (xa != x)
+            HIPCHK(c, hipMemcpyAsync(x, xa, (size_t)c->g.N * 8, hipMemcpyDeviceToDevice, c->st));
+        if (iters) *iters = inner_total;
+        return HF_OK;
+    }
     if (c->p.precision != HF_PREC_FP64 || !c->hx)
-        return set_err(c, HF_ERR_STATE, "adjoint solve needs an FP64 context");
+        return set_err(c, HF_ERR_STATE, "solve: no FP64 operator");
     RhsParams rp{0.0, tol, itmax, 0, 0};
     launch_rhs_res_solve64(c->g, rp, x, b, c->d, c->cx, c->cy, c->cz, c->r64, c->sc, c->partials,
                            c->st);

@@ -68,6 +68,10 @@ void launch_rhs_res_solve64(const Dims &g, const RhsParams &rp, const double *x,
                             const double *b, const double *d, const double *cx,
                             const double *cy, const double *cz, double *rhat, Scal *sc,
                             double *partials, hipStream_t st);
+void launch_rhs_res_solve32(const Dims &g, const RhsParams &rp, const double *x,
+                            const double *b, const double *d, const double *cx,
+                            const double *cy, const double *cz, float *rhat, Scal *sc,
+                            double *partials, hipStream_t st);
 template <typename T>
 void launch_cg_a(const Dims &g, const T *r, const T *pold, T *pnew, T *q, const T *hx,
                  const T *hy, const T *hz, Scal *sc, double *partials, hipStream_t st);

@@ -294,6 +294,15 @@ void launch_rhs_res_solve64(const Dims &g, const RhsParams &rp, const double *x,
                partials);
 }
 
+void launch_rhs_res_solve32(const Dims &g, const RhsParams &rp, const double *x,
+                            const double *b, const double *d, const double *cx,
+                            const double *cy, const double *cz, float *rhat, Scal *sc,
+                            double *partials, hipStream_t st) {
+    HF_ZLAUNCH((k_rhs_res_z<float, false, false, false>), g, st, g, rp, nullptr, nullptr, x, b,
+               nullptr, nullptr, nullptr, nullptr, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
+               partials);
+}
+
 template <typename T>
 void launch_cg_a(const Dims &g, const T *r, const T *pold, T *pnew, T *q, const T *hx,
                  const T *hy, const T *hz, Scal *sc, double *partials, hipStream_t st) {

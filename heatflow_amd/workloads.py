@@ -8,6 +8,7 @@ numpy.random.Generator(PCG64(seed)); the seeds are part of the returned descript
   c2  256^3 two-phase Bernoulli(0.3) voxels, 32 heaters x 1e-5 W               (50 steps)
   c3  512^3 log-normal kappa=exp(N(ln 50, 1)), Gaussian heating, 6 Dirichlet faces (20 steps)
   c4  128^3 kappa=100(1+0.1U(-1,1)), 8 heaters x 1e-5 W, 64 probes             (200 steps)
+  c5  384^3 layered slabs + spherical inclusions, 32 heaters, 256 probes          (500 steps)
 Any builder takes `n` to scale the grid down (parity tests) with the same physics.
 """
 from __future__ import annotations
@@ -97,12 +98,14 @@ def near_cells(n: int, centres: np.ndarray, radius: int) -> np.ndarray:
 
 
 def c4(n: int = 128, icattaneo: int = 1, nheat: int = 8, nprobe: int = 64, seed_k: int = 4,
-       seed_kstar: int = 40, seed_probe: int = 41, seed_heat: int = 42):
+       seed_kstar: int = 40, seed_probe: int = 41, seed_heat: int = 42, t_ref: float = 300.0):
     """kappa = 100 (1 + 0.1 U(-1,1)) (trainable), 1 um cells, rho c_p = 1.631e6, BCs as c1,
     `nheat` heaters of 1e-5 W each entering as per-cell power density P/V (trainable),
     `nprobe` probe cells drawn from the heaters' neighbourhoods (Chebyshev radius 2: over 200
     steps of 0.1 ns heat diffuses about one cell, so farther probes would not see the heaters),
     targets from a ground-truth kappa* (seed_kstar); dt=1e-10, 200 steps, checkpoints every 20.
+    `t_ref` is the bath and initial temperature (300 K); the operator is linear, so t_ref = 0
+    gives the same field minus 300 K, resolved to ~1e-17 K instead of ~6e-14 K.
     Returns the system (with kappa) and kappa*, probes, powers."""
     N = n ** 3
     L = n * 1.0e-6
@@ -120,10 +123,59 @@ def c4(n: int = 128, icattaneo: int = 1, nheat: int = 8, nprobe: int = 64, seed_
     power = np.zeros(N)
     power[heat] = 1.0e-5
     p = Params(nx=n, ny=n, nz=n, time_step=1.0e-10, icattaneo=icattaneo, isteady=0,
-               kappa_bound=(150.0, 150.0, 0.0, 0.0, 0.0, 0.0), t_bath=(300.0,) * 6,
-               t_system=300.0, ntime=200)
+               kappa_bound=(150.0, 150.0, 0.0, 0.0, 0.0, 0.0), t_bath=(t_ref,) * 6,
+               t_system=t_ref, ntime=200)
     s = System.uniform(p, (L, L, L), kappa=kappa, rho=RHO_A, heat_capacity=CP_A,
                        tau=1.0e-12 if icattaneo else 0.0, qdens_extra=power / vol)
     return s, dict(name="c4", steps=200, ckpt=20, kstar=kstar, probes=probes, heaters=heat,
                    power=power, volume=vol, near=near,
                    seeds=dict(kappa=seed_k, kstar=seed_kstar, probes=seed_probe, heat=seed_heat))
+
+
+def c5(n: int = 384, nheat: int = 32, nprobe: int = 256, seed_geom: int = 5,
+       seed_heat: int = 52, seed_probe: int = 51):
+    """Layered stack: 8 z-slabs of n/8 planes with kappa cycling 150, 1.4, 30 W/(m K), plus
+    seeded spherical inclusions (radius 4..12 cells) of kappa 0.2 until 5% of the volume;
+    rho c_p = 1.631e6, tau = 1e-12, 0.5 um cells, Dirichlet 300 K on all six faces
+    (kappa_b = 50), `nheat` heaters of 1e-5 W (per-cell power density), `nprobe` probes in the
+    heaters' neighbourhoods (radius 2); dt = 5e-11 s, 500 steps."""
+    N = n ** 3
+    L = n * 0.5e-6
+    kap = np.empty((n, n, n))                      # (z, y, x)
+    vals = (150.0, 1.4, 30.0)
+    edges = np.linspace(0, n, 9).astype(int)
+    for s in range(8):
+        kap[edges[s]:edges[s + 1]] = vals[s % 3]
+    rng = _rng(seed_geom)
+    incl = np.zeros((n, n, n), bool)
+    target = int(0.05 * N)
+    count = 0
+    while count < target:
+        c = rng.integers(0, n, 3)
+        r = int(rng.integers(4, 13))
+        lo = np.maximum(c - r, 0)
+        hi = np.minimum(c + r + 1, n)
+        zz, yy, xx = np.ogrid[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        ball = (zz - c[0]) ** 2 + (yy - c[1]) ** 2 + (xx - c[2]) ** 2 <= r * r
+        sub = incl[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        count += int(np.count_nonzero(ball & ~sub))
+        sub |= ball
+    kap[incl] = 0.2
+    kappa = kap.reshape(-1)
+    lo_, hi_ = (2, n - 2) if n > 6 else (0, n)
+    side = hi_ - lo_
+    hsel = _rng(seed_heat).choice(side ** 3, size=min(nheat, side ** 3), replace=False)
+    hx, hy, hz = hsel % side + lo_, (hsel // side) % side + lo_, hsel // (side * side) + lo_
+    heat = (hx + n * (hy + n * hz)).astype(np.int64)
+    near = near_cells(n, heat, 2)
+    probes = _rng(seed_probe).choice(near, size=min(nprobe, len(near)), replace=False).astype(np.int64)
+    vol = (L / n) ** 3
+    power = np.zeros(N)
+    power[heat] = 1.0e-5
+    p = Params(nx=n, ny=n, nz=n, time_step=5.0e-11, icattaneo=1, isteady=0,
+               kappa_bound=(50.0,) * 6, t_bath=(300.0,) * 6, t_system=300.0, ntime=500)
+    s = System.uniform(p, (L, L, L), kappa=kappa, rho=RHO_A, heat_capacity=CP_A, tau=1.0e-12,
+                       qdens_extra=power / vol)
+    return s, dict(name="c5", steps=500, probes=probes, heaters=heat, power=power, volume=vol,
+                   near=near, inclusion_fraction=count / N,
+                   seeds=dict(geometry=seed_geom, heat=seed_heat, probes=seed_probe))

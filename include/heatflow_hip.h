@@ -187,9 +187,10 @@ int hf_set_solver(hf_ctx *ctx, double tol, int64_t itmax, int32_t precision, dou
  * final state.  hf_adjoint then runs the reverse sweep (each segment recomputed from its
  * checkpoint) seeded with dL/dT^{final} (length N), and returns the gradients with respect to
  * the per-cell conductivity kappa (dl_dkappa, length N) and to a per-cell power P_I that enters
- * as Qdens_I += P_I / volume_I (dl_dpower, length N).  Requires an FP64 context, per-axis
- * uniform cell lengths; the BR switch (x BR where neighbouring kappas differ) is treated as
- * locally constant.  The context is left at the end state of the forward run. */
+ * as Qdens_I += P_I / volume_I (dl_dpower, length N).  The adjoint solves use the context's
+ * precision mode (FP64, or FP32 inner CG inside FP64 refinement) and tolerance.  Requires
+ * per-axis uniform cell lengths; the BR switch (x BR where neighbouring kappas differ) is
+ * treated as locally constant.  The context is left at the end state of the forward run. */
 int hf_forward_checkpointed(hf_ctx *ctx, int64_t itime0, int64_t nsteps, int64_t ckpt_every);
 int hf_adjoint(hf_ctx *ctx, const double *dl_dt_final, double *dl_dkappa, double *dl_dpower);
 

@@ -74,51 +74,112 @@ def test_autograd_function_on_device():
 
 @pytest.mark.parametrize("icat", [0, 1])
 def test_adjoint_central_fd(icat):
-    """Central differences (relative step 1e-6) of the GPU forward, 32^3, 40 steps, on 16
-    seeded cells next to the heaters and on 4 heater powers.  A central difference of L carries
-    the forward's round-off: |dL_noise| <= 2 sum|T - t| * dT with dT ~ 1e-13 * 300 K (the
-    FP64 solve converges to ~1e-16 relative; 1e-13 leaves margin), so the bar for each cell is
-    1e-6 relative plus that floor divided by 2h."""
+    """Central differences of the GPU forward, 32^3, 40 steps: the 16 cells with the largest
+    |dL/dkappa|, kappa along a seeded direction, and 4 heater powers, each to 1e-6 relative;
+    targets come from the ground-truth kappa*.
+
+    The check runs on the system with temperatures measured from the bath (baths and initial
+    state 0 K; the operator is linear, so the field is the 300 K field minus 300 K, checked
+    here to 1e-10 K).  There the loss, sum (T - T*)^2 with T - T* ~ 1e-6..1e-5 K, is resolved
+    to ~1e-17 K; at 300 K FP64 resolves T only to 6e-14 K, which bounds how well the loss and
+    its gradient are defined at all.  The 300 K adjoint gradient is compared with the 0 K one
+    (measured ~1e-9 apart; bar 1e-7).  Relative step 1e-5 for kappa (the h^2 truncation is
+    ~1e-10 there; a 1e-6 step moves the loss by a few ulps of T at 40 steps), 1e-3 for the
+    powers (T is affine in them, so the difference is exact up to round-off)."""
     s, m = workloads.c4(32, icattaneo=icat, nprobe=64)
+    s0, _ = workloads.c4(32, icattaneo=icat, nprobe=64, t_ref=0.0)
     steps, ckpt = 40, 10
     probes = m["probes"]
+    kap = s.cells.kappa.copy()
+    vol = s.cells.volume
     sim = Simulation(s, tol=1e-32)
+    sim0 = Simulation(s0, tol=1e-32)
+
+    def final_t(sm, kv, pv):
+        sm.set_fields(kv, pv / vol)
+        sm.sparse_Hmatrix()
+        sm.initial_evolve()
+        sm.run(1, steps)
+        return sm.get_state()[0]
+
     try:
-        sim.initial_evolve()
-        sim.run(1, steps)
-        T0, _ = sim.get_state()
-        targets = T0[probes] - 0.01
+        targets = final_t(sim, m["kstar"], m["power"])[probes]
+        targets0 = final_t(sim0, m["kstar"], m["power"])[probes]
+        T300 = final_t(sim, kap, m["power"])
+        T0 = final_t(sim0, kap, m["power"])
+        assert np.abs(T0 - (T300 - 300.0)).max() <= 1e-10
         sim.initial_evolve()
         L, gk, gp = sim.grad(1, steps, ckpt, probes, targets)
-        near1 = workloads.near_cells(32, m["heaters"], 1)
-        cells = np.random.default_rng(43).choice(near1, 16, replace=False)
-        kap = s.cells.kappa.copy()
-        vol = s.cells.volume
+        assert abs(_loss_np(T300, probes, targets) - L) <= 1e-12 * L
+        sim0.initial_evolve()
+        L0, gk0, gp0 = sim0.grad(1, steps, ckpt, probes, targets0)
+        assert abs(_loss_np(T0, probes, targets0) - L0) <= 1e-12 * L0
+        # 300 K vs 0 K: the seed 2 (T - T*) at 300 K carries the 300 K field's round-off, dT
+        dT = float(np.abs(T0[probes] - (T300[probes] - 300.0)).max()
+                   + np.abs(targets0 - (targets - 300.0)).max())
+        h = m["heaters"]
+        ek = np.abs(gk - gk0).max() / np.abs(gk0).max()
+        ep = np.abs(gp[h] - gp0[h]).max() / np.abs(gp0[h]).max()
+        print(f"300 K vs 0 K: dT {dT:.3e} K, loss {abs(L - L0) / L0:.3e}, "
+              f"dL/dkappa {ek:.3e}, dL/dP {ep:.3e}", flush=True)
+        bound = 2 * np.abs(T0[probes] - targets0).sum() * dT + len(probes) * dT * dT
+        assert abs(L - L0) <= bound + 1e-12 * L0
+        assert ek <= 1e-7 and ep <= 1e-7
 
-        def loss_at(kv, pv):
-            sim.set_fields(kv, pv / vol)
-            sim.sparse_Hmatrix()
-            sim.initial_evolve()
-            sim.run(1, steps)
-            T, _ = sim.get_state()
-            return _loss_np(T, probes, targets), T
+        def loss0(kv, pv):
+            return _loss_np(final_t(sim0, kv, pv), probes, targets0)
 
-        L0, T0b = loss_at(kap, m["power"])
-        assert abs(L0 - L) <= 1e-12 * L
-        noise = 2.0 * np.sum(np.abs(T0b[probes] - targets)) * 1e-13 * 300.0
-        for c in cells:
-            h = 1e-6 * kap[c]
+        for c in np.argsort(-np.abs(gk0))[:16]:
+            hk = 1e-5 * kap[c]
             kp, km = kap.copy(), kap.copy()
-            kp[c] += h
-            km[c] -= h
-            fd = (loss_at(kp, m["power"])[0] - loss_at(km, m["power"])[0]) / (2 * h)
-            assert abs(fd - gk[c]) <= 1e-6 * abs(gk[c]) + noise / (2 * h), (c, fd, gk[c])
-        for hc in m["heaters"][:4]:
-            hp = 1e-6 * m["power"][hc]
+            kp[c] += hk
+            km[c] -= hk
+            fd = (loss0(kp, m["power"]) - loss0(km, m["power"])) / (2 * hk)
+            assert abs(fd - gk0[c]) <= 1e-6 * abs(gk0[c]), (c, fd, gk0[c])
+        u = np.random.default_rng(44).uniform(-1.0, 1.0, kap.size)
+        eps = 1e-5
+        fd = (loss0(kap * (1 + eps * u), m["power"]) - loss0(kap * (1 - eps * u), m["power"])) / (2 * eps)
+        dd = float(np.dot(gk0, kap * u))
+        assert abs(fd - dd) <= 1e-6 * abs(dd), (fd, dd)
+        for hc in h[:4]:
+            hp = 1e-3 * m["power"][hc]
             pp, pm = m["power"].copy(), m["power"].copy()
             pp[hc] += hp
             pm[hc] -= hp
-            fdp = (loss_at(kap, pp)[0] - loss_at(kap, pm)[0]) / (2 * hp)
-            assert abs(fdp - gp[hc]) <= 1e-6 * abs(gp[hc]) + noise / (2 * hp), (hc, fdp, gp[hc])
+            fdp = (loss0(kap, pp) - loss0(kap, pm)) / (2 * hp)
+            assert abs(fdp - gp0[hc]) <= 1e-6 * abs(gp0[hc]), (hc, fdp, gp0[hc])
     finally:
         sim.close()
+        sim0.close()
+
+
+def test_adjoint_mixed_precision_matches_fp64():
+    """FP32-inner / FP64-refined forward and adjoint solves give the FP64 gradient to the
+    solver tolerance (layered-slab workload with inclusions, scaled to 24^3, 40 steps).
+
+    Both modes stop at the same scaled residual, 1e-13.  The loss difference is bounded by
+    the measured probe temperature difference dT: |L32 - L64| <= 2 sum|T64 - t| dT + m dT^2."""
+    s, m = workloads.c5(24, nheat=4, nprobe=40)
+    probes = m["probes"]
+    steps = 40
+    with Simulation(s, precision="fp64", tol=1e-13) as sim:
+        sim.initial_evolve()
+        sim.run(1, steps)
+        t_end, _ = sim.get_state()
+    targets = t_end[probes] - 0.01
+    out = {}
+    for prec in ("fp32_ir", "fp64"):
+        with Simulation(s, precision=prec, tol=1e-13) as sim:
+            sim.initial_evolve()
+            loss, gk, gp = sim.grad(1, steps, 7, probes, targets)
+            tp, _ = sim.get_state()
+            out[prec] = (loss, gk, gp, tp)
+    L32, gk32, gp32, T32 = out["fp32_ir"]
+    L64, gk64, gp64, T64 = out["fp64"]
+    assert np.max(np.abs(T32 - T64) / np.abs(T64)) <= 1e-10
+    dT = float(np.abs(T32[probes] - T64[probes]).max())
+    bound = 2 * np.abs(T64[probes] - targets).sum() * dT + len(probes) * dT ** 2
+    assert abs(L32 - L64) <= bound + 1e-14 * abs(L64)
+    assert np.abs(gk32 - gk64).max() <= 1e-6 * np.abs(gk64).max()
+    h = m["heaters"]
+    assert np.abs(gp32[h] - gp64[h]).max() <= 1e-6 * np.abs(gp64[h]).max()

@@ -171,6 +171,7 @@ static int ensure(hf_ctx *c, double **p, size_t n) {
 // restore (T^n, T^{n-1}) into the context state and recompute the TINY flag
 static int restore_state(hf_ctx *c, const double *tp, const double *tpp) {
     const size_t B8 = (size_t)c->g.N * 8;
+    c->carry = false;
     AHIP(c, hipMemcpyAsync(c->T[0], tp, B8, hipMemcpyDeviceToDevice, c->st));
     AHIP(c, hipMemcpyAsync(c->T[1], tpp, B8, hipMemcpyDeviceToDevice, c->st));
     launch_step_begin(c->sc, c->st);

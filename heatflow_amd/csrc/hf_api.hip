@@ -66,6 +66,9 @@ static int check_launch(hf_ctx *c, const char *what) {
     return HF_OK;
 }
 
+// CG solves may stop after their first iteration unless debug bit 1 turns that off
+static bool allow_short(const hf_ctx *c) { return (c->debug & 2) == 0; }
+
 // ---- kernel timing -----------------------------------------------------------------------
 
 // Opens a timed launch: records the start event on the context stream; returns the slot or -1.
@@ -111,22 +114,11 @@ static void make_dims(Dims &g, const hf_params &p) {
     g.tpb = (uint32_t)((g.ntiles + nb - 1) / nb);
     nb = (g.ntiles + g.tpb - 1) / g.tpb;
     g.nblk = (uint32_t)((nb + 7) / 8 * 8);
-    // z-marching tiles: enough z-chunks for ~`zcap` blocks (HF_ZBLK overrides); 128 x 4 tiles
-    // with ~8M threads per launch measured best at 512^3 (64 planes per chunk)
+    // z-marching tiles of kZTX x kZTY columns: enough z-chunks for ~`zcap` blocks (HF_ZBLK
+    // overrides)
     uint32_t zcap = 16384;
     if (const char *e = getenv("HF_ZBLK")) zcap = std::max(8u, (uint32_t)atoi(e));
-    // tile shape: tile_x (power of two) by tile_y threads; HF_TILE="tx,ty" overrides
-    uint32_t tx = 128, ty = 4;
-    if (const char *e = getenv("HF_TILE")) {
-        unsigned a = 0, b = 0;
-        if (sscanf(e, "%u,%u", &a, &b) == 2 && a >= 1 && b >= 1 && a * b <= 1024 && a * b >= 64 &&
-            (a & (a - 1)) == 0)
-            tx = a, ty = b;
-    }
-    g.tile_x = tx;
-    g.tile_y = ty;
-    g.tile_xs = 0;
-    while ((1u << g.tile_xs) < tx) ++g.tile_xs;
+    const uint32_t tx = kZTX, ty = kZTY;
     g.ztx = (g.nx + tx - 1) / tx;
     g.zty = (g.ny + ty - 1) / ty;
     const uint64_t per_plane = (uint64_t)g.ztx * g.zty;
@@ -187,6 +179,7 @@ static int alloc_mode(hf_ctx *c) {
         TRY(dalloc(c, &c->r32, N)); TRY(dalloc(c, &c->p32[0], N)); TRY(dalloc(c, &c->p32[1], N));
         TRY(dalloc(c, &c->q32, N)); TRY(dalloc(c, &c->y32, N));
         TRY(dalloc(c, &c->S, N)); TRY(dalloc(c, &c->T[2], N));
+        TRY(dalloc(c, &c->rn[0], N)); TRY(dalloc(c, &c->rn[1], N)); TRY(dalloc(c, &c->qb, N));
     } else {
         TRY(dalloc(c, &c->hx, N)); TRY(dalloc(c, &c->hy, N)); TRY(dalloc(c, &c->hz, N));
         TRY(dalloc(c, &c->r64, N)); TRY(dalloc(c, &c->p64[0], N)); TRY(dalloc(c, &c->p64[1], N));
@@ -299,6 +292,7 @@ extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho
                                const double *len_z, const double *volume,
                                const int64_t *iheater, const double *qdens_extra) {
     if (!c) return HF_ERR_ARG;
+    c->carry = false;   // device state or scratch changes: no carried residual
     if (!kappa || !rho || !heat_capacity || !tau_scaled || !len_x || !len_y || !len_z || !volume)
         return set_err(c, HF_ERR_ARG, "hf_upload_cells: required array is NULL");
     const size_t N = c->g.N, B8 = N * sizeof(double);
@@ -418,6 +412,7 @@ static int sync_scal(hf_ctx *c, Scal *dev, Scal *host) {
 
 extern "C" int hf_assemble(hf_ctx *c) {
     if (!c) return HF_ERR_ARG;
+    c->carry = false;   // device state or scratch changes: no carried residual
     if (!c->uploaded) return set_err(c, HF_ERR_STATE, "hf_assemble: cells not uploaded");
     launch_assemble(c->g, c->ap, c->kappa, c->lx, c->ly, c->lz, c->rhoc, c->tau_s, c->d, c->cx,
                     c->cy, c->cz, c->bvec, c->sc, c->st);
@@ -442,6 +437,7 @@ extern "C" int hf_assemble(hf_ctx *c) {
 extern "C" int hf_export_coo(hf_ctx *c, int64_t *n, int64_t *len, int64_t *count, double *val,
                              int64_t *irow, int64_t *jcol) {
     if (!c) return HF_ERR_ARG;
+    c->carry = false;   // device state or scratch changes: no carried residual
     if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_export_coo: not assembled");
     const hf_params &p = c->p;
     int64_t L = 7 * p.nx * p.ny * p.nz - 2 * (p.nx * p.ny + p.ny * p.nz + p.nz * p.nx);
@@ -491,6 +487,7 @@ extern "C" int hf_export_coo(hf_ctx *c, int64_t *n, int64_t *len, int64_t *count
 
 extern "C" int hf_set_state(hf_ctx *c, const double *tp, const double *tpp) {
     if (!c || !tp || !tpp) return set_err(c, HF_ERR_ARG, "hf_set_state: null argument");
+    c->carry = false;   // device state or scratch changes: no carried residual
     const size_t B8 = (size_t)c->g.N * sizeof(double);
     HIPCHK(c, hipMemcpyAsync(c->T[0], tp, B8, hipMemcpyDefault, c->st));
     HIPCHK(c, hipMemcpyAsync(c->T[1], tpp, B8, hipMemcpyDefault, c->st));
@@ -523,6 +520,21 @@ static int flags_to_rc(hf_ctx *c, int32_t fl) {
     return HF_OK;
 }
 
+// Timing slots of CG launches: iteration k of the solve, and whether it is the k_cg_b launch.
+// Launches queued past convergence return at once and their slots are dropped; so is the
+// first k_cg_b of a solve that stopped on its first iteration (Scal::shortcut).
+struct SlotRec {
+    long slot;
+    int64_t k;
+    bool is_b;
+};
+
+static void drop_idle_slots(hf_ctx *c, const std::vector<SlotRec> &slots, int64_t iters,
+                            bool shortcut) {
+    for (const SlotRec &sl : slots)
+        if (sl.k >= iters || (sl.is_b && shortcut && sl.k == 0)) c->tpool[sl.slot].valid = false;
+}
+
 // CG on A^ y = r^ until sc->done, in batches of `pred` iterations per host synchronisation.
 // Launches queued after convergence return at once; their timing slots are invalidated.
 template <typename T>
@@ -532,7 +544,7 @@ static int run_cg(hf_ctx *c, Scal *sc, Scal *h, T *r, T **p, T *q, T *y, const T
     int cur = 0;
     int64_t batch = std::max<int64_t>(1, *pred);
     int64_t queued = 0;
-    std::vector<std::pair<long, int64_t>> slots;
+    std::vector<SlotRec> slots;
     for (;;) {
         for (int64_t b = 0; b < batch; ++b, ++queued) {
             const bool first = queued == 0;
@@ -546,8 +558,8 @@ static int run_cg(hf_ctx *c, Scal *sc, Scal *h, T *r, T **p, T *q, T *y, const T
             launch_cg_b<T>(c->g, y, p[cur ^ 1], r, q, sc, c->partials, c->st);
             tend(c, sb);
             launch_fin_b(c->g, c->partials, sc, c->st);
-            if (sa >= 0) slots.push_back({sa, queued});
-            if (sb >= 0) slots.push_back({sb, queued});
+            if (sa >= 0) slots.push_back({sa, queued, false});
+            if (sb >= 0) slots.push_back({sb, queued, true});
             cur ^= 1;
         }
         TRY(check_launch(c, "k_cg"));
@@ -555,8 +567,8 @@ static int run_cg(hf_ctx *c, Scal *sc, Scal *h, T *r, T **p, T *q, T *y, const T
         if (h->done) break;
         batch = std::max<int64_t>(2, batch);
     }
-    for (auto &sl : slots)
-        if (sl.second >= h->iter) c->tpool[sl.first].valid = false;
+    drop_idle_slots(c, slots, h->iter, h->shortcut != 0);
+    if (h->shortcut) c->stats.first_iter_stops += 1;
     *iters = h->iter;
     *pred = std::max<int64_t>(1, h->iter + 1);
     return HF_OK;
@@ -580,7 +592,7 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
                            c->cx, c->cy, c->cz, c->r64, (c->debug & 1) ? c->S : nullptr, c->sc,
                            c->partials, c->st);
     tend(c, sr);
-    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
+    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
     TRY(check_launch(c, "k_rhs_res"));
     int64_t it = 0;
     TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
@@ -591,7 +603,7 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bnorm2) : 0.0;
     // k_finalize: read Tp, Tpp, s, y; write T^{n+1}
     long sf = tbeg(c, HF_K_FINAL, N * 8 * 5);
-    launch_finalize(c->g, true, c->T[0], c->T[1], c->sv, c->y64, c->sc, c->st);
+    launch_finalize(c->g, true, c->T[0], c->T[1], c->sv, c->y64, c->p64[1], c->sc, c->st);
     tend(c, sf);
     TRY(check_launch(c, "k_finalize"));
     std::swap(c->T[0], c->T[1]);  // Temp_pp <- Temp_p, Temp_p <- x (mod_evolve.f90:169-170)
@@ -603,7 +615,123 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     return HF_OK;
 }
 
+// Inner FP32 CG of one refinement sweep on r (in place); returns the iterations it took.
+static int inner_solve(hf_ctx *c, float *r, int64_t *batch_out, std::vector<SlotRec> &slots) {
+    const hf_params &p = c->p;
+    const double N = (double)c->g.N;
+    launch_inner_begin(c->sc, c->sci, p.inner_tol, p.tol, p.inner_itmax, 0.25, allow_short(c),
+                       c->st);
+    int cur = 0;
+    const int64_t batch = std::max<int64_t>(1, c->pred_inner);
+    for (int64_t b = 0; b < batch; ++b) {
+        // k_cg_a: read r, p_old (not on the first iteration), 3 couplings; write p, q
+        long sa = tbeg(c, HF_K_CG_A32, N * 4 * (b == 0 ? 6 : 7));
+        launch_cg_a<float>(c->g, r, c->p32[cur], c->p32[cur ^ 1], c->q32, c->fx, c->fy, c->fz,
+                           c->sci, c->partials, c->st);
+        tend(c, sa);
+        launch_fin_a(c->g, c->partials, c->sci, c->st);
+        // k_cg_b: read y (not on the first iteration), p, r, q; write y, r
+        long sb = tbeg(c, HF_K_CG_B32, N * 4 * (b == 0 ? 5 : 6));
+        launch_cg_b<float>(c->g, c->y32, c->p32[cur ^ 1], r, c->q32, c->sci, c->partials, c->st);
+        tend(c, sb);
+        launch_fin_b(c->g, c->partials, c->sci, c->st);
+        if (sa >= 0) slots.push_back({sa, b, false});
+        if (sb >= 0) slots.push_back({sb, b, true});
+        cur ^= 1;
+    }
+    *batch_out = batch;
+    return check_launch(c, "k_cg32");
+}
+
+// Mixed-precision step with the refinement residual fused into the next step's right-hand
+// side (k_ir_step_z): per step one inner solve and one fused pass; the first step after a
+// change of state starts from k_rhs_res.  Needs a per-step constant Qdens (no time-dependent
+// heater modes or radiation: those recompute Qdens from T^n first).
+static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
+    (void)itime;
+    const hf_params &p = c->p;
+    const double N = (double)c->g.N;
+    launch_step_begin(c->sc, c->st);
+    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
+    float *rin;
+    if (c->carry) {
+        launch_carry_begin(c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
+        rin = c->carry_r;
+    } else {
+        // k_rhs_res: read 10 FP64 arrays; write r^ (FP32)
+        long sr = tbeg(c, HF_K_RHS32, N * (8 * 10 + 4));
+        launch_rhs_res_state32(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec,
+                               c->d, c->cx, c->cy, c->cz, c->r32, nullptr, nullptr, c->sc,
+                               c->partials, c->st);
+        tend(c, sr);
+        launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
+        // Qdens and B only change through calls that end a carried run
+        launch_sum2(c->g, c->qd, c->bvec, c->qb, c->st);
+        TRY(check_launch(c, "k_rhs_res"));
+        rin = c->r32;
+    }
+    c->carry = false;
+    const StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo};
+    const int64_t sweeps_max = std::min<int64_t>(p.itmax, 64);
+    // refined x ping-pongs between the spare state buffer and S (unused by this path)
+    double *xbuf[2] = {c->T[2], c->S};
+    double *xa = nullptr;
+    int nb = 0;
+    int64_t inner_total = 0, sweeps = 0;
+    for (;;) {
+        std::vector<SlotRec> slots;
+        int64_t batch = 0;
+        TRY(inner_solve(c, rin, &batch, slots));
+        double *xo = xbuf[nb];
+        // k_ir_step: read T^n, T^{n-1} (or x), y or p (FP32), s, rho c_p, tau' rho c_p,
+        // Qdens + B, diag, 3 couplings; write x', three FP32 residuals
+        long si = tbeg(c, HF_K_IR_STEP, N * (8 * 10 + 4 + 8 + 3 * 4) + (xa ? N * 8 : 0));
+        launch_ir_step(c->g, sp, xa == nullptr, c->T[0], c->T[1], xa, c->y32, c->p32[1], c->sci,
+                       c->sv, c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, c->r32,
+                       c->rn[0], c->rn[1], c->sc, c->partials, c->st);
+        tend(c, si);
+        launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->st);
+        TRY(check_launch(c, "k_ir_step"));
+        HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
+        TRY(sync_scal(c, c->sc, c->h_sc));
+        const int64_t in_it = c->h_sci->iter;
+        drop_idle_slots(c, slots, in_it, c->h_sci->shortcut != 0);
+        if (c->h_sci->shortcut) c->stats.first_iter_stops += 1;
+        inner_total += in_it;
+        sweeps += 1;
+        c->pred_inner = c->h_sci->done ? std::max<int64_t>(1, in_it + 1) : batch * 2;
+        xa = xo;
+        nb ^= 1;
+        rin = c->r32;
+        int32_t fl = c->h_sc->flags | c->setup_flags;
+        if (fl & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S))
+            return flags_to_rc(c, fl);
+        if (c->h_sc->done) break;
+    }
+    // rotation: Temp_pp <- Temp_p, Temp_p <- x'; the two spares take the old Temp_pp buffer
+    // and the unused refinement buffer
+    double *old_tp = c->T[0], *old_tpp = c->T[1];
+    double *other = xbuf[nb];   // the buffer the last sweep did not write
+    c->T[0] = xa;
+    c->T[1] = old_tp;
+    c->T[2] = old_tpp;
+    c->S = other;
+    // the next step starts from the residual this pass left for its TINY case
+    c->carry = true;
+    c->carry_r = (c->h_sc->flags & FLAG_TINY_NEXT) ? c->rn[1] : c->rn[0];
+    const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->outer_rr / c->h_sc->bnorm2) : 0.0;
+    c->stats.steps += 1;
+    c->stats.outer_iters += sweeps;
+    c->stats.inner_iters += inner_total;
+    c->stats.last_err = e;
+    if (iter) *iter = inner_total + sweeps;
+    if (err) *err = e;
+    return HF_OK;
+}
+
 static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
+    if (!c->dyn_heat && !(c->debug & 1)) return step_ir_fused(c, itime, iter, err);
+    c->carry = false;
     const hf_params &p = c->p;
     launch_step_begin(c->sc, c->st);
     const double N = (double)c->g.N;
@@ -621,7 +749,7 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     launch_rhs_res_state32(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec, c->d,
                            c->cx, c->cy, c->cz, c->r32, c->S, c->T[2], c->sc, c->partials, c->st);
     tend(c, sr);
-    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
+    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
     TRY(check_launch(c, "k_rhs_res"));
     double *xa = c->T[2], *xb = c->T[1];  // T[1] (Temp_pp) is free once x0 and S exist
     // T[1] is still needed: the refined x of the last sweep must not overwrite Temp_pp before
@@ -631,30 +759,13 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     IrParams ip{std::min<int64_t>(p.itmax, 64), 1};
     int64_t inner_total = 0, sweeps = 0;
     for (;;) {
-        launch_inner_begin(c->sc, c->sci, p.inner_tol, p.tol, p.inner_itmax, 0.25, c->st);
-        int cur = 0;
-        int64_t batch = std::max<int64_t>(1, c->pred_inner);
-        std::vector<std::pair<long, int64_t>> slots;
-        for (int64_t b = 0; b < batch; ++b) {
-            long sa = tbeg(c, HF_K_CG_A32, N * 4 * (b == 0 ? 6 : 7));
-            launch_cg_a<float>(c->g, c->r32, c->p32[cur], c->p32[cur ^ 1], c->q32, c->fx, c->fy,
-                               c->fz, c->sci, c->partials, c->st);
-            tend(c, sa);
-            launch_fin_a(c->g, c->partials, c->sci, c->st);
-            long sb = tbeg(c, HF_K_CG_B32, N * 4 * (b == 0 ? 5 : 6));
-            launch_cg_b<float>(c->g, c->y32, c->p32[cur ^ 1], c->r32, c->q32, c->sci, c->partials,
-                               c->st);
-            tend(c, sb);
-            launch_fin_b(c->g, c->partials, c->sci, c->st);
-            if (sa >= 0) slots.push_back({sa, b});
-            if (sb >= 0) slots.push_back({sb, b});
-            cur ^= 1;
-        }
-        TRY(check_launch(c, "k_cg32"));
+        std::vector<SlotRec> slots;
+        int64_t batch = 0;
+        TRY(inner_solve(c, c->r32, &batch, slots));
         // k_ir_res: read x, y (FP32), s, S, diag, 3 couplings, Tp; write x, r^ (FP32)
         long si = tbeg(c, HF_K_IR_RES, N * (8 * 8 + 4 + 8 + 4));
-        launch_ir_res(c->g, ip, xa, c->y32, c->sv, c->S, c->d, c->cx, c->cy, c->cz, c->T[0], xb,
-                      c->r32, c->sc, c->partials, c->st);
+        launch_ir_res(c->g, ip, xa, c->y32, c->p32[1], c->sci, c->sv, c->S, c->d, c->cx, c->cy,
+                      c->cz, c->T[0], xb, c->r32, c->sc, c->partials, c->st);
         tend(c, si);
         launch_fin_ir(c->g, c->partials, c->sc, ip.itmax, c->st);
         TRY(check_launch(c, "k_ir_res"));
@@ -662,8 +773,8 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
         TRY(sync_scal(c, c->sc, c->h_sc));
         const int64_t in_it = c->h_sci->iter;
-        for (auto &sl : slots)
-            if (sl.second >= in_it) c->tpool[sl.first].valid = false;
+        drop_idle_slots(c, slots, in_it, c->h_sci->shortcut != 0);
+        if (c->h_sci->shortcut) c->stats.first_iter_stops += 1;
         inner_total += in_it;
         sweeps += 1;
         // next prediction: what this sweep needed (+1), or a longer batch if it was cut short
@@ -728,6 +839,7 @@ extern "C" int hf_solve(hf_ctx *c, const double *b, double *x, int32_t itol, dou
                         int64_t itmax, int64_t *iter, double *err) {
     (void)itol;
     if (!c || !b || !x) return set_err(c, HF_ERR_ARG, "hf_solve: null argument");
+    c->carry = false;   // device state or scratch changes: no carried residual
     if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_solve: not assembled");
     if (c->p.precision != HF_PREC_FP64 || !c->hx)
         return set_err(c, HF_ERR_STATE, "hf_solve: needs an FP64 context");
@@ -742,12 +854,12 @@ extern "C" int hf_solve(hf_ctx *c, const double *b, double *x, int32_t itol, dou
     RhsParams rp{0.0, tol, itmax, 0, 0};
     launch_rhs_res_solve64(c->g, rp, xd, bd, c->d, c->cx, c->cy, c->cz, c->r64, c->sc,
                            c->partials, c->st);
-    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
+    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
     TRY(check_launch(c, "k_rhs_res(solve)"));
     int64_t it = 0, pred = std::max<int64_t>(4, c->pred_iters);
     TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
                        &pred, &it, HF_K_CG_A64, HF_K_CG_B64));
-    launch_finalize(c->g, false, nullptr, xd, c->sv, c->y64, c->sc, c->st);
+    launch_finalize(c->g, false, nullptr, xd, c->sv, c->y64, c->p64[1], c->sc, c->st);
     TRY(check_launch(c, "k_finalize(solve)"));
     HIPCHK(c, hipMemcpyAsync(x, xd, B8, hipMemcpyDefault, c->st));
     TRY(sync_scal(c, c->sc, c->h_sc));
@@ -778,6 +890,7 @@ extern "C" int hf_get_vector(hf_ctx *c, int32_t which, double *out) {
 
 extern "C" int hf_set_debug(hf_ctx *c, int32_t flags) {
     if (!c) return HF_ERR_ARG;
+    c->carry = false;   // device state or scratch changes: no carried residual
     c->debug = flags;
     if (flags & 1) TRY(dalloc(c, &c->S, c->g.N));
     return HF_OK;
@@ -826,6 +939,7 @@ extern "C" int hf_kernel_times(hf_ctx *c, double *ms, int64_t *launches, double 
 extern "C" int hf_set_solver(hf_ctx *c, double tol, int64_t itmax, int32_t precision,
                              double inner_tol, int32_t inner_itmax) {
     if (!c) return HF_ERR_ARG;
+    c->carry = false;   // device state or scratch changes: no carried residual
     if (precision != HF_PREC_FP64 && precision != HF_PREC_FP32_IR)
         return set_err(c, HF_ERR_ARG, "hf_set_solver: unknown precision");
     const int32_t old = c->p.precision;
@@ -857,18 +971,20 @@ int ctx_alloc(hf_ctx *c, double **ptr, size_t n) { return dalloc(c, ptr, n); }
 
 int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itmax,
                   int64_t *iters) {
+    c->carry = false;   // the solve reuses the step scratch
     if (c->p.precision == HF_PREC_FP32_IR) {
         // FP32 inner CG inside FP64 refinement, x refined in place (ping-pong with T[2])
         RhsParams rp{0.0, tol, itmax, 0, 0};
         launch_rhs_res_solve32(c->g, rp, x, b, c->d, c->cx, c->cy, c->cz, c->r32, c->sc,
                                c->partials, c->st);
-        launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
+        launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
         TRY(check_launch(c, "k_rhs_res(solve32)"));
         IrParams ip{std::min<int64_t>(itmax, 64), 0};
         double *xa = x, *xb = c->T[2];
         int64_t inner_total = 0;
         for (;;) {
-            launch_inner_begin(c->sc, c->sci, c->p.inner_tol, tol, c->p.inner_itmax, 0.25, c->st);
+            launch_inner_begin(c->sc, c->sci, c->p.inner_tol, tol, c->p.inner_itmax, 0.25,
+                               allow_short(c), c->st);
             int cur = 0;
             const int64_t batch = std::max<int64_t>(1, c->pred_adj);
             for (int64_t k = 0; k < batch; ++k) {
@@ -880,8 +996,8 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
                 launch_fin_b(c->g, c->partials, c->sci, c->st);
                 cur ^= 1;
             }
-            launch_ir_res(c->g, ip, xa, c->y32, c->sv, b, c->d, c->cx, c->cy, c->cz, nullptr, xb,
-                          c->r32, c->sc, c->partials, c->st);
+            launch_ir_res(c->g, ip, xa, c->y32, c->p32[1], c->sci, c->sv, b, c->d, c->cx, c->cy,
+                          c->cz, nullptr, xb, c->r32, c->sc, c->partials, c->st);
             launch_fin_ir(c->g, c->partials, c->sc, ip.itmax, c->st);
             TRY(check_launch(c, "k_ir_res(solve)"));
             std::swap(xa, xb);
@@ -901,12 +1017,12 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
     RhsParams rp{0.0, tol, itmax, 0, 0};
     launch_rhs_res_solve64(c->g, rp, x, b, c->d, c->cx, c->cy, c->cz, c->r64, c->sc, c->partials,
                            c->st);
-    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, c->st);
+    launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
     TRY(check_launch(c, "k_rhs_res(solve)"));
     int64_t it = 0;
     TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
                        &c->pred_adj, &it, HF_K_CG_A64, HF_K_CG_B64));
-    launch_finalize(c->g, false, nullptr, x, c->sv, c->y64, c->sc, c->st);
+    launch_finalize(c->g, false, nullptr, x, c->sv, c->y64, c->p64[1], c->sc, c->st);
     TRY(check_launch(c, "k_finalize(solve)"));
     if (iters) *iters = it;
     return HF_OK;
@@ -927,6 +1043,7 @@ extern "C" int hf_adjoint(hf_ctx *c, const double *dl_dt_final, double *dl_dkapp
 
 extern "C" int hf_set_fields(hf_ctx *c, const double *kappa, const double *qdens_extra) {
     if (!c) return HF_ERR_ARG;
+    c->carry = false;   // device state or scratch changes: no carried residual
     if (!c->uploaded) return set_err(c, HF_ERR_STATE, "hf_set_fields: cells not uploaded");
     const size_t B8 = (size_t)c->g.N * 8;
     if (kappa) HIPCHK(c, hipMemcpyAsync(c->kappa, kappa, B8, hipMemcpyDefault, c->st));

@@ -46,6 +46,13 @@ struct hf_ctx {
     double *r64 = nullptr, *p64[2] = {nullptr, nullptr}, *q64 = nullptr, *y64 = nullptr;
     // FP32 inner CG
     float *r32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr, *y32 = nullptr;
+    // fused mixed step: the next step's r^0 without / with the TINY shift; `carry` is set when
+    // the last step left them (and the scalars) valid for the current state, `carry_r` is the
+    // one the next step starts from
+    float *rn[2] = {nullptr, nullptr};
+    double *qb = nullptr;        // Qdens + B for the fused step (set when a carried run starts)
+    bool carry = false;
+    float *carry_r = nullptr;
     // reductions / scalars
     Scal *sc = nullptr, *sci = nullptr;
     double *partials = nullptr;

@@ -16,6 +16,8 @@ namespace hf {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxWaves = 16;            // largest block: 1024 threads (z-marching kernels)
+// z-marching stencil tile: kZTX x kZTY threads, one column each
+constexpr uint32_t kZTX = 128, kZTY = 4;
 constexpr double kTiny = 1.0e-12;        // constants::TINY, src/heatflow/mod_constants.f90:17
 constexpr double kStefBoltz = 5.670373e-8;  // src/heatflow/mod_constants.f90:16
 
@@ -41,10 +43,9 @@ struct Dims {
     uint32_t tpb;        // tiles per block (chunked layout)
     uint32_t nblk;       // launch grid (multiple of 8)
     int32_t layout;      // LAYOUT_* bits
-    // z-marching stencil kernels: blocks of tile_x * tile_y threads (tile_x = 1 << tile_xs),
-    // ztx x zty tiles per plane, z-chunks of `zchunk` planes, ztiles = ztx * zty * chunks,
-    // zblk = ztiles rounded up to a multiple of 8
-    uint32_t tile_x, tile_y, tile_xs;
+    // z-marching stencil kernels: blocks of kZTX x kZTY threads, ztx x zty tiles per plane,
+    // z-chunks of `zchunk` planes, ztiles = ztx * zty * chunks, zblk = ztiles rounded up to a
+    // multiple of 8
     uint32_t ztx, zty, zchunk, ztiles, zblk;
     int32_t per[3];      // periodic axes (length >= 3)
     FastDiv dnx, dny;
@@ -70,7 +71,11 @@ struct Scal {
     int32_t first;    // next CG iteration is the first one (p = r, y = 0)
     int32_t flags;    // HF_FLAG_* bits (sticky until the host clears them)
     int32_t tiny;     // x0 += TINY for the next step (mod_evolve.f90:141)
-    int32_t pad[2];
+    int32_t shortcut; // the solve stopped after its first iteration: y = alpha p (p = r0)
+    int32_t flags_next;  // NaN bits of the next step's S, computed ahead by the fused step
+    int32_t allow_short; // k_fin_a may end a solve after its first iteration
+    // next step's ||b^||^2 and ||r^0||^2 without / with the TINY shift (fused mixed step)
+    double next_bnorm2, next_rr[2];
 };
 
 enum : int32_t {

@@ -7,6 +7,7 @@
 //   k_qdens        per-step heater power density                      (mod_heating.f90:27-169)
 //   k_cg_b         y += alpha p, r -= alpha q, r.r                     (mod_Sparse.f90:412-420)
 //   k_finalize     x = x0 + s*y, NaN guard, TINY flag of the next step (mod_evolve.f90:140-170)
+//   k_fin_step     refinement test and next-step norms after the fused mixed step
 //   k_fin_*        one-block finishing kernels of the reductions (scalars, convergence)
 // The stencil kernels (fused RHS/residual, CG SpMV, refinement residual) are in hf_stencil.hip.
 //
@@ -300,7 +301,8 @@ k_qdens(Dims g, HeatParams hp, const int8_t *__restrict__ iheater,
 
 // Start of a solve: ||b^||^2, r.r, thresholds (after k_rhs_res).
 __global__ void __launch_bounds__(kBlock)
-k_fin_start(uint32_t nb, const double *partials, Scal *sc, double tol, int64_t itmax) {
+k_fin_start(uint32_t nb, const double *partials, Scal *sc, double tol, int64_t itmax,
+            int32_t allow_short) {
     double v[2];
     sum_partials<2>(partials, nb, v);
     if (threadIdx.x != 0) return;
@@ -315,6 +317,8 @@ k_fin_start(uint32_t nb, const double *partials, Scal *sc, double tol, int64_t i
     sc->pq = 0.0;
     sc->alpha = 0.0;
     sc->beta = 0.0;
+    sc->shortcut = 0;
+    sc->allow_short = allow_short;
     // An exactly zero residual needs no iteration (the reference would divide 0/0).
     sc->done = (v[1] == 0.0) || isnan(v[1]) || itmax <= 0;
     if (isnan(v[1])) sc->flags |= FLAG_BREAKDOWN;
@@ -322,12 +326,15 @@ k_fin_start(uint32_t nb, const double *partials, Scal *sc, double tol, int64_t i
 
 // ---- CG on A^ = I - C^ (unit diagonal; the Jacobi-preconditioned CG of linbcg in scaled form)
 
-// alpha = r.r / p.q (after k_cg_a)
+// alpha = r.r / p.q (after k_cg_a).  On the first iteration p = r0, so the next residual's
+// norm is known without forming it: |r0 - alpha q|^2 = rr - 2 alpha p.q + alpha^2 q.q
+// = alpha^2 q.q - rr.  If that already meets the stop test the solve ends here with
+// y = alpha p (`shortcut`; the consumers of y read p), and k_cg_b is skipped.
 __global__ void __launch_bounds__(kBlock)
 k_fin_a(uint32_t nb, const double *partials, Scal *sc) {
     if (sc->done) return;
-    double v[1];
-    sum_partials<1>(partials, nb, v);
+    double v[2];
+    sum_partials<2>(partials, nb, v);
     if (threadIdx.x != 0) return;
     sc->pq = v[0];
     sc->alpha = sc->rr / v[0];
@@ -335,6 +342,17 @@ k_fin_a(uint32_t nb, const double *partials, Scal *sc) {
         sc->done = 1;
         sc->alpha = 0.0;
         sc->flags |= FLAG_BREAKDOWN;
+        return;
+    }
+    if (sc->first && sc->allow_short) {
+        const double a = sc->alpha;
+        const double rr1 = a * a * v[1] - sc->rr;
+        if (rr1 <= sc->thresh2 && sc->itmax >= 1) {
+            sc->rr = rr1 > 0.0 ? rr1 : 0.0;
+            sc->iter = 1;
+            sc->shortcut = 1;
+            sc->done = 1;
+        }
     }
 }
 
@@ -380,9 +398,15 @@ k_fin_b(uint32_t nb, const double *partials, Scal *sc) {
 template <bool FromState>
 __global__ void __launch_bounds__(kBlock)
 k_finalize(Dims g, const double *__restrict__ tp, double *__restrict__ tpp_io,
-           const double *__restrict__ sv, const double *__restrict__ y, Scal *sc) {
+           const double *__restrict__ sv, const double *__restrict__ y,
+           const double *__restrict__ p0, Scal *sc) {
     const bool tiny = sc->tiny != 0;
     const bool have_y = sc->iter > 0;
+    // a solve that ended on its first iteration left y = alpha p0 unformed
+    const bool sh = sc->shortcut != 0;
+    const double alpha = sc->alpha;
+    const double *__restrict__ ys = sh ? p0 : y;
+    const double ysc = sh ? alpha : 1.0;
     int32_t fl = 0;
     HF_FOR_CELLS(g, I) {
         double x0, tpi = 0.0;
@@ -393,7 +417,7 @@ k_finalize(Dims g, const double *__restrict__ tp, double *__restrict__ tpp_io,
         } else {
             x0 = ldnt(&tpp_io[I]);
         }
-        const double x = have_y ? x0 + ldnt(&sv[I]) * ldnt(&y[I]) : x0;
+        const double x = have_y ? x0 + ldnt(&sv[I]) * (ysc * ldnt(&ys[I])) : x0;
         stnt(&tpp_io[I], x);
         if (isnan(x)) fl |= FLAG_NAN_X;
         if (FromState && (x + (x - tpi)) - x < kTiny) fl |= FLAG_TINY_NEXT;
@@ -419,8 +443,9 @@ k_fin_ir(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max) {
 
 // Starts an inner FP32 solve from the outer scalars: thresholds relative to this sweep's
 // residual, floored by the outer target (no point solving further than the outer test needs).
+// Launched with one thread.
 __global__ void k_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
-                              int64_t inner_itmax, double safety) {
+                              int64_t inner_itmax, double safety, int32_t allow_short) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double rr = outer->outer_rr;
     double t_rel = inner_tol * inner_tol * rr;
@@ -434,14 +459,69 @@ __global__ void k_inner_begin(Scal *outer, Scal *inner, double inner_tol, double
     inner->first = 1;
     inner->beta = 0.0;
     inner->alpha = 0.0;
+    inner->shortcut = 0;
+    inner->allow_short = allow_short;
     inner->done = (rr == 0.0) || isnan(rr) || inner_itmax <= 0;
+    // the refinement pass after this solve recomputes the next step's TINY test and S: only
+    // the last sweep of a step may set them
+    outer->flags &= ~FLAG_TINY_NEXT;
+    outer->flags_next = 0;
 }
 
-// Clears the per-step flag bits, keeping TINY_NEXT -> tiny for this step.
+// Clears the per-step flag bits, keeping TINY_NEXT -> tiny for this step; NaN bits of this
+// step's S found ahead by the previous fused step become current.
 __global__ void k_step_begin(Scal *sc) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     sc->tiny = (sc->flags & FLAG_TINY_NEXT) ? 1 : 0;
-    sc->flags = sc->flags & FLAG_NAN_B;
+    sc->flags = (sc->flags & FLAG_NAN_B) | sc->flags_next;
+    sc->flags_next = 0;
+}
+
+// Start of a carried mixed-precision step: the previous fused pass left this step's ||b^||^2
+// and its ||r^0||^2 for both TINY cases (as k_fin_start for a residual from the state).
+__global__ void k_carry_begin(Scal *sc, double tol, int64_t itmax, int32_t allow_short) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double rr = sc->next_rr[sc->tiny ? 1 : 0];
+    sc->bnorm2 = sc->next_bnorm2;
+    sc->rr = rr;
+    sc->rr_start = rr;
+    sc->outer_rr = rr;
+    sc->thresh2 = tol * tol * sc->next_bnorm2;
+    sc->iter = 0;
+    sc->itmax = itmax;
+    sc->first = 1;
+    sc->pq = 0.0;
+    sc->alpha = 0.0;
+    sc->beta = 0.0;
+    sc->shortcut = 0;
+    sc->allow_short = allow_short;
+    sc->done = (rr == 0.0) || isnan(rr) || itmax <= 0;
+    if (isnan(rr)) sc->flags |= FLAG_BREAKDOWN;
+}
+
+// After the fused mixed step: FP64 residual of this step (refinement test as k_fin_ir) and
+// the next step's norms.  partials: ||r^||^2, ||b^_next||^2, ||r^0_next||^2 (no TINY / TINY).
+__global__ void __launch_bounds__(kBlock)
+k_fin_step(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max) {
+    double v[4];
+    sum_partials<4>(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    const double prev = sc->outer_rr;
+    sc->outer_rr = v[0];
+    sc->iter += 1;
+    const bool stalled = sc->iter > 1 && v[0] > 0.25 * prev;
+    sc->done = (v[0] <= sc->thresh2) || sc->iter >= sweeps_max || stalled || isnan(v[0]);
+    if (isnan(v[0])) sc->flags |= FLAG_BREAKDOWN;
+    sc->next_bnorm2 = v[1];
+    sc->next_rr[0] = v[2];
+    sc->next_rr[1] = v[3];
+}
+
+// out = a + b (the fused mixed step's per-step constant part of S: Qdens + B)
+__global__ void __launch_bounds__(kBlock)
+k_sum2(Dims g, const double *__restrict__ a, const double *__restrict__ b,
+       double *__restrict__ out) {
+    HF_FOR_CELLS(g, I) { stnt(&out[I], ldnt(&a[I]) + ldnt(&b[I])); }
 }
 
 // TINY rule from an arbitrary state (first step after hf_set_state)
@@ -518,18 +598,19 @@ template void launch_cg_b<float>(const Dims &, float *, const float *, float *, 
                                  Scal *, double *, hipStream_t);
 
 void launch_finalize(const Dims &g, bool from_state, const double *tp, double *tpp_io,
-                     const double *sv, const double *y, Scal *sc, hipStream_t st) {
+                     const double *sv, const double *y, const double *p0, Scal *sc,
+                     hipStream_t st) {
     if (from_state)
-        HF_LAUNCH(k_finalize<true>, g, st, g, tp, tpp_io, sv, y, sc);
+        HF_LAUNCH(k_finalize<true>, g, st, g, tp, tpp_io, sv, y, p0, sc);
     else
-        HF_LAUNCH(k_finalize<false>, g, st, g, tp, tpp_io, sv, y, sc);
+        HF_LAUNCH(k_finalize<false>, g, st, g, tp, tpp_io, sv, y, p0, sc);
 }
 
 
 // the stencil kernels (k_rhs_res_z, k_cg_a_z, k_ir_res_z) run g.zblk blocks, the flat ones g.nblk
 void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double tol, int64_t itmax,
-                      hipStream_t st) {
-    HF_FIN(k_fin_start, g.zblk, st, partials, sc, tol, itmax);
+                      bool allow_short, hipStream_t st) {
+    HF_FIN(k_fin_start, g.zblk, st, partials, sc, tol, itmax, (int32_t)allow_short);
 }
 void launch_fin_a(const Dims &g, const double *partials, Scal *sc, hipStream_t st) {
     HF_FIN(k_fin_a, g.zblk, st, partials, sc);
@@ -541,15 +622,27 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
                    hipStream_t st) {
     HF_FIN(k_fin_ir, g.zblk, st, partials, sc, sweeps_max);
 }
+void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
+                     hipStream_t st) {
+    HF_FIN(k_fin_step, g.zblk, st, partials, sc, sweeps_max);
+}
+void launch_carry_begin(Scal *sc, double tol, int64_t itmax, bool allow_short, hipStream_t st) {
+    hipLaunchKernelGGL(k_carry_begin, dim3(1), dim3(64), 0, st, sc, tol, itmax,
+                       (int32_t)allow_short);
+}
 
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
-                        int64_t inner_itmax, double safety, hipStream_t st) {
+                        int64_t inner_itmax, double safety, bool allow_short, hipStream_t st) {
     hipLaunchKernelGGL(k_inner_begin, dim3(1), dim3(64), 0, st, outer, inner, inner_tol,
-                       outer_tol, inner_itmax, safety);
+                       outer_tol, inner_itmax, safety, (int32_t)allow_short);
 }
 
 void launch_step_begin(Scal *sc, hipStream_t st) {
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, st, sc);
+}
+
+void launch_sum2(const Dims &g, const double *a, const double *b, double *out, hipStream_t st) {
+    HF_LAUNCH(k_sum2, g, st, g, a, b, out);
 }
 
 void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,

@@ -39,6 +39,12 @@ struct IrParams {
     int32_t from_state;
 };
 
+// the fused mixed-precision step (k_ir_step_z)
+struct StepParams {
+    double inverse_time;
+    int32_t isteady, icattaneo;
+};
+
 void launch_assemble(const Dims &g, const AsmParams &ap, const double *kappa, const double *lx,
                      const double *ly, const double *lz, const double *rhoc,
                      const double *tau_s, double *d, double *cx, double *cy, double *cz,
@@ -78,23 +84,42 @@ void launch_cg_a(const Dims &g, const T *r, const T *pold, T *pnew, T *q, const 
 template <typename T>
 void launch_cg_b(const Dims &g, T *y, const T *p, T *r, const T *q, Scal *sc, double *partials,
                  hipStream_t st);
+// y of a solve is read from `y`, or as alpha * p0 when the solve ended on its first iteration
+// (Scal::shortcut of the solve's scalars)
 void launch_finalize(const Dims &g, bool from_state, const double *tp, double *tpp_io,
-                     const double *sv, const double *y, Scal *sc, hipStream_t st);
+                     const double *sv, const double *y, const double *p0, Scal *sc,
+                     hipStream_t st);
 void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const float *y,
-                   const double *sv, const double *S, const double *d, const double *cx,
-                   const double *cy, const double *cz, const double *tp, double *xout,
-                   float *rhat, Scal *sc, double *partials, hipStream_t st);
+                   const float *p0, const Scal *inner, const double *sv, const double *S,
+                   const double *d, const double *cx, const double *cy, const double *cz,
+                   const double *tp, double *xout, float *rhat, Scal *sc, double *partials,
+                   hipStream_t st);
+// Fused mixed-precision step: x' = base + s y (base = the state's x0, or xb), this step's
+// FP64 residual at x' (FP32 -> rres), x' -> xout, and the next step's scaled r^0 from
+// (x', T^n) without and with the TINY shift (FP32 -> rnext0 / rnext1); 4 partials.
+// qb = Qdens + B (per-step constant in this path)
+void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const double *tp,
+                    const double *tpp, const double *xb, const float *y, const float *p0,
+                    const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
+                    const double *qb, const double *d, const double *cx, const double *cy,
+                    const double *cz, double *xout, float *rres, float *rnext0, float *rnext1,
+                    Scal *sc, double *partials, hipStream_t st);
 // one-block finishing kernels: sum the per-block partials in block order, update the scalars
+// allow_short: the solve may end after its first iteration (k_fin_a)
 void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double tol, int64_t itmax,
-                      hipStream_t st);
+                      bool allow_short, hipStream_t st);
 void launch_fin_a(const Dims &g, const double *partials, Scal *sc, hipStream_t st);
 void launch_fin_b(const Dims &g, const double *partials, Scal *sc, hipStream_t st);
 void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                    hipStream_t st);
+void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
+                     hipStream_t st);
+void launch_carry_begin(Scal *sc, double tol, int64_t itmax, bool allow_short, hipStream_t st);
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
-                        int64_t inner_itmax, double safety, hipStream_t st);
+                        int64_t inner_itmax, double safety, bool allow_short, hipStream_t st);
 void launch_step_begin(Scal *sc, hipStream_t st);
 void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,
                             hipStream_t st);
+void launch_sum2(const Dims &g, const double *a, const double *b, double *out, hipStream_t st);
 
 }  // namespace hf

@@ -1,11 +1,16 @@
-// hf_stencil.hip -- the 7-point stencil kernels of the time step, z-marching.
+// hf_stencil.hip -- the 7-point stencil kernels of the time step: z-marching with the operand
+// plane staged in LDS.
 //
-// A block is a tile_x x tile_y (x by y) tile of columns; each thread walks its column through a chunk
-// of z-planes and keeps the stencil operand of planes z-1, z, z+1 -- and the z coupling of
-// plane z-1 -- in registers, so the z neighbours are never re-read; the x and y neighbours of
-// plane z come from the same tile rows (L1) or the adjacent tile (same XCD, L2).  Tiles are
-// numbered x fastest, then y, then z-chunk, and hardware block h runs logical tile
-// (h % 8) * (zblk / 8) + h / 8: each XCD owns a contiguous range of tiles, i.e. whole z-chunks.
+// A block of kZTX x kZTY threads owns a tile of columns and walks it through a chunk of
+// z-planes.  Each thread keeps the stencil operand of planes z-1, z, z+1 (and the z coupling of
+// plane z-1) of its own column in registers, so z neighbours are never re-read.  The operand of
+// plane z, with a one-cell halo ring around the tile, sits in LDS: every cell of the tile and
+// ring is loaded once per plane and the four in-plane neighbours are read from LDS.  Two LDS
+// buffers alternate, so plane z+1 is staged while plane z is computed, with one barrier per
+// plane.  Periodic wrap neighbours (boundary columns of a periodic axis) are read from memory.
+// Tiles are numbered x fastest, then y, then z-chunk; hardware block h runs logical tile
+// (h % 8) * (zblk / 8) + h / 8, so each XCD owns a contiguous range of tiles (whole z-chunks)
+// and the ring rows of neighbouring tiles come from the same L2.
 //
 // Sums run in the ascending-column order of the reference's serial SpMV (z-, y-, x-, diag,
 // x+, y+, z+; periodic wrap terms last), the same as the per-cell formulation.
@@ -13,6 +18,7 @@
 //   k_rhs_res_z  S, x0 = 2 T^n - T^{n-1} (+TINY), r^ = -s (S - H x0), norms  (mod_evolve.f90:112-141)
 //   k_cg_a_z     p = r + beta p, q = (I - C^) p, p.q                        (mod_Sparse.f90:404-411)
 //   k_ir_res_z   x' = x + s y, r^ = -s (S - H x'), ||r^||^2  (mixed-precision refinement)
+//   k_ir_step_z  the refinement residual fused with the next step's right-hand side
 #include "hf_device.hpp"
 #include "hf_kernels.hpp"
 
@@ -20,7 +26,9 @@ namespace hf {
 
 struct ZCol {
     uint32_t col, x, y, z0, z1;
-    bool active;
+    uint32_t bx, by, lx, ly;      // tile coordinates, thread position in the tile
+    bool active;                  // this thread's column is inside the grid
+    bool zvalid;                  // this block has planes to walk
     // x / y neighbours: existence and index offsets (constant along the column)
     bool hxm, hxp, hym, hyp;
     int32_t oxm, oxp, oym, oyp;   // signed offsets from I
@@ -34,12 +42,16 @@ __device__ __forceinline__ ZCol zcolumn(const Dims &g) {
     const uint32_t per_plane = g.ztx * g.zty;
     const uint32_t zc = lt / per_plane;
     const uint32_t rem = lt - zc * per_plane;
-    const uint32_t by = rem / g.ztx, bx = rem - by * g.ztx;
-    c.x = bx * g.tile_x + (threadIdx.x & (g.tile_x - 1));
-    c.y = by * g.tile_y + (threadIdx.x >> g.tile_xs);
+    c.by = rem / g.ztx;
+    c.bx = rem - c.by * g.ztx;
+    c.lx = threadIdx.x % kZTX;
+    c.ly = threadIdx.x / kZTX;
+    c.x = c.bx * kZTX + c.lx;
+    c.y = c.by * kZTY + c.ly;
     c.z0 = zc * g.zchunk;
     c.z1 = c.z0 + g.zchunk < g.nz ? c.z0 + g.zchunk : g.nz;
-    c.active = lt < g.ztiles && c.x < g.nx && c.y < g.ny && c.z0 < g.nz;
+    c.zvalid = lt < g.ztiles && c.z0 < g.nz;
+    c.active = c.zvalid && c.x < g.nx && c.y < g.ny;
     c.col = c.x + g.nx * c.y;
     c.hxm = c.x > 0 || g.per[0];
     c.hxp = c.x + 1 < g.nx || g.per[0];
@@ -56,22 +68,143 @@ __device__ __forceinline__ ZCol zcolumn(const Dims &g) {
     return c;
 }
 
-// Sum of the seven stencil terms of H x in the reference's serial order; xm_z/xp_z and cz_m are
-// the z neighbours' operands and the z- coupling carried in registers.
-template <class XF>
+// LDS planes of the operand: two buffers of (kZTY + 2) x (kZTX + 2) values (tile + halo ring).
+template <typename V>
+struct PlaneBuf {
+    V v[2][kZTY + 2][kZTX + 2];
+};
+
+// In-plane neighbours of a cell, read from the staged LDS plane when used (c points at the
+// cell's own slot); meaningful where the neighbour exists and is not a periodic wrap.
+template <typename V>
+struct InPlane {
+    const V *c;
+    __device__ __forceinline__ V xm() const { return c[-1]; }
+    __device__ __forceinline__ V xp() const { return c[1]; }
+    __device__ __forceinline__ V ym() const { return c[-(int)(kZTX + 2)]; }
+    __device__ __forceinline__ V yp() const { return c[(int)(kZTX + 2)]; }
+};
+
+// A pair of stencil operands staged together (the fused step: x' and the next x0).
+struct D2 {
+    double a, b;
+    __device__ D2() = default;
+    __device__ constexpr D2(double v) : a(v), b(v) {}
+    __device__ constexpr D2(double x, double y) : a(x), b(y) {}
+};
+
+// One component of the in-plane neighbours of a staged pair.
+template <int K>
+struct InPlaneOf {
+    const D2 *c;
+    __device__ __forceinline__ static double pick(const D2 &v) { return K == 0 ? v.a : v.b; }
+    __device__ __forceinline__ double xm() const { return pick(c[-1]); }
+    __device__ __forceinline__ double xp() const { return pick(c[1]); }
+    __device__ __forceinline__ double ym() const { return pick(c[-(int)(kZTX + 2)]); }
+    __device__ __forceinline__ double yp() const { return pick(c[(int)(kZTX + 2)]); }
+};
+
+// The halo-ring cell a thread stages (threads < kZRing; corners are not needed): its slot in
+// the LDS plane and its column, or kNoRing.  Ring cells outside the grid are not loaded: a
+// wrap neighbour is read from memory by its user.
+struct Ring {
+    int slot;
+    uint32_t col;
+};
+constexpr uint32_t kNoRing = 0xffffffffu;
+
+constexpr int kZRing = 2 * (int)kZTX + 2 * (int)kZTY;
+static_assert(kZRing <= (int)(kZTX * kZTY), "ring loader needs one thread per ring cell");
+
+__device__ __forceinline__ Ring ring_cell(const Dims &g, const ZCol &c) {
+    Ring r{0, kNoRing};
+    const int t = (int)threadIdx.x;
+    const int TX = (int)kZTX, TY = (int)kZTY;
+    if (t >= kZRing || !c.zvalid) return r;
+    int lx, ly;
+    if (t < TX) {
+        lx = t + 1; ly = 0;
+    } else if (t < 2 * TX) {
+        lx = t - TX + 1; ly = TY + 1;
+    } else if (t < 2 * TX + TY) {
+        lx = 0; ly = t - 2 * TX + 1;
+    } else {
+        lx = TX + 1; ly = t - 2 * TX - TY + 1;
+    }
+    const int64_t gx = (int64_t)c.bx * TX + lx - 1, gy = (int64_t)c.by * TY + ly - 1;
+    r.slot = ly * (TX + 2) + lx;
+    if (gx >= 0 && gx < (int64_t)g.nx && gy >= 0 && gy < (int64_t)g.ny)
+        r.col = (uint32_t)gx + g.nx * (uint32_t)gy;
+    return r;
+}
+
+// index of the z-1 / z+1 neighbour of (col, z) (periodic wrap; only used when it exists)
+__device__ __forceinline__ uint32_t zm_index(const Dims &g, uint32_t col, uint32_t z) {
+    return z > 0 ? col + g.nxny * (z - 1) : col + g.nxny * (g.nz - 1);
+}
+__device__ __forceinline__ uint32_t zp_index(const Dims &g, uint32_t col, uint32_t z) {
+    return z + 1 < g.nz ? col + g.nxny * (z + 1) : col;
+}
+
+// Walks this block's z-chunk.  X(J) is the stencil operand at cell J (it may combine several
+// arrays); cell(I, z, xi, xz_m, xz_p, nb) runs for every active cell with its own operand, the
+// z neighbours' operands and the in-plane neighbours from LDS.  Every thread of the block runs
+// the same number of trips (one barrier each); inactive threads only stage.
+template <typename V, class XF, class CF>
+__device__ __forceinline__ void zmarch(const Dims &g, const ZCol &c, PlaneBuf<V> &sp, XF X,
+                                       CF cell) {
+    const Ring rg = ring_cell(g, c);
+    const bool per_z = g.per[2];
+    V xz_m = (V)0, xi = (V)0;
+    if (c.active) {
+        if (c.z0 > 0 || per_z) xz_m = X(zm_index(g, c.col, c.z0));
+        xi = X(c.col + g.nxny * c.z0);
+    }
+    if (c.zvalid) {
+        sp.v[0][c.ly + 1][c.lx + 1] = xi;
+        if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = X(rg.col + g.nxny * c.z0);
+    }
+    int buf = 0;
+    uint32_t I = c.col + g.nxny * c.z0;
+    for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
+        const bool more = z + 1 < c.z1;
+        V xz_p = (V)0, rnext = (V)0;
+        if (c.active && (z + 1 < g.nz || per_z)) xz_p = X(zp_index(g, c.col, z));
+        if (more && rg.col != kNoRing) rnext = X(rg.col + g.nxny * (z + 1));
+        __syncthreads();   // plane z is staged in sp.v[buf]
+        if (c.active) {
+            const InPlane<V> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
+            cell(I, z, xi, xz_m, xz_p, nb);
+        }
+        if (more) {
+            // sp.v[buf ^ 1] was last read before this trip's barrier
+            sp.v[buf ^ 1][c.ly + 1][c.lx + 1] = xz_p;
+            if (rg.col != kNoRing) (&sp.v[buf ^ 1][0][0])[rg.slot] = rnext;
+        }
+        buf ^= 1;
+        xz_m = xi;
+        xi = xz_p;
+    }
+}
+
+// Sum of the seven stencil terms of H x in the reference's serial order: xz_m / xz_p and cz_m
+// are the z neighbours' operands and the z- coupling carried in registers, nb the in-plane
+// neighbours from LDS; periodic wrap terms (last) read the operand X from memory.
+template <class NB, class XF>
 __device__ __forceinline__ double hx_sum(const Dims &g, const ZCol &c, uint32_t I, uint32_t z,
                                          double di, const double *__restrict__ cx,
                                          const double *__restrict__ cy, double czc, double xi,
-                                         double xz_m, double xz_p, double cz_m, XF X) {
+                                         double xz_m, double xz_p, double cz_m, const NB &nb,
+                                         XF X) {
     const bool hzm = z > 0 || g.per[2], hzp = z + 1 < g.nz || g.per[2];
     const bool wzm = z == 0, wzp = z + 1 == g.nz;
     double h = 0.0;
     if (hzm && !wzm) h = h + cz_m * xz_m;
-    if (c.hym && !c.wym) h = h + cy[I + c.oym] * X(I + c.oym);
-    if (c.hxm && !c.wxm) h = h + cx[I + c.oxm] * X(I + c.oxm);
+    if (c.hym && !c.wym) h = h + cy[I + c.oym] * nb.ym();
+    if (c.hxm && !c.wxm) h = h + cx[I + c.oxm] * nb.xm();
     h = h + di * xi;
-    if (c.hxp && !c.wxp) h = h + cx[I] * X(I + c.oxp);
-    if (c.hyp && !c.wyp) h = h + cy[I] * X(I + c.oyp);
+    if (c.hxp && !c.wxp) h = h + cx[I] * nb.xp();
+    if (c.hyp && !c.wyp) h = h + cy[I] * nb.yp();
     if (hzp && !wzp) h = h + czc * xz_p;
     // periodic wrap couplings
     if (c.hxm && c.wxm) h = h + cx[I + c.oxm] * X(I + c.oxm);
@@ -83,16 +216,8 @@ __device__ __forceinline__ double hx_sum(const Dims &g, const ZCol &c, uint32_t 
     return h;
 }
 
-// index of the z-1 / z+1 neighbour of (col, z) (periodic wrap; only used when it exists)
-__device__ __forceinline__ uint32_t zm_index(const Dims &g, uint32_t col, uint32_t z) {
-    return z > 0 ? col + g.nxny * (z - 1) : col + g.nxny * (g.nz - 1);
-}
-__device__ __forceinline__ uint32_t zp_index(const Dims &g, uint32_t col, uint32_t z) {
-    return z + 1 < g.nz ? col + g.nxny * (z + 1) : col;
-}
-
 template <typename OutT, bool FromState, bool StoreS, bool StoreX>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kZTX * kZTY)
 k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *__restrict__ tpp,
             const double *__restrict__ xin, const double *__restrict__ bin,
             const double *__restrict__ rhoc, const double *__restrict__ cvec,
@@ -101,6 +226,7 @@ k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *_
             const double *__restrict__ cy, const double *__restrict__ cz,
             OutT *__restrict__ rhat, double *__restrict__ sout, double *__restrict__ xout,
             Scal *sc, double *partials) {
+    __shared__ PlaneBuf<double> sp;
     const bool tiny = sc->tiny != 0;
     auto X = [&](uint32_t J) -> double {
         if (FromState) {
@@ -114,152 +240,241 @@ k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *_
     double acc[2] = {0.0, 0.0};
     int32_t fl = 0;
     const ZCol c = zcolumn(g);
-    if (c.active) {
-        const bool per_z = g.per[2];
-        const bool h0 = c.z0 > 0 || per_z;
-        double xz_m = 0.0, cz_m = 0.0;
-        if (h0) {
-            const uint32_t J = zm_index(g, c.col, c.z0);
-            xz_m = X(J);
-            cz_m = cz[J];
-        }
-        uint32_t I = c.col + g.nxny * c.z0;
-        double xi = X(I);
-        for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
-            const bool hzp = z + 1 < g.nz || per_z;
-            const double xz_p = hzp ? X(zp_index(g, c.col, z)) : 0.0;
-            double S;
-            if (FromState) {
-                // mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36
-                if (rp.isteady == 0) {
-                    S = -(rp.inverse_time * tp[I] * ldnt(&rhoc[I])) - ldnt(&qd[I]) - ldnt(&bvec[I]);
-                    if (rp.icattaneo == 1) {
-                        double scat = (tpp[I] - 2.0 * tp[I]) * ldnt(&cvec[I]);
-                        if (isnan(scat)) fl |= FLAG_NAN_SCAT;
-                        S = S + scat;
-                    }
-                } else {
-                    S = -ldnt(&qd[I]) - ldnt(&bvec[I]);
+    double cz_m = 0.0;
+    if (c.active && (c.z0 > 0 || g.per[2])) cz_m = cz[zm_index(g, c.col, c.z0)];
+    zmarch<double>(g, c, sp, X, [&](uint32_t I, uint32_t z, double xi, double xz_m, double xz_p,
+                                    const InPlane<double> &nb) {
+        double S;
+        if (FromState) {
+            // mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36
+            if (rp.isteady == 0) {
+                S = -(rp.inverse_time * tp[I] * ldnt(&rhoc[I])) - ldnt(&qd[I]) - ldnt(&bvec[I]);
+                if (rp.icattaneo == 1) {
+                    double scat = (tpp[I] - 2.0 * tp[I]) * ldnt(&cvec[I]);
+                    if (isnan(scat)) fl |= FLAG_NAN_SCAT;
+                    S = S + scat;
                 }
-                if (isnan(S)) fl |= FLAG_NAN_S;
             } else {
-                S = ldnt(&bin[I]);
+                S = -ldnt(&qd[I]) - ldnt(&bvec[I]);
             }
-            const double di = ldnt(&d[I]);
-            const double czc = ldnt(&cz[I]);
-            const double hxv = hx_sum(g, c, I, z, di, cx, cy, czc, xi, xz_m, xz_p, cz_m, X);
-            const double r = S - hxv;
-            const double s = 1.0 / sqrt(-di);
-            const double rh = -s * r;
-            stnt(&rhat[I], (OutT)rh);
-            if (StoreS) stnt(&sout[I], S);
-            if (StoreX) stnt(&xout[I], xi);
-            const double bh = s * S;
-            const double rq = (double)(OutT)rh;   // r.r of the stored (possibly FP32) residual
-            acc[0] += bh * bh;
-            acc[1] += rq * rq;
-            xz_m = xi;
-            xi = xz_p;
-            cz_m = czc;
+            if (isnan(S)) fl |= FLAG_NAN_S;
+        } else {
+            S = ldnt(&bin[I]);
         }
-    }
+        const double di = ldnt(&d[I]);
+        const double czc = ldnt(&cz[I]);
+        const double hxv = hx_sum(g, c, I, z, di, cx, cy, czc, xi, xz_m, xz_p, cz_m, nb, X);
+        const double r = S - hxv;
+        const double s = 1.0 / sqrt(-di);
+        const double rh = -s * r;
+        stnt(&rhat[I], (OutT)rh);
+        if (StoreS) stnt(&sout[I], S);
+        if (StoreX) stnt(&xout[I], xi);
+        const double bh = s * S;
+        const double rq = (double)(OutT)rh;   // r.r of the stored (possibly FP32) residual
+        acc[0] += bh * bh;
+        acc[1] += rq * rq;
+        cz_m = czc;
+    });
     block_flags(fl, &sc->flags);
     block_partials<2>(acc, partials);
 }
 
-// p = r + beta * p_old (p = r on the first iteration), q = A^ p, partial p.q.
+// p = r + beta * p_old (p = r on the first iteration), q = A^ p, partials p.q and q.q (the
+// latter for the first-iteration stop test of k_fin_a).
 template <typename T>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kZTX * kZTY)
 k_cg_a_z(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
          T *__restrict__ q, const T *__restrict__ hx, const T *__restrict__ hy,
          const T *__restrict__ hz, Scal *sc, double *partials) {
+    __shared__ PlaneBuf<T> sp;
     if (sc->done) return;
     const bool first = sc->first != 0;
     const T beta = (T)sc->beta;
     auto P = [&](uint32_t J) -> T { return first ? r[J] : (T)(r[J] + beta * pold[J]); };
-    double acc[1] = {0.0};
+    double acc[2] = {0.0, 0.0};
     const ZCol c = zcolumn(g);
-    if (c.active) {
-        const bool per_z = g.per[2];
-        T pz_m = (T)0, hz_m = (T)0;
-        if (c.z0 > 0 || per_z) {
-            const uint32_t J = zm_index(g, c.col, c.z0);
-            pz_m = P(J);
-            hz_m = hz[J];
-        }
-        uint32_t I = c.col + g.nxny * c.z0;
-        T pi = P(I);
-        for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
-            const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
-            const T pz_p = hzp ? P(zp_index(g, c.col, z)) : (T)0;
-            const T hzc = ldnt(&hz[I]);
-            T s = (T)0;
-            if (hzm) s = s + hz_m * pz_m;
-            if (c.hym) s = s + hy[I + c.oym] * P(I + c.oym);
-            if (c.hxm) s = s + hx[I + c.oxm] * P(I + c.oxm);
-            if (c.hxp) s = s + hx[I] * P(I + c.oxp);
-            if (c.hyp) s = s + hy[I] * P(I + c.oyp);
-            if (hzp) s = s + hzc * pz_p;
-            const T qi = pi - s;
-            stnt(&pnew[I], pi);
-            stnt(&q[I], qi);
-            acc[0] += (double)pi * (double)qi;
-            pz_m = pi;
-            pi = pz_p;
-            hz_m = hzc;
-        }
-    }
-    block_partials<1>(acc, partials);
+    const bool per_z = g.per[2];
+    T hz_m = (T)0;
+    if (c.active && (c.z0 > 0 || per_z)) hz_m = hz[zm_index(g, c.col, c.z0)];
+    zmarch<T>(g, c, sp, P, [&](uint32_t I, uint32_t z, T pi, T pz_m, T pz_p,
+                               const InPlane<T> &nb) {
+        const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
+        const T hzc = ldnt(&hz[I]);
+        T s = (T)0;
+        if (hzm) s = s + hz_m * pz_m;
+        if (c.hym) s = s + hy[I + c.oym] * (c.wym ? P(I + c.oym) : nb.ym());
+        if (c.hxm) s = s + hx[I + c.oxm] * (c.wxm ? P(I + c.oxm) : nb.xm());
+        if (c.hxp) s = s + hx[I] * (c.wxp ? P(I + c.oxp) : nb.xp());
+        if (c.hyp) s = s + hy[I] * (c.wyp ? P(I + c.oyp) : nb.yp());
+        if (hzp) s = s + hzc * pz_p;
+        const T qi = pi - s;
+        stnt(&pnew[I], pi);
+        stnt(&q[I], qi);
+        acc[0] += (double)pi * (double)qi;
+        acc[1] += (double)qi * (double)qi;
+        hz_m = hzc;
+    });
+    block_partials<2>(acc, partials);
 }
 
 // Mixed-precision refinement: x' = x + s y (y FP32) formed on the fly, r^ = -s (S - H x')
 // stored as FP32 for the next inner solve, ||r^||^2, NaN and next-step TINY tests on x'.
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kZTX * kZTY)
 k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__restrict__ y,
-           const double *__restrict__ sv, const double *__restrict__ S,
-           const double *__restrict__ d, const double *__restrict__ cx,
-           const double *__restrict__ cy, const double *__restrict__ cz,
-           const double *__restrict__ tp, double *__restrict__ xout, float *__restrict__ rhat,
-           Scal *sc, double *partials) {
-    auto X = [&](uint32_t J) -> double { return x[J] + sv[J] * (double)y[J]; };
+           const float *__restrict__ p0, const Scal *inner, const double *__restrict__ sv,
+           const double *__restrict__ S, const double *__restrict__ d,
+           const double *__restrict__ cx, const double *__restrict__ cy,
+           const double *__restrict__ cz, const double *__restrict__ tp,
+           double *__restrict__ xout, float *__restrict__ rhat, Scal *sc, double *partials) {
+    __shared__ PlaneBuf<double> sp;
+    // an inner solve that ended on its first iteration left y = alpha p0 unformed; it is
+    // formed here as k_cg_b would have (FP32 product)
+    const bool sh = inner->shortcut != 0;
+    const float *__restrict__ ys = sh ? p0 : y;
+    const float ysc = sh ? (float)inner->alpha : 1.0f;
+    auto X = [&](uint32_t J) -> double { return x[J] + sv[J] * (double)(ysc * ys[J]); };
     double acc[1] = {0.0};
     int32_t fl = 0;
     const ZCol c = zcolumn(g);
-    if (c.active) {
-        const bool per_z = g.per[2];
-        double xz_m = 0.0, cz_m = 0.0;
-        if (c.z0 > 0 || per_z) {
-            const uint32_t J = zm_index(g, c.col, c.z0);
-            xz_m = X(J);
-            cz_m = cz[J];
-        }
-        uint32_t I = c.col + g.nxny * c.z0;
-        double xi = X(I);
-        for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
-            const bool hzp = z + 1 < g.nz || per_z;
-            const double xz_p = hzp ? X(zp_index(g, c.col, z)) : 0.0;
-            const double czc = ldnt(&cz[I]);
-            const double hxv = hx_sum(g, c, I, z, ldnt(&d[I]), cx, cy, czc, xi, xz_m, xz_p, cz_m, X);
-            const double rh = -sv[I] * (ldnt(&S[I]) - hxv);
-            const float rf = (float)rh;
-            stnt(&rhat[I], rf);
-            stnt(&xout[I], xi);
-            acc[0] += (double)rf * (double)rf;
-            if (isnan(xi)) fl |= FLAG_NAN_X;
-            if (ip.from_state && (xi + (xi - ldnt(&tp[I]))) - xi < kTiny) fl |= FLAG_TINY_NEXT;
-            xz_m = xi;
-            xi = xz_p;
-            cz_m = czc;
-        }
-    }
+    double cz_m = 0.0;
+    if (c.active && (c.z0 > 0 || g.per[2])) cz_m = cz[zm_index(g, c.col, c.z0)];
+    zmarch<double>(g, c, sp, X, [&](uint32_t I, uint32_t z, double xi, double xz_m, double xz_p,
+                                    const InPlane<double> &nb) {
+        const double czc = ldnt(&cz[I]);
+        const double hxv =
+            hx_sum(g, c, I, z, ldnt(&d[I]), cx, cy, czc, xi, xz_m, xz_p, cz_m, nb, X);
+        const double rh = -sv[I] * (ldnt(&S[I]) - hxv);
+        const float rf = (float)rh;
+        stnt(&rhat[I], rf);
+        stnt(&xout[I], xi);
+        acc[0] += (double)rf * (double)rf;
+        if (isnan(xi)) fl |= FLAG_NAN_X;
+        if (ip.from_state && (xi + (xi - ldnt(&tp[I]))) - xi < kTiny) fl |= FLAG_TINY_NEXT;
+        cz_m = czc;
+    });
     block_flags(fl, &sc->flags);
     block_partials<1>(acc, partials);
+}
+
+// Fused step of the mixed-precision path, one pass over the grid after an inner solve:
+//   x' = base + s y               base = x0 of this step from the state (2 T^n - T^{n-1},
+//                                 +TINY, mod_evolve.f90:140-141) or a refined xb
+//   r^ = -s (S_n - H x')          this step's FP64 refinement residual -> rres (FP32)
+//   x1 = x' + (x' - T^n)          the next step's initial guess (no TINY yet)
+//   r^0 = -s (S_{n+1} - H x1)     the next step's residual, S_{n+1} from (x', T^n) -> rnext0;
+//                                 (S uses qb = Qdens + B, constant over the run)
+//                                 with the TINY shift x1 + TINY: r^0 + TINY s (H 1) -> rnext1
+// The TINY test of the next step (any x1 - x' < TINY) is an OR over the grid, known only after
+// the pass, so both next residuals are kept and the host takes the one the flag selects.  x'
+// and x1 are staged together in LDS (two stencil operands, one halo ring).
+// Two staged operands need ~86 VGPRs (5 waves per SIMD, i.e. 2 blocks of 8 waves per CU); held
+// to 80 (6 waves, 3 blocks per CU, a few spilled loop invariants) it runs ~10% faster at 512^3.
+template <bool FromState>
+__global__ void __launch_bounds__(kZTX * kZTY) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
+            const double *__restrict__ tpp, const double *__restrict__ xb,
+            const float *__restrict__ y, const float *__restrict__ p0, const Scal *inner,
+            const double *__restrict__ sv, const double *__restrict__ rhoc,
+            const double *__restrict__ cvec, const double *__restrict__ qb,
+            const double *__restrict__ d, const double *__restrict__ cx,
+            const double *__restrict__ cy, const double *__restrict__ cz,
+            double *__restrict__ xout, float *__restrict__ rres,
+            float *__restrict__ rnext0, float *__restrict__ rnext1, Scal *sc,
+            double *partials) {
+    __shared__ PlaneBuf<D2> sp;
+    const bool tiny = sc->tiny != 0;
+    // y = alpha p0 of an inner solve that ended on its first iteration, as k_cg_b forms it
+    const bool sh = inner->shortcut != 0;
+    const float *__restrict__ ys = sh ? p0 : y;
+    const float ysc = sh ? (float)inner->alpha : 1.0f;
+    auto X = [&](uint32_t J) -> D2 {
+        const double tpj = tp[J];
+        double base;
+        if (FromState) {
+            base = tpj + (tpj - tpp[J]);
+            if (tiny) base = base + kTiny;
+        } else {
+            base = xb[J];
+        }
+        const double xn = base + sv[J] * (double)(ysc * ys[J]);
+        return D2(xn, xn + (xn - tpj));
+    };
+    auto Xa = [&](uint32_t J) -> double { return X(J).a; };
+    auto Xb = [&](uint32_t J) -> double { return X(J).b; };
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    int32_t fl = 0, fl_next = 0;
+    const ZCol c = zcolumn(g);
+    double cz_m = 0.0;
+    if (c.active && (c.z0 > 0 || g.per[2])) cz_m = cz[zm_index(g, c.col, c.z0)];
+    zmarch<D2>(g, c, sp, X, [&](uint32_t I, uint32_t z, D2 xi, D2 xz_m, D2 xz_p,
+                                const InPlane<D2> &nb) {
+        const double di = ldnt(&d[I]);
+        const double czc = ldnt(&cz[I]);
+        const double qbi = ldnt(&qb[I]);
+        const double tpi = tp[I];
+        double rc = 0.0, cv = 0.0;
+        if (sp_.isteady == 0) {
+            rc = ldnt(&rhoc[I]);
+            if (sp_.icattaneo == 1) cv = ldnt(&cvec[I]);
+        }
+        // S of this step (mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36) and of the
+        // next, with Qdens + B summed once per run (qb)
+        double Sn, Sn1;
+        if (sp_.isteady == 0) {
+            Sn = -(sp_.inverse_time * tpi * rc) - qbi;
+            Sn1 = -(sp_.inverse_time * xi.a * rc) - qbi;
+            if (sp_.icattaneo == 1) {
+                Sn = Sn + (ldnt(&tpp[I]) - 2.0 * tpi) * cv;
+                const double scat = (tpi - 2.0 * xi.a) * cv;
+                if (isnan(scat)) fl_next |= FLAG_NAN_SCAT;
+                Sn1 = Sn1 + scat;
+            }
+        } else {
+            Sn = -qbi;
+            Sn1 = Sn;
+        }
+        if (isnan(Sn1)) fl_next |= FLAG_NAN_S;
+        const double h1 = hx_sum(g, c, I, z, di, cx, cy, czc, xi.a, xz_m.a, xz_p.a, cz_m,
+                                 InPlaneOf<0>{nb.c}, Xa);
+        const double h2 = hx_sum(g, c, I, z, di, cx, cy, czc, xi.b, xz_m.b, xz_p.b, cz_m,
+                                 InPlaneOf<1>{nb.c}, Xb);
+        // H 1: the row sum over the same terms
+        const bool hzm = z > 0 || g.per[2], hzp = z + 1 < g.nz || g.per[2];
+        double h1s = di;
+        if (hzm) h1s = h1s + cz_m;
+        if (c.hym) h1s = h1s + cy[I + c.oym];
+        if (c.hxm) h1s = h1s + cx[I + c.oxm];
+        if (c.hxp) h1s = h1s + cx[I];
+        if (c.hyp) h1s = h1s + cy[I];
+        if (hzp) h1s = h1s + czc;
+        const double s = 1.0 / sqrt(-di);
+        const float rf = (float)(-s * (Sn - h1));
+        const double ra = -s * (Sn1 - h2);
+        const float fa = (float)ra;
+        const float fb = (float)(ra + kTiny * s * h1s);
+        stnt(&xout[I], xi.a);
+        stnt(&rres[I], rf);
+        stnt(&rnext0[I], fa);
+        stnt(&rnext1[I], fb);
+        const double bh = s * Sn1;
+        acc[0] += (double)rf * (double)rf;
+        acc[1] += bh * bh;
+        acc[2] += (double)fa * (double)fa;
+        acc[3] += (double)fb * (double)fb;
+        if (isnan(xi.a)) fl |= FLAG_NAN_X;
+        if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;   // mod_evolve.f90:141 for the next step
+        cz_m = czc;
+    });
+    block_flags(fl, &sc->flags);
+    block_flags(fl_next, &sc->flags_next);
+    block_partials<4>(acc, partials);
 }
 
 // ---- launchers -----------------------------------------------------------------------------
 
 #define HF_ZLAUNCH(kern, g, st, ...) \
-    hipLaunchKernelGGL(kern, dim3((g).zblk), dim3((g).tile_x * (g).tile_y), 0, st, __VA_ARGS__)
+    hipLaunchKernelGGL(kern, dim3((g).zblk), dim3(kZTX * kZTY), 0, st, __VA_ARGS__)
 
 void launch_rhs_res_state64(const Dims &g, const RhsParams &rp, const double *tp,
                             const double *tpp, const double *rhoc, const double *cvec,
@@ -281,8 +496,13 @@ void launch_rhs_res_state32(const Dims &g, const RhsParams &rp, const double *tp
                             const double *cx, const double *cy, const double *cz, float *rhat,
                             double *sout, double *xout, Scal *sc, double *partials,
                             hipStream_t st) {
-    HF_ZLAUNCH((k_rhs_res_z<float, true, true, true>), g, st, g, rp, tp, tpp, nullptr, nullptr,
-               rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, sout, xout, sc, partials);
+    if (sout && xout)
+        HF_ZLAUNCH((k_rhs_res_z<float, true, true, true>), g, st, g, rp, tp, tpp, nullptr,
+                   nullptr, rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, sout, xout, sc, partials);
+    else
+        HF_ZLAUNCH((k_rhs_res_z<float, true, false, false>), g, st, g, rp, tp, tpp, nullptr,
+                   nullptr, rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
+                   partials);
 }
 
 void launch_rhs_res_solve64(const Dims &g, const RhsParams &rp, const double *x,
@@ -316,10 +536,26 @@ template void launch_cg_a<float>(const Dims &, const float *, const float *, flo
                                  hipStream_t);
 
 void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const float *y,
-                   const double *sv, const double *S, const double *d, const double *cx,
-                   const double *cy, const double *cz, const double *tp, double *xout,
-                   float *rhat, Scal *sc, double *partials, hipStream_t st) {
-    HF_ZLAUNCH(k_ir_res_z, g, st, g, ip, x, y, sv, S, d, cx, cy, cz, tp, xout, rhat, sc, partials);
+                   const float *p0, const Scal *inner, const double *sv, const double *S,
+                   const double *d, const double *cx, const double *cy, const double *cz,
+                   const double *tp, double *xout, float *rhat, Scal *sc, double *partials,
+                   hipStream_t st) {
+    HF_ZLAUNCH(k_ir_res_z, g, st, g, ip, x, y, p0, inner, sv, S, d, cx, cy, cz, tp, xout, rhat,
+               sc, partials);
+}
+
+void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const double *tp,
+                    const double *tpp, const double *xb, const float *y, const float *p0,
+                    const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
+                    const double *qb, const double *d, const double *cx, const double *cy,
+                    const double *cz, double *xout, float *rres, float *rnext0, float *rnext1,
+                    Scal *sc, double *partials, hipStream_t st) {
+    if (from_state)
+        HF_ZLAUNCH(k_ir_step_z<true>, g, st, g, sp, tp, tpp, xb, y, p0, inner, sv, rhoc, cvec, qb,
+                   d, cx, cy, cz, xout, rres, rnext0, rnext1, sc, partials);
+    else
+        HF_ZLAUNCH(k_ir_step_z<false>, g, st, g, sp, tp, tpp, xb, y, p0, inner, sv, rhoc, cvec,
+                   qb, d, cx, cy, cz, xout, rres, rnext0, rnext1, sc, partials);
 }
 
 }  // namespace hf

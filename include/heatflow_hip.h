@@ -150,8 +150,10 @@ int hf_solve(hf_ctx *ctx, const double *b, double *x, int32_t itol, double tol, 
  *         3: diagonal of H, 4/5/6: forward couplings in x/y/z (0 where absent) */
 int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
 
-/* Debug switches (no reference counterpart): bit 0 keeps S of every FP64 step for
- * hf_get_vector(2) (costs one extra N-vector write per step). */
+/* Debug switches (no reference counterpart): bit 0 keeps S of every step for hf_get_vector(2)
+ * (costs one extra N-vector write per step; mixed mode then runs the unfused step); bit 1
+ * turns off the first-iteration stop of CG (a solve whose first iteration already meets the
+ * tolerance ends there, with the next residual norm predicted from p.q and q.q). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */
@@ -160,6 +162,7 @@ typedef struct hf_stats {
     int64_t outer_iters;      /* FP64 CG iterations (or refinement sweeps in mixed mode) */
     int64_t inner_iters;      /* FP32 inner CG iterations (mixed mode)                 */
     double last_err;          /* final relative residual of the last solve             */
+    int64_t first_iter_stops; /* CG solves that ended on their first iteration          */
 } hf_stats;
 int hf_get_stats(hf_ctx *ctx, hf_stats *st);
 int hf_reset_stats(hf_ctx *ctx);
@@ -172,7 +175,7 @@ int hf_reset_stats(hf_ctx *ctx);
  * arrays each kernel touches, every array element counted once).  Reading resets the pool. */
 enum {
     HF_K_QDENS = 0, HF_K_RHS64, HF_K_RHS32, HF_K_CG_A64, HF_K_CG_B64, HF_K_CG_A32, HF_K_CG_B32,
-    HF_K_FINAL, HF_K_IR_RES, HF_K_COUNT
+    HF_K_FINAL, HF_K_IR_RES, HF_K_IR_STEP, HF_K_COUNT
 };
 int hf_set_timing(hf_ctx *ctx, int32_t on);
 int hf_kernel_times(hf_ctx *ctx, double *ms, int64_t *launches, double *bytes);

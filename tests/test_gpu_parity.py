@@ -263,3 +263,94 @@ def test_fp32_ir_matches_fp64(n):
     err = np.abs(t32 - t64) / np.abs(t64)
     assert err.max() <= 1e-10
     assert st.inner_iters > 0 and st.outer_iters >= 20
+
+
+MIXED_GEOMETRIES = GEOMETRIES + [dict(nx=5, ny=4, nz=3, isteady=1)]
+
+
+@pytest.mark.parametrize("gi", range(len(MIXED_GEOMETRIES)))
+def test_fp32_ir_steps_match_oracle(gi):
+    """Mixed precision on every geometry (periodic axes, CG faces, TINY-dropped couplings,
+    heater modes 1-6 with radiation, Fourier, steady state), 6 steps in one hf_run from a
+    random state, against the oracle at the reference tolerance.  Per-step constant heating
+    takes the fused step (refinement residual + next right-hand side in one pass, residual
+    carried between steps); time-dependent heating or radiation the unfused one."""
+    s = random_system(700 + gi, **MIXED_GEOMETRIES[gi])
+    tp0, tpp0 = random_state(gi, s.params.n)
+    steps = [1, 2, 3, 4, 5, 6]
+    rc, otp, otpp, _ = oracle.OracleSystem(s).simulate_steps(steps, tp0, tpp0)
+    assert rc == 0
+    with Simulation(s, precision="fp32_ir", tol=1e-14) as sim:
+        sim.set_state(tp0, tpp0)
+        sim.run(1, len(steps))
+        tp, tpp = sim.get_state()
+        st = sim.stats()
+    assert relerr(tp, otp) <= 1e-11 and relerr(tpp, otpp) <= 1e-11
+    assert st.outer_iters >= len(steps)
+
+
+def test_fp32_ir_carried_residual_matches_fresh_start():
+    """hf_run carries each step's initial residual from the previous step's fused pass; with
+    the state set again before every step it is recomputed from the state by k_rhs_res.  Both
+    match the oracle (log-normal kappa, Gaussian heating, 20^3, 12 steps) and each other."""
+    s, _ = workloads.c3(20)
+    n = s.params.n
+    t0 = np.full(n, s.params.t_system)
+    rc, otp, _, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 13)), t0, t0)
+    assert rc == 0
+    with Simulation(s, precision="fp32_ir", tol=1e-14) as a:
+        a.initial_evolve()
+        a.run(1, 12)
+        ta, _ = a.get_state()
+    with Simulation(s, precision="fp32_ir", tol=1e-14) as b:
+        b.initial_evolve()
+        for k in range(1, 13):
+            b.simulate(k)
+            tp, tpp = b.get_state()
+            b.set_state(tp, tpp)
+        tb, _ = b.get_state()
+    assert relerr(ta, otp) <= 1e-12
+    assert relerr(tb, otp) <= 1e-12
+    assert relerr(ta, tb) <= 1e-12
+
+
+def test_fp32_ir_refinement_sweeps():
+    """A loose inner tolerance (1e-2) forces several refinement sweeps per step, so the fused
+    pass also runs from a refined iterate (not only from the state's initial guess); the
+    result still meets the oracle at the reference tolerance."""
+    s, _ = workloads.c3(24)
+    n = s.params.n
+    t0 = np.full(n, s.params.t_system)
+    rc, otp, _, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 9)), t0, t0)
+    assert rc == 0
+    with Simulation(s, precision="fp32_ir", tol=1e-14, inner_tol=1e-2) as sim:
+        sim.initial_evolve()
+        sim.run(1, 8)
+        tp, _ = sim.get_state()
+        st = sim.stats()
+    assert st.outer_iters >= 2 * 8
+    assert relerr(tp, otp) <= 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
+def test_first_iteration_stop_is_exact(prec):
+    """A CG solve whose first iteration already meets the tolerance stops there: the next
+    residual norm is predicted as alpha^2 q.q - r.r and y = alpha p is left to its consumer,
+    which forms it with k_cg_b's own product.  With that stop turned off (debug bit 1) the
+    same iteration runs in full, so the temperatures must agree bit for bit; over 20 steps
+    at least 5 solves take the stop (inner tolerance 0.5 for the mixed inner solves, so that
+    nearly every inner solve stops after one iteration and the refinement does the rest)."""
+    s, _ = workloads.c3(24)
+    out = {}
+    for dbg in (0, 2):
+        with Simulation(s, precision=prec, tol=1e-12,
+                        inner_tol=0.5 if prec == "fp32_ir" else 1e-6) as sim:
+            sim.set_debug(dbg)
+            sim.initial_evolve()
+            its = [sim.simulate(k)[0] for k in range(1, 21)]
+            out[dbg] = (sim.get_state()[0], its, sim.stats().first_iter_stops)
+    t0, its0, n0 = out[0]
+    t2, its2, n2 = out[2]
+    assert its0 == its2
+    assert n0 >= 5 and n2 == 0, (n0, n2, its0)
+    assert np.array_equal(bits(t0), bits(t2))

@@ -131,6 +131,39 @@ def cpu_baseline(n_full: int, tol: float, sample_n: int, sample_steps: int) -> d
                 sample_seconds=round(dt, 3))
 
 
+def stiff_bench(s, steps: int, tol: float, dt_factor: float = 1000.0) -> dict:
+    """The 512^3 workload at a 1000x longer time step (5e-8 s): about 60 CG iterations per step
+    instead of one, so the solve, not the per-step passes, dominates.  Both precisions at
+    `tol`, errors against FP64 at the reference tolerance."""
+    import copy
+    from heatflow_amd import Simulation
+    st_sys = copy.copy(s)
+    st_sys.params = copy.copy(s.params)
+    st_sys.params.time_step = s.params.time_step * dt_factor
+    out = dict(time_step=st_sys.params.time_step, steps=steps, tol=tol)
+    states = {}
+    for prec, t in (("fp32_ir", tol), ("fp64", tol), ("fp64", 1e-32)):
+        key = f"{prec}@{t:g}"
+        sim = Simulation(st_sys, precision=prec, tol=t)
+        try:
+            dt, st, kt, tp = timed_run(sim, steps, 1)
+        finally:
+            sim.close()
+        states[key] = tp
+        iters = st.inner_iters if prec == "fp32_ir" else st.outer_iters
+        out[key] = dict(timesteps_per_s=steps / dt, cg_iters=int(iters),
+                        refine_sweeps=int(st.outer_iters) if prec == "fp32_ir" else None,
+                        cg_iters_per_s=iters / dt, kernels=kernel_summary(kt, st_sys.params.n))
+        log(f"[bench] stiff {key}: {steps / dt:.2f} steps/s, {iters} CG iterations")
+    ref = states["fp64@1e-32"]
+    for key in (f"fp32_ir@{tol:g}", f"fp64@{tol:g}"):
+        mx, rms = rel_errors(states[key], ref)
+        out[key]["err_vs_fp64_reftol"] = dict(max=mx, rms=rms)
+    out["fp32_ir_over_fp64"] = (out[f"fp32_ir@{tol:g}"]["timesteps_per_s"]
+                                / out[f"fp64@{tol:g}"]["timesteps_per_s"])
+    return out
+
+
 def grad_bench(n: int, steps: int, ckpt: int, fd_cells: int = 16) -> dict:
     """Forward+adjoint gradient of the 128^3 / 200-step workload (Fourier and Cattaneo, FP64):
     loss = sum over the probes of (T - T*)^2 at the last step, T* from the same run with the
@@ -338,6 +371,8 @@ def main():
     ap.add_argument("--grad-n", type=int, default=128)
     ap.add_argument("--c5-n", type=int, default=384)
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-stiff", action="store_true")
+    ap.add_argument("--stiff-steps", type=int, default=3)
     ap.add_argument("--tol-sweep", action="store_true",
                     help="only run the 384^3 accuracy-vs-tolerance sweep and print it")
     ap.add_argument("--copy-probe", action="store_true",
@@ -418,6 +453,7 @@ def main():
 
     head = results.get(f"fp32_ir@{args.tol:g}") or next(iter(results.values()))
     fp64 = results.get(f"fp64@{args.tol:g}")
+    stiff = None if (args.no_stiff or args.only) else stiff_bench(s, args.stiff_steps, args.tol)
     grad = None if (args.no_grad or args.only) else grad_bench(args.grad_n, 200, 20, fd_cells=12)
     grad5 = None if (args.no_grad or args.no_c5 or args.only) else grad_c5_bench(args.c5_n, 500)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, args.tol, args.cpu_n,
@@ -446,6 +482,7 @@ def main():
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "grad_128": grad,
+        "stiff_dt_x1000": stiff,
         "grad_c5_384": grad5,
     }
     print(json.dumps(line), flush=True)

@@ -179,10 +179,12 @@ static int alloc_mode(hf_ctx *c) {
         TRY(dalloc(c, &c->r32, N)); TRY(dalloc(c, &c->p32[0], N)); TRY(dalloc(c, &c->p32[1], N));
         TRY(dalloc(c, &c->q32, N)); TRY(dalloc(c, &c->y32, N));
         TRY(dalloc(c, &c->S, N)); TRY(dalloc(c, &c->T[2], N));
-        TRY(dalloc(c, &c->rn[0], N)); TRY(dalloc(c, &c->rn[1], N)); TRY(dalloc(c, &c->qb, N));
+        TRY(dalloc(c, &c->rn[0], N)); TRY(dalloc(c, &c->rn[1], N)); TRY(dalloc(c, &c->rn[2], N));
+        TRY(dalloc(c, &c->rw32, N)); TRY(dalloc(c, &c->qb, N));
     } else {
         TRY(dalloc(c, &c->hx, N)); TRY(dalloc(c, &c->hy, N)); TRY(dalloc(c, &c->hz, N));
         TRY(dalloc(c, &c->r64, N)); TRY(dalloc(c, &c->p64[0], N)); TRY(dalloc(c, &c->p64[1], N));
+        TRY(dalloc(c, &c->rw64, N));
         TRY(dalloc(c, &c->q64, N)); TRY(dalloc(c, &c->y64, N));
     }
     return HF_OK;
@@ -535,33 +537,66 @@ static void drop_idle_slots(hf_ctx *c, const std::vector<SlotRec> &slots, int64_
         if (sl.k >= iters || (sl.is_b && shortcut && sl.k == 0)) c->tpool[sl.slot].valid = false;
 }
 
-// CG on A^ y = r^ until sc->done, in batches of `pred` iterations per host synchronisation.
-// Launches queued after convergence return at once; their timing slots are invalidated.
+// Vectors of one CG solve on A^ y = rin.
 template <typename T>
-static int run_cg(hf_ctx *c, Scal *sc, Scal *h, T *r, T **p, T *q, T *y, const T *hx,
-                  const T *hy, const T *hz, int64_t *pred, int64_t *iters, int ida, int idb) {
+struct CgVecs {
+    const T *rin;              // right-hand side, left unchanged (p_old of iteration 1)
+    T *rw, *p[2], *q, *y;      // residual from iteration 1 on, directions, A^ p, solution
+    const T *hx, *hy, *hz;     // scaled couplings
+};
+
+// Queues CG iterations k0 .. k0+count-1 (launches past convergence return at once).
+// Iteration 0: k_cg_a (first: p = rin, only p.q and q.q), k_fin_a (which may end the solve:
+// y = alpha rin is then formed by the consumer), k_cg_b1 (q recomputed; y, rw), k_fin_b.
+// Iteration k >= 1: k_cg_a (p = rw + beta p_old, p_old = rin for k = 1), k_cg_b on rw.
+template <typename T>
+static void queue_cg(hf_ctx *c, Scal *sc, const CgVecs<T> &v, int64_t k0, int64_t count,
+                     std::vector<SlotRec> *slots, int ida, int idb) {
     const double N = (double)c->g.N, w = (double)sizeof(T);
-    int cur = 0;
+    for (int64_t k = k0; k < k0 + count; ++k) {
+        long sa, sb;
+        if (k == 0) {
+            // k_cg_a (first): read r, 3 couplings
+            sa = tbeg(c, ida, N * w * 4);
+            launch_cg_a<T>(c->g, true, v.rin, nullptr, nullptr, nullptr, v.hx, v.hy, v.hz, sc,
+                           c->partials, c->st);
+            tend(c, sa);
+            launch_fin_a(c->g, c->partials, sc, c->st);
+            // k_cg_b1: read r, 3 couplings; write y, rw
+            sb = tbeg(c, idb, N * w * 6);
+            launch_cg_b1<T>(c->g, v.rin, v.hx, v.hy, v.hz, v.y, v.rw, sc, c->partials, c->st);
+            tend(c, sb);
+            launch_fin_b1(c->g, c->partials, sc, c->st);
+        } else {
+            const T *pold = k == 1 ? v.rin : v.p[k & 1];
+            T *pnew = v.p[(k - 1) & 1];
+            // k_cg_a: read r, p_old, 3 couplings; write p, q
+            sa = tbeg(c, ida, N * w * 7);
+            launch_cg_a<T>(c->g, false, v.rw, pold, pnew, v.q, v.hx, v.hy, v.hz, sc, c->partials,
+                           c->st);
+            tend(c, sa);
+            launch_fin_a(c->g, c->partials, sc, c->st);
+            // k_cg_b: read y, p, r, q; write y, r
+            sb = tbeg(c, idb, N * w * 6);
+            launch_cg_b<T>(c->g, v.y, pnew, v.rw, v.q, sc, c->partials, c->st);
+            tend(c, sb);
+            launch_fin_b(c->g, c->partials, sc, c->st);
+        }
+        if (slots && sa >= 0) slots->push_back({sa, k, false});
+        if (slots && sb >= 0) slots->push_back({sb, k, true});
+    }
+}
+
+// CG on A^ y = rin until sc->done, in batches of `pred` iterations per host synchronisation.
+template <typename T>
+static int run_cg(hf_ctx *c, Scal *sc, Scal *h, const CgVecs<T> &v, int64_t *pred,
+                  int64_t *iters, int ida, int idb) {
     int64_t batch = std::max<int64_t>(1, *pred);
     int64_t queued = 0;
     std::vector<SlotRec> slots;
     for (;;) {
-        for (int64_t b = 0; b < batch; ++b, ++queued) {
-            const bool first = queued == 0;
-            // k_cg_a: read r, p_old (not on the first iteration), 3 couplings; write p, q
-            long sa = tbeg(c, ida, N * w * (first ? 6 : 7));
-            launch_cg_a<T>(c->g, r, p[cur], p[cur ^ 1], q, hx, hy, hz, sc, c->partials, c->st);
-            tend(c, sa);
-            launch_fin_a(c->g, c->partials, sc, c->st);
-            // k_cg_b: read y (not on the first iteration), p, r, q; write y, r
-            long sb = tbeg(c, idb, N * w * (first ? 5 : 6));
-            launch_cg_b<T>(c->g, y, p[cur ^ 1], r, q, sc, c->partials, c->st);
-            tend(c, sb);
-            launch_fin_b(c->g, c->partials, sc, c->st);
-            if (sa >= 0) slots.push_back({sa, queued, false});
-            if (sb >= 0) slots.push_back({sb, queued, true});
-            cur ^= 1;
-        }
+        queue_cg<T>(c, sc, v, queued, batch, &slots, ida, idb);
+        queued += batch;
         TRY(check_launch(c, "k_cg"));
         TRY(sync_scal(c, sc, h));
         if (h->done) break;
@@ -572,6 +607,13 @@ static int run_cg(hf_ctx *c, Scal *sc, Scal *h, T *r, T **p, T *q, T *y, const T
     *iters = h->iter;
     *pred = std::max<int64_t>(1, h->iter + 1);
     return HF_OK;
+}
+
+static CgVecs<double> cg64(hf_ctx *c, const double *rin) {
+    return CgVecs<double>{rin, c->rw64, {c->p64[0], c->p64[1]}, c->q64, c->y64, c->hx, c->hy, c->hz};
+}
+static CgVecs<float> cg32(hf_ctx *c, const float *rin) {
+    return CgVecs<float>{rin, c->rw32, {c->p32[0], c->p32[1]}, c->q32, c->y32, c->fx, c->fy, c->fz};
 }
 
 static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
@@ -595,15 +637,15 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
     TRY(check_launch(c, "k_rhs_res"));
     int64_t it = 0;
-    TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
-                       &c->pred_iters, &it, HF_K_CG_A64, HF_K_CG_B64));
+    TRY(run_cg<double>(c, c->sc, c->h_sc, cg64(c, c->r64), &c->pred_iters, &it, HF_K_CG_A64,
+                       HF_K_CG_B64));
     int32_t fl = c->h_sc->flags | c->setup_flags;
     if (fl & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S))
         return flags_to_rc(c, fl);
     const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bnorm2) : 0.0;
     // k_finalize: read Tp, Tpp, s, y; write T^{n+1}
     long sf = tbeg(c, HF_K_FINAL, N * 8 * 5);
-    launch_finalize(c->g, true, c->T[0], c->T[1], c->sv, c->y64, c->p64[1], c->sc, c->st);
+    launch_finalize(c->g, true, c->T[0], c->T[1], c->sv, c->y64, c->r64, c->sc, c->st);
     tend(c, sf);
     TRY(check_launch(c, "k_finalize"));
     std::swap(c->T[0], c->T[1]);  // Temp_pp <- Temp_p, Temp_p <- x (mod_evolve.f90:169-170)
@@ -615,32 +657,25 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     return HF_OK;
 }
 
-// Inner FP32 CG of one refinement sweep on r (in place); returns the iterations it took.
-static int inner_solve(hf_ctx *c, float *r, int64_t *batch_out, std::vector<SlotRec> &slots) {
+// Queues the inner FP32 CG of one refinement sweep on A^ y = r (r unchanged); *batch_out is
+// the number of iterations queued.
+static int inner_solve(hf_ctx *c, const float *r, int64_t *batch_out, std::vector<SlotRec> &slots) {
     const hf_params &p = c->p;
-    const double N = (double)c->g.N;
     launch_inner_begin(c->sc, c->sci, p.inner_tol, p.tol, p.inner_itmax, 0.25, allow_short(c),
                        c->st);
-    int cur = 0;
     const int64_t batch = std::max<int64_t>(1, c->pred_inner);
-    for (int64_t b = 0; b < batch; ++b) {
-        // k_cg_a: read r, p_old (not on the first iteration), 3 couplings; write p, q
-        long sa = tbeg(c, HF_K_CG_A32, N * 4 * (b == 0 ? 6 : 7));
-        launch_cg_a<float>(c->g, r, c->p32[cur], c->p32[cur ^ 1], c->q32, c->fx, c->fy, c->fz,
-                           c->sci, c->partials, c->st);
-        tend(c, sa);
-        launch_fin_a(c->g, c->partials, c->sci, c->st);
-        // k_cg_b: read y (not on the first iteration), p, r, q; write y, r
-        long sb = tbeg(c, HF_K_CG_B32, N * 4 * (b == 0 ? 5 : 6));
-        launch_cg_b<float>(c->g, c->y32, c->p32[cur ^ 1], r, c->q32, c->sci, c->partials, c->st);
-        tend(c, sb);
-        launch_fin_b(c->g, c->partials, c->sci, c->st);
-        if (sa >= 0) slots.push_back({sa, b, false});
-        if (sb >= 0) slots.push_back({sb, b, true});
-        cur ^= 1;
-    }
+    queue_cg<float>(c, c->sci, cg32(c, r), 0, batch, &slots, HF_K_CG_A32, HF_K_CG_B32);
     *batch_out = batch;
     return check_launch(c, "k_cg32");
+}
+
+// the first mixed-mode residual vector that is none of the given ones
+static float *other_rvec(hf_ctx *c, const float *a, const float *b = nullptr,
+                         const float *d = nullptr) {
+    float *all[4] = {c->r32, c->rn[0], c->rn[1], c->rn[2]};
+    for (float *v : all)
+        if (v != a && v != b && v != d) return v;
+    return nullptr;
 }
 
 // Mixed-precision step with the refinement residual fused into the next step's right-hand
@@ -653,7 +688,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     const double N = (double)c->g.N;
     launch_step_begin(c->sc, c->st);
     RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
-    float *rin;
+    const float *rin;
     if (c->carry) {
         launch_carry_begin(c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
         rin = c->carry_r;
@@ -683,12 +718,17 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         int64_t batch = 0;
         TRY(inner_solve(c, rin, &batch, slots));
         double *xo = xbuf[nb];
-        // k_ir_step: read T^n, T^{n-1} (or x), y or p (FP32), s, rho c_p, tau' rho c_p,
+        // the pass reads y (or the right-hand side, for y = alpha r) and writes three residual
+        // vectors: none of them may be the right-hand side
+        float *rres = other_rvec(c, rin);
+        float *rn0 = other_rvec(c, rin, rres);
+        float *rn1 = other_rvec(c, rin, rres, rn0);
+        // k_ir_step: read T^n, T^{n-1} (or x), y or r (FP32), s, rho c_p, tau' rho c_p,
         // Qdens + B, diag, 3 couplings; write x', three FP32 residuals
         long si = tbeg(c, HF_K_IR_STEP, N * (8 * 10 + 4 + 8 + 3 * 4) + (xa ? N * 8 : 0));
-        launch_ir_step(c->g, sp, xa == nullptr, c->T[0], c->T[1], xa, c->y32, c->p32[1], c->sci,
-                       c->sv, c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, c->r32,
-                       c->rn[0], c->rn[1], c->sc, c->partials, c->st);
+        launch_ir_step(c->g, sp, xa == nullptr, c->T[0], c->T[1], xa, c->y32, rin, c->sci, c->sv,
+                       c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, rres, rn0, rn1,
+                       c->sc, c->partials, c->st);
         tend(c, si);
         launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->st);
         TRY(check_launch(c, "k_ir_step"));
@@ -702,7 +742,8 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         c->pred_inner = c->h_sci->done ? std::max<int64_t>(1, in_it + 1) : batch * 2;
         xa = xo;
         nb ^= 1;
-        rin = c->r32;
+        rin = rres;
+        c->carry_r = (c->h_sc->flags & FLAG_TINY_NEXT) ? rn1 : rn0;
         int32_t fl = c->h_sc->flags | c->setup_flags;
         if (fl & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S))
             return flags_to_rc(c, fl);
@@ -716,9 +757,8 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     c->T[1] = old_tp;
     c->T[2] = old_tpp;
     c->S = other;
-    // the next step starts from the residual this pass left for its TINY case
+    // the next step starts from the residual the last pass left for its TINY case
     c->carry = true;
-    c->carry_r = (c->h_sc->flags & FLAG_TINY_NEXT) ? c->rn[1] : c->rn[0];
     const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->outer_rr / c->h_sc->bnorm2) : 0.0;
     c->stats.steps += 1;
     c->stats.outer_iters += sweeps;
@@ -758,18 +798,21 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     // suffice; the cap guards tolerances below the FP64 round-off floor
     IrParams ip{std::min<int64_t>(p.itmax, 64), 1};
     int64_t inner_total = 0, sweeps = 0;
+    float *rin = c->r32;
     for (;;) {
         std::vector<SlotRec> slots;
         int64_t batch = 0;
-        TRY(inner_solve(c, c->r32, &batch, slots));
+        TRY(inner_solve(c, rin, &batch, slots));
+        float *rout = other_rvec(c, rin);   // the pass may read y as alpha * rin
         // k_ir_res: read x, y (FP32), s, S, diag, 3 couplings, Tp; write x, r^ (FP32)
         long si = tbeg(c, HF_K_IR_RES, N * (8 * 8 + 4 + 8 + 4));
-        launch_ir_res(c->g, ip, xa, c->y32, c->p32[1], c->sci, c->sv, c->S, c->d, c->cx, c->cy,
-                      c->cz, c->T[0], xb, c->r32, c->sc, c->partials, c->st);
+        launch_ir_res(c->g, ip, xa, c->y32, rin, c->sci, c->sv, c->S, c->d, c->cx, c->cy, c->cz,
+                      c->T[0], xb, rout, c->sc, c->partials, c->st);
         tend(c, si);
         launch_fin_ir(c->g, c->partials, c->sc, ip.itmax, c->st);
         TRY(check_launch(c, "k_ir_res"));
         std::swap(xa, xb);
+        rin = rout;
         HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
         TRY(sync_scal(c, c->sc, c->h_sc));
         const int64_t in_it = c->h_sci->iter;
@@ -857,9 +900,9 @@ extern "C" int hf_solve(hf_ctx *c, const double *b, double *x, int32_t itol, dou
     launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
     TRY(check_launch(c, "k_rhs_res(solve)"));
     int64_t it = 0, pred = std::max<int64_t>(4, c->pred_iters);
-    TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
+    TRY(run_cg<double>(c, c->sc, c->h_sc, cg64(c, c->r64),
                        &pred, &it, HF_K_CG_A64, HF_K_CG_B64));
-    launch_finalize(c->g, false, nullptr, xd, c->sv, c->y64, c->p64[1], c->sc, c->st);
+    launch_finalize(c->g, false, nullptr, xd, c->sv, c->y64, c->r64, c->sc, c->st);
     TRY(check_launch(c, "k_finalize(solve)"));
     HIPCHK(c, hipMemcpyAsync(x, xd, B8, hipMemcpyDefault, c->st));
     TRY(sync_scal(c, c->sc, c->h_sc));
@@ -981,26 +1024,20 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
         TRY(check_launch(c, "k_rhs_res(solve32)"));
         IrParams ip{std::min<int64_t>(itmax, 64), 0};
         double *xa = x, *xb = c->T[2];
+        float *rin = c->r32;
         int64_t inner_total = 0;
         for (;;) {
             launch_inner_begin(c->sc, c->sci, c->p.inner_tol, tol, c->p.inner_itmax, 0.25,
                                allow_short(c), c->st);
-            int cur = 0;
             const int64_t batch = std::max<int64_t>(1, c->pred_adj);
-            for (int64_t k = 0; k < batch; ++k) {
-                launch_cg_a<float>(c->g, c->r32, c->p32[cur], c->p32[cur ^ 1], c->q32, c->fx,
-                                   c->fy, c->fz, c->sci, c->partials, c->st);
-                launch_fin_a(c->g, c->partials, c->sci, c->st);
-                launch_cg_b<float>(c->g, c->y32, c->p32[cur ^ 1], c->r32, c->q32, c->sci,
-                                   c->partials, c->st);
-                launch_fin_b(c->g, c->partials, c->sci, c->st);
-                cur ^= 1;
-            }
-            launch_ir_res(c->g, ip, xa, c->y32, c->p32[1], c->sci, c->sv, b, c->d, c->cx, c->cy,
-                          c->cz, nullptr, xb, c->r32, c->sc, c->partials, c->st);
+            queue_cg<float>(c, c->sci, cg32(c, rin), 0, batch, nullptr, HF_K_CG_A32, HF_K_CG_B32);
+            float *rout = other_rvec(c, rin);
+            launch_ir_res(c->g, ip, xa, c->y32, rin, c->sci, c->sv, b, c->d, c->cx, c->cy, c->cz,
+                          nullptr, xb, rout, c->sc, c->partials, c->st);
             launch_fin_ir(c->g, c->partials, c->sc, ip.itmax, c->st);
             TRY(check_launch(c, "k_ir_res(solve)"));
             std::swap(xa, xb);
+            rin = rout;
             HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
             TRY(sync_scal(c, c->sc, c->h_sc));
             inner_total += c->h_sci->iter;
@@ -1020,9 +1057,9 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
     launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
     TRY(check_launch(c, "k_rhs_res(solve)"));
     int64_t it = 0;
-    TRY(run_cg<double>(c, c->sc, c->h_sc, c->r64, c->p64, c->q64, c->y64, c->hx, c->hy, c->hz,
+    TRY(run_cg<double>(c, c->sc, c->h_sc, cg64(c, c->r64),
                        &c->pred_adj, &it, HF_K_CG_A64, HF_K_CG_B64));
-    launch_finalize(c->g, false, nullptr, x, c->sv, c->y64, c->p64[1], c->sc, c->st);
+    launch_finalize(c->g, false, nullptr, x, c->sv, c->y64, c->r64, c->sc, c->st);
     TRY(check_launch(c, "k_finalize(solve)"));
     if (iters) *iters = it;
     return HF_OK;

@@ -42,14 +42,17 @@ struct hf_ctx {
     // state: T[0] = Temp_p, T[1] = Temp_pp, T[2] = spare (mixed-mode refined x)
     double *T[3] = {nullptr, nullptr, nullptr};
     double *S = nullptr;
-    // FP64 CG
-    double *r64 = nullptr, *p64[2] = {nullptr, nullptr}, *q64 = nullptr, *y64 = nullptr;
+    // FP64 CG (r64: right-hand side of a solve, rw64: its residual from iteration 1 on)
+    double *r64 = nullptr, *rw64 = nullptr, *p64[2] = {nullptr, nullptr}, *q64 = nullptr,
+           *y64 = nullptr;
     // FP32 inner CG
-    float *r32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr, *y32 = nullptr;
-    // fused mixed step: the next step's r^0 without / with the TINY shift; `carry` is set when
-    // the last step left them (and the scalars) valid for the current state, `carry_r` is the
-    // one the next step starts from
-    float *rn[2] = {nullptr, nullptr};
+    float *r32 = nullptr, *rw32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr,
+          *y32 = nullptr;
+    // mixed-mode residual vectors besides r32: a refinement pass writes its residuals into
+    // vectors other than the right-hand side it read y from.  `carry` is set when the last
+    // fused step left the next step's r^0 (and the scalars) valid for the current state;
+    // `carry_r` is that r^0 (without or with the TINY shift)
+    float *rn[3] = {nullptr, nullptr, nullptr};
     double *qb = nullptr;        // Qdens + B for the fused step (set when a carried run starts)
     bool carry = false;
     float *carry_r = nullptr;

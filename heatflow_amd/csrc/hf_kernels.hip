@@ -618,6 +618,9 @@ void launch_fin_a(const Dims &g, const double *partials, Scal *sc, hipStream_t s
 void launch_fin_b(const Dims &g, const double *partials, Scal *sc, hipStream_t st) {
     HF_FIN(k_fin_b, g.nblk, st, partials, sc);
 }
+void launch_fin_b1(const Dims &g, const double *partials, Scal *sc, hipStream_t st) {
+    HF_FIN(k_fin_b, g.zblk, st, partials, sc);
+}
 void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                    hipStream_t st) {
     HF_FIN(k_fin_ir, g.zblk, st, partials, sc, sweeps_max);

@@ -78,9 +78,15 @@ void launch_rhs_res_solve32(const Dims &g, const RhsParams &rp, const double *x,
                             const double *b, const double *d, const double *cx,
                             const double *cy, const double *cz, float *rhat, Scal *sc,
                             double *partials, hipStream_t st);
+// first: the solve's first iteration (p = r; reductions only, nothing stored)
 template <typename T>
-void launch_cg_a(const Dims &g, const T *r, const T *pold, T *pnew, T *q, const T *hx,
-                 const T *hy, const T *hz, Scal *sc, double *partials, hipStream_t st);
+void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, T *q,
+                 const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
+                 hipStream_t st);
+// the rest of the first iteration: y = alpha r, rw = r - alpha A^ r (r unchanged)
+template <typename T>
+void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *y, T *rw,
+                  Scal *sc, double *partials, hipStream_t st);
 template <typename T>
 void launch_cg_b(const Dims &g, T *y, const T *p, T *r, const T *q, Scal *sc, double *partials,
                  hipStream_t st);
@@ -110,6 +116,7 @@ void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double to
                       bool allow_short, hipStream_t st);
 void launch_fin_a(const Dims &g, const double *partials, Scal *sc, hipStream_t st);
 void launch_fin_b(const Dims &g, const double *partials, Scal *sc, hipStream_t st);
+void launch_fin_b1(const Dims &g, const double *partials, Scal *sc, hipStream_t st);  // after k_cg_b1
 void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                    hipStream_t st);
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,

@@ -17,6 +17,7 @@
 //
 //   k_rhs_res_z  S, x0 = 2 T^n - T^{n-1} (+TINY), r^ = -s (S - H x0), norms  (mod_evolve.f90:112-141)
 //   k_cg_a_z     p = r + beta p, q = (I - C^) p, p.q                        (mod_Sparse.f90:404-411)
+//   k_cg_b1_z    the first iteration's y, r updates with q recomputed
 //   k_ir_res_z   x' = x + s y, r^ = -s (S - H x'), ||r^||^2  (mixed-precision refinement)
 //   k_ir_step_z  the refinement residual fused with the next step's right-hand side
 #include "hf_device.hpp"
@@ -280,42 +281,83 @@ k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *_
     block_partials<2>(acc, partials);
 }
 
-// p = r + beta * p_old (p = r on the first iteration), q = A^ p, partials p.q and q.q (the
-// latter for the first-iteration stop test of k_fin_a).
-template <typename T>
+// q = A^ p = p - C^ p with the z coupling of plane z-1 carried in hz_m (order z-, y-, x-, x+,
+// y+, z+; periodic wrap neighbours read through P from memory)
+template <typename T, class PF>
+__device__ __forceinline__ T ahat(const Dims &g, const ZCol &c, uint32_t I, uint32_t z, T pi,
+                                  T pz_m, T pz_p, T hz_m, T hzc, const T *__restrict__ hx,
+                                  const T *__restrict__ hy, const InPlane<T> &nb, PF P) {
+    const bool per_z = g.per[2];
+    const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
+    T s = (T)0;
+    if (hzm) s = s + hz_m * pz_m;
+    if (c.hym) s = s + hy[I + c.oym] * (c.wym ? P(I + c.oym) : nb.ym());
+    if (c.hxm) s = s + hx[I + c.oxm] * (c.wxm ? P(I + c.oxm) : nb.xm());
+    if (c.hxp) s = s + hx[I] * (c.wxp ? P(I + c.oxp) : nb.xp());
+    if (c.hyp) s = s + hy[I] * (c.wyp ? P(I + c.oyp) : nb.yp());
+    if (hzp) s = s + hzc * pz_p;
+    return pi - s;
+}
+
+// CG iteration k >= 1: p = r + beta p_old, q = A^ p, partials p.q and q.q.
+// First iteration (First): p = r, and only the partials: p and q are not stored, because the
+// solve either stops here (k_fin_a's first-iteration test; y = alpha r is formed by its
+// consumer) or k_cg_b1_z recomputes q.
+template <typename T, bool First>
 __global__ void __launch_bounds__(kZTX * kZTY)
 k_cg_a_z(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
          T *__restrict__ q, const T *__restrict__ hx, const T *__restrict__ hy,
          const T *__restrict__ hz, Scal *sc, double *partials) {
     __shared__ PlaneBuf<T> sp;
     if (sc->done) return;
-    const bool first = sc->first != 0;
     const T beta = (T)sc->beta;
-    auto P = [&](uint32_t J) -> T { return first ? r[J] : (T)(r[J] + beta * pold[J]); };
+    auto P = [&](uint32_t J) -> T { return First ? r[J] : (T)(r[J] + beta * pold[J]); };
     double acc[2] = {0.0, 0.0};
     const ZCol c = zcolumn(g);
-    const bool per_z = g.per[2];
     T hz_m = (T)0;
-    if (c.active && (c.z0 > 0 || per_z)) hz_m = hz[zm_index(g, c.col, c.z0)];
+    if (c.active && (c.z0 > 0 || g.per[2])) hz_m = hz[zm_index(g, c.col, c.z0)];
     zmarch<T>(g, c, sp, P, [&](uint32_t I, uint32_t z, T pi, T pz_m, T pz_p,
                                const InPlane<T> &nb) {
-        const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
         const T hzc = ldnt(&hz[I]);
-        T s = (T)0;
-        if (hzm) s = s + hz_m * pz_m;
-        if (c.hym) s = s + hy[I + c.oym] * (c.wym ? P(I + c.oym) : nb.ym());
-        if (c.hxm) s = s + hx[I + c.oxm] * (c.wxm ? P(I + c.oxm) : nb.xm());
-        if (c.hxp) s = s + hx[I] * (c.wxp ? P(I + c.oxp) : nb.xp());
-        if (c.hyp) s = s + hy[I] * (c.wyp ? P(I + c.oyp) : nb.yp());
-        if (hzp) s = s + hzc * pz_p;
-        const T qi = pi - s;
-        stnt(&pnew[I], pi);
-        stnt(&q[I], qi);
+        const T qi = ahat(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P);
+        if (!First) {
+            stnt(&pnew[I], pi);
+            stnt(&q[I], qi);
+        }
         acc[0] += (double)pi * (double)qi;
         acc[1] += (double)qi * (double)qi;
         hz_m = hzc;
     });
     block_partials<2>(acc, partials);
+}
+
+// Second half of the first CG iteration when the solve goes on: q = A^ r recomputed,
+// y = alpha r, rw = r - alpha q (into a separate vector: r is the stencil operand), partial
+// rw.rw.  The right-hand side r is left unchanged; it is p_old of iteration 1.
+template <typename T>
+__global__ void __launch_bounds__(kZTX * kZTY)
+k_cg_b1_z(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *__restrict__ hy,
+          const T *__restrict__ hz, T *__restrict__ y, T *__restrict__ rw, Scal *sc,
+          double *partials) {
+    __shared__ PlaneBuf<T> sp;
+    if (sc->done) return;
+    const T alpha = (T)sc->alpha;
+    auto P = [&](uint32_t J) -> T { return r[J]; };
+    double acc[1] = {0.0};
+    const ZCol c = zcolumn(g);
+    T hz_m = (T)0;
+    if (c.active && (c.z0 > 0 || g.per[2])) hz_m = hz[zm_index(g, c.col, c.z0)];
+    zmarch<T>(g, c, sp, P, [&](uint32_t I, uint32_t z, T pi, T pz_m, T pz_p,
+                               const InPlane<T> &nb) {
+        const T hzc = ldnt(&hz[I]);
+        const T qi = ahat(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P);
+        const T ri = pi - alpha * qi;
+        stnt(&y[I], alpha * pi);
+        stnt(&rw[I], ri);
+        acc[0] += (double)ri * (double)ri;
+        hz_m = hzc;
+    });
+    block_partials<1>(acc, partials);
 }
 
 // Mixed-precision refinement: x' = x + s y (y FP32) formed on the fly, r^ = -s (S - H x')
@@ -524,16 +566,32 @@ void launch_rhs_res_solve32(const Dims &g, const RhsParams &rp, const double *x,
 }
 
 template <typename T>
-void launch_cg_a(const Dims &g, const T *r, const T *pold, T *pnew, T *q, const T *hx,
-                 const T *hy, const T *hz, Scal *sc, double *partials, hipStream_t st) {
-    HF_ZLAUNCH(k_cg_a_z<T>, g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, T *q,
+                 const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
+                 hipStream_t st) {
+    if (first)
+        HF_ZLAUNCH((k_cg_a_z<T, true>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+    else
+        HF_ZLAUNCH((k_cg_a_z<T, false>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
 }
-template void launch_cg_a<double>(const Dims &, const double *, const double *, double *,
+template void launch_cg_a<double>(const Dims &, bool, const double *, const double *, double *,
                                   double *, const double *, const double *, const double *,
                                   Scal *, double *, hipStream_t);
-template void launch_cg_a<float>(const Dims &, const float *, const float *, float *, float *,
-                                 const float *, const float *, const float *, Scal *, double *,
-                                 hipStream_t);
+template void launch_cg_a<float>(const Dims &, bool, const float *, const float *, float *,
+                                 float *, const float *, const float *, const float *, Scal *,
+                                 double *, hipStream_t);
+
+template <typename T>
+void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *y, T *rw,
+                  Scal *sc, double *partials, hipStream_t st) {
+    HF_ZLAUNCH(k_cg_b1_z<T>, g, st, g, r, hx, hy, hz, y, rw, sc, partials);
+}
+template void launch_cg_b1<double>(const Dims &, const double *, const double *, const double *,
+                                   const double *, double *, double *, Scal *, double *,
+                                   hipStream_t);
+template void launch_cg_b1<float>(const Dims &, const float *, const float *, const float *,
+                                  const float *, float *, float *, Scal *, double *,
+                                  hipStream_t);
 
 void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const float *y,
                    const float *p0, const Scal *inner, const double *sv, const double *S,

@@ -119,15 +119,22 @@ static void make_dims(Dims &g, const hf_params &p) {
     uint32_t zcap = 16384;
     if (const char *e = getenv("HF_ZBLK")) zcap = std::max(8u, (uint32_t)atoi(e));
     const uint32_t tx = kZTX, ty = kZTY;
-    g.ztx = (g.nx + tx - 1) / tx;
     g.zty = (g.ny + ty - 1) / ty;
-    const uint64_t per_plane = (uint64_t)g.ztx * g.zty;
-    uint64_t nzc = std::max<uint64_t>(1, (zcap + per_plane - 1) / per_plane);
-    nzc = std::min<uint64_t>(nzc, g.nz);
-    g.zchunk = (uint32_t)((g.nz + nzc - 1) / nzc);
-    nzc = (g.nz + g.zchunk - 1) / g.zchunk;
-    g.ztiles = (uint32_t)(per_plane * nzc);
-    g.zblk = (g.ztiles + 7) / 8 * 8;
+    // tiles of `w` cells in x: chunk length and tile count for ~zcap blocks
+    auto zlayout = [&](uint32_t w, uint32_t *ztx, uint32_t *zchunk, uint32_t *ztiles) {
+        *ztx = (g.nx + w - 1) / w;
+        const uint64_t per_plane = (uint64_t)*ztx * g.zty;
+        uint64_t nzc = std::max<uint64_t>(1, (zcap + per_plane - 1) / per_plane);
+        nzc = std::min<uint64_t>(nzc, g.nz);
+        *zchunk = (uint32_t)((g.nz + nzc - 1) / nzc);
+        nzc = (g.nz + *zchunk - 1) / *zchunk;
+        *ztiles = (uint32_t)(per_plane * nzc);
+    };
+    zlayout(tx, &g.ztx, &g.zchunk, &g.ztiles);
+    g.pairs = (g.nx % 2 == 0) && getenv("HF_NO_PAIRS") == nullptr;
+    zlayout(2 * tx, &g.ztx2, &g.zchunk2, &g.ztiles2);
+    const uint32_t zt = g.pairs ? std::max(g.ztiles, g.ztiles2) : g.ztiles;
+    g.zblk = (zt + 7) / 8 * 8;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -44,9 +44,14 @@ struct Dims {
     uint32_t nblk;       // launch grid (multiple of 8)
     int32_t layout;      // LAYOUT_* bits
     // z-marching stencil kernels: blocks of kZTX x kZTY threads, ztx x zty tiles per plane,
-    // z-chunks of `zchunk` planes, ztiles = ztx * zty * chunks, zblk = ztiles rounded up to a
-    // multiple of 8
+    // z-chunks of `zchunk` planes, ztiles = ztx * zty * chunks.  The FP32 kernels take two
+    // x-neighbouring cells per thread when nx is even (`pairs`): tiles of 2 kZTX cells, ztx2 per
+    // row, chunks of zchunk2 planes, ztiles2 tiles.  Every stencil launch runs zblk blocks (the
+    // larger tile count rounded up to a multiple of 8; blocks past a kernel's tiles only write
+    // zero partials), so the finishing kernels see one partial count.
     uint32_t ztx, zty, zchunk, ztiles, zblk;
+    uint32_t ztx2, zchunk2, ztiles2;
+    int32_t pairs;
     int32_t per[3];      // periodic axes (length >= 3)
     FastDiv dnx, dny;
 };

@@ -356,22 +356,47 @@ k_fin_a(uint32_t nb, const double *partials, Scal *sc) {
     }
 }
 
-// y += alpha p, r -= alpha q, partial r.r; convergence test on ||r^|| / ||b^||.
+// y += alpha p, r -= alpha q, partial r.r, with 16-byte accesses: block b (XCD-mapped) takes a
+// contiguous chunk of the 16-byte vectors; the cells past the last full vector go to logical
+// block 0.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_cg_b(Dims g, T *__restrict__ y, const T *__restrict__ p, T *__restrict__ r,
-       const T *__restrict__ q, Scal *sc, double *partials) {
+         const T *__restrict__ q, Scal *sc, double *partials) {
+    constexpr int W = 16 / sizeof(T);
+    using V = T __attribute__((ext_vector_type(W)));
     if (sc->done) return;
     const bool first = sc->first != 0;
     const T alpha = (T)sc->alpha;
     double acc[1] = {0.0};
-    HF_FOR_CELLS(g, I) {
-        const T pi = ldnt(&p[I]);
-        const T yi = first ? alpha * pi : ldnt(&y[I]) + alpha * pi;
-        const T ri = ldnt(&r[I]) - alpha * ldnt(&q[I]);
-        stnt(&y[I], yi);
-        stnt(&r[I], ri);
-        acc[0] += (double)ri * (double)ri;
+    const uint32_t nv = g.N / W;
+    const uint32_t b = logical_block(g);
+    const uint32_t per = (nv + g.nblk - 1) / g.nblk;
+    const uint64_t v0 = (uint64_t)b * per;
+    const uint32_t v1 = (uint32_t)(v0 + per < nv ? v0 + per : nv);
+    for (uint32_t v = (uint32_t)(v0 < nv ? v0 : nv) + threadIdx.x; v < v1; v += kBlock) {
+        const V pv = __builtin_nontemporal_load((const V *)p + v);
+        const V qv = __builtin_nontemporal_load((const V *)q + v);
+        V rv = __builtin_nontemporal_load((const V *)r + v);
+        V yv = first ? (V)(T)0 : __builtin_nontemporal_load((const V *)y + v);
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            yv[k] = first ? alpha * pv[k] : yv[k] + alpha * pv[k];
+            rv[k] = rv[k] - alpha * qv[k];
+            acc[0] += (double)rv[k] * (double)rv[k];
+        }
+        __builtin_nontemporal_store(yv, (V *)y + v);
+        __builtin_nontemporal_store(rv, (V *)r + v);
+    }
+    if (b == 0) {
+        for (uint32_t I = nv * W + threadIdx.x; I < g.N; I += kBlock) {
+            const T pi = p[I];
+            const T yi = first ? alpha * pi : y[I] + alpha * pi;
+            const T ri = r[I] - alpha * q[I];
+            y[I] = yi;
+            r[I] = ri;
+            acc[0] += (double)ri * (double)ri;
+        }
     }
     block_partials<1>(acc, partials);
 }

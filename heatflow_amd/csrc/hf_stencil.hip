@@ -20,6 +20,8 @@
 //   k_cg_b1_z    the first iteration's y, r updates with q recomputed
 //   k_ir_res_z   x' = x + s y, r^ = -s (S - H x'), ||r^||^2  (mixed-precision refinement)
 //   k_ir_step_z  the refinement residual fused with the next step's right-hand side
+#include <type_traits>
+
 #include "hf_device.hpp"
 #include "hf_kernels.hpp"
 
@@ -513,6 +515,236 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     block_partials<4>(acc, partials);
 }
 
+// ---- FP32 kernels with two x-neighbouring cells per thread (nx even) ------------------------
+//
+// FP32 arrays moved one float per lane per instruction run well below the FP64 kernels'
+// bandwidth; with pairs every lane moves 8 bytes per access.  A block of kZTX x kZTY threads
+// owns 2 kZTX x kZTY columns (tiles numbered as above, g.ztx2 per row, chunks of g.zchunk2
+// planes).  The LDS plane holds pairs: the x neighbours of a pair (x, x+1) are the .y of the
+// pair on its left and the .x of the pair on its right.  Sums per cell are the scalar kernels'
+// (same terms, same order).
+
+using f2 = float __attribute__((ext_vector_type(2)));
+
+struct ZCol2 {
+    uint32_t col, x, y, z0, z1;   // x: the even (left) cell a of the pair; b = x + 1
+    uint32_t bx, by, lx, ly;
+    bool active, zvalid;
+    bool hym, hyp, wym, wyp;
+    int32_t oym, oyp;
+    bool hxm_a, wxm_a;            // a's x- neighbour exists / is the periodic wrap (x == 0)
+    bool hxp_b, wxp_b;            // b's x+ neighbour exists / is the wrap (x + 2 == nx)
+    int32_t oxm_a, oxp_b;         // offsets from a's index of a's x- and b's x+ neighbours
+};
+
+__device__ __forceinline__ ZCol2 zcolumn2(const Dims &g) {
+    ZCol2 c;
+    const uint32_t h = blockIdx.x;
+    const uint32_t lt = (h & 7u) * (g.zblk >> 3) + (h >> 3);
+    const uint32_t per_plane = g.ztx2 * g.zty;
+    const uint32_t zc = lt / per_plane;
+    const uint32_t rem = lt - zc * per_plane;
+    c.by = rem / g.ztx2;
+    c.bx = rem - c.by * g.ztx2;
+    c.lx = threadIdx.x % kZTX;
+    c.ly = threadIdx.x / kZTX;
+    c.x = c.bx * (2 * kZTX) + 2 * c.lx;
+    c.y = c.by * kZTY + c.ly;
+    c.z0 = zc * g.zchunk2;
+    c.z1 = c.z0 + g.zchunk2 < g.nz ? c.z0 + g.zchunk2 : g.nz;
+    c.zvalid = lt < g.ztiles2 && c.z0 < g.nz;
+    c.active = c.zvalid && c.x < g.nx && c.y < g.ny;   // nx even: x + 1 < nx as well
+    c.col = c.x + g.nx * c.y;
+    c.hym = c.y > 0 || g.per[1];
+    c.hyp = c.y + 1 < g.ny || g.per[1];
+    c.wym = c.y == 0;
+    c.wyp = c.y + 1 == g.ny;
+    c.oym = c.wym ? (int32_t)((g.ny - 1) * g.nx) : -(int32_t)g.nx;
+    c.oyp = c.wyp ? -(int32_t)((g.ny - 1) * g.nx) : (int32_t)g.nx;
+    c.wxm_a = c.x == 0;
+    c.hxm_a = c.x > 0 || g.per[0];
+    c.oxm_a = c.wxm_a ? (int32_t)(g.nx - 1) : -1;
+    c.wxp_b = c.x + 2 == g.nx;
+    c.hxp_b = c.x + 2 < g.nx || g.per[0];
+    c.oxp_b = c.wxp_b ? -(int32_t)(g.nx - 2) : 2;
+    return c;
+}
+
+__device__ __forceinline__ Ring ring_cell2(const Dims &g, const ZCol2 &c) {
+    Ring r{0, kNoRing};
+    const int t = (int)threadIdx.x;
+    const int TX = (int)kZTX, TY = (int)kZTY;
+    if (t >= kZRing || !c.zvalid) return r;
+    int lx, ly;
+    if (t < TX) {
+        lx = t + 1; ly = 0;
+    } else if (t < 2 * TX) {
+        lx = t - TX + 1; ly = TY + 1;
+    } else if (t < 2 * TX + TY) {
+        lx = 0; ly = t - 2 * TX + 1;
+    } else {
+        lx = TX + 1; ly = t - 2 * TX - TY + 1;
+    }
+    const int64_t px = (int64_t)c.bx * TX + lx - 1, gy = (int64_t)c.by * TY + ly - 1;
+    r.slot = ly * (TX + 2) + lx;
+    if (px >= 0 && 2 * px < (int64_t)g.nx && gy >= 0 && gy < (int64_t)g.ny)
+        r.col = (uint32_t)(2 * px) + g.nx * (uint32_t)gy;
+    return r;
+}
+
+// zmarch for pairs: X(J) gives the operand pair at the even cell J.
+template <class XF, class CF>
+__device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<f2> &sp, XF X,
+                                        CF cell) {
+    const Ring rg = ring_cell2(g, c);
+    const bool per_z = g.per[2];
+    f2 xz_m = (f2)0.0f, xi = (f2)0.0f;
+    if (c.active) {
+        if (c.z0 > 0 || per_z) xz_m = X(zm_index(g, c.col, c.z0));
+        xi = X(c.col + g.nxny * c.z0);
+    }
+    if (c.zvalid) {
+        sp.v[0][c.ly + 1][c.lx + 1] = xi;
+        if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = X(rg.col + g.nxny * c.z0);
+    }
+    int buf = 0;
+    uint32_t I = c.col + g.nxny * c.z0;
+    for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
+        const bool more = z + 1 < c.z1;
+        f2 xz_p = (f2)0.0f, rnext = (f2)0.0f;
+        if (c.active && (z + 1 < g.nz || per_z)) xz_p = X(zp_index(g, c.col, z));
+        if (more && rg.col != kNoRing) rnext = X(rg.col + g.nxny * (z + 1));
+        __syncthreads();   // plane z is staged in sp.v[buf]
+        if (c.active) {
+            const InPlane<f2> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
+            cell(I, z, xi, xz_m, xz_p, nb);
+        }
+        if (more) {
+            sp.v[buf ^ 1][c.ly + 1][c.lx + 1] = xz_p;
+            if (rg.col != kNoRing) (&sp.v[buf ^ 1][0][0])[rg.slot] = rnext;
+        }
+        buf ^= 1;
+        xz_m = xi;
+        xi = xz_p;
+    }
+}
+
+__device__ __forceinline__ f2 ld2(const float *p) { return *(const f2 *)p; }
+__device__ __forceinline__ f2 ldnt2(const float *p) {
+    return __builtin_nontemporal_load((const f2 *)p);
+}
+__device__ __forceinline__ void stnt2(float *p, f2 v) { __builtin_nontemporal_store(v, (f2 *)p); }
+
+// q = A^ p for the pair (a, b) = (p.x, p.y), the scalar ahat's terms and order per cell; P1
+// reads the operand of one (wrap) cell from memory, P2 the pair at an even cell.
+template <class P1F, class P2F>
+__device__ __forceinline__ f2 ahat2(const Dims &g, const ZCol2 &c, uint32_t I, uint32_t z, f2 pi,
+                                    f2 pz_m, f2 pz_p, f2 hz_m, f2 hzc, const float *__restrict__ hx,
+                                    const float *__restrict__ hy, const InPlane<f2> &nb, P1F P1,
+                                    P2F P2) {
+    const bool per_z = g.per[2];
+    const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
+    const f2 hx2 = ld2(&hx[I]);           // a-b coupling, b-(b+1) coupling
+    float sa = 0.0f, sb = 0.0f;
+    if (hzm) {
+        sa = sa + hz_m.x * pz_m.x;
+        sb = sb + hz_m.y * pz_m.y;
+    }
+    if (c.hym) {
+        const f2 cy = ld2(&hy[I + c.oym]);
+        const f2 v = c.wym ? P2(I + c.oym) : nb.ym();
+        sa = sa + cy.x * v.x;
+        sb = sb + cy.y * v.y;
+    }
+    // x-: a's left neighbour, b's is a
+    if (c.hxm_a) sa = sa + hx[I + c.oxm_a] * (c.wxm_a ? P1(I + c.oxm_a) : nb.xm().y);
+    sb = sb + hx2.x * pi.x;
+    // x+: a's right neighbour is b, b's is the next pair's a
+    sa = sa + hx2.x * pi.y;
+    if (c.hxp_b) sb = sb + hx2.y * (c.wxp_b ? P1(I + c.oxp_b) : nb.xp().x);
+    if (c.hyp) {
+        const f2 cy = ld2(&hy[I]);
+        const f2 v = c.wyp ? P2(I + c.oyp) : nb.yp();
+        sa = sa + cy.x * v.x;
+        sb = sb + cy.y * v.y;
+    }
+    if (hzp) {
+        sa = sa + hzc.x * pz_p.x;
+        sb = sb + hzc.y * pz_p.y;
+    }
+    f2 q;
+    q.x = pi.x - sa;
+    q.y = pi.y - sb;
+    return q;
+}
+
+template <bool First>
+__global__ void __launch_bounds__(kZTX * kZTY)
+k_cg_a_z2(Dims g, const float *__restrict__ r, const float *__restrict__ pold,
+          float *__restrict__ pnew, float *__restrict__ q, const float *__restrict__ hx,
+          const float *__restrict__ hy, const float *__restrict__ hz, Scal *sc,
+          double *partials) {
+    __shared__ PlaneBuf<f2> sp;
+    if (sc->done) return;
+    const float beta = (float)sc->beta;
+    auto P1 = [&](uint32_t J) -> float { return First ? r[J] : (float)(r[J] + beta * pold[J]); };
+    auto P2 = [&](uint32_t J) -> f2 {
+        if (First) return ld2(&r[J]);
+        const f2 a = ld2(&r[J]), b = ld2(&pold[J]);
+        f2 v;
+        v.x = a.x + beta * b.x;
+        v.y = a.y + beta * b.y;
+        return v;
+    };
+    double acc[2] = {0.0, 0.0};
+    const ZCol2 c = zcolumn2(g);
+    f2 hz_m = (f2)0.0f;
+    if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ld2(&hz[zm_index(g, c.col, c.z0)]);
+    zmarch2(g, c, sp, P2, [&](uint32_t I, uint32_t z, f2 pi, f2 pz_m, f2 pz_p,
+                              const InPlane<f2> &nb) {
+        const f2 hzc = ldnt2(&hz[I]);
+        const f2 qi = ahat2(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
+        if (!First) {
+            stnt2(&pnew[I], pi);
+            stnt2(&q[I], qi);
+        }
+        acc[0] += (double)pi.x * (double)qi.x + (double)pi.y * (double)qi.y;
+        acc[1] += (double)qi.x * (double)qi.x + (double)qi.y * (double)qi.y;
+        hz_m = hzc;
+    });
+    block_partials<2>(acc, partials);
+}
+
+__global__ void __launch_bounds__(kZTX * kZTY)
+k_cg_b1_z2(Dims g, const float *__restrict__ r, const float *__restrict__ hx,
+           const float *__restrict__ hy, const float *__restrict__ hz, float *__restrict__ y,
+           float *__restrict__ rw, Scal *sc, double *partials) {
+    __shared__ PlaneBuf<f2> sp;
+    if (sc->done) return;
+    const float alpha = (float)sc->alpha;
+    auto P1 = [&](uint32_t J) -> float { return r[J]; };
+    auto P2 = [&](uint32_t J) -> f2 { return ld2(&r[J]); };
+    double acc[1] = {0.0};
+    const ZCol2 c = zcolumn2(g);
+    f2 hz_m = (f2)0.0f;
+    if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ld2(&hz[zm_index(g, c.col, c.z0)]);
+    zmarch2(g, c, sp, P2, [&](uint32_t I, uint32_t z, f2 pi, f2 pz_m, f2 pz_p,
+                              const InPlane<f2> &nb) {
+        const f2 hzc = ldnt2(&hz[I]);
+        const f2 qi = ahat2(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
+        f2 yi, ri;
+        yi.x = alpha * pi.x;
+        yi.y = alpha * pi.y;
+        ri.x = pi.x - alpha * qi.x;
+        ri.y = pi.y - alpha * qi.y;
+        stnt2(&y[I], yi);
+        stnt2(&rw[I], ri);
+        acc[0] += (double)ri.x * (double)ri.x + (double)ri.y * (double)ri.y;
+        hz_m = hzc;
+    });
+    block_partials<1>(acc, partials);
+}
+
 // ---- launchers -----------------------------------------------------------------------------
 
 #define HF_ZLAUNCH(kern, g, st, ...) \
@@ -569,6 +801,15 @@ template <typename T>
 void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, T *q,
                  const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
                  hipStream_t st) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (g.pairs) {
+            if (first)
+                HF_ZLAUNCH(k_cg_a_z2<true>, g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+            else
+                HF_ZLAUNCH(k_cg_a_z2<false>, g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+            return;
+        }
+    }
     if (first)
         HF_ZLAUNCH((k_cg_a_z<T, true>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
     else
@@ -584,6 +825,12 @@ template void launch_cg_a<float>(const Dims &, bool, const float *, const float 
 template <typename T>
 void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *y, T *rw,
                   Scal *sc, double *partials, hipStream_t st) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (g.pairs) {
+            HF_ZLAUNCH(k_cg_b1_z2, g, st, g, r, hx, hy, hz, y, rw, sc, partials);
+            return;
+        }
+    }
     HF_ZLAUNCH(k_cg_b1_z<T>, g, st, g, r, hx, hy, hz, y, rw, sc, partials);
 }
 template void launch_cg_b1<double>(const Dims &, const double *, const double *, const double *,

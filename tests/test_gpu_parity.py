@@ -265,7 +265,11 @@ def test_fp32_ir_matches_fp64(n):
     assert st.inner_iters > 0 and st.outer_iters >= 20
 
 
-MIXED_GEOMETRIES = GEOMETRIES + [dict(nx=5, ny=4, nz=3, isteady=1)]
+# GEOMETRIES plus a steady-state case and an even nx with periodic x and y (the FP32 stencil
+# kernels take two x-neighbouring cells per thread when nx is even, including the wrap pair)
+MIXED_GEOMETRIES = GEOMETRIES + [dict(nx=5, ny=4, nz=3, isteady=1),
+                                 dict(nx=6, ny=4, nz=3, periodic=(True, True, False)),
+                                 dict(nx=8, ny=3, nz=4, cg=(False, True, True, False, False, True))]
 
 
 @pytest.mark.parametrize("gi", range(len(MIXED_GEOMETRIES)))
@@ -354,3 +358,22 @@ def test_first_iteration_stop_is_exact(prec):
     assert its0 == its2
     assert n0 >= 5 and n2 == 0, (n0, n2, its0)
     assert np.array_equal(bits(t0), bits(t2))
+
+
+def test_fp32_pair_kernels_match_scalar(monkeypatch):
+    """With an even nx the FP32 CG kernels take two x-neighbouring cells per thread; with
+    HF_NO_PAIRS set they take one.  The sums per cell are the same; only the order of the
+    per-block partial sums differs, so both runs meet the tolerance and agree to it
+    (log-normal kappa, Gaussian heating, 24^3 and a 2-cell-wide periodic strip)."""
+    for s in (workloads.c3(24)[0], random_system(31, 8, 6, 5, periodic=(True, False, True))):
+        out = []
+        for no_pairs in (False, True):
+            if no_pairs:
+                monkeypatch.setenv("HF_NO_PAIRS", "1")
+            else:
+                monkeypatch.delenv("HF_NO_PAIRS", raising=False)
+            with Simulation(s, precision="fp32_ir", tol=1e-14) as sim:
+                sim.initial_evolve()
+                sim.run(1, 10)
+                out.append(sim.get_state()[0])
+        assert relerr(out[0], out[1]) <= 1e-12

@@ -201,14 +201,19 @@ __device__ __forceinline__ double hx_sum(const Dims &g, const ZCol &c, uint32_t 
                                          XF X) {
     const bool hzm = z > 0 || g.per[2], hzp = z + 1 < g.nz || g.per[2];
     const bool wzm = z == 0, wzp = z + 1 == g.nz;
+    // Terms of neighbours inside the grid, without branches: an absent neighbour contributes
+    // coefficient * 0 = +-0, which leaves the sum unchanged, and its coefficient is read at I
+    // (a valid address).  Wrap terms, last, keep their branches (periodic axes only).
+    const bool ym_in = c.y > 0, yp_in = c.y + 1 < g.ny;
+    const bool xm_in = c.x > 0, xp_in = c.x + 1 < g.nx;
     double h = 0.0;
-    if (hzm && !wzm) h = h + cz_m * xz_m;
-    if (c.hym && !c.wym) h = h + cy[I + c.oym] * nb.ym();
-    if (c.hxm && !c.wxm) h = h + cx[I + c.oxm] * nb.xm();
+    h = h + cz_m * (z > 0 ? xz_m : 0.0);
+    h = h + cy[ym_in ? I - g.nx : I] * (ym_in ? nb.ym() : 0.0);
+    h = h + cx[xm_in ? I - 1 : I] * (xm_in ? nb.xm() : 0.0);
     h = h + di * xi;
-    if (c.hxp && !c.wxp) h = h + cx[I] * nb.xp();
-    if (c.hyp && !c.wyp) h = h + cy[I] * nb.yp();
-    if (hzp && !wzp) h = h + czc * xz_p;
+    h = h + cx[I] * (xp_in ? nb.xp() : 0.0);
+    h = h + cy[I] * (yp_in ? nb.yp() : 0.0);
+    h = h + czc * (z + 1 < g.nz ? xz_p : 0.0);
     // periodic wrap couplings
     if (c.hxm && c.wxm) h = h + cx[I + c.oxm] * X(I + c.oxm);
     if (c.hxp && c.wxp) h = h + cx[I] * X(I + c.oxp);
@@ -291,13 +296,20 @@ __device__ __forceinline__ T ahat(const Dims &g, const ZCol &c, uint32_t I, uint
                                   const T *__restrict__ hy, const InPlane<T> &nb, PF P) {
     const bool per_z = g.per[2];
     const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
+    // absent neighbours contribute coefficient * 0 (coefficient read at I); wrap neighbours are
+    // read from memory
+    T vym = (T)0, vxm = (T)0, vxp = (T)0, vyp = (T)0;
+    if (c.hym) vym = c.wym ? P(I + c.oym) : nb.ym();
+    if (c.hxm) vxm = c.wxm ? P(I + c.oxm) : nb.xm();
+    if (c.hxp) vxp = c.wxp ? P(I + c.oxp) : nb.xp();
+    if (c.hyp) vyp = c.wyp ? P(I + c.oyp) : nb.yp();
     T s = (T)0;
-    if (hzm) s = s + hz_m * pz_m;
-    if (c.hym) s = s + hy[I + c.oym] * (c.wym ? P(I + c.oym) : nb.ym());
-    if (c.hxm) s = s + hx[I + c.oxm] * (c.wxm ? P(I + c.oxm) : nb.xm());
-    if (c.hxp) s = s + hx[I] * (c.wxp ? P(I + c.oxp) : nb.xp());
-    if (c.hyp) s = s + hy[I] * (c.wyp ? P(I + c.oyp) : nb.yp());
-    if (hzp) s = s + hzc * pz_p;
+    s = s + hz_m * (hzm ? pz_m : (T)0);
+    s = s + hy[c.hym ? I + c.oym : I] * vym;
+    s = s + hx[c.hxm ? I + c.oxm : I] * vxm;
+    s = s + hx[I] * vxp;
+    s = s + hy[I] * vyp;
+    s = s + hzc * (hzp ? pz_p : (T)0);
     return pi - s;
 }
 
@@ -483,15 +495,14 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                                  InPlaneOf<0>{nb.c}, Xa);
         const double h2 = hx_sum(g, c, I, z, di, cx, cy, czc, xi.b, xz_m.b, xz_p.b, cz_m,
                                  InPlaneOf<1>{nb.c}, Xb);
-        // H 1: the row sum over the same terms
-        const bool hzm = z > 0 || g.per[2], hzp = z + 1 < g.nz || g.per[2];
+        // H 1: the row sum over the same terms (absent couplings are stored as 0)
         double h1s = di;
-        if (hzm) h1s = h1s + cz_m;
-        if (c.hym) h1s = h1s + cy[I + c.oym];
-        if (c.hxm) h1s = h1s + cx[I + c.oxm];
-        if (c.hxp) h1s = h1s + cx[I];
-        if (c.hyp) h1s = h1s + cy[I];
-        if (hzp) h1s = h1s + czc;
+        h1s = h1s + cz_m;
+        h1s = h1s + cy[c.hym ? I + c.oym : I] * (c.hym ? 1.0 : 0.0);
+        h1s = h1s + cx[c.hxm ? I + c.oxm : I] * (c.hxm ? 1.0 : 0.0);
+        h1s = h1s + cx[I];
+        h1s = h1s + cy[I];
+        h1s = h1s + czc;
         const double s = 1.0 / sqrt(-di);
         const float rf = (float)(-s * (Sn - h1));
         const double ra = -s * (Sn1 - h2);
@@ -645,33 +656,31 @@ __device__ __forceinline__ f2 ahat2(const Dims &g, const ZCol2 &c, uint32_t I, u
     const bool per_z = g.per[2];
     const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
     const f2 hx2 = ld2(&hx[I]);           // a-b coupling, b-(b+1) coupling
+    // absent neighbours contribute coefficient * 0 (coefficients read at I); wrap neighbours
+    // are read from memory
+    f2 vym = (f2)0.0f, vyp = (f2)0.0f;
+    float vxm = 0.0f, vxp = 0.0f;
+    if (c.hym) vym = c.wym ? P2(I + c.oym) : nb.ym();
+    if (c.hyp) vyp = c.wyp ? P2(I + c.oyp) : nb.yp();
+    if (c.hxm_a) vxm = c.wxm_a ? P1(I + c.oxm_a) : nb.xm().y;
+    if (c.hxp_b) vxp = c.wxp_b ? P1(I + c.oxp_b) : nb.xp().x;
+    const f2 zm = hzm ? pz_m : (f2)0.0f, zp = hzp ? pz_p : (f2)0.0f;
+    const f2 cym = ld2(&hy[c.hym ? I + c.oym : I]), cyp = ld2(&hy[I]);
+    const float cxm = hx[c.hxm_a ? I + c.oxm_a : I];
     float sa = 0.0f, sb = 0.0f;
-    if (hzm) {
-        sa = sa + hz_m.x * pz_m.x;
-        sb = sb + hz_m.y * pz_m.y;
-    }
-    if (c.hym) {
-        const f2 cy = ld2(&hy[I + c.oym]);
-        const f2 v = c.wym ? P2(I + c.oym) : nb.ym();
-        sa = sa + cy.x * v.x;
-        sb = sb + cy.y * v.y;
-    }
-    // x-: a's left neighbour, b's is a
-    if (c.hxm_a) sa = sa + hx[I + c.oxm_a] * (c.wxm_a ? P1(I + c.oxm_a) : nb.xm().y);
+    sa = sa + hz_m.x * zm.x;
+    sb = sb + hz_m.y * zm.y;
+    sa = sa + cym.x * vym.x;
+    sb = sb + cym.y * vym.y;
+    // x-: a's left neighbour, b's is a; x+: a's right neighbour is b, b's is the next pair's a
+    sa = sa + cxm * vxm;
     sb = sb + hx2.x * pi.x;
-    // x+: a's right neighbour is b, b's is the next pair's a
     sa = sa + hx2.x * pi.y;
-    if (c.hxp_b) sb = sb + hx2.y * (c.wxp_b ? P1(I + c.oxp_b) : nb.xp().x);
-    if (c.hyp) {
-        const f2 cy = ld2(&hy[I]);
-        const f2 v = c.wyp ? P2(I + c.oyp) : nb.yp();
-        sa = sa + cy.x * v.x;
-        sb = sb + cy.y * v.y;
-    }
-    if (hzp) {
-        sa = sa + hzc.x * pz_p.x;
-        sb = sb + hzc.y * pz_p.y;
-    }
+    sb = sb + hx2.y * vxp;
+    sa = sa + cyp.x * vyp.x;
+    sb = sb + cyp.y * vyp.y;
+    sa = sa + hzc.x * zp.x;
+    sb = sb + hzc.y * zp.y;
     f2 q;
     q.x = pi.x - sa;
     q.y = pi.y - sb;

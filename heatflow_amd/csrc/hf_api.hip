@@ -612,7 +612,9 @@ static int run_cg(hf_ctx *c, Scal *sc, Scal *h, const CgVecs<T> &v, int64_t *pre
     drop_idle_slots(c, slots, h->iter, h->shortcut != 0);
     if (h->shortcut) c->stats.first_iter_stops += 1;
     *iters = h->iter;
-    *pred = std::max<int64_t>(1, h->iter + 1);
+    // next batch: what this solve needed, +1 unless it stopped on its first iteration (a batch
+    // cut short only costs one more host synchronisation)
+    *pred = std::max<int64_t>(1, h->iter + (h->shortcut ? 0 : 1));
     return HF_OK;
 }
 
@@ -746,6 +748,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         if (c->h_sci->shortcut) c->stats.first_iter_stops += 1;
         inner_total += in_it;
         sweeps += 1;
+        // next batch: what this sweep needed + 1 (a batch cut short costs a refinement sweep)
         c->pred_inner = c->h_sci->done ? std::max<int64_t>(1, in_it + 1) : batch * 2;
         xa = xo;
         nb ^= 1;
@@ -828,6 +831,7 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         inner_total += in_it;
         sweeps += 1;
         // next prediction: what this sweep needed (+1), or a longer batch if it was cut short
+        // next batch: what this sweep needed + 1 (a batch cut short costs a refinement sweep)
         c->pred_inner = c->h_sci->done ? std::max<int64_t>(1, in_it + 1) : batch * 2;
         int32_t fl = c->h_sc->flags | c->setup_flags;
         if (fl & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S))

@@ -105,7 +105,7 @@ static void make_dims(Dims &g, const hf_params &p) {
     g.dny.init(g.ny);
     // launch layout: tiles of kBlock cells over a grid of at most `cap` blocks (multiple of 8);
     // HF_NBLK / HF_LAYOUT override the defaults (tuning sweeps)
-    uint32_t cap = 8192;
+    uint32_t cap = 16384;
     if (const char *e = getenv("HF_NBLK")) cap = std::max(8u, (uint32_t)atoi(e) / 8u * 8u);
     g.layout = LAYOUT_CHUNKED | LAYOUT_XCD;
     if (const char *e = getenv("HF_LAYOUT")) g.layout = atoi(e);
@@ -116,7 +116,8 @@ static void make_dims(Dims &g, const hf_params &p) {
     g.nblk = (uint32_t)((nb + 7) / 8 * 8);
     // z-marching tiles of kZTX x kZTY columns: enough z-chunks for ~`zcap` blocks (HF_ZBLK
     // overrides)
-    uint32_t zcap = 16384;
+    // (at 512^3 sweeps of 4096 .. 32768 blocks favoured the fewest, longest z-chunks: 64 planes)
+    uint32_t zcap = 4096;
     if (const char *e = getenv("HF_ZBLK")) zcap = std::max(8u, (uint32_t)atoi(e));
     const uint32_t tx = kZTX, ty = kZTY;
     g.zty = (g.ny + ty - 1) / ty;

@@ -98,39 +98,62 @@ def rel_errors(a, ref):
     return float(e.max()), float(np.sqrt(np.mean(e * e)))
 
 
+def _omp_threads() -> int:
+    """Threads the reference's OpenMP build runs with (omp_get_max_threads of the libomp it
+    links)."""
+    import oracle
+    return int(oracle.rlib(True).omp_get_max_threads())
+
+
 def cpu_baseline(n_full: int, tol: float, sample_n: int, sample_steps: int) -> dict:
-    """The reference Fortran path (oracle/_ref, amdflang -O3, serial) on the same physics at a
-    reduced grid; scaled to timesteps/s at n_full^3 by cell count (iterations per step do not
-    grow with N for this mass-dominated operator).  Falls back to the C restatement."""
+    """The reference Fortran path (oracle/_ref, amdflang -O3) on the same physics at a reduced
+    grid, serial and with the reference's OpenMP directives on this host's threads; scaled to
+    timesteps/s at n_full^3 by cell count (iterations per step do not grow with N for this
+    mass-dominated operator).  The faster build is the reported value.  Also times 2 steps at
+    the reference's own tolerance, 1e-32.  Falls back to the C restatement (serial)."""
     import oracle
     from heatflow_amd import workloads
     s, _ = workloads.c3(sample_n)
     n = s.params.n
     tp = np.full(n, 300.0)
-    kind = "reference"
-    iters = []
-    if oracle.ref_available():
-        r = oracle.RefSystem(s)
+
+    def ref_rate(omp: bool, steps: int, t: float):
+        r = oracle.RefSystem(s, omp=omp)
         r.set_state(tp, tp)
         t0 = time.perf_counter()
-        for it in range(1, sample_steps + 1):
-            iters.append(r.simulate_qextra(it, s.cells.qdens_extra, tol))
+        its = [r.simulate_qextra(k, s.cells.qdens_extra, t) for k in range(1, steps + 1)]
         dt = time.perf_counter() - t0
-    else:
-        kind = "port"
+        return n * steps / dt / float(n_full) ** 3, dt, sum(its)
+
+    if not oracle.ref_available():
         o = oracle.OracleSystem(s)
         t0 = time.perf_counter()
         rc, _, _, tot = o.run(sample_steps, tol=tol)
         dt = time.perf_counter() - t0
-        iters = [tot]
-    cell_steps = n * sample_steps / dt
-    return dict(value=round(cell_steps / float(n_full) ** 3, 6), unit="timesteps/s", cores=1,
-                kind=kind,
-                sample=(f"{'reference Fortran modules (amdflang -O3, serial COO BiCG)' if kind == 'reference' else 'C restatement'}"
-                        f" on the same workload at {sample_n}^3, {sample_steps} steps, tol {tol:g}: "
-                        f"{dt:.2f} s, {sum(iters)} solver iterations; scaled by cell count to "
+        return dict(value=round(n * sample_steps / dt / float(n_full) ** 3, 6),
+                    unit="timesteps/s", cores=1, kind="port",
+                    sample=(f"C restatement on the same workload at {sample_n}^3, {sample_steps} "
+                            f"steps, tol {tol:g}: {dt:.2f} s, {tot} solver iterations; scaled by "
+                            f"cell count to {n_full}^3"), sample_seconds=round(dt, 3))
+    v1, dt1, it1 = ref_rate(False, sample_steps, tol)
+    out = dict(serial=dict(value=round(v1, 6), cores=1, seconds=round(dt1, 3), iterations=it1))
+    best, cores, dtb, itb, label = v1, 1, dt1, it1, "serial"
+    if oracle.ref_omp_available():
+        threads = _omp_threads()
+        v2, dt2, it2 = ref_rate(True, sample_steps, tol)
+        out["openmp"] = dict(value=round(v2, 6), cores=threads, seconds=round(dt2, 3),
+                             iterations=it2)
+        if v2 > v1:
+            best, cores, dtb, itb, label = v2, threads, dt2, it2, f"OpenMP, {threads} threads"
+    v3, dt3, it3 = ref_rate(cores > 1, 2, 1e-32)
+    out["reference_tol_1e-32"] = dict(value=round(v3, 6), steps=2, seconds=round(dt3, 3),
+                                      iterations=it3)
+    return dict(value=round(best, 6), unit="timesteps/s", cores=cores, kind="reference",
+                sample=(f"reference Fortran modules (amdflang -O3, COO BiCG, {label}) on the "
+                        f"same workload at {sample_n}^3, {sample_steps} steps, tol {tol:g}: "
+                        f"{dtb:.2f} s, {itb} solver iterations; scaled by cell count to "
                         f"{n_full}^3"),
-                sample_seconds=round(dt, 3))
+                sample_seconds=round(dtb, 3), builds=out)
 
 
 def stiff_bench(s, steps: int, tol: float, dt_factor: float = 1000.0) -> dict:

@@ -9,9 +9,11 @@
 !!
 !! On the first call the module gathers the per-cell fields of grid(:,:,:) into
 !! structure-of-arrays, creates a device context, uploads and assembles H on the device.  Each
-!! call pushes Temp_p/Temp_pp (the host copy stays authoritative for the driver: initial_evolve
-!! and restarts write it), runs one step and pulls the new history back, so data_write and
-!! final_print see the same arrays as with the CPU solver.  The reference's fatal NaN guards are
+!! call runs one step on the device and leaves the same Temp_p/Temp_pp on the host as the CPU
+!! solver, so data_write and final_print see the same arrays: the old Temp_p becomes Temp_pp by
+!! moving the allocation, and only the new Temp_p is copied back.  The host history goes to the
+!! device on the first call and at step 1 (after initial_evolve, which may read a restart);
+!! the driver does not change it between steps otherwise.  The reference's fatal NaN guards are
 !! reproduced: the message of mod_evolve.f90:70-158 and `stop 1`.
 module evolution
   use iso_c_binding
@@ -73,7 +75,8 @@ module evolution
      integer(c_int) function hf_get_state(ctx, tp, tpp) bind(C, name="hf_get_state")
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: ctx
-       real(c_double), intent(out) :: tp(*), tpp(*)
+       real(c_double), intent(out) :: tp(*)
+       real(c_double), intent(out), optional :: tpp(*)     ! absent: not copied
      end function
      integer(c_int) function hf_step(ctx, itime, iter, err) bind(C, name="hf_step")
        import :: c_ptr, c_int, c_int64_t, c_double
@@ -89,6 +92,8 @@ module evolution
   end interface
 
   type(c_ptr), save :: ctx = c_null_ptr
+  !! the device holds Temp_p / Temp_pp as the host last received them
+  logical, save :: device_current = .false.
 
 contains
 
@@ -99,18 +104,32 @@ contains
     integer(c_int64_t) :: iter
     real(c_double) :: err
 
+    real(c_double), allocatable :: moved(:)
+
     if (.not. c_associated(ctx)) then
        call device_setup()
+       device_current = .false.
     else if (TempDepProp .eq. 1) then
        ! ChangeProp (mod_tempdep.f90:38-65) re-runs sparse_Hmatrix after every step
        call upload_cells()
     end if
-    rc = hf_set_state(ctx, Temp_p, Temp_pp)
-    if (rc /= 0) call fail(rc)
+    ! The driver changes the history only through initial_evolve, before step 1
+    ! (heatflow.f90:79-80, mod_init_evolve.f90:34-51); otherwise the device copy is current.
+    if (itime == 1) device_current = .false.
+    if (.not. device_current) then
+       rc = hf_set_state(ctx, Temp_p, Temp_pp)
+       if (rc /= 0) call fail(rc)
+    end if
     rc = hf_step(ctx, int(itime, c_int64_t), iter, err)
     if (rc /= 0) call fail(rc)
-    rc = hf_get_state(ctx, Temp_p, Temp_pp)
+    ! Temp_pp = Temp_p; Temp_p = x (mod_evolve.f90:169-170): the old Temp_p moves to Temp_pp
+    ! without a copy and only the new Temp_p comes back from the device
+    call move_alloc(Temp_pp, moved)
+    call move_alloc(Temp_p, Temp_pp)
+    call move_alloc(moved, Temp_p)
+    rc = hf_get_state(ctx, Temp_p)
     if (rc /= 0) call fail(rc)
+    device_current = .true.
   end subroutine simulate
 
   subroutine device_setup()

@@ -27,6 +27,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libhfref.so")
+REF_OMP_SO = os.path.join(HERE, "_ref", "libhfref_omp.so")   # the same with -fopenmp
 REF_EXE = os.path.join(HERE, "_ref", "heatflow_ref")
 REF_SRC = "/root/reference"
 
@@ -178,7 +179,6 @@ class OracleSystem:
 
 # ---- the reference build (Fortran modules + ref_harness.f90) ------------------------------
 
-_rlib = None
 # simulate() and linbcg() keep their N-vectors as automatic arrays on the stack
 # (src/heatflow/mod_evolve.f90:47, src/heatflow/mod_Sparse.f90:384): every call into the
 # reference build runs on a thread with a large stack.
@@ -212,20 +212,30 @@ def ref_available() -> bool:
     return os.path.exists(REF_SO)
 
 
-def rlib() -> C.CDLL:
-    global _rlib
-    if _rlib is None:
-        if not os.path.exists(REF_SO):
+_rlibs = {}
+
+
+def rlib(omp: bool = False) -> C.CDLL:
+    """The reference build: serial, or with the reference's OpenMP directives (-fopenmp; its
+    SpMV loops, mod_Sparse.f90:306-336, run in parallel with atomic updates)."""
+    if omp not in _rlibs:
+        path = REF_OMP_SO if omp else REF_SO
+        if not os.path.exists(path):
             raise RuntimeError("reference build missing: run `make -C oracle ref`")
-        _rlib = C.CDLL(REF_SO)
-    return _rlib
+        _rlibs[omp] = C.CDLL(path)
+    return _rlibs[omp]
+
+
+def ref_omp_available() -> bool:
+    return os.path.exists(REF_OMP_SO)
 
 
 class RefSystem:
     """The reference modules set up with a System (one at a time: module globals)."""
 
-    def __init__(self, system):
+    def __init__(self, system, omp: bool = False):
         p, c = system.params, system.cells
+        self.lib = rlib(omp)
         self.n = p.n
         self.params = p
         f = lambda a: np.ascontiguousarray(a, np.float64)
@@ -236,59 +246,59 @@ class RefSystem:
         self.keep = [ip, dp, f(c.kappa), f(c.rho), f(c.heat_capacity), f(c.tau), f(c.em),
                      f(c.len_x), f(c.len_y), f(c.len_z),
                      np.ascontiguousarray(c.iheater, np.int64)]
-        _big_stack(rlib().hfr_setup, *[_p(a) for a in self.keep])
+        _big_stack(self.lib.hfr_setup, *[_p(a) for a in self.keep])
 
     def coo(self):
         n, L = C.c_int64(), C.c_int64()
-        rlib().hfr_coo_sizes(C.byref(n), C.byref(L))
+        self.lib.hfr_coo_sizes(C.byref(n), C.byref(L))
         val = np.empty(L.value)
         irow = np.empty(L.value, np.int64)
         jcol = np.empty(L.value, np.int64)
-        rlib().hfr_coo(_p(val), _p(irow), _p(jcol))
+        self.lib.hfr_coo(_p(val), _p(irow), _p(jcol))
         return n.value, L.value, val, irow, jcol
 
     def set_state(self, tp, tpp):
         a = np.ascontiguousarray(tp, np.float64)
         b = np.ascontiguousarray(tpp, np.float64)
-        rlib().hfr_set_state(_p(a), _p(b))
+        self.lib.hfr_set_state(_p(a), _p(b))
 
     def get_state(self):
         a = np.empty(self.n)
         b = np.empty(self.n)
-        rlib().hfr_get_state(_p(a), _p(b))
+        self.lib.hfr_get_state(_p(a), _p(b))
         return a, b
 
     def simulate(self, itime: int):
-        _big_stack(rlib().hfr_simulate, C.c_int64(itime))
+        _big_stack(self.lib.hfr_simulate, C.c_int64(itime))
 
     def simulate_qextra(self, itime: int, qextra, tol: float = 1e-32) -> int:
         q = np.ascontiguousarray(qextra, np.float64)
         it = C.c_int64()
-        _big_stack(rlib().hfr_simulate_qextra, C.c_int64(itime), _p(q), C.c_double(tol),
+        _big_stack(self.lib.hfr_simulate_qextra, C.c_int64(itime), _p(q), C.c_double(tol),
                    C.byref(it))
         return it.value
 
     def boundary(self):
         b = np.empty(self.n)
-        _big_stack(rlib().hfr_boundary, _p(b))
+        _big_stack(self.lib.hfr_boundary, _p(b))
         return b
 
     def heater(self, itime: int):
         q = np.empty(self.n)
         qd = np.empty(self.n)
         hv = C.c_double()
-        _big_stack(rlib().hfr_heater, C.c_int64(itime), _p(q), _p(qd), C.byref(hv))
+        _big_stack(self.lib.hfr_heater, C.c_int64(itime), _p(q), _p(qd), C.byref(hv))
         return q, qd, hv.value
 
     def scat(self):
         s = np.empty(self.n)
-        _big_stack(rlib().hfr_scat, _p(s))
+        _big_stack(self.lib.hfr_scat, _p(s))
         return s
 
     def linbcg(self, b, x, tol=1e-32, itmax=50000, itol=1):
         bb = np.ascontiguousarray(b, np.float64)
         xx = np.array(x, np.float64)
         it, err = C.c_int64(), C.c_double()
-        _big_stack(rlib().hfr_linbcg, _p(bb), _p(xx), C.c_int64(itol), C.c_double(tol),
+        _big_stack(self.lib.hfr_linbcg, _p(bb), _p(xx), C.c_int64(itol), C.c_double(tol),
                    C.c_int64(itmax), C.byref(it), C.byref(err))
         return xx, it.value, err.value

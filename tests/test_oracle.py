@@ -82,6 +82,23 @@ def test_oracle_matches_reference_build(gi):
         assert np.array_equal(otp, rtp) and np.array_equal(otpp, rtpp)
 
 
+@pytest.mark.skipif(not oracle.ref_omp_available(),
+                    reason="OpenMP reference build oracle/_ref/libhfref_omp.so not present")
+def test_reference_openmp_build_matches_serial():
+    """The OpenMP build of the reference (the CPU baseline's parallel leg) solves the same
+    steps: its atomic SpMV only reorders sums, so temperatures agree to round-off."""
+    s = random_system(11, 12, 10, 9, nheat=4)
+    tp0, tpp0 = random_state(3, s.params.n)
+    out = []
+    for omp in (False, True):
+        r = oracle.RefSystem(s, omp=omp)
+        r.set_state(tp0, tpp0)
+        for k in (1, 2, 3):
+            r.simulate(k)
+        out.append(r.get_state()[0])
+    np.testing.assert_allclose(out[1], out[0], rtol=1e-12)
+
+
 @needs_ref
 def test_oracle_matches_reference_qextra():
     s = random_system(7, 5, 4, 3, qextra=True)

@@ -190,6 +190,52 @@ __device__ __forceinline__ void zmarch(const Dims &g, const ZCol &c, PlaneBuf<V>
     }
 }
 
+// zmarch with a per-cell stash: when the own column's operand of plane z+1 is formed, stash(J,
+// slot) keeps values of that cell (loads of the same addresses, which the compiler shares with
+// X's) in an LDS slot; cell() then receives the slot of plane z.  Values a cell needs again one
+// plane after X loaded them would otherwise be fetched a second time: one plane of a block's
+// traffic across an XCD's resident blocks exceeds its L2.
+template <typename V, typename O, class XF, class SF, class CF>
+__device__ __forceinline__ void zmarch_stash(const Dims &g, const ZCol &c, PlaneBuf<V> &sp,
+                                             O (&ob)[2][kZTY][kZTX], XF X, SF stash, CF cell) {
+    const Ring rg = ring_cell(g, c);
+    const bool per_z = g.per[2];
+    V xz_m = (V)0, xi = (V)0;
+    if (c.active) {
+        if (c.z0 > 0 || per_z) xz_m = X(zm_index(g, c.col, c.z0));
+        xi = X(c.col + g.nxny * c.z0);
+        stash(c.col + g.nxny * c.z0, ob[0][c.ly][c.lx]);
+    }
+    if (c.zvalid) {
+        sp.v[0][c.ly + 1][c.lx + 1] = xi;
+        if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = X(rg.col + g.nxny * c.z0);
+    }
+    int buf = 0;
+    uint32_t I = c.col + g.nxny * c.z0;
+    for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
+        const bool more = z + 1 < c.z1;
+        V xz_p = (V)0, rnext = (V)0;
+        if (c.active && (z + 1 < g.nz || per_z)) {
+            const uint32_t J = zp_index(g, c.col, z);
+            xz_p = X(J);
+            if (more) stash(J, ob[buf ^ 1][c.ly][c.lx]);
+        }
+        if (more && rg.col != kNoRing) rnext = X(rg.col + g.nxny * (z + 1));
+        __syncthreads();   // plane z is staged in sp.v[buf]
+        if (c.active) {
+            const InPlane<V> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
+            cell(I, z, xi, xz_m, xz_p, nb, ob[buf][c.ly][c.lx]);
+        }
+        if (more) {
+            sp.v[buf ^ 1][c.ly + 1][c.lx + 1] = xz_p;
+            if (rg.col != kNoRing) (&sp.v[buf ^ 1][0][0])[rg.slot] = rnext;
+        }
+        buf ^= 1;
+        xz_m = xi;
+        xi = xz_p;
+    }
+}
+
 // Sum of the seven stencil terms of H x in the reference's serial order: xz_m / xz_p and cz_m
 // are the z neighbours' operands and the z- coupling carried in registers, nb the in-plane
 // neighbours from LDS; periodic wrap terms (last) read the operand X from memory.
@@ -439,6 +485,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             float *__restrict__ rnext0, float *__restrict__ rnext1, Scal *sc,
             double *partials) {
     __shared__ PlaneBuf<D2> sp;
+    __shared__ D2 own[2][kZTY][kZTX];   // T^n, T^{n-1} of the cell (state pass)
     const bool tiny = sc->tiny != 0;
     // y = alpha p0 of an inner solve that ended on its first iteration, as k_cg_b forms it
     const bool sh = inner->shortcut != 0;
@@ -458,17 +505,20 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     };
     auto Xa = [&](uint32_t J) -> double { return X(J).a; };
     auto Xb = [&](uint32_t J) -> double { return X(J).b; };
+    // T^n and T^{n-1} of the own column, loaded by X one plane ahead
+    auto stash = [&](uint32_t J, D2 &slot) { slot = D2(tp[J], FromState ? tpp[J] : 0.0); };
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     int32_t fl = 0, fl_next = 0;
     const ZCol c = zcolumn(g);
     double cz_m = 0.0;
     if (c.active && (c.z0 > 0 || g.per[2])) cz_m = cz[zm_index(g, c.col, c.z0)];
-    zmarch<D2>(g, c, sp, X, [&](uint32_t I, uint32_t z, D2 xi, D2 xz_m, D2 xz_p,
-                                const InPlane<D2> &nb) {
+    zmarch_stash<D2>(g, c, sp, own, X, stash, [&](uint32_t I, uint32_t z, D2 xi, D2 xz_m,
+                                                  D2 xz_p, const InPlane<D2> &nb,
+                                                  const D2 &ot) {
         const double di = ldnt(&d[I]);
         const double czc = ldnt(&cz[I]);
         const double qbi = ldnt(&qb[I]);
-        const double tpi = tp[I];
+        const double tpi = ot.a;
         double rc = 0.0, cv = 0.0;
         if (sp_.isteady == 0) {
             rc = ldnt(&rhoc[I]);
@@ -481,7 +531,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             Sn = -(sp_.inverse_time * tpi * rc) - qbi;
             Sn1 = -(sp_.inverse_time * xi.a * rc) - qbi;
             if (sp_.icattaneo == 1) {
-                Sn = Sn + (ldnt(&tpp[I]) - 2.0 * tpi) * cv;
+                Sn = Sn + ((FromState ? ot.b : tpp[I]) - 2.0 * tpi) * cv;
                 const double scat = (tpi - 2.0 * xi.a) * cv;
                 if (isnan(scat)) fl_next |= FLAG_NAN_SCAT;
                 Sn1 = Sn1 + scat;

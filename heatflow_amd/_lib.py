@@ -56,7 +56,7 @@ class HfParams(C.Structure):
 class HfStats(C.Structure):
     _fields_ = [
         ("steps", C.c_int64), ("outer_iters", C.c_int64), ("inner_iters", C.c_int64),
-        ("last_err", C.c_double), ("first_iter_stops", C.c_int64),
+        ("last_err", C.c_double), ("first_iter_stops", C.c_int64), ("tiny_misses", C.c_int64),
     ]
 
 

@@ -187,7 +187,7 @@ static int alloc_mode(hf_ctx *c) {
         TRY(dalloc(c, &c->r32, N)); TRY(dalloc(c, &c->p32[0], N)); TRY(dalloc(c, &c->p32[1], N));
         TRY(dalloc(c, &c->q32, N)); TRY(dalloc(c, &c->y32, N));
         TRY(dalloc(c, &c->S, N)); TRY(dalloc(c, &c->T[2], N));
-        TRY(dalloc(c, &c->rn[0], N)); TRY(dalloc(c, &c->rn[1], N)); TRY(dalloc(c, &c->rn[2], N));
+        TRY(dalloc(c, &c->rn[0], N)); TRY(dalloc(c, &c->rn[1], N));
         TRY(dalloc(c, &c->rw32, N)); TRY(dalloc(c, &c->qb, N));
     } else {
         TRY(dalloc(c, &c->hx, N)); TRY(dalloc(c, &c->hy, N)); TRY(dalloc(c, &c->hz, N));
@@ -680,11 +680,10 @@ static int inner_solve(hf_ctx *c, const float *r, int64_t *batch_out, std::vecto
 }
 
 // the first mixed-mode residual vector that is none of the given ones
-static float *other_rvec(hf_ctx *c, const float *a, const float *b = nullptr,
-                         const float *d = nullptr) {
-    float *all[4] = {c->r32, c->rn[0], c->rn[1], c->rn[2]};
+static float *other_rvec(hf_ctx *c, const float *a, const float *b = nullptr) {
+    float *all[3] = {c->r32, c->rn[0], c->rn[1]};
     for (float *v : all)
-        if (v != a && v != b && v != d) return v;
+        if (v != a && v != b) return v;
     return nullptr;
 }
 
@@ -692,6 +691,9 @@ static float *other_rvec(hf_ctx *c, const float *a, const float *b = nullptr,
 // side (k_ir_step_z): per step one inner solve and one fused pass; the first step after a
 // change of state starts from k_rhs_res.  Needs a per-step constant Qdens (no time-dependent
 // heater modes or radiation: those recompute Qdens from T^n first).
+// The next step's residual depends on its TINY test, an OR over the grid that the pass itself
+// decides: the pass forms it for the outcome of the last test (c->tiny_guess), and a wrong
+// guess makes the next step start from k_rhs_res.
 static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     (void)itime;
     const hf_params &p = c->p;
@@ -716,7 +718,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         rin = c->r32;
     }
     c->carry = false;
-    const StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo};
+    const StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo, c->tiny_guess ? 1 : 0};
     const int64_t sweeps_max = std::min<int64_t>(p.itmax, 64);
     // refined x ping-pongs between the spare state buffer and S (unused by this path)
     double *xbuf[2] = {c->T[2], c->S};
@@ -728,19 +730,18 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         int64_t batch = 0;
         TRY(inner_solve(c, rin, &batch, slots));
         double *xo = xbuf[nb];
-        // the pass reads y (or the right-hand side, for y = alpha r) and writes three residual
-        // vectors: none of them may be the right-hand side
+        // the pass reads y (or the right-hand side, for y = alpha r) and writes two residual
+        // vectors: neither may be the right-hand side
         float *rres = other_rvec(c, rin);
-        float *rn0 = other_rvec(c, rin, rres);
-        float *rn1 = other_rvec(c, rin, rres, rn0);
+        float *rn = other_rvec(c, rin, rres);
         // k_ir_step: read T^n, T^{n-1} (or x), y or r (FP32), s, rho c_p, tau' rho c_p,
-        // Qdens + B, diag, 3 couplings; write x', three FP32 residuals
-        long si = tbeg(c, HF_K_IR_STEP, N * (8 * 10 + 4 + 8 + 3 * 4) + (xa ? N * 8 : 0));
+        // Qdens + B, diag, 3 couplings; write x', two FP32 residuals
+        long si = tbeg(c, HF_K_IR_STEP, N * (8 * 10 + 4 + 8 + 2 * 4) + (xa ? N * 8 : 0));
         launch_ir_step(c->g, sp, xa == nullptr, c->T[0], c->T[1], xa, c->y32, rin, c->sci, c->sv,
-                       c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, rres, rn0, rn1,
-                       c->sc, c->partials, c->st);
+                       c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, rres, rn, c->sc,
+                       c->partials, c->st);
         tend(c, si);
-        launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->st);
+        launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->tiny_guess, c->st);
         TRY(check_launch(c, "k_ir_step"));
         HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
         TRY(sync_scal(c, c->sc, c->h_sc));
@@ -754,7 +755,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         xa = xo;
         nb ^= 1;
         rin = rres;
-        c->carry_r = (c->h_sc->flags & FLAG_TINY_NEXT) ? rn1 : rn0;
+        c->carry_r = rn;
         int32_t fl = c->h_sc->flags | c->setup_flags;
         if (fl & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S))
             return flags_to_rc(c, fl);
@@ -768,8 +769,11 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     c->T[1] = old_tp;
     c->T[2] = old_tpp;
     c->S = other;
-    // the next step starts from the residual the last pass left for its TINY case
-    c->carry = true;
+    // the next step starts from the residual the last pass left if its TINY case was guessed
+    const bool tiny_next = (c->h_sc->flags & FLAG_TINY_NEXT) != 0;
+    c->carry = tiny_next == c->tiny_guess;
+    if (!c->carry) c->stats.tiny_misses += 1;
+    c->tiny_guess = tiny_next;
     const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->outer_rr / c->h_sc->bnorm2) : 0.0;
     c->stats.steps += 1;
     c->stats.outer_iters += sweeps;

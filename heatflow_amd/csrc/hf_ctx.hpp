@@ -51,10 +51,11 @@ struct hf_ctx {
     // mixed-mode residual vectors besides r32: a refinement pass writes its residuals into
     // vectors other than the right-hand side it read y from.  `carry` is set when the last
     // fused step left the next step's r^0 (and the scalars) valid for the current state;
-    // `carry_r` is that r^0 (without or with the TINY shift)
-    float *rn[3] = {nullptr, nullptr, nullptr};
+    // `carry_r` is that r^0, formed for the TINY outcome `tiny_guess` (the last one seen)
+    float *rn[2] = {nullptr, nullptr};
     double *qb = nullptr;        // Qdens + B for the fused step (set when a carried run starts)
     bool carry = false;
+    bool tiny_guess = true;
     float *carry_r = nullptr;
     // reductions / scalars
     Scal *sc = nullptr, *sci = nullptr;

@@ -503,7 +503,8 @@ __global__ void k_step_begin(Scal *sc) {
 }
 
 // Start of a carried mixed-precision step: the previous fused pass left this step's ||b^||^2
-// and its ||r^0||^2 for both TINY cases (as k_fin_start for a residual from the state).
+// and its ||r^0||^2 for the TINY case it guessed (as k_fin_start for a residual from the
+// state).  The host carries only a right guess; the other case's norm is NaN.
 __global__ void k_carry_begin(Scal *sc, double tol, int64_t itmax, int32_t allow_short) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double rr = sc->next_rr[sc->tiny ? 1 : 0];
@@ -525,11 +526,12 @@ __global__ void k_carry_begin(Scal *sc, double tol, int64_t itmax, int32_t allow
 }
 
 // After the fused mixed step: FP64 residual of this step (refinement test as k_fin_ir) and
-// the next step's norms.  partials: ||r^||^2, ||b^_next||^2, ||r^0_next||^2 (no TINY / TINY).
+// the next step's norms.  partials: ||r^||^2, ||b^_next||^2, ||r^0_next||^2 (TINY case
+// `guess`).
 __global__ void __launch_bounds__(kBlock)
-k_fin_step(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max) {
-    double v[4];
-    sum_partials<4>(partials, nb, v);
+k_fin_step(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max, int32_t guess) {
+    double v[3];
+    sum_partials<3>(partials, nb, v);
     if (threadIdx.x != 0) return;
     const double prev = sc->outer_rr;
     sc->outer_rr = v[0];
@@ -538,8 +540,8 @@ k_fin_step(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max) {
     sc->done = (v[0] <= sc->thresh2) || sc->iter >= sweeps_max || stalled || isnan(v[0]);
     if (isnan(v[0])) sc->flags |= FLAG_BREAKDOWN;
     sc->next_bnorm2 = v[1];
-    sc->next_rr[0] = v[2];
-    sc->next_rr[1] = v[3];
+    sc->next_rr[guess ? 1 : 0] = v[2];
+    sc->next_rr[guess ? 0 : 1] = __builtin_nan("");
 }
 
 // out = a + b (the fused mixed step's per-step constant part of S: Qdens + B)
@@ -651,8 +653,8 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
     HF_FIN(k_fin_ir, g.zblk, st, partials, sc, sweeps_max);
 }
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
-                     hipStream_t st) {
-    HF_FIN(k_fin_step, g.zblk, st, partials, sc, sweeps_max);
+                     bool tiny_guess, hipStream_t st) {
+    HF_FIN(k_fin_step, g.zblk, st, partials, sc, sweeps_max, (int32_t)tiny_guess);
 }
 void launch_carry_begin(Scal *sc, double tol, int64_t itmax, bool allow_short, hipStream_t st) {
     hipLaunchKernelGGL(k_carry_begin, dim3(1), dim3(64), 0, st, sc, tol, itmax,

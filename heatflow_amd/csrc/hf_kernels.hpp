@@ -43,6 +43,7 @@ struct IrParams {
 struct StepParams {
     double inverse_time;
     int32_t isteady, icattaneo;
+    int32_t tiny_guess;        // form the next residual for this outcome of its TINY test
 };
 
 void launch_assemble(const Dims &g, const AsmParams &ap, const double *kappa, const double *lx,
@@ -102,14 +103,14 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
                    hipStream_t st);
 // Fused mixed-precision step: x' = base + s y (base = the state's x0, or xb), this step's
 // FP64 residual at x' (FP32 -> rres), x' -> xout, and the next step's scaled r^0 from
-// (x', T^n) without and with the TINY shift (FP32 -> rnext0 / rnext1); 4 partials.
+// (x', T^n), with the TINY shift if sp.tiny_guess (FP32 -> rnext); 3 partials.
 // qb = Qdens + B (per-step constant in this path)
 void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const double *tp,
                     const double *tpp, const double *xb, const float *y, const float *p0,
                     const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
                     const double *qb, const double *d, const double *cx, const double *cy,
-                    const double *cz, double *xout, float *rres, float *rnext0, float *rnext1,
-                    Scal *sc, double *partials, hipStream_t st);
+                    const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
+                    double *partials, hipStream_t st);
 // one-block finishing kernels: sum the per-block partials in block order, update the scalars
 // allow_short: the solve may end after its first iteration (k_fin_a)
 void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double tol, int64_t itmax,
@@ -119,8 +120,9 @@ void launch_fin_b(const Dims &g, const double *partials, Scal *sc, hipStream_t s
 void launch_fin_b1(const Dims &g, const double *partials, Scal *sc, hipStream_t st);  // after k_cg_b1
 void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                    hipStream_t st);
+// tiny_guess: the TINY case the step's rnext was formed for
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
-                     hipStream_t st);
+                     bool tiny_guess, hipStream_t st);
 void launch_carry_begin(Scal *sc, double tol, int64_t itmax, bool allow_short, hipStream_t st);
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
                         int64_t inner_itmax, double safety, bool allow_short, hipStream_t st);

@@ -464,11 +464,12 @@ k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__res
 //                                 +TINY, mod_evolve.f90:140-141) or a refined xb
 //   r^ = -s (S_n - H x')          this step's FP64 refinement residual -> rres (FP32)
 //   x1 = x' + (x' - T^n)          the next step's initial guess (no TINY yet)
-//   r^0 = -s (S_{n+1} - H x1)     the next step's residual, S_{n+1} from (x', T^n) -> rnext0;
-//                                 (S uses qb = Qdens + B, constant over the run)
-//                                 with the TINY shift x1 + TINY: r^0 + TINY s (H 1) -> rnext1
+//   r^0 = -s (S_{n+1} - H x1)     the next step's residual, S_{n+1} from (x', T^n) -> rnext
+//                                 (S uses qb = Qdens + B, constant over the run), or with
+//                                 the TINY shift x1 + TINY: r^0 + TINY s (H 1)
 // The TINY test of the next step (any x1 - x' < TINY) is an OR over the grid, known only after
-// the pass, so both next residuals are kept and the host takes the one the flag selects.  x'
+// the pass: rnext is formed for the guessed outcome (StepParams::tiny_guess), and the flag the
+// pass raises tells the host whether the guess held (else the next step starts afresh).  x'
 // and x1 are staged together in LDS (two stencil operands, one halo ring).
 // Two staged operands need ~86 VGPRs (5 waves per SIMD, i.e. 2 blocks of 8 waves per CU); held
 // to 80 (6 waves, 3 blocks per CU, a few spilled loop invariants) it runs ~10% faster at 512^3.
@@ -481,9 +482,8 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             const double *__restrict__ cvec, const double *__restrict__ qb,
             const double *__restrict__ d, const double *__restrict__ cx,
             const double *__restrict__ cy, const double *__restrict__ cz,
-            double *__restrict__ xout, float *__restrict__ rres,
-            float *__restrict__ rnext0, float *__restrict__ rnext1, Scal *sc,
-            double *partials) {
+            double *__restrict__ xout, float *__restrict__ rres, float *__restrict__ rnext,
+            Scal *sc, double *partials) {
     __shared__ PlaneBuf<D2> sp;
     __shared__ D2 own[2][kZTY][kZTX];   // T^n, T^{n-1} of the cell (state pass)
     const bool tiny = sc->tiny != 0;
@@ -507,7 +507,8 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     auto Xb = [&](uint32_t J) -> double { return X(J).b; };
     // T^n and T^{n-1} of the own column, loaded by X one plane ahead
     auto stash = [&](uint32_t J, D2 &slot) { slot = D2(tp[J], FromState ? tpp[J] : 0.0); };
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const bool guess = sp_.tiny_guess != 0;
+    double acc[3] = {0.0, 0.0, 0.0};
     int32_t fl = 0, fl_next = 0;
     const ZCol c = zcolumn(g);
     double cz_m = 0.0;
@@ -545,35 +546,35 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                                  InPlaneOf<0>{nb.c}, Xa);
         const double h2 = hx_sum(g, c, I, z, di, cx, cy, czc, xi.b, xz_m.b, xz_p.b, cz_m,
                                  InPlaneOf<1>{nb.c}, Xb);
-        // H 1: the row sum over the same terms (absent couplings are stored as 0)
-        double h1s = di;
-        h1s = h1s + cz_m;
-        h1s = h1s + cy[c.hym ? I + c.oym : I] * (c.hym ? 1.0 : 0.0);
-        h1s = h1s + cx[c.hxm ? I + c.oxm : I] * (c.hxm ? 1.0 : 0.0);
-        h1s = h1s + cx[I];
-        h1s = h1s + cy[I];
-        h1s = h1s + czc;
         const double s = 1.0 / sqrt(-di);
         const float rf = (float)(-s * (Sn - h1));
-        const double ra = -s * (Sn1 - h2);
-        const float fa = (float)ra;
-        const float fb = (float)(ra + kTiny * s * h1s);
+        double ra = -s * (Sn1 - h2);
+        if (guess) {
+            // H 1: the row sum over the same terms (absent couplings are stored as 0)
+            double h1s = di;
+            h1s = h1s + cz_m;
+            h1s = h1s + cy[c.hym ? I + c.oym : I] * (c.hym ? 1.0 : 0.0);
+            h1s = h1s + cx[c.hxm ? I + c.oxm : I] * (c.hxm ? 1.0 : 0.0);
+            h1s = h1s + cx[I];
+            h1s = h1s + cy[I];
+            h1s = h1s + czc;
+            ra = ra + kTiny * s * h1s;
+        }
+        const float fn = (float)ra;
         stnt(&xout[I], xi.a);
         stnt(&rres[I], rf);
-        stnt(&rnext0[I], fa);
-        stnt(&rnext1[I], fb);
+        stnt(&rnext[I], fn);
         const double bh = s * Sn1;
         acc[0] += (double)rf * (double)rf;
         acc[1] += bh * bh;
-        acc[2] += (double)fa * (double)fa;
-        acc[3] += (double)fb * (double)fb;
+        acc[2] += (double)fn * (double)fn;
         if (isnan(xi.a)) fl |= FLAG_NAN_X;
         if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;   // mod_evolve.f90:141 for the next step
         cz_m = czc;
     });
     block_flags(fl, &sc->flags);
     block_flags(fl_next, &sc->flags_next);
-    block_partials<4>(acc, partials);
+    block_partials<3>(acc, partials);
 }
 
 // ---- FP32 kernels with two x-neighbouring cells per thread (nx even) ------------------------
@@ -912,14 +913,14 @@ void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const 
                     const double *tpp, const double *xb, const float *y, const float *p0,
                     const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
                     const double *qb, const double *d, const double *cx, const double *cy,
-                    const double *cz, double *xout, float *rres, float *rnext0, float *rnext1,
-                    Scal *sc, double *partials, hipStream_t st) {
+                    const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
+                    double *partials, hipStream_t st) {
     if (from_state)
         HF_ZLAUNCH(k_ir_step_z<true>, g, st, g, sp, tp, tpp, xb, y, p0, inner, sv, rhoc, cvec, qb,
-                   d, cx, cy, cz, xout, rres, rnext0, rnext1, sc, partials);
+                   d, cx, cy, cz, xout, rres, rnext, sc, partials);
     else
         HF_ZLAUNCH(k_ir_step_z<false>, g, st, g, sp, tp, tpp, xb, y, p0, inner, sv, rhoc, cvec,
-                   qb, d, cx, cy, cz, xout, rres, rnext0, rnext1, sc, partials);
+                   qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);
 }
 
 }  // namespace hf

@@ -163,6 +163,8 @@ typedef struct hf_stats {
     int64_t inner_iters;      /* FP32 inner CG iterations (mixed mode)                 */
     double last_err;          /* final relative residual of the last solve             */
     int64_t first_iter_stops; /* CG solves that ended on their first iteration          */
+    int64_t tiny_misses;      /* mixed-mode steps whose successor's TINY test was guessed
+                                 wrong (that successor started from the state)          */
 } hf_stats;
 int hf_get_stats(hf_ctx *ctx, hf_stats *st);
 int hf_reset_stats(hf_ctx *ctx);

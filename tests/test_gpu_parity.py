@@ -318,6 +318,31 @@ def test_fp32_ir_carried_residual_matches_fresh_start():
     assert relerr(ta, tb) <= 1e-12
 
 
+@pytest.mark.parametrize("heating", ["uniform", "gaussian"])
+def test_fp32_ir_tiny_guess(heating):
+    """The fused pass forms the next step's residual for the last TINY outcome seen (first
+    guess: TINY).  Uniform heating of an insulated log-normal grid raises every cell each
+    step, so TINY holds only at step 1 (x0 = T^n from a flat start): the first guess misses
+    once and the next step starts from the state.  Gaussian heating with 300 K faces leaves
+    cells that do not warm, so TINY holds every step and the guess never misses.  Both match
+    the oracle at the reference tolerance."""
+    s, _ = workloads.c3(16)
+    p = s.params
+    if heating == "uniform":
+        p.kappa_bound = (0.0,) * 6
+        s.cells.qdens_extra = np.full(p.n, 1.0e12)
+    t0 = np.full(p.n, p.t_system)
+    rc, otp, _, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 11)), t0, t0)
+    assert rc == 0
+    with Simulation(s, precision="fp32_ir", tol=1e-14) as sim:
+        sim.initial_evolve()
+        sim.run(1, 10)
+        tp, _ = sim.get_state()
+        st = sim.stats()
+    assert st.tiny_misses == (1 if heating == "uniform" else 0)
+    assert relerr(tp, otp) <= 1e-12
+
+
 def test_fp32_ir_refinement_sweeps():
     """A loose inner tolerance (1e-2) forces several refinement sweeps per step, so the fused
     pass also runs from a refined iterate (not only from the state's initial guess); the

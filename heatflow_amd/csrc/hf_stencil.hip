@@ -190,56 +190,11 @@ __device__ __forceinline__ void zmarch(const Dims &g, const ZCol &c, PlaneBuf<V>
     }
 }
 
-// zmarch with a per-cell stash: when the own column's operand of plane z+1 is formed, stash(J,
-// slot) keeps values of that cell (loads of the same addresses, which the compiler shares with
-// X's) in an LDS slot; cell() then receives the slot of plane z.  Values a cell needs again one
-// plane after X loaded them would otherwise be fetched a second time: one plane of a block's
-// traffic across an XCD's resident blocks exceeds its L2.
-template <typename V, typename O, class XF, class SF, class CF>
-__device__ __forceinline__ void zmarch_stash(const Dims &g, const ZCol &c, PlaneBuf<V> &sp,
-                                             O (&ob)[2][kZTY][kZTX], XF X, SF stash, CF cell) {
-    const Ring rg = ring_cell(g, c);
-    const bool per_z = g.per[2];
-    V xz_m = (V)0, xi = (V)0;
-    if (c.active) {
-        if (c.z0 > 0 || per_z) xz_m = X(zm_index(g, c.col, c.z0));
-        xi = X(c.col + g.nxny * c.z0);
-        stash(c.col + g.nxny * c.z0, ob[0][c.ly][c.lx]);
-    }
-    if (c.zvalid) {
-        sp.v[0][c.ly + 1][c.lx + 1] = xi;
-        if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = X(rg.col + g.nxny * c.z0);
-    }
-    int buf = 0;
-    uint32_t I = c.col + g.nxny * c.z0;
-    for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
-        const bool more = z + 1 < c.z1;
-        V xz_p = (V)0, rnext = (V)0;
-        if (c.active && (z + 1 < g.nz || per_z)) {
-            const uint32_t J = zp_index(g, c.col, z);
-            xz_p = X(J);
-            if (more) stash(J, ob[buf ^ 1][c.ly][c.lx]);
-        }
-        if (more && rg.col != kNoRing) rnext = X(rg.col + g.nxny * (z + 1));
-        __syncthreads();   // plane z is staged in sp.v[buf]
-        if (c.active) {
-            const InPlane<V> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
-            cell(I, z, xi, xz_m, xz_p, nb, ob[buf][c.ly][c.lx]);
-        }
-        if (more) {
-            sp.v[buf ^ 1][c.ly + 1][c.lx + 1] = xz_p;
-            if (rg.col != kNoRing) (&sp.v[buf ^ 1][0][0])[rg.slot] = rnext;
-        }
-        buf ^= 1;
-        xz_m = xi;
-        xi = xz_p;
-    }
-}
-
 // Sum of the seven stencil terms of H x in the reference's serial order: xz_m / xz_p and cz_m
 // are the z neighbours' operands and the z- coupling carried in registers, nb the in-plane
-// neighbours from LDS; periodic wrap terms (last) read the operand X from memory.
-template <class NB, class XF>
+// neighbours from LDS; periodic wrap terms (last) read the operand X from memory.  Wrap =
+// false: no axis is periodic (the wrap terms are left out).
+template <bool Wrap = true, class NB, class XF>
 __device__ __forceinline__ double hx_sum(const Dims &g, const ZCol &c, uint32_t I, uint32_t z,
                                          double di, const double *__restrict__ cx,
                                          const double *__restrict__ cy, double czc, double xi,
@@ -260,6 +215,7 @@ __device__ __forceinline__ double hx_sum(const Dims &g, const ZCol &c, uint32_t 
     h = h + cx[I] * (xp_in ? nb.xp() : 0.0);
     h = h + cy[I] * (yp_in ? nb.yp() : 0.0);
     h = h + czc * (z + 1 < g.nz ? xz_p : 0.0);
+    if (!Wrap) return h;
     // periodic wrap couplings
     if (c.hxm && c.wxm) h = h + cx[I + c.oxm] * X(I + c.oxm);
     if (c.hxp && c.wxp) h = h + cx[I] * X(I + c.oxp);
@@ -459,6 +415,19 @@ k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__res
     block_partials<1>(acc, partials);
 }
 
+// operand inputs of the fused step: T^n, T^{n-1} (or the refined x), s, y
+struct StepRaw {
+    double tp, b, s;
+    float y;
+};
+
+// the other per-cell inputs of the fused step: diag, z+ coupling, Qdens + B, rho c_p,
+// tau' rho c_p, the x+ / y+ couplings and those of the x- / y- neighbours (stored at the
+// neighbour; read at the cell itself where that neighbour is absent), T^{n-1} (refined pass)
+struct StepCell {
+    double d, cz, qb, rc, cv, cx, cy, cxm, cym, tpp;
+};
+
 // Fused step of the mixed-precision path, one pass over the grid after an inner solve:
 //   x' = base + s y               base = x0 of this step from the state (2 T^n - T^{n-1},
 //                                 +TINY, mod_evolve.f90:140-141) or a refined xb
@@ -471,10 +440,16 @@ k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__res
 // the pass: rnext is formed for the guessed outcome (StepParams::tiny_guess), and the flag the
 // pass raises tells the host whether the guess held (else the next step starts afresh).  x'
 // and x1 are staged together in LDS (two stencil operands, one halo ring).
-// Two staged operands need ~86 VGPRs (5 waves per SIMD, i.e. 2 blocks of 8 waves per CU); held
-// to 80 (6 waves, 3 blocks per CU, a few spilled loop invariants) it runs ~10% faster at 512^3.
-template <bool FromState>
-__global__ void __launch_bounds__(kZTX * kZTY) __attribute__((amdgpu_waves_per_eu(6, 8)))
+//
+// The z-march is software-pipelined: while plane z is computed, the loads of plane z+1's cell
+// inputs and halo ring and of plane z+2's own operand inputs are in flight, so a trip waits on
+// loads issued one trip earlier.  That takes ~128 VGPRs: 4 waves per SIMD (2 blocks of 8 waves
+// per CU), which the staged pair of operands allowed anyway; at 512^3 it runs ~8% faster than
+// the same pass with each trip's loads issued after its barrier.  Transient is isteady == 0,
+// Cattaneo icattaneo == 1, Wrap some axis periodic (compile-time, so the trip is straight-line
+// code the scheduler can order freely).
+template <bool FromState, bool Transient, bool Cattaneo, bool Wrap>
+__global__ void __launch_bounds__(kZTX * kZTY) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             const double *__restrict__ tpp, const double *__restrict__ xb,
             const float *__restrict__ y, const float *__restrict__ p0, const Scal *inner,
@@ -485,93 +460,188 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             double *__restrict__ xout, float *__restrict__ rres, float *__restrict__ rnext,
             Scal *sc, double *partials) {
     __shared__ PlaneBuf<D2> sp;
-    __shared__ D2 own[2][kZTY][kZTX];   // T^n, T^{n-1} of the cell (state pass)
     const bool tiny = sc->tiny != 0;
     // y = alpha p0 of an inner solve that ended on its first iteration, as k_cg_b forms it
     const bool sh = inner->shortcut != 0;
     const float *__restrict__ ys = sh ? p0 : y;
     const float ysc = sh ? (float)inner->alpha : 1.0f;
-    auto X = [&](uint32_t J) -> D2 {
-        const double tpj = tp[J];
+    const ZCol c = zcolumn(g);
+    const Ring rg = ring_cell(g, c);
+    const bool per_z = Wrap && g.per[2];
+    const bool xm_in = c.x > 0, xp_in = c.x + 1 < g.nx;
+    const bool ym_in = c.y > 0, yp_in = c.y + 1 < g.ny;
+    auto load = [&](uint32_t J) -> StepRaw {
+        return StepRaw{tp[J], FromState ? tpp[J] : xb[J], sv[J], ys[J]};
+    };
+    auto form = [&](const StepRaw &r) -> D2 {
         double base;
         if (FromState) {
-            base = tpj + (tpj - tpp[J]);
+            base = r.tp + (r.tp - r.b);
             if (tiny) base = base + kTiny;
         } else {
-            base = xb[J];
+            base = r.b;
         }
-        const double xn = base + sv[J] * (double)(ysc * ys[J]);
-        return D2(xn, xn + (xn - tpj));
+        const double xn = base + r.s * (double)(ysc * r.y);
+        return D2(xn, xn + (xn - r.tp));
     };
-    auto Xa = [&](uint32_t J) -> double { return X(J).a; };
-    auto Xb = [&](uint32_t J) -> double { return X(J).b; };
-    // T^n and T^{n-1} of the own column, loaded by X one plane ahead
-    auto stash = [&](uint32_t J, D2 &slot) { slot = D2(tp[J], FromState ? tpp[J] : 0.0); };
+    auto X = [&](uint32_t J) -> D2 { return form(load(J)); };
+    // offsets of the x- / y- neighbours' couplings (0 where absent or the thread is inactive),
+    // opaque to the compiler: it would otherwise turn cx[I + oxm] into a load under a branch
+    // that reuses cx[I], and a branch stops the loads being issued ahead
+    int32_t oxm = c.active && xm_in ? -1 : 0;
+    int32_t oym = c.active && ym_in ? -(int32_t)g.nx : 0;
+    asm volatile("" : "+v"(oxm), "+v"(oym));
+    // I must be a cell of this thread's column (active threads) or 0
+    auto cload = [&](uint32_t I) -> StepCell {
+        StepCell v;
+        v.d = ldnt(&d[I]);
+        v.cz = ldnt(&cz[I]);
+        v.qb = ldnt(&qb[I]);
+        v.rc = Transient ? ldnt(&rhoc[I]) : 0.0;
+        v.cv = (Transient && Cattaneo) ? ldnt(&cvec[I]) : 0.0;
+        v.cx = cx[I];
+        v.cy = cy[I];
+        v.cxm = cx[I + oxm];
+        v.cym = cy[I + oym];
+        v.tpp = FromState ? 0.0 : ldnt(&tpp[I]);
+        return v;
+    };
+    // H v of the cell in the reference's serial order (as hx_sum), K picks x' or x1
+    auto hsum = [&](uint32_t I, uint32_t z, const StepCell &cv, double cz_m, const D2 &xi,
+                    const D2 &xz_m, const D2 &xz_p, const D2 *nbc, auto pick) -> double {
+        double h = 0.0;
+        h = h + cz_m * (z > 0 ? pick(xz_m) : 0.0);
+        h = h + cv.cym * (ym_in ? pick(nbc[-(int)(kZTX + 2)]) : 0.0);
+        h = h + cv.cxm * (xm_in ? pick(nbc[-1]) : 0.0);
+        h = h + cv.d * pick(xi);
+        h = h + cv.cx * (xp_in ? pick(nbc[1]) : 0.0);
+        h = h + cv.cy * (yp_in ? pick(nbc[(int)(kZTX + 2)]) : 0.0);
+        h = h + cv.cz * (z + 1 < g.nz ? pick(xz_p) : 0.0);
+        if (Wrap) {
+            const bool hzm = z > 0 || g.per[2], hzp = z + 1 < g.nz || g.per[2];
+            if (c.hxm && c.wxm) h = h + cx[I + c.oxm] * pick(X(I + c.oxm));
+            if (c.hxp && c.wxp) h = h + cv.cx * pick(X(I + c.oxp));
+            if (c.hym && c.wym) h = h + cy[I + c.oym] * pick(X(I + c.oym));
+            if (c.hyp && c.wyp) h = h + cv.cy * pick(X(I + c.oyp));
+            if (hzm && z == 0) h = h + cz_m * pick(xz_m);
+            if (hzp && z + 1 == g.nz) h = h + cv.cz * pick(xz_p);
+        }
+        return h;
+    };
+    auto pa = [](const D2 &v) { return v.a; };
+    auto pb = [](const D2 &v) { return v.b; };
     const bool guess = sp_.tiny_guess != 0;
     double acc[3] = {0.0, 0.0, 0.0};
     int32_t fl = 0, fl_next = 0;
-    const ZCol c = zcolumn(g);
     double cz_m = 0.0;
-    if (c.active && (c.z0 > 0 || g.per[2])) cz_m = cz[zm_index(g, c.col, c.z0)];
-    zmarch_stash<D2>(g, c, sp, own, X, stash, [&](uint32_t I, uint32_t z, D2 xi, D2 xz_m,
-                                                  D2 xz_p, const InPlane<D2> &nb,
-                                                  const D2 &ot) {
-        const double di = ldnt(&d[I]);
-        const double czc = ldnt(&cz[I]);
-        const double qbi = ldnt(&qb[I]);
-        const double tpi = ot.a;
-        double rc = 0.0, cv = 0.0;
-        if (sp_.isteady == 0) {
-            rc = ldnt(&rhoc[I]);
-            if (sp_.icattaneo == 1) cv = ldnt(&cvec[I]);
+    D2 xz_m = 0.0, xi = 0.0;
+    StepRaw ri{}, r1{};   // own operand inputs of planes z and z+1
+    StepCell cur{};       // cell inputs of plane z
+    if (c.active) {
+        if (c.z0 > 0 || per_z) {
+            const uint32_t J = zm_index(g, c.col, c.z0);
+            xz_m = X(J);
+            cz_m = cz[J];
         }
-        // S of this step (mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36) and of the
-        // next, with Qdens + B summed once per run (qb)
-        double Sn, Sn1;
-        if (sp_.isteady == 0) {
-            Sn = -(sp_.inverse_time * tpi * rc) - qbi;
-            Sn1 = -(sp_.inverse_time * xi.a * rc) - qbi;
-            if (sp_.icattaneo == 1) {
-                Sn = Sn + ((FromState ? ot.b : tpp[I]) - 2.0 * tpi) * cv;
-                const double scat = (tpi - 2.0 * xi.a) * cv;
-                if (isnan(scat)) fl_next |= FLAG_NAN_SCAT;
-                Sn1 = Sn1 + scat;
+        ri = load(c.col + g.nxny * c.z0);
+        xi = form(ri);
+        if (c.z0 + 1 < g.nz || per_z) r1 = load(zp_index(g, c.col, c.z0));
+        cur = cload(c.col + g.nxny * c.z0);
+    }
+    if (c.zvalid) {
+        sp.v[0][c.ly + 1][c.lx + 1] = xi;
+        if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = X(rg.col + g.nxny * c.z0);
+    }
+    int buf = 0;
+    // One trip: plane z (cell I of the column) with its own operand inputs of plane z+1 (r1)
+    // and its cell inputs (cur), loaded one trip earlier; loads r2 (z+2) and cn (z+1).
+    auto trip = [&](uint32_t z, uint32_t I, const StepRaw &r1, const StepCell &cur, StepRaw &r2,
+                    StepCell &cn) {
+        const bool more = z + 1 < c.z1;
+        const bool has_p = c.active && (z + 1 < g.nz || per_z);
+        const bool has_p2 = c.active && more && (z + 2 < g.nz || per_z);
+        const bool has_r = more && rg.col != kNoRing;
+        // The next trips' loads are issued on every trip by every thread, without branches, so
+        // that the compiler's wait counts know how many are in flight and this trip waits only
+        // for the previous trip's.  Where there is nothing to load they read an address this
+        // thread read recently (cache hits): its own cell, or cell 0 (inactive threads).
+        const uint32_t own = c.active ? I : 0u;
+        r2 = load(has_p2 ? zp_index(g, c.col, z + 1) : own);
+        const StepRaw rr = load(rg.col != kNoRing ? rg.col + g.nxny * (more ? z + 1 : z) : 0u);
+        cn = cload(c.active && more ? I + g.nxny : own);
+        __syncthreads();   // plane z is staged in sp.v[buf]
+        D2 xz_p = 0.0;
+        if (c.active) {
+            xz_p = has_p ? form(r1) : D2(0.0);
+            const D2 *nbc = &sp.v[buf][c.ly + 1][c.lx + 1];
+            const double tpi = ri.tp;
+            const double tppi = FromState ? ri.b : cur.tpp;
+            // S of this step (mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36) and of
+            // the next, with Qdens + B summed once per run (qb)
+            double Sn, Sn1;
+            if (Transient) {
+                Sn = -(sp_.inverse_time * tpi * cur.rc) - cur.qb;
+                Sn1 = -(sp_.inverse_time * xi.a * cur.rc) - cur.qb;
+                if (Cattaneo) {
+                    Sn = Sn + (tppi - 2.0 * tpi) * cur.cv;
+                    const double scat = (tpi - 2.0 * xi.a) * cur.cv;
+                    if (isnan(scat)) fl_next |= FLAG_NAN_SCAT;
+                    Sn1 = Sn1 + scat;
+                }
+            } else {
+                Sn = -cur.qb;
+                Sn1 = Sn;
             }
-        } else {
-            Sn = -qbi;
-            Sn1 = Sn;
+            if (isnan(Sn1)) fl_next |= FLAG_NAN_S;
+            const double h1 = hsum(I, z, cur, cz_m, xi, xz_m, xz_p, nbc, pa);
+            const double h2 = hsum(I, z, cur, cz_m, xi, xz_m, xz_p, nbc, pb);
+            const double s = 1.0 / sqrt(-cur.d);
+            const float rf = (float)(-s * (Sn - h1));
+            double ra = -s * (Sn1 - h2);
+            if (guess) {
+                // H 1: the row sum over the same terms (absent couplings are stored as 0)
+                double h1s = cur.d;
+                h1s = h1s + cz_m;
+                h1s = h1s + (Wrap ? cy[c.hym ? I + c.oym : I] : cur.cym) * (c.hym ? 1.0 : 0.0);
+                h1s = h1s + (Wrap ? cx[c.hxm ? I + c.oxm : I] : cur.cxm) * (c.hxm ? 1.0 : 0.0);
+                h1s = h1s + cur.cx;
+                h1s = h1s + cur.cy;
+                h1s = h1s + cur.cz;
+                ra = ra + kTiny * s * h1s;
+            }
+            const float fn = (float)ra;
+            stnt(&xout[I], xi.a);
+            stnt(&rres[I], rf);
+            stnt(&rnext[I], fn);
+            const double bh = s * Sn1;
+            acc[0] += (double)rf * (double)rf;
+            acc[1] += bh * bh;
+            acc[2] += (double)fn * (double)fn;
+            if (isnan(xi.a)) fl |= FLAG_NAN_X;
+            if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;   // mod_evolve.f90:141, next step
+            cz_m = cur.cz;
         }
-        if (isnan(Sn1)) fl_next |= FLAG_NAN_S;
-        const double h1 = hx_sum(g, c, I, z, di, cx, cy, czc, xi.a, xz_m.a, xz_p.a, cz_m,
-                                 InPlaneOf<0>{nb.c}, Xa);
-        const double h2 = hx_sum(g, c, I, z, di, cx, cy, czc, xi.b, xz_m.b, xz_p.b, cz_m,
-                                 InPlaneOf<1>{nb.c}, Xb);
-        const double s = 1.0 / sqrt(-di);
-        const float rf = (float)(-s * (Sn - h1));
-        double ra = -s * (Sn1 - h2);
-        if (guess) {
-            // H 1: the row sum over the same terms (absent couplings are stored as 0)
-            double h1s = di;
-            h1s = h1s + cz_m;
-            h1s = h1s + cy[c.hym ? I + c.oym : I] * (c.hym ? 1.0 : 0.0);
-            h1s = h1s + cx[c.hxm ? I + c.oxm : I] * (c.hxm ? 1.0 : 0.0);
-            h1s = h1s + cx[I];
-            h1s = h1s + cy[I];
-            h1s = h1s + czc;
-            ra = ra + kTiny * s * h1s;
+        if (more) {
+            // sp.v[buf ^ 1] was last read before this trip's barrier
+            sp.v[buf ^ 1][c.ly + 1][c.lx + 1] = xz_p;
+            if (has_r) (&sp.v[buf ^ 1][0][0])[rg.slot] = form(rr);
         }
-        const float fn = (float)ra;
-        stnt(&xout[I], xi.a);
-        stnt(&rres[I], rf);
-        stnt(&rnext[I], fn);
-        const double bh = s * Sn1;
-        acc[0] += (double)rf * (double)rf;
-        acc[1] += bh * bh;
-        acc[2] += (double)fn * (double)fn;
-        if (isnan(xi.a)) fl |= FLAG_NAN_X;
-        if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;   // mod_evolve.f90:141 for the next step
-        cz_m = czc;
-    });
+        buf ^= 1;
+        xz_m = xi;
+        xi = xz_p;
+        ri = r1;
+    };
+    // Two trips per iteration, alternating the register sets (A, B) that carry the loads
+    // between trips: a copy from one set to the other at the end of each trip would wait for
+    // the loads just issued.  (The compiler does not unroll a loop with a barrier by itself.)
+    StepRaw rB{};
+    StepCell cB{};
+    uint32_t I = c.col + g.nxny * c.z0;
+    for (uint32_t z = c.z0; z < c.z1; z += 2, I += 2 * g.nxny) {
+        trip(z, I, r1, cur, rB, cB);
+        if (z + 1 >= c.z1) break;
+        trip(z + 1, I + g.nxny, rB, cB, r1, cur);
+    }
     block_flags(fl, &sc->flags);
     block_flags(fl_next, &sc->flags_next);
     block_partials<3>(acc, partials);
@@ -915,12 +985,27 @@ void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const 
                     const double *qb, const double *d, const double *cx, const double *cy,
                     const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
                     double *partials, hipStream_t st) {
-    if (from_state)
-        HF_ZLAUNCH(k_ir_step_z<true>, g, st, g, sp, tp, tpp, xb, y, p0, inner, sv, rhoc, cvec, qb,
-                   d, cx, cy, cz, xout, rres, rnext, sc, partials);
-    else
-        HF_ZLAUNCH(k_ir_step_z<false>, g, st, g, sp, tp, tpp, xb, y, p0, inner, sv, rhoc, cvec,
-                   qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);
+#define HF_IR_STEP(F, T, C)                                                                   \
+    do {                                                                                      \
+        if (wrap)                                                                             \
+            HF_ZLAUNCH((k_ir_step_z<F, T, C, true>), g, st, g, sp, tp, tpp, xb, y, p0, inner, \
+                       sv, rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);   \
+        else                                                                                  \
+            HF_ZLAUNCH((k_ir_step_z<F, T, C, false>), g, st, g, sp, tp, tpp, xb, y, p0, inner,\
+                       sv, rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);   \
+    } while (0)
+    const bool tr = sp.isteady == 0, cat = sp.icattaneo == 1;
+    const bool wrap = g.per[0] || g.per[1] || g.per[2];
+    if (from_state) {
+        if (!tr) HF_IR_STEP(true, false, false);
+        else if (cat) HF_IR_STEP(true, true, true);
+        else HF_IR_STEP(true, true, false);
+    } else {
+        if (!tr) HF_IR_STEP(false, false, false);
+        else if (cat) HF_IR_STEP(false, true, true);
+        else HF_IR_STEP(false, true, false);
+    }
+#undef HF_IR_STEP
 }
 
 }  // namespace hf

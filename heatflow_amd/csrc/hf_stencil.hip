@@ -8,6 +8,8 @@
 // ring is loaded once per plane and the four in-plane neighbours are read from LDS.  Two LDS
 // buffers alternate, so plane z+1 is staged while plane z is computed, with one barrier per
 // plane.  Periodic wrap neighbours (boundary columns of a periodic axis) are read from memory.
+// The kernels with many inputs per cell (k_rhs_res_z, k_ir_step_z) are software-pipelined: the
+// next plane's loads are issued before the barrier of the current one (zmarch_pipe).
 // Tiles are numbered x fastest, then y, then z-chunk; hardware block h runs logical tile
 // (h % 8) * (zblk / 8) + h / 8, so each XCD owns a contiguous range of tiles (whole z-chunks)
 // and the ring rows of neighbouring tiles come from the same L2.
@@ -190,6 +192,124 @@ __device__ __forceinline__ void zmarch(const Dims &g, const ZCol &c, PlaneBuf<V>
     }
 }
 
+// zmarch, software-pipelined.  The operand is formed from a raw record R = load(J) by form(R);
+// the cell's other inputs are a record C = cload(I) (I a cell of the thread's column, or 0 for
+// an inactive thread).  While plane z is computed, the loads of plane z+1's cell record and
+// halo ring and of plane z+2's own operand record are in flight, so a trip waits only on loads
+// issued one trip earlier.  cell(I, z, xi, xz_m, xz_p, nb, cur, ri) runs for every active
+// cell; ri is its own operand record of plane z.
+// The next trips' loads are issued on every trip by every thread, without branches, so that
+// the compiler's wait counts know how many are in flight; where there is nothing to load they
+// read an address the thread read recently (cache hits): its own cell, or cell 0.  Two trips
+// per loop iteration alternate the register sets that carry loads between trips (a copy at
+// the end of a trip would wait for the loads just issued; the compiler does not unroll a loop
+// with a barrier by itself).  Wrap = false: no axis is periodic.
+template <typename V, typename R, typename C, bool Wrap, class XL, class XF, class CL, class CF>
+__device__ __forceinline__ void zmarch_pipe(const Dims &g, const ZCol &c, PlaneBuf<V> &sp,
+                                            XL load, XF form, CL cload, CF cell) {
+    const Ring rg = ring_cell(g, c);
+    const bool per_z = Wrap && g.per[2];
+    V xz_m = (V)0, xi = (V)0;
+    R rA{}, rB{}, ri{};
+    C cA{}, cB{};
+    if (c.active) {
+        if (c.z0 > 0 || per_z) xz_m = form(load(zm_index(g, c.col, c.z0)));
+        ri = load(c.col + g.nxny * c.z0);
+        xi = form(ri);
+        if (c.z0 + 1 < g.nz || per_z) rA = load(zp_index(g, c.col, c.z0));
+        cA = cload(c.col + g.nxny * c.z0);
+    }
+    if (c.zvalid) {
+        sp.v[0][c.ly + 1][c.lx + 1] = xi;
+        if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = form(load(rg.col + g.nxny * c.z0));
+    }
+    int buf = 0;
+    auto trip = [&](uint32_t z, uint32_t I, const R &r1, const C &cur, R &r2, C &cn) {
+        const bool more = z + 1 < c.z1;
+        const bool has_p = c.active && (z + 1 < g.nz || per_z);
+        const bool has_p2 = c.active && more && (z + 2 < g.nz || per_z);
+        const bool has_r = more && rg.col != kNoRing;
+        const uint32_t own = c.active ? I : 0u;
+        r2 = load(has_p2 ? zp_index(g, c.col, z + 1) : own);
+        const R rr = load(rg.col != kNoRing ? rg.col + g.nxny * (more ? z + 1 : z) : 0u);
+        cn = cload(c.active && more ? I + g.nxny : own);
+        __syncthreads();   // plane z is staged in sp.v[buf]
+        V xz_p = (V)0;
+        if (c.active) {
+            xz_p = has_p ? form(r1) : (V)0;
+            const InPlane<V> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
+            cell(I, z, xi, xz_m, xz_p, nb, cur, ri);
+        }
+        if (more) {
+            // sp.v[buf ^ 1] was last read before this trip's barrier
+            sp.v[buf ^ 1][c.ly + 1][c.lx + 1] = xz_p;
+            if (has_r) (&sp.v[buf ^ 1][0][0])[rg.slot] = form(rr);
+        }
+        buf ^= 1;
+        xz_m = xi;
+        xi = xz_p;
+        ri = r1;
+    };
+    uint32_t I = c.col + g.nxny * c.z0;
+    for (uint32_t z = c.z0; z < c.z1; z += 2, I += 2 * g.nxny) {
+        trip(z, I, rA, cA, rB, cB);
+        if (z + 1 >= c.z1) break;
+        trip(z + 1, I + g.nxny, rB, cB, rA, cA);
+    }
+}
+
+// Offsets from a cell to its in-grid x- / y- neighbours' couplings (stored at the neighbour;
+// periodic wrap terms are added apart): 0 where the neighbour is absent or the thread
+// inactive.  Opaque to the compiler, which would otherwise turn a load at I + offset into a
+// load under a branch reusing the one at I (and a branch stops loads being issued ahead).
+struct NbOff {
+    int32_t xm, ym;
+};
+__device__ __forceinline__ NbOff nb_offsets_in(const Dims &g, const ZCol &c) {
+    int32_t xm = c.active && c.x > 0 ? -1 : 0;
+    int32_t ym = c.active && c.y > 0 ? -(int32_t)g.nx : 0;
+    asm volatile("" : "+v"(xm), "+v"(ym));
+    return NbOff{xm, ym};
+}
+
+// the couplings of H a cell needs: diag, its own x+ / y+ / z+ ones and those of its in-grid
+// x- / y- neighbours (read at the cell itself where that neighbour is absent)
+struct HCell {
+    double d, cx, cy, cz, cxm, cym;
+};
+__device__ __forceinline__ HCell hcell_load(uint32_t I, const NbOff &o, const double *d,
+                                            const double *cx, const double *cy,
+                                            const double *cz) {
+    return HCell{ldnt(&d[I]), cx[I], cy[I], ldnt(&cz[I]), cx[I + o.xm], cy[I + o.ym]};
+}
+
+// hx_sum with the couplings preloaded (same terms, same order; wrap terms read from memory)
+template <bool Wrap, class NB, class XF>
+__device__ __forceinline__ double hx_sum_v(const Dims &g, const ZCol &c, uint32_t I, uint32_t z,
+                                           const HCell &hc, const double *__restrict__ cx,
+                                           const double *__restrict__ cy, double xi, double xz_m,
+                                           double xz_p, double cz_m, const NB &nb, XF X) {
+    const bool ym_in = c.y > 0, yp_in = c.y + 1 < g.ny;
+    const bool xm_in = c.x > 0, xp_in = c.x + 1 < g.nx;
+    double h = 0.0;
+    h = h + cz_m * (z > 0 ? xz_m : 0.0);
+    h = h + hc.cym * (ym_in ? nb.ym() : 0.0);
+    h = h + hc.cxm * (xm_in ? nb.xm() : 0.0);
+    h = h + hc.d * xi;
+    h = h + hc.cx * (xp_in ? nb.xp() : 0.0);
+    h = h + hc.cy * (yp_in ? nb.yp() : 0.0);
+    h = h + hc.cz * (z + 1 < g.nz ? xz_p : 0.0);
+    if (!Wrap) return h;
+    const bool hzm = z > 0 || g.per[2], hzp = z + 1 < g.nz || g.per[2];
+    if (c.hxm && c.wxm) h = h + cx[I + c.oxm] * X(I + c.oxm);
+    if (c.hxp && c.wxp) h = h + hc.cx * X(I + c.oxp);
+    if (c.hym && c.wym) h = h + cy[I + c.oym] * X(I + c.oym);
+    if (c.hyp && c.wyp) h = h + hc.cy * X(I + c.oyp);
+    if (hzm && z == 0) h = h + cz_m * xz_m;
+    if (hzp && z + 1 == g.nz) h = h + hc.cz * xz_p;
+    return h;
+}
+
 // Sum of the seven stencil terms of H x in the reference's serial order: xz_m / xz_p and cz_m
 // are the z neighbours' operands and the z- coupling carried in registers, nb the in-plane
 // neighbours from LDS; periodic wrap terms (last) read the operand X from memory.  Wrap =
@@ -226,7 +346,26 @@ __device__ __forceinline__ double hx_sum(const Dims &g, const ZCol &c, uint32_t 
     return h;
 }
 
-template <typename OutT, bool FromState, bool StoreS, bool StoreX>
+// Mode of S (compile-time): kSteady (isteady /= 0), kFourier (transient, icattaneo /= 1),
+// kCattaneo (transient, icattaneo == 1).  Kernels built for a solve (not FromState) use kSteady.
+enum : int { kSteady = 0, kFourier = 1, kCattaneo = 2 };
+__host__ __device__ constexpr int s_mode(int32_t isteady, int32_t icattaneo) {
+    return isteady != 0 ? kSteady : (icattaneo == 1 ? kCattaneo : kFourier);
+}
+
+// operand inputs of k_rhs_res_z: T^n, T^{n-1} (state) or x (solve)
+struct RhsRaw {
+    double a, b;
+};
+// the cell's other inputs: S inputs (state: rho c_p, tau' rho c_p, Qdens, B; solve: b in qd)
+// and the couplings
+struct RhsCell {
+    double rc, cv, qd, bv;
+    HCell h;
+};
+
+// (software-pipelined z-march; Wrap = some axis is periodic)
+template <typename OutT, bool FromState, bool StoreS, bool StoreX, int Mode, bool Wrap>
 __global__ void __launch_bounds__(kZTX * kZTY)
 k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *__restrict__ tpp,
             const double *__restrict__ xin, const double *__restrict__ bin,
@@ -238,54 +377,69 @@ k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *_
             Scal *sc, double *partials) {
     __shared__ PlaneBuf<double> sp;
     const bool tiny = sc->tiny != 0;
-    auto X = [&](uint32_t J) -> double {
+    auto load = [&](uint32_t J) -> RhsRaw {
+        return FromState ? RhsRaw{tp[J], tpp[J]} : RhsRaw{xin[J], 0.0};
+    };
+    auto form = [&](const RhsRaw &v) -> double {
         if (FromState) {
-            double x0 = tp[J] + (tp[J] - tpp[J]);  // mod_evolve.f90:140
-            if (tiny) x0 = x0 + kTiny;             // :141
+            double x0 = v.a + (v.a - v.b);   // mod_evolve.f90:140
+            if (tiny) x0 = x0 + kTiny;       // :141
             return x0;
         } else {
-            return xin[J];
+            return v.a;
         }
     };
+    auto X = [&](uint32_t J) -> double { return form(load(J)); };
     double acc[2] = {0.0, 0.0};
     int32_t fl = 0;
     const ZCol c = zcolumn(g);
+    const NbOff o = nb_offsets_in(g, c);
+    auto cload = [&](uint32_t I) -> RhsCell {
+        RhsCell v;
+        v.rc = (FromState && Mode != kSteady) ? ldnt(&rhoc[I]) : 0.0;
+        v.cv = (FromState && Mode == kCattaneo) ? ldnt(&cvec[I]) : 0.0;
+        v.qd = FromState ? ldnt(&qd[I]) : ldnt(&bin[I]);
+        v.bv = FromState ? ldnt(&bvec[I]) : 0.0;
+        v.h = hcell_load(I, o, d, cx, cy, cz);
+        return v;
+    };
     double cz_m = 0.0;
-    if (c.active && (c.z0 > 0 || g.per[2])) cz_m = cz[zm_index(g, c.col, c.z0)];
-    zmarch<double>(g, c, sp, X, [&](uint32_t I, uint32_t z, double xi, double xz_m, double xz_p,
-                                    const InPlane<double> &nb) {
-        double S;
-        if (FromState) {
-            // mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36
-            if (rp.isteady == 0) {
-                S = -(rp.inverse_time * tp[I] * ldnt(&rhoc[I])) - ldnt(&qd[I]) - ldnt(&bvec[I]);
-                if (rp.icattaneo == 1) {
-                    double scat = (tpp[I] - 2.0 * tp[I]) * ldnt(&cvec[I]);
-                    if (isnan(scat)) fl |= FLAG_NAN_SCAT;
-                    S = S + scat;
+    if (c.active && (c.z0 > 0 || (Wrap && g.per[2]))) cz_m = cz[zm_index(g, c.col, c.z0)];
+    zmarch_pipe<double, RhsRaw, RhsCell, Wrap>(
+        g, c, sp, load, form, cload,
+        [&](uint32_t I, uint32_t z, double xi, double xz_m, double xz_p,
+            const InPlane<double> &nb, const RhsCell &cv, const RhsRaw &ri) {
+            double S;
+            if (FromState) {
+                // mod_evolve.f90:112-119, S_catS mod_cattaneo.f90:35-36
+                if (Mode != kSteady) {
+                    S = -(rp.inverse_time * ri.a * cv.rc) - cv.qd - cv.bv;
+                    if (Mode == kCattaneo) {
+                        const double scat = (ri.b - 2.0 * ri.a) * cv.cv;
+                        if (isnan(scat)) fl |= FLAG_NAN_SCAT;
+                        S = S + scat;
+                    }
+                } else {
+                    S = -cv.qd - cv.bv;
                 }
+                if (isnan(S)) fl |= FLAG_NAN_S;
             } else {
-                S = -ldnt(&qd[I]) - ldnt(&bvec[I]);
+                S = cv.qd;
             }
-            if (isnan(S)) fl |= FLAG_NAN_S;
-        } else {
-            S = ldnt(&bin[I]);
-        }
-        const double di = ldnt(&d[I]);
-        const double czc = ldnt(&cz[I]);
-        const double hxv = hx_sum(g, c, I, z, di, cx, cy, czc, xi, xz_m, xz_p, cz_m, nb, X);
-        const double r = S - hxv;
-        const double s = 1.0 / sqrt(-di);
-        const double rh = -s * r;
-        stnt(&rhat[I], (OutT)rh);
-        if (StoreS) stnt(&sout[I], S);
-        if (StoreX) stnt(&xout[I], xi);
-        const double bh = s * S;
-        const double rq = (double)(OutT)rh;   // r.r of the stored (possibly FP32) residual
-        acc[0] += bh * bh;
-        acc[1] += rq * rq;
-        cz_m = czc;
-    });
+            const double hxv = hx_sum_v<Wrap>(g, c, I, z, cv.h, cx, cy, xi, xz_m, xz_p, cz_m, nb,
+                                              X);
+            const double r = S - hxv;
+            const double s = 1.0 / sqrt(-cv.h.d);
+            const double rh = -s * r;
+            stnt(&rhat[I], (OutT)rh);
+            if (StoreS) stnt(&sout[I], S);
+            if (StoreX) stnt(&xout[I], xi);
+            const double bh = s * S;
+            const double rq = (double)(OutT)rh;   // r.r of the stored (possibly FP32) residual
+            acc[0] += bh * bh;
+            acc[1] += rq * rq;
+            cz_m = cv.h.cz;
+        });
     block_flags(fl, &sc->flags);
     block_partials<2>(acc, partials);
 }
@@ -485,12 +639,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         return D2(xn, xn + (xn - r.tp));
     };
     auto X = [&](uint32_t J) -> D2 { return form(load(J)); };
-    // offsets of the x- / y- neighbours' couplings (0 where absent or the thread is inactive),
-    // opaque to the compiler: it would otherwise turn cx[I + oxm] into a load under a branch
-    // that reuses cx[I], and a branch stops the loads being issued ahead
-    int32_t oxm = c.active && xm_in ? -1 : 0;
-    int32_t oym = c.active && ym_in ? -(int32_t)g.nx : 0;
-    asm volatile("" : "+v"(oxm), "+v"(oym));
+    const NbOff o = nb_offsets_in(g, c);
     // I must be a cell of this thread's column (active threads) or 0
     auto cload = [&](uint32_t I) -> StepCell {
         StepCell v;
@@ -501,8 +650,8 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         v.cv = (Transient && Cattaneo) ? ldnt(&cvec[I]) : 0.0;
         v.cx = cx[I];
         v.cy = cy[I];
-        v.cxm = cx[I + oxm];
-        v.cym = cy[I + oym];
+        v.cxm = cx[I + o.xm];
+        v.cym = cy[I + o.ym];
         v.tpp = FromState ? 0.0 : ldnt(&tpp[I]);
         return v;
     };
@@ -880,18 +1029,41 @@ k_cg_b1_z2(Dims g, const float *__restrict__ r, const float *__restrict__ hx,
 #define HF_ZLAUNCH(kern, g, st, ...) \
     hipLaunchKernelGGL(kern, dim3((g).zblk), dim3(kZTX * kZTY), 0, st, __VA_ARGS__)
 
+// k_rhs_res_z for the run-time S mode and periodicity
+template <typename OutT, bool FS, bool SS, bool SX>
+static void launch_rhs(const Dims &g, const RhsParams &rp, const double *tp, const double *tpp,
+                       const double *xin, const double *bin, const double *rhoc,
+                       const double *cvec, const double *qd, const double *bvec, const double *d,
+                       const double *cx, const double *cy, const double *cz, OutT *rhat,
+                       double *sout, double *xout, Scal *sc, double *partials, hipStream_t st) {
+#define HF_RHS(M, W)                                                                        \
+    HF_ZLAUNCH((k_rhs_res_z<OutT, FS, SS, SX, M, W>), g, st, g, rp, tp, tpp, xin, bin, rhoc, \
+               cvec, qd, bvec, d, cx, cy, cz, rhat, sout, xout, sc, partials)
+    const int mode = FS ? s_mode(rp.isteady, rp.icattaneo) : kSteady;
+    const bool wrap = g.per[0] || g.per[1] || g.per[2];
+    if (mode == kCattaneo) {
+        if (wrap) HF_RHS(kCattaneo, true); else HF_RHS(kCattaneo, false);
+    } else if (mode == kFourier) {
+        if (wrap) HF_RHS(kFourier, true); else HF_RHS(kFourier, false);
+    } else {
+        if (wrap) HF_RHS(kSteady, true); else HF_RHS(kSteady, false);
+    }
+#undef HF_RHS
+}
+
 void launch_rhs_res_state64(const Dims &g, const RhsParams &rp, const double *tp,
                             const double *tpp, const double *rhoc, const double *cvec,
                             const double *qd, const double *bvec, const double *d,
                             const double *cx, const double *cy, const double *cz, double *rhat,
                             double *sout, Scal *sc, double *partials, hipStream_t st) {
     if (sout)
-        HF_ZLAUNCH((k_rhs_res_z<double, true, true, false>), g, st, g, rp, tp, tpp, nullptr,
-                   nullptr, rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, sout, nullptr, sc, partials);
+        launch_rhs<double, true, true, false>(g, rp, tp, tpp, nullptr, nullptr, rhoc, cvec, qd,
+                                              bvec, d, cx, cy, cz, rhat, sout, nullptr, sc,
+                                              partials, st);
     else
-        HF_ZLAUNCH((k_rhs_res_z<double, true, false, false>), g, st, g, rp, tp, tpp, nullptr,
-                   nullptr, rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
-                   partials);
+        launch_rhs<double, true, false, false>(g, rp, tp, tpp, nullptr, nullptr, rhoc, cvec, qd,
+                                               bvec, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
+                                               partials, st);
 }
 
 void launch_rhs_res_state32(const Dims &g, const RhsParams &rp, const double *tp,
@@ -901,30 +1073,31 @@ void launch_rhs_res_state32(const Dims &g, const RhsParams &rp, const double *tp
                             double *sout, double *xout, Scal *sc, double *partials,
                             hipStream_t st) {
     if (sout && xout)
-        HF_ZLAUNCH((k_rhs_res_z<float, true, true, true>), g, st, g, rp, tp, tpp, nullptr,
-                   nullptr, rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, sout, xout, sc, partials);
+        launch_rhs<float, true, true, true>(g, rp, tp, tpp, nullptr, nullptr, rhoc, cvec, qd,
+                                            bvec, d, cx, cy, cz, rhat, sout, xout, sc, partials,
+                                            st);
     else
-        HF_ZLAUNCH((k_rhs_res_z<float, true, false, false>), g, st, g, rp, tp, tpp, nullptr,
-                   nullptr, rhoc, cvec, qd, bvec, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
-                   partials);
+        launch_rhs<float, true, false, false>(g, rp, tp, tpp, nullptr, nullptr, rhoc, cvec, qd,
+                                              bvec, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
+                                              partials, st);
 }
 
 void launch_rhs_res_solve64(const Dims &g, const RhsParams &rp, const double *x,
                             const double *b, const double *d, const double *cx,
                             const double *cy, const double *cz, double *rhat, Scal *sc,
                             double *partials, hipStream_t st) {
-    HF_ZLAUNCH((k_rhs_res_z<double, false, false, false>), g, st, g, rp, nullptr, nullptr, x, b,
-               nullptr, nullptr, nullptr, nullptr, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
-               partials);
+    launch_rhs<double, false, false, false>(g, rp, nullptr, nullptr, x, b, nullptr, nullptr,
+                                            nullptr, nullptr, d, cx, cy, cz, rhat, nullptr,
+                                            nullptr, sc, partials, st);
 }
 
 void launch_rhs_res_solve32(const Dims &g, const RhsParams &rp, const double *x,
                             const double *b, const double *d, const double *cx,
                             const double *cy, const double *cz, float *rhat, Scal *sc,
                             double *partials, hipStream_t st) {
-    HF_ZLAUNCH((k_rhs_res_z<float, false, false, false>), g, st, g, rp, nullptr, nullptr, x, b,
-               nullptr, nullptr, nullptr, nullptr, d, cx, cy, cz, rhat, nullptr, nullptr, sc,
-               partials);
+    launch_rhs<float, false, false, false>(g, rp, nullptr, nullptr, x, b, nullptr, nullptr,
+                                           nullptr, nullptr, d, cx, cy, cz, rhat, nullptr,
+                                           nullptr, sc, partials, st);
 }
 
 template <typename T>

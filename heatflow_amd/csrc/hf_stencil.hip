@@ -796,16 +796,17 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     block_partials<3>(acc, partials);
 }
 
-// ---- FP32 kernels with two x-neighbouring cells per thread (nx even) ------------------------
+// ---- CG kernels with two x-neighbouring cells per thread (nx even) --------------------------
 //
 // FP32 arrays moved one float per lane per instruction run well below the FP64 kernels'
-// bandwidth; with pairs every lane moves 8 bytes per access.  A block of kZTX x kZTY threads
+// bandwidth; with pairs every lane moves 8 bytes per access (16 for FP64).  A block of kZTX x kZTY threads
 // owns 2 kZTX x kZTY columns (tiles numbered as above, g.ztx2 per row, chunks of g.zchunk2
 // planes).  The LDS plane holds pairs: the x neighbours of a pair (x, x+1) are the .y of the
 // pair on its left and the .x of the pair on its right.  Sums per cell are the scalar kernels'
 // (same terms, same order).
 
-using f2 = float __attribute__((ext_vector_type(2)));
+template <typename T>
+using vec2 = T __attribute__((ext_vector_type(2)));
 
 struct ZCol2 {
     uint32_t col, x, y, z0, z1;   // x: the even (left) cell a of the pair; b = x + 1
@@ -874,12 +875,13 @@ __device__ __forceinline__ Ring ring_cell2(const Dims &g, const ZCol2 &c) {
 }
 
 // zmarch for pairs: X(J) gives the operand pair at the even cell J.
-template <class XF, class CF>
-__device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<f2> &sp, XF X,
-                                        CF cell) {
+template <typename T, class XF, class CF>
+__device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<vec2<T>> &sp,
+                                        XF X, CF cell) {
+    using T2 = vec2<T>;
     const Ring rg = ring_cell2(g, c);
     const bool per_z = g.per[2];
-    f2 xz_m = (f2)0.0f, xi = (f2)0.0f;
+    T2 xz_m = (T2)(T)0, xi = (T2)(T)0;
     if (c.active) {
         if (c.z0 > 0 || per_z) xz_m = X(zm_index(g, c.col, c.z0));
         xi = X(c.col + g.nxny * c.z0);
@@ -892,12 +894,12 @@ __device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<
     uint32_t I = c.col + g.nxny * c.z0;
     for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
         const bool more = z + 1 < c.z1;
-        f2 xz_p = (f2)0.0f, rnext = (f2)0.0f;
+        T2 xz_p = (T2)(T)0, rnext = (T2)(T)0;
         if (c.active && (z + 1 < g.nz || per_z)) xz_p = X(zp_index(g, c.col, z));
         if (more && rg.col != kNoRing) rnext = X(rg.col + g.nxny * (z + 1));
         __syncthreads();   // plane z is staged in sp.v[buf]
         if (c.active) {
-            const InPlane<f2> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
+            const InPlane<T2> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
             cell(I, z, xi, xz_m, xz_p, nb);
         }
         if (more) {
@@ -910,34 +912,41 @@ __device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<
     }
 }
 
-__device__ __forceinline__ f2 ld2(const float *p) { return *(const f2 *)p; }
-__device__ __forceinline__ f2 ldnt2(const float *p) {
-    return __builtin_nontemporal_load((const f2 *)p);
+template <typename T>
+__device__ __forceinline__ vec2<T> ld2(const T *p) { return *(const vec2<T> *)p; }
+template <typename T>
+__device__ __forceinline__ vec2<T> ldnt2(const T *p) {
+    return __builtin_nontemporal_load((const vec2<T> *)p);
 }
-__device__ __forceinline__ void stnt2(float *p, f2 v) { __builtin_nontemporal_store(v, (f2 *)p); }
+template <typename T>
+__device__ __forceinline__ void stnt2(T *p, vec2<T> v) {
+    __builtin_nontemporal_store(v, (vec2<T> *)p);
+}
 
 // q = A^ p for the pair (a, b) = (p.x, p.y), the scalar ahat's terms and order per cell; P1
 // reads the operand of one (wrap) cell from memory, P2 the pair at an even cell.
-template <class P1F, class P2F>
-__device__ __forceinline__ f2 ahat2(const Dims &g, const ZCol2 &c, uint32_t I, uint32_t z, f2 pi,
-                                    f2 pz_m, f2 pz_p, f2 hz_m, f2 hzc, const float *__restrict__ hx,
-                                    const float *__restrict__ hy, const InPlane<f2> &nb, P1F P1,
-                                    P2F P2) {
+template <typename T, class P1F, class P2F>
+__device__ __forceinline__ vec2<T> ahat2(const Dims &g, const ZCol2 &c, uint32_t I, uint32_t z,
+                                         vec2<T> pi, vec2<T> pz_m, vec2<T> pz_p, vec2<T> hz_m,
+                                         vec2<T> hzc, const T *__restrict__ hx,
+                                         const T *__restrict__ hy, const InPlane<vec2<T>> &nb,
+                                         P1F P1, P2F P2) {
+    using T2 = vec2<T>;
     const bool per_z = g.per[2];
     const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
-    const f2 hx2 = ld2(&hx[I]);           // a-b coupling, b-(b+1) coupling
+    const T2 hx2 = ld2(&hx[I]);           // a-b coupling, b-(b+1) coupling
     // absent neighbours contribute coefficient * 0 (coefficients read at I); wrap neighbours
     // are read from memory
-    f2 vym = (f2)0.0f, vyp = (f2)0.0f;
-    float vxm = 0.0f, vxp = 0.0f;
+    T2 vym = (T2)(T)0, vyp = (T2)(T)0;
+    T vxm = (T)0, vxp = (T)0;
     if (c.hym) vym = c.wym ? P2(I + c.oym) : nb.ym();
     if (c.hyp) vyp = c.wyp ? P2(I + c.oyp) : nb.yp();
     if (c.hxm_a) vxm = c.wxm_a ? P1(I + c.oxm_a) : nb.xm().y;
     if (c.hxp_b) vxp = c.wxp_b ? P1(I + c.oxp_b) : nb.xp().x;
-    const f2 zm = hzm ? pz_m : (f2)0.0f, zp = hzp ? pz_p : (f2)0.0f;
-    const f2 cym = ld2(&hy[c.hym ? I + c.oym : I]), cyp = ld2(&hy[I]);
-    const float cxm = hx[c.hxm_a ? I + c.oxm_a : I];
-    float sa = 0.0f, sb = 0.0f;
+    const T2 zm = hzm ? pz_m : (T2)(T)0, zp = hzp ? pz_p : (T2)(T)0;
+    const T2 cym = ld2(&hy[c.hym ? I + c.oym : I]), cyp = ld2(&hy[I]);
+    const T cxm = hx[c.hxm_a ? I + c.oxm_a : I];
+    T sa = (T)0, sb = (T)0;
     sa = sa + hz_m.x * zm.x;
     sb = sb + hz_m.y * zm.y;
     sa = sa + cym.x * vym.x;
@@ -951,38 +960,38 @@ __device__ __forceinline__ f2 ahat2(const Dims &g, const ZCol2 &c, uint32_t I, u
     sb = sb + cyp.y * vyp.y;
     sa = sa + hzc.x * zp.x;
     sb = sb + hzc.y * zp.y;
-    f2 q;
+    T2 q;
     q.x = pi.x - sa;
     q.y = pi.y - sb;
     return q;
 }
 
-template <bool First>
+template <typename T, bool First>
 __global__ void __launch_bounds__(kZTX * kZTY)
-k_cg_a_z2(Dims g, const float *__restrict__ r, const float *__restrict__ pold,
-          float *__restrict__ pnew, float *__restrict__ q, const float *__restrict__ hx,
-          const float *__restrict__ hy, const float *__restrict__ hz, Scal *sc,
-          double *partials) {
-    __shared__ PlaneBuf<f2> sp;
+k_cg_a_z2(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
+          T *__restrict__ q, const T *__restrict__ hx, const T *__restrict__ hy,
+          const T *__restrict__ hz, Scal *sc, double *partials) {
+    using T2 = vec2<T>;
+    __shared__ PlaneBuf<T2> sp;
     if (sc->done) return;
-    const float beta = (float)sc->beta;
-    auto P1 = [&](uint32_t J) -> float { return First ? r[J] : (float)(r[J] + beta * pold[J]); };
-    auto P2 = [&](uint32_t J) -> f2 {
+    const T beta = (T)sc->beta;
+    auto P1 = [&](uint32_t J) -> T { return First ? r[J] : (T)(r[J] + beta * pold[J]); };
+    auto P2 = [&](uint32_t J) -> T2 {
         if (First) return ld2(&r[J]);
-        const f2 a = ld2(&r[J]), b = ld2(&pold[J]);
-        f2 v;
+        const T2 a = ld2(&r[J]), b = ld2(&pold[J]);
+        T2 v;
         v.x = a.x + beta * b.x;
         v.y = a.y + beta * b.y;
         return v;
     };
     double acc[2] = {0.0, 0.0};
     const ZCol2 c = zcolumn2(g);
-    f2 hz_m = (f2)0.0f;
+    T2 hz_m = (T2)(T)0;
     if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ld2(&hz[zm_index(g, c.col, c.z0)]);
-    zmarch2(g, c, sp, P2, [&](uint32_t I, uint32_t z, f2 pi, f2 pz_m, f2 pz_p,
-                              const InPlane<f2> &nb) {
-        const f2 hzc = ldnt2(&hz[I]);
-        const f2 qi = ahat2(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
+    zmarch2<T>(g, c, sp, P2, [&](uint32_t I, uint32_t z, T2 pi, T2 pz_m, T2 pz_p,
+                                 const InPlane<T2> &nb) {
+        const T2 hzc = ldnt2(&hz[I]);
+        const T2 qi = ahat2<T>(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
         if (!First) {
             stnt2(&pnew[I], pi);
             stnt2(&q[I], qi);
@@ -994,24 +1003,26 @@ k_cg_a_z2(Dims g, const float *__restrict__ r, const float *__restrict__ pold,
     block_partials<2>(acc, partials);
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kZTX * kZTY)
-k_cg_b1_z2(Dims g, const float *__restrict__ r, const float *__restrict__ hx,
-           const float *__restrict__ hy, const float *__restrict__ hz, float *__restrict__ y,
-           float *__restrict__ rw, Scal *sc, double *partials) {
-    __shared__ PlaneBuf<f2> sp;
+k_cg_b1_z2(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *__restrict__ hy,
+           const T *__restrict__ hz, T *__restrict__ y, T *__restrict__ rw, Scal *sc,
+           double *partials) {
+    using T2 = vec2<T>;
+    __shared__ PlaneBuf<T2> sp;
     if (sc->done) return;
-    const float alpha = (float)sc->alpha;
-    auto P1 = [&](uint32_t J) -> float { return r[J]; };
-    auto P2 = [&](uint32_t J) -> f2 { return ld2(&r[J]); };
+    const T alpha = (T)sc->alpha;
+    auto P1 = [&](uint32_t J) -> T { return r[J]; };
+    auto P2 = [&](uint32_t J) -> T2 { return ld2(&r[J]); };
     double acc[1] = {0.0};
     const ZCol2 c = zcolumn2(g);
-    f2 hz_m = (f2)0.0f;
+    T2 hz_m = (T2)(T)0;
     if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ld2(&hz[zm_index(g, c.col, c.z0)]);
-    zmarch2(g, c, sp, P2, [&](uint32_t I, uint32_t z, f2 pi, f2 pz_m, f2 pz_p,
-                              const InPlane<f2> &nb) {
-        const f2 hzc = ldnt2(&hz[I]);
-        const f2 qi = ahat2(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
-        f2 yi, ri;
+    zmarch2<T>(g, c, sp, P2, [&](uint32_t I, uint32_t z, T2 pi, T2 pz_m, T2 pz_p,
+                                 const InPlane<T2> &nb) {
+        const T2 hzc = ldnt2(&hz[I]);
+        const T2 qi = ahat2<T>(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
+        T2 yi, ri;
         yi.x = alpha * pi.x;
         yi.y = alpha * pi.y;
         ri.x = pi.x - alpha * qi.x;
@@ -1104,14 +1115,13 @@ template <typename T>
 void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, T *q,
                  const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
                  hipStream_t st) {
-    if constexpr (std::is_same<T, float>::value) {
-        if (g.pairs) {
-            if (first)
-                HF_ZLAUNCH(k_cg_a_z2<true>, g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
-            else
-                HF_ZLAUNCH(k_cg_a_z2<false>, g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
-            return;
-        }
+    if (g.pairs) {
+        if (first)
+            HF_ZLAUNCH((k_cg_a_z2<T, true>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+        else
+            HF_ZLAUNCH((k_cg_a_z2<T, false>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc,
+                       partials);
+        return;
     }
     if (first)
         HF_ZLAUNCH((k_cg_a_z<T, true>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
@@ -1128,11 +1138,9 @@ template void launch_cg_a<float>(const Dims &, bool, const float *, const float 
 template <typename T>
 void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *y, T *rw,
                   Scal *sc, double *partials, hipStream_t st) {
-    if constexpr (std::is_same<T, float>::value) {
-        if (g.pairs) {
-            HF_ZLAUNCH(k_cg_b1_z2, g, st, g, r, hx, hy, hz, y, rw, sc, partials);
-            return;
-        }
+    if (g.pairs) {
+        HF_ZLAUNCH(k_cg_b1_z2<T>, g, st, g, r, hx, hy, hz, y, rw, sc, partials);
+        return;
     }
     HF_ZLAUNCH(k_cg_b1_z<T>, g, st, g, r, hx, hy, hz, y, rw, sc, partials);
 }

@@ -385,11 +385,12 @@ def test_first_iteration_stop_is_exact(prec):
     assert np.array_equal(bits(t0), bits(t2))
 
 
-def test_fp32_pair_kernels_match_scalar(monkeypatch):
-    """With an even nx the FP32 CG kernels take two x-neighbouring cells per thread; with
-    HF_NO_PAIRS set they take one.  The sums per cell are the same; only the order of the
-    per-block partial sums differs, so both runs meet the tolerance and agree to it
-    (log-normal kappa, Gaussian heating, 24^3 and a 2-cell-wide periodic strip)."""
+@pytest.mark.parametrize("prec", ["fp32_ir", "fp64"])
+def test_pair_kernels_match_scalar(monkeypatch, prec):
+    """With an even nx the CG kernels (FP32 inner solve, FP64 solve) take two x-neighbouring
+    cells per thread; with HF_NO_PAIRS set they take one.  The sums per cell are the same;
+    only the order of the per-block partial sums differs, so both runs meet the tolerance and
+    agree to it (log-normal kappa, Gaussian heating, 24^3 and a periodic 8 x 6 x 5 grid)."""
     for s in (workloads.c3(24)[0], random_system(31, 8, 6, 5, periodic=(True, False, True))):
         out = []
         for no_pairs in (False, True):
@@ -397,7 +398,7 @@ def test_fp32_pair_kernels_match_scalar(monkeypatch):
                 monkeypatch.setenv("HF_NO_PAIRS", "1")
             else:
                 monkeypatch.delenv("HF_NO_PAIRS", raising=False)
-            with Simulation(s, precision="fp32_ir", tol=1e-14) as sim:
+            with Simulation(s, precision=prec, tol=1e-14) as sim:
                 sim.initial_evolve()
                 sim.run(1, 10)
                 out.append(sim.get_state()[0])

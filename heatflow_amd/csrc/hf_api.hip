@@ -693,7 +693,9 @@ static float *other_rvec(hf_ctx *c, const float *a, const float *b = nullptr) {
 // heater modes or radiation: those recompute Qdens from T^n first).
 // The next step's residual depends on its TINY test, an OR over the grid that the pass itself
 // decides: the pass forms it for the outcome of the last test (c->tiny_guess), and a wrong
-// guess makes the next step start from k_rhs_res.
+// guess makes the next step start from k_rhs_res.  The pass stores this step's residual only
+// for a sweep that the last step followed with another (c->last_sweeps); when a sweep then
+// needs one after all, the same pass runs again with the store.
 static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     (void)itime;
     const hf_params &p = c->p;
@@ -718,7 +720,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         rin = c->r32;
     }
     c->carry = false;
-    const StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo, c->tiny_guess ? 1 : 0};
+    StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo, c->tiny_guess ? 1 : 0, 0};
     const int64_t sweeps_max = std::min<int64_t>(p.itmax, 64);
     // refined x ping-pongs between the spare state buffer and S (unused by this path)
     double *xbuf[2] = {c->T[2], c->S};
@@ -735,12 +737,17 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         float *rres = other_rvec(c, rin);
         float *rn = other_rvec(c, rin, rres);
         // k_ir_step: read T^n, T^{n-1} (or x), y or r (FP32), s, rho c_p, tau' rho c_p,
-        // Qdens + B, diag, 3 couplings; write x', two FP32 residuals
-        long si = tbeg(c, HF_K_IR_STEP, N * (8 * 10 + 4 + 8 + 2 * 4) + (xa ? N * 8 : 0));
-        launch_ir_step(c->g, sp, xa == nullptr, c->T[0], c->T[1], xa, c->y32, rin, c->sci, c->sv,
-                       c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, rres, rn, c->sc,
-                       c->partials, c->st);
-        tend(c, si);
+        // Qdens + B, diag, 3 couplings; write x', the next FP32 residual (and this step's)
+        sp.store_res = sweeps + 1 < c->last_sweeps ? 1 : 0;
+        auto pass = [&]() {
+            long si = tbeg(c, HF_K_IR_STEP,
+                           N * (8 * 10 + 4 + 8 + 4 + (sp.store_res ? 4 : 0)) + (xa ? N * 8 : 0));
+            launch_ir_step(c->g, sp, xa == nullptr, c->T[0], c->T[1], xa, c->y32, rin, c->sci,
+                           c->sv, c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, rres,
+                           rn, c->sc, c->partials, c->st);
+            tend(c, si);
+        };
+        pass();
         launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->tiny_guess, c->st);
         TRY(check_launch(c, "k_ir_step"));
         HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
@@ -752,6 +759,14 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         sweeps += 1;
         // next batch: what this sweep needed + 1 (a batch cut short costs a refinement sweep)
         c->pred_inner = c->h_sci->done ? std::max<int64_t>(1, in_it + 1) : batch * 2;
+        if (!c->h_sc->done && !sp.store_res) {
+            // a further sweep needs this step's residual: the same pass again, with the store
+            // (its other outputs, partials and flags come out the same; the scalars stay)
+            sp.store_res = 1;
+            pass();
+            TRY(check_launch(c, "k_ir_step"));
+            c->stats.residual_reruns += 1;
+        }
         xa = xo;
         nb ^= 1;
         rin = rres;
@@ -774,6 +789,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     c->carry = tiny_next == c->tiny_guess;
     if (!c->carry) c->stats.tiny_misses += 1;
     c->tiny_guess = tiny_next;
+    c->last_sweeps = sweeps;
     const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->outer_rr / c->h_sc->bnorm2) : 0.0;
     c->stats.steps += 1;
     c->stats.outer_iters += sweeps;

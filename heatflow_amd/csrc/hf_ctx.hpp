@@ -56,6 +56,7 @@ struct hf_ctx {
     double *qb = nullptr;        // Qdens + B for the fused step (set when a carried run starts)
     bool carry = false;
     bool tiny_guess = true;
+    int64_t last_sweeps = 1;     // refinement sweeps the last fused step needed
     float *carry_r = nullptr;
     // reductions / scalars
     Scal *sc = nullptr, *sci = nullptr;

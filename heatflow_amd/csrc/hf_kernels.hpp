@@ -44,6 +44,7 @@ struct StepParams {
     double inverse_time;
     int32_t isteady, icattaneo;
     int32_t tiny_guess;        // form the next residual for this outcome of its TINY test
+    int32_t store_res;         // store this step's residual (needed only for a further sweep)
 };
 
 void launch_assemble(const Dims &g, const AsmParams &ap, const double *kappa, const double *lx,
@@ -102,8 +103,9 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
                    const double *tp, double *xout, float *rhat, Scal *sc, double *partials,
                    hipStream_t st);
 // Fused mixed-precision step: x' = base + s y (base = the state's x0, or xb), this step's
-// FP64 residual at x' (FP32 -> rres), x' -> xout, and the next step's scaled r^0 from
-// (x', T^n), with the TINY shift if sp.tiny_guess (FP32 -> rnext); 3 partials.
+// FP64 residual at x' (FP32 -> rres if sp.store_res; its norm always), x' -> xout, and the next
+// step's scaled r^0 from (x', T^n), with the TINY shift if sp.tiny_guess (FP32 -> rnext);
+// 3 partials.  The pass is a function of its inputs: run again, it writes the same values.
 // qb = Qdens + B (per-step constant in this path)
 void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const double *tp,
                     const double *tpp, const double *xb, const float *y, const float *p0,

@@ -592,7 +592,9 @@ struct StepCell {
 //                                 the TINY shift x1 + TINY: r^0 + TINY s (H 1)
 // The TINY test of the next step (any x1 - x' < TINY) is an OR over the grid, known only after
 // the pass: rnext is formed for the guessed outcome (StepParams::tiny_guess), and the flag the
-// pass raises tells the host whether the guess held (else the next step starts afresh).  x'
+// pass raises tells the host whether the guess held (else the next step starts afresh).
+// rres is read only by a further refinement sweep, also known only after the pass: it is
+// stored when the host expects one (StepParams::store_res), else the pass is run again.  x'
 // and x1 are staged together in LDS (two stencil operands, one halo ring).
 //
 // The z-march is software-pipelined: while plane z is computed, the loads of plane z+1's cell
@@ -760,7 +762,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             }
             const float fn = (float)ra;
             stnt(&xout[I], xi.a);
-            stnt(&rres[I], rf);
+            if (sp_.store_res) stnt(&rres[I], rf);
             stnt(&rnext[I], fn);
             const double bh = s * Sn1;
             acc[0] += (double)rf * (double)rf;

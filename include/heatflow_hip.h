@@ -165,6 +165,8 @@ typedef struct hf_stats {
     int64_t first_iter_stops; /* CG solves that ended on their first iteration          */
     int64_t tiny_misses;      /* mixed-mode steps whose successor's TINY test was guessed
                                  wrong (that successor started from the state)          */
+    int64_t residual_reruns;  /* mixed-mode passes run again to store the residual a
+                                 further refinement sweep needed                        */
 } hf_stats;
 int hf_get_stats(hf_ctx *ctx, hf_stats *st);
 int hf_reset_stats(hf_ctx *ctx);

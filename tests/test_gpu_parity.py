@@ -346,7 +346,9 @@ def test_fp32_ir_tiny_guess(heating):
 def test_fp32_ir_refinement_sweeps():
     """A loose inner tolerance (1e-2) forces several refinement sweeps per step, so the fused
     pass also runs from a refined iterate (not only from the state's initial guess); the
-    result still meets the oracle at the reference tolerance."""
+    result still meets the oracle at the reference tolerance.  The pass stores a step's
+    residual only for sweeps the previous step followed with another; the first step (and any
+    step needing more sweeps than the last) runs a pass again to store it."""
     s, _ = workloads.c3(24)
     n = s.params.n
     t0 = np.full(n, s.params.t_system)
@@ -358,6 +360,7 @@ def test_fp32_ir_refinement_sweeps():
         tp, _ = sim.get_state()
         st = sim.stats()
     assert st.outer_iters >= 2 * 8
+    assert 1 <= st.residual_reruns < st.outer_iters - 8
     assert relerr(tp, otp) <= 1e-12
 
 

@@ -172,11 +172,12 @@ static bool allocs_base(hf_ctx *c, int *rc) {
         (r = dalloc(c, &c->cx, N)) || (r = dalloc(c, &c->cy, N)) || (r = dalloc(c, &c->cz, N)) ||
         (r = dalloc(c, &c->bvec, N)) || (r = dalloc(c, &c->sv, N)) || (r = dalloc(c, &c->qd, N)) ||
         (r = dalloc(c, &c->T[0], N)) || (r = dalloc(c, &c->T[1], N)) ||
-        (r = dalloc(c, &c->sc, 1)) || (r = dalloc(c, &c->sci, 1)) ||
+        (r = dalloc(c, &c->sc, 2)) ||
         (r = dalloc(c, &c->partials, (size_t)4 * std::max(c->g.nblk, c->g.zblk)))) {
         *rc = r;
         return false;
     }
+    c->sci = c->sc + 1;   // outer and inner solver scalars side by side: one read-back
     return true;
 }
 
@@ -237,14 +238,13 @@ extern "C" int hf_create(hf_ctx **out, const hf_params *p) {
     }
     int rc = HF_OK;
     if (!allocs_base(c, &rc) || (rc = alloc_mode(c)) != HF_OK ||
-        hipHostMalloc((void **)&c->h_sc, sizeof(Scal), 0) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_sci, sizeof(Scal), 0) != hipSuccess) {
+        hipHostMalloc((void **)&c->h_sc, 2 * sizeof(Scal), 0) != hipSuccess) {
         std::string m = c->err.empty() ? "hf_create: allocation failed" : c->err;
         hf_destroy(c);
         return set_err(nullptr, rc ? rc : HF_ERR_ALLOC, m);
     }
-    hipMemsetAsync(c->sc, 0, sizeof(Scal), c->st);
-    hipMemsetAsync(c->sci, 0, sizeof(Scal), c->st);
+    c->h_sci = c->h_sc + 1;
+    hipMemsetAsync(c->sc, 0, 2 * sizeof(Scal), c->st);
     if (hipStreamSynchronize(c->st) != hipSuccess) {
         hf_destroy(c);
         return set_err(nullptr, HF_ERR_HIP, "hf_create: stream sync failed");
@@ -275,8 +275,7 @@ extern "C" void hf_destroy(hf_ctx *c) {
         hipEventDestroy(t.b);
     }
     for (void *v : c->allocs) hipFree(v);
-    if (c->h_sc) hipHostFree(c->h_sc);
-    if (c->h_sci) hipHostFree(c->h_sci);
+    if (c->h_sc) hipHostFree(c->h_sc);   // h_sci is h_sc + 1
     if (c->st) hipStreamDestroy(c->st);
     delete c;
 }
@@ -414,8 +413,9 @@ static HeatParams heat_params(const hf_ctx *c, int64_t itime) {
     return hp;
 }
 
-static int sync_scal(hf_ctx *c, Scal *dev, Scal *host) {
-    HIPCHK(c, hipMemcpyAsync(host, dev, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
+// reads back `count` consecutive scalar blocks
+static int sync_scal(hf_ctx *c, Scal *dev, Scal *host, int count = 1) {
+    HIPCHK(c, hipMemcpyAsync(host, dev, count * sizeof(Scal), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return HF_OK;
 }
@@ -667,12 +667,17 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     return HF_OK;
 }
 
+// inner CG target floored at this fraction of the outer tolerance
+constexpr double kInnerSafety = 0.25;
+
 // Queues the inner FP32 CG of one refinement sweep on A^ y = r (r unchanged); *batch_out is
-// the number of iterations queued.
-static int inner_solve(hf_ctx *c, const float *r, int64_t *batch_out, std::vector<SlotRec> &slots) {
+// the number of iterations queued.  begin: set up the inner scalars (else already done).
+static int inner_solve(hf_ctx *c, const float *r, int64_t *batch_out, std::vector<SlotRec> &slots,
+                       bool begin = true) {
     const hf_params &p = c->p;
-    launch_inner_begin(c->sc, c->sci, p.inner_tol, p.tol, p.inner_itmax, 0.25, allow_short(c),
-                       c->st);
+    if (begin)
+        launch_inner_begin(c->sc, c->sci, p.inner_tol, p.tol, p.inner_itmax, kInnerSafety,
+                           allow_short(c), c->st);
     const int64_t batch = std::max<int64_t>(1, c->pred_inner);
     queue_cg<float>(c, c->sci, cg32(c, r), 0, batch, &slots, HF_K_CG_A32, HF_K_CG_B32);
     *batch_out = batch;
@@ -700,13 +705,17 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     (void)itime;
     const hf_params &p = c->p;
     const double N = (double)c->g.N;
-    launch_step_begin(c->sc, c->st);
     RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
     const float *rin;
+    bool inner_started = false;
     if (c->carry) {
-        launch_carry_begin(c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
+        // step, carried scalars and the first inner solve's scalars in one launch
+        launch_carry_start(c->sc, c->sci, rp.tol, rp.itmax, allow_short(c), p.inner_tol,
+                           p.inner_itmax, kInnerSafety, c->st);
+        inner_started = true;
         rin = c->carry_r;
     } else {
+        launch_step_begin(c->sc, c->st);
         // k_rhs_res: read 10 FP64 arrays; write r^ (FP32)
         long sr = tbeg(c, HF_K_RHS32, N * (8 * 10 + 4));
         launch_rhs_res_state32(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec,
@@ -730,7 +739,8 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     for (;;) {
         std::vector<SlotRec> slots;
         int64_t batch = 0;
-        TRY(inner_solve(c, rin, &batch, slots));
+        TRY(inner_solve(c, rin, &batch, slots, !inner_started));
+        inner_started = false;
         double *xo = xbuf[nb];
         // the pass reads y (or the right-hand side, for y = alpha r) and writes two residual
         // vectors: neither may be the right-hand side
@@ -750,8 +760,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         pass();
         launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->tiny_guess, c->st);
         TRY(check_launch(c, "k_ir_step"));
-        HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
-        TRY(sync_scal(c, c->sc, c->h_sc));
+        TRY(sync_scal(c, c->sc, c->h_sc, 2));   // outer and inner scalars
         const int64_t in_it = c->h_sci->iter;
         drop_idle_slots(c, slots, in_it, c->h_sci->shortcut != 0);
         if (c->h_sci->shortcut) c->stats.first_iter_stops += 1;
@@ -844,8 +853,7 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         TRY(check_launch(c, "k_ir_res"));
         std::swap(xa, xb);
         rin = rout;
-        HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
-        TRY(sync_scal(c, c->sc, c->h_sc));
+        TRY(sync_scal(c, c->sc, c->h_sc, 2));   // outer and inner scalars
         const int64_t in_it = c->h_sci->iter;
         drop_idle_slots(c, slots, in_it, c->h_sci->shortcut != 0);
         if (c->h_sci->shortcut) c->stats.first_iter_stops += 1;
@@ -1059,7 +1067,7 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
         float *rin = c->r32;
         int64_t inner_total = 0;
         for (;;) {
-            launch_inner_begin(c->sc, c->sci, c->p.inner_tol, tol, c->p.inner_itmax, 0.25,
+            launch_inner_begin(c->sc, c->sci, c->p.inner_tol, tol, c->p.inner_itmax, kInnerSafety,
                                allow_short(c), c->st);
             const int64_t batch = std::max<int64_t>(1, c->pred_adj);
             queue_cg<float>(c, c->sci, cg32(c, rin), 0, batch, nullptr, HF_K_CG_A32, HF_K_CG_B32);

@@ -468,10 +468,9 @@ k_fin_ir(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max) {
 
 // Starts an inner FP32 solve from the outer scalars: thresholds relative to this sweep's
 // residual, floored by the outer target (no point solving further than the outer test needs).
-// Launched with one thread.
-__global__ void k_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
-                              int64_t inner_itmax, double safety, int32_t allow_short) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Run by one thread.
+static __device__ void inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
+                                   int64_t inner_itmax, double safety, int32_t allow_short) {
     const double rr = outer->outer_rr;
     double t_rel = inner_tol * inner_tol * rr;
     double t_abs = safety * safety * outer_tol * outer_tol * outer->bnorm2;
@@ -492,21 +491,28 @@ __global__ void k_inner_begin(Scal *outer, Scal *inner, double inner_tol, double
     outer->flags &= ~FLAG_TINY_NEXT;
     outer->flags_next = 0;
 }
+__global__ void k_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
+                              int64_t inner_itmax, double safety, int32_t allow_short) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    inner_begin(outer, inner, inner_tol, outer_tol, inner_itmax, safety, allow_short);
+}
 
 // Clears the per-step flag bits, keeping TINY_NEXT -> tiny for this step; NaN bits of this
 // step's S found ahead by the previous fused step become current.
-__global__ void k_step_begin(Scal *sc) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+static __device__ void step_begin(Scal *sc) {
     sc->tiny = (sc->flags & FLAG_TINY_NEXT) ? 1 : 0;
     sc->flags = (sc->flags & FLAG_NAN_B) | sc->flags_next;
     sc->flags_next = 0;
+}
+__global__ void k_step_begin(Scal *sc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    step_begin(sc);
 }
 
 // Start of a carried mixed-precision step: the previous fused pass left this step's ||b^||^2
 // and its ||r^0||^2 for the TINY case it guessed (as k_fin_start for a residual from the
 // state).  The host carries only a right guess; the other case's norm is NaN.
-__global__ void k_carry_begin(Scal *sc, double tol, int64_t itmax, int32_t allow_short) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+static __device__ void carry_begin(Scal *sc, double tol, int64_t itmax, int32_t allow_short) {
     const double rr = sc->next_rr[sc->tiny ? 1 : 0];
     sc->bnorm2 = sc->next_bnorm2;
     sc->rr = rr;
@@ -523,6 +529,17 @@ __global__ void k_carry_begin(Scal *sc, double tol, int64_t itmax, int32_t allow
     sc->allow_short = allow_short;
     sc->done = (rr == 0.0) || isnan(rr) || itmax <= 0;
     if (isnan(rr)) sc->flags |= FLAG_BREAKDOWN;
+}
+
+// Start of a carried mixed-precision step and of its first inner solve in one launch:
+// step_begin, carry_begin, inner_begin.
+__global__ void k_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax,
+                              int32_t allow_short, double inner_tol, int64_t inner_itmax,
+                              double safety) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    step_begin(sc);
+    carry_begin(sc, tol, itmax, allow_short);
+    inner_begin(sc, inner, inner_tol, tol, inner_itmax, safety, allow_short);
 }
 
 // After the fused mixed step: FP64 residual of this step (refinement test as k_fin_ir) and
@@ -656,9 +673,10 @@ void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sw
                      bool tiny_guess, hipStream_t st) {
     HF_FIN(k_fin_step, g.zblk, st, partials, sc, sweeps_max, (int32_t)tiny_guess);
 }
-void launch_carry_begin(Scal *sc, double tol, int64_t itmax, bool allow_short, hipStream_t st) {
-    hipLaunchKernelGGL(k_carry_begin, dim3(1), dim3(64), 0, st, sc, tol, itmax,
-                       (int32_t)allow_short);
+void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
+                        double inner_tol, int64_t inner_itmax, double safety, hipStream_t st) {
+    hipLaunchKernelGGL(k_carry_start, dim3(1), dim3(64), 0, st, sc, inner, tol, itmax,
+                       (int32_t)allow_short, inner_tol, inner_itmax, safety);
 }
 
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,

@@ -125,7 +125,11 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
 // tiny_guess: the TINY case the step's rnext was formed for
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                      bool tiny_guess, hipStream_t st);
-void launch_carry_begin(Scal *sc, double tol, int64_t itmax, bool allow_short, hipStream_t st);
+// start of a carried mixed-precision step (k_step_begin; the scalars from the norms the last
+// fused pass left, as launch_fin_start for a residual from the state) and the first inner
+// solve's scalars (as launch_inner_begin(sc, inner, ..., outer_tol = tol)), in one launch
+void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
+                        double inner_tol, int64_t inner_itmax, double safety, hipStream_t st);
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
                         int64_t inner_itmax, double safety, bool allow_short, hipStream_t st);
 void launch_step_begin(Scal *sc, hipStream_t st);

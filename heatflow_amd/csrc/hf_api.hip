@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/heatflow_hip.h"
@@ -114,28 +115,55 @@ static void make_dims(Dims &g, const hf_params &p) {
     g.tpb = (uint32_t)((g.ntiles + nb - 1) / nb);
     nb = (g.ntiles + g.tpb - 1) / g.tpb;
     g.nblk = (uint32_t)((nb + 7) / 8 * 8);
-    // z-marching tiles of kZTX x kZTY columns: enough z-chunks for ~`zcap` blocks (HF_ZBLK
-    // overrides)
-    // (at 512^3 sweeps of 4096 .. 32768 blocks favoured the fewest, longest z-chunks: 64 planes)
-    uint32_t zcap = 4096;
-    if (const char *e = getenv("HF_ZBLK")) zcap = std::max(8u, (uint32_t)atoi(e));
+    // z-marching tiles of kZTX x kZTY columns: enough z-chunks for ~`zcap` blocks, one chunking
+    // per kernel class (measured at 512^3, 1 to 7% apart; HF_ZBLK, HF_ZBLK_PIPE, HF_ZBLK_COL
+    // override).  Fewer, longer chunks cost fewer prologue planes; the FP32 CG kernels, with
+    // the least work per cell, ran best at ~4096 blocks, the pipelined kernels at ~2048 and the
+    // FP64 two-cell CG kernels with whole columns (one round of resident blocks).
     const uint32_t tx = kZTX, ty = kZTY;
     g.zty = (g.ny + ty - 1) / ty;
-    // tiles of `w` cells in x: chunk length and tile count for ~zcap blocks
-    auto zlayout = [&](uint32_t w, uint32_t *ztx, uint32_t *zchunk, uint32_t *ztiles) {
-        *ztx = (g.nx + w - 1) / w;
-        const uint64_t per_plane = (uint64_t)*ztx * g.zty;
+    g.ztx = (g.nx + tx - 1) / tx;
+    g.ztx2 = (g.nx + 2 * tx - 1) / (2 * tx);
+    g.pairs = (g.nx % 2 == 0) && getenv("HF_NO_PAIRS") == nullptr;
+    // chunk length and tile count for ~zcap blocks, tiles of `ztx` x zty per plane
+    auto chunking = [&](uint32_t zcap, uint32_t ztx, uint32_t *zchunk, uint32_t *ztiles) {
+        const uint64_t per_plane = (uint64_t)ztx * g.zty;
         uint64_t nzc = std::max<uint64_t>(1, (zcap + per_plane - 1) / per_plane);
         nzc = std::min<uint64_t>(nzc, g.nz);
         *zchunk = (uint32_t)((g.nz + nzc - 1) / nzc);
         nzc = (g.nz + *zchunk - 1) / *zchunk;
         *ztiles = (uint32_t)(per_plane * nzc);
     };
-    zlayout(tx, &g.ztx, &g.zchunk, &g.ztiles);
-    g.pairs = (g.nx % 2 == 0) && getenv("HF_NO_PAIRS") == nullptr;
-    zlayout(2 * tx, &g.ztx2, &g.zchunk2, &g.ztiles2);
-    const uint32_t zt = g.pairs ? std::max(g.ztiles, g.ztiles2) : g.ztiles;
-    g.zblk = (zt + 7) / 8 * 8;
+    auto env_cap = [](const char *name, uint32_t dflt) {
+        const char *e = getenv(name);
+        return e ? std::max(8u, (uint32_t)atoi(e)) : dflt;
+    };
+    const uint32_t caps[3] = {env_cap("HF_ZBLK", 4096), env_cap("HF_ZBLK_PIPE", 2048),
+                              env_cap("HF_ZBLK_COL", 512)};
+    for (int k = 0; k < 3; ++k) {
+        Dims::ZLayout &l = g.zlay[k];
+        chunking(caps[k], g.ztx, &l.zchunk, &l.ztiles);
+        chunking(caps[k], g.ztx2, &l.zchunk2, &l.ztiles2);
+        const uint32_t zt = g.pairs ? std::max(l.ztiles, l.ztiles2) : l.ztiles;
+        l.zblk = (zt + 7) / 8 * 8;
+    }
+    const Dims::ZLayout &l0 = g.zlay[kZDefault];
+    g.zchunk = l0.zchunk;
+    g.ztiles = l0.ztiles;
+    g.zchunk2 = l0.zchunk2;
+    g.ztiles2 = l0.ztiles2;
+    g.zblk = l0.zblk;
+}
+
+// the largest z-march grid over the chunkings (partial sums per block)
+static uint32_t max_zblk(const Dims &g) {
+    return std::max(g.zlay[0].zblk, std::max(g.zlay[1].zblk, g.zlay[2].zblk));
+}
+
+// the z-chunking of the CG kernels in precision T
+template <typename T>
+static Dims cg_dims(const Dims &g) {
+    return std::is_same<T, double>::value && g.pairs ? zdims(g, kZColumn) : g;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -173,7 +201,7 @@ static bool allocs_base(hf_ctx *c, int *rc) {
         (r = dalloc(c, &c->bvec, N)) || (r = dalloc(c, &c->sv, N)) || (r = dalloc(c, &c->qd, N)) ||
         (r = dalloc(c, &c->T[0], N)) || (r = dalloc(c, &c->T[1], N)) ||
         (r = dalloc(c, &c->sc, 2)) ||
-        (r = dalloc(c, &c->partials, (size_t)4 * std::max(c->g.nblk, c->g.zblk)))) {
+        (r = dalloc(c, &c->partials, (size_t)4 * std::max(c->g.nblk, max_zblk(c->g))))) {
         *rc = r;
         return false;
     }
@@ -561,29 +589,30 @@ template <typename T>
 static void queue_cg(hf_ctx *c, Scal *sc, const CgVecs<T> &v, int64_t k0, int64_t count,
                      std::vector<SlotRec> *slots, int ida, int idb) {
     const double N = (double)c->g.N, w = (double)sizeof(T);
+    const Dims gc = cg_dims<T>(c->g);   // z-chunking of the stencil CG kernels
     for (int64_t k = k0; k < k0 + count; ++k) {
         long sa, sb;
         if (k == 0) {
             // k_cg_a (first): read r, 3 couplings
             sa = tbeg(c, ida, N * w * 4);
-            launch_cg_a<T>(c->g, true, v.rin, nullptr, nullptr, nullptr, v.hx, v.hy, v.hz, sc,
+            launch_cg_a<T>(gc, true, v.rin, nullptr, nullptr, nullptr, v.hx, v.hy, v.hz, sc,
                            c->partials, c->st);
             tend(c, sa);
-            launch_fin_a(c->g, c->partials, sc, c->st);
+            launch_fin_a(gc, c->partials, sc, c->st);
             // k_cg_b1: read r, 3 couplings; write y, rw
             sb = tbeg(c, idb, N * w * 6);
-            launch_cg_b1<T>(c->g, v.rin, v.hx, v.hy, v.hz, v.y, v.rw, sc, c->partials, c->st);
+            launch_cg_b1<T>(gc, v.rin, v.hx, v.hy, v.hz, v.y, v.rw, sc, c->partials, c->st);
             tend(c, sb);
-            launch_fin_b1(c->g, c->partials, sc, c->st);
+            launch_fin_b1(gc, c->partials, sc, c->st);
         } else {
             const T *pold = k == 1 ? v.rin : v.p[k & 1];
             T *pnew = v.p[(k - 1) & 1];
             // k_cg_a: read r, p_old, 3 couplings; write p, q
             sa = tbeg(c, ida, N * w * 7);
-            launch_cg_a<T>(c->g, false, v.rw, pold, pnew, v.q, v.hx, v.hy, v.hz, sc, c->partials,
+            launch_cg_a<T>(gc, false, v.rw, pold, pnew, v.q, v.hx, v.hy, v.hz, sc, c->partials,
                            c->st);
             tend(c, sa);
-            launch_fin_a(c->g, c->partials, sc, c->st);
+            launch_fin_a(gc, c->partials, sc, c->st);
             // k_cg_b: read y, p, r, q; write y, r
             sb = tbeg(c, idb, N * w * 6);
             launch_cg_b<T>(c->g, v.y, pnew, v.rw, v.q, sc, c->partials, c->st);

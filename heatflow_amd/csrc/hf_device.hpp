@@ -54,7 +54,27 @@ struct Dims {
     int32_t pairs;
     int32_t per[3];      // periodic axes (length >= 3)
     FastDiv dnx, dny;
+    // z-chunkings for ~4096 (the active one above), ~2048 and ~512 blocks at 512^3; zdims()
+    // selects one for a kernel and its finishing kernel
+    struct ZLayout {
+        uint32_t zchunk, ztiles, zchunk2, ztiles2, zblk;
+    } zlay[3];
 };
+
+// z-chunkings (Dims::zlay): kZDefault; kZPipe for the software-pipelined kernels
+// (k_rhs_res_z, k_ir_step_z), which favour fewer, longer chunks; kZColumn, whole columns (512
+// blocks at 512^3, all resident at once) for the FP64 two-cell CG kernels.
+enum : int { kZDefault = 0, kZPipe = 1, kZColumn = 2 };
+inline Dims zdims(const Dims &g, int cls) {
+    Dims d = g;
+    const Dims::ZLayout &l = g.zlay[cls];
+    d.zchunk = l.zchunk;
+    d.ztiles = l.ztiles;
+    d.zchunk2 = l.zchunk2;
+    d.ztiles2 = l.ztiles2;
+    d.zblk = l.zblk;
+    return d;
+}
 
 // Tile-to-block layouts.  CHUNKED: block b owns tiles [b*tpb, (b+1)*tpb); otherwise block b
 // takes tiles b, b+nblk, b+2 nblk, ... (all blocks sweep the grid together).  XCD: hardware

@@ -650,11 +650,12 @@ void launch_finalize(const Dims &g, bool from_state, const double *tp, double *t
         HF_LAUNCH(k_finalize<false>, g, st, g, tp, tpp_io, sv, y, p0, sc);
 }
 
-
-// the stencil kernels (k_rhs_res_z, k_cg_a_z, k_ir_res_z) run g.zblk blocks, the flat ones g.nblk
+// the stencil kernels run g.zblk blocks of the Dims they were launched with (their z-chunking),
+// the flat ones g.nblk; a finishing kernel reads that many partials
 void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double tol, int64_t itmax,
                       bool allow_short, hipStream_t st) {
-    HF_FIN(k_fin_start, g.zblk, st, partials, sc, tol, itmax, (int32_t)allow_short);
+    // after k_rhs_res_z, which runs the pipelined kernels' z-chunking
+    HF_FIN(k_fin_start, g.zlay[kZPipe].zblk, st, partials, sc, tol, itmax, (int32_t)allow_short);
 }
 void launch_fin_a(const Dims &g, const double *partials, Scal *sc, hipStream_t st) {
     HF_FIN(k_fin_a, g.zblk, st, partials, sc);
@@ -671,7 +672,8 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
 }
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                      bool tiny_guess, hipStream_t st) {
-    HF_FIN(k_fin_step, g.zblk, st, partials, sc, sweeps_max, (int32_t)tiny_guess);
+    // after k_ir_step_z (pipelined z-chunking)
+    HF_FIN(k_fin_step, g.zlay[kZPipe].zblk, st, partials, sc, sweeps_max, (int32_t)tiny_guess);
 }
 void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
                         double inner_tol, int64_t inner_itmax, double safety, hipStream_t st) {

@@ -1042,13 +1042,15 @@ k_cg_b1_z2(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *_
 #define HF_ZLAUNCH(kern, g, st, ...) \
     hipLaunchKernelGGL(kern, dim3((g).zblk), dim3(kZTX * kZTY), 0, st, __VA_ARGS__)
 
-// k_rhs_res_z for the run-time S mode and periodicity
+// k_rhs_res_z for the run-time S mode and periodicity, with the pipelined kernels' z-chunking
+// (launch_fin_start reads its partials)
 template <typename OutT, bool FS, bool SS, bool SX>
-static void launch_rhs(const Dims &g, const RhsParams &rp, const double *tp, const double *tpp,
+static void launch_rhs(const Dims &g0, const RhsParams &rp, const double *tp, const double *tpp,
                        const double *xin, const double *bin, const double *rhoc,
                        const double *cvec, const double *qd, const double *bvec, const double *d,
                        const double *cx, const double *cy, const double *cz, OutT *rhat,
                        double *sout, double *xout, Scal *sc, double *partials, hipStream_t st) {
+    const Dims g = zdims(g0, kZPipe);
 #define HF_RHS(M, W)                                                                        \
     HF_ZLAUNCH((k_rhs_res_z<OutT, FS, SS, SX, M, W>), g, st, g, rp, tp, tpp, xin, bin, rhoc, \
                cvec, qd, bvec, d, cx, cy, cz, rhat, sout, xout, sc, partials)
@@ -1162,12 +1164,14 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
                sc, partials);
 }
 
-void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const double *tp,
+// (pipelined z-chunking; launch_fin_step reads its partials)
+void launch_ir_step(const Dims &g0, const StepParams &sp, bool from_state, const double *tp,
                     const double *tpp, const double *xb, const float *y, const float *p0,
                     const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
                     const double *qb, const double *d, const double *cx, const double *cy,
                     const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
                     double *partials, hipStream_t st) {
+    const Dims g = zdims(g0, kZPipe);
 #define HF_IR_STEP(F, T, C)                                                                   \
     do {                                                                                      \
         if (wrap)                                                                             \

@@ -595,27 +595,28 @@ static void queue_cg(hf_ctx *c, Scal *sc, const CgVecs<T> &v, int64_t k0, int64_
         if (k == 0) {
             // k_cg_a (first): read r, 3 couplings
             sa = tbeg(c, ida, N * w * 4);
-            launch_cg_a<T>(gc, true, v.rin, nullptr, nullptr, nullptr, v.hx, v.hy, v.hz, sc,
-                           c->partials, c->st);
+            launch_cg_a<T>(gc, true, v.rin, nullptr, nullptr, nullptr, nullptr, false, v.hx, v.hy,
+                           v.hz, sc, c->partials, c->st);
             tend(c, sa);
             launch_fin_a(gc, c->partials, sc, c->st);
-            // k_cg_b1: read r, 3 couplings; write y, rw
-            sb = tbeg(c, idb, N * w * 6);
-            launch_cg_b1<T>(gc, v.rin, v.hx, v.hy, v.hz, v.y, v.rw, sc, c->partials, c->st);
+            // k_cg_b1: read r, 3 couplings; write rw
+            sb = tbeg(c, idb, N * w * 5);
+            launch_cg_b1<T>(gc, v.rin, v.hx, v.hy, v.hz, v.rw, sc, c->partials, c->st);
             tend(c, sb);
             launch_fin_b1(gc, c->partials, sc, c->st);
         } else {
+            // p_k lives in p[(k - 1) & 1]; p_0 is the right-hand side
             const T *pold = k == 1 ? v.rin : v.p[k & 1];
             T *pnew = v.p[(k - 1) & 1];
-            // k_cg_a: read r, p_old, 3 couplings; write p, q
-            sa = tbeg(c, ida, N * w * 7);
-            launch_cg_a<T>(gc, false, v.rw, pold, pnew, v.q, v.hx, v.hy, v.hz, sc, c->partials,
-                           c->st);
+            // k_cg_a: read r, p_old, (y), 3 couplings; write p, q, y (iteration k-1's term)
+            sa = tbeg(c, ida, N * w * (k == 1 ? 8 : 9));
+            launch_cg_a<T>(gc, false, v.rw, pold, pnew, v.q, v.y, k == 1, v.hx, v.hy, v.hz, sc,
+                           c->partials, c->st);
             tend(c, sa);
             launch_fin_a(gc, c->partials, sc, c->st);
-            // k_cg_b: read y, p, r, q; write y, r
-            sb = tbeg(c, idb, N * w * 6);
-            launch_cg_b<T>(c->g, v.y, pnew, v.rw, v.q, sc, c->partials, c->st);
+            // k_cg_b: read r, q; write r
+            sb = tbeg(c, idb, N * w * 3);
+            launch_cg_b<T>(c->g, v.rw, v.q, sc, c->partials, c->st);
             tend(c, sb);
             launch_fin_b(c->g, c->partials, sc, c->st);
         }
@@ -638,6 +639,11 @@ static int run_cg(hf_ctx *c, Scal *sc, Scal *h, const CgVecs<T> &v, int64_t *pre
         TRY(sync_scal(c, sc, h));
         if (h->done) break;
         batch = std::max<int64_t>(2, batch);
+    }
+    // the last iteration's term of y (iterations >= 1 leave it to their successor)
+    if (h->ypend && h->yset) {
+        launch_cg_tail<T>(c->g, v.y, v.p[0], v.p[1], sc, c->st);
+        TRY(check_launch(c, "k_cg_tail"));
     }
     drop_idle_slots(c, slots, h->iter, h->shortcut != 0);
     if (h->shortcut) c->stats.first_iter_stops += 1;
@@ -709,6 +715,9 @@ static int inner_solve(hf_ctx *c, const float *r, int64_t *batch_out, std::vecto
                            allow_short(c), c->st);
     const int64_t batch = std::max<int64_t>(1, c->pred_inner);
     queue_cg<float>(c, c->sci, cg32(c, r), 0, batch, &slots, HF_K_CG_A32, HF_K_CG_B32);
+    // the last iteration's term of y, when the solve went past its first iteration (the
+    // kernel returns at once otherwise)
+    launch_cg_tail<float>(c->g, c->y32, c->p32[0], c->p32[1], c->sci, c->st);
     *batch_out = batch;
     return check_launch(c, "k_cg32");
 }
@@ -1100,6 +1109,7 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
                                allow_short(c), c->st);
             const int64_t batch = std::max<int64_t>(1, c->pred_adj);
             queue_cg<float>(c, c->sci, cg32(c, rin), 0, batch, nullptr, HF_K_CG_A32, HF_K_CG_B32);
+            launch_cg_tail<float>(c->g, c->y32, c->p32[0], c->p32[1], c->sci, c->st);
             float *rout = other_rvec(c, rin);
             launch_ir_res(c->g, ip, xa, c->y32, rin, c->sci, c->sv, b, c->d, c->cx, c->cy, c->cz,
                           nullptr, xb, rout, c->sc, c->partials, c->st);

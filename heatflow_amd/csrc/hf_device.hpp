@@ -101,7 +101,33 @@ struct Scal {
     int32_t allow_short; // k_fin_a may end a solve after its first iteration
     // next step's ||b^||^2 and ||r^0||^2 without / with the TINY shift (fused mixed step)
     double next_bnorm2, next_rr[2];
+    // the solution y (CgY below): y holds at least one term (yset); the term alpha p_kpend
+    // (alpha = this Scal's alpha) is still to be added (ypend)
+    int32_t yset, ypend;
+    int64_t kpend;
 };
+
+// The solution y of a CG solve on A^ y = r0, as its consumers read it.  The SpMV pass of
+// iteration k >= 1 adds the previous iteration's term alpha_{k-1} p_{k-1} to y, and k_cg_tail,
+// queued after a solve, adds the term of its last iteration K >= 1.  A solve that ended in its
+// first iteration (K = 0) leaves y = alpha r0 unformed (its consumer reads r0 with that scale);
+// one that needed no iteration leaves y = 0.  The values are those an eager update forms.
+template <typename T>
+struct CgY {
+    const T *v;      // y, or r0 (K = 0)
+    T scale;         // 1, or alpha (K = 0)
+    bool any;        // y is not 0
+    __device__ __forceinline__ T at(uint32_t J) const { return scale * v[J]; }
+};
+template <typename T>
+__device__ __forceinline__ CgY<T> cg_y(const Scal *sc, const T *y, const T *r0) {
+    CgY<T> c;
+    const bool k0 = sc->ypend && !sc->yset;   // only p_0 = r0's term
+    c.v = k0 ? r0 : y;
+    c.scale = k0 ? (T)sc->alpha : (T)1;
+    c.any = sc->ypend || sc->yset;
+    return c;
+}
 
 enum : int32_t {
     FLAG_NAN_B = 1, FLAG_NAN_Q = 2, FLAG_NAN_QDENS = 4, FLAG_NAN_SCAT = 8, FLAG_NAN_S = 16,

@@ -319,6 +319,9 @@ k_fin_start(uint32_t nb, const double *partials, Scal *sc, double tol, int64_t i
     sc->beta = 0.0;
     sc->shortcut = 0;
     sc->allow_short = allow_short;
+    sc->yset = 0;
+    sc->ypend = 0;
+    sc->kpend = 0;
     // An exactly zero residual needs no iteration (the reference would divide 0/0).
     sc->done = (v[1] == 0.0) || isnan(v[1]) || itmax <= 0;
     if (isnan(v[1])) sc->flags |= FLAG_BREAKDOWN;
@@ -329,13 +332,18 @@ k_fin_start(uint32_t nb, const double *partials, Scal *sc, double tol, int64_t i
 // alpha = r.r / p.q (after k_cg_a).  On the first iteration p = r0, so the next residual's
 // norm is known without forming it: |r0 - alpha q|^2 = rr - 2 alpha p.q + alpha^2 q.q
 // = alpha^2 q.q - rr.  If that already meets the stop test the solve ends here with
-// y = alpha p (`shortcut`; the consumers of y read p), and k_cg_b is skipped.
+// y = alpha p0 (`shortcut`; the term stays pending for the consumers), and k_cg_b is skipped.
+// After iteration k >= 1's k_cg_a, y holds the terms of iterations 0 .. k-1.
 __global__ void __launch_bounds__(kBlock)
 k_fin_a(uint32_t nb, const double *partials, Scal *sc) {
     if (sc->done) return;
     double v[2];
     sum_partials<2>(partials, nb, v);
     if (threadIdx.x != 0) return;
+    if (!sc->first) {
+        sc->yset = 1;
+        sc->ypend = 0;
+    }
     sc->pq = v[0];
     sc->alpha = sc->rr / v[0];
     if (!(v[0] > 0.0)) {  // A^ is SPD: p.q <= 0 or NaN is a breakdown
@@ -351,22 +359,22 @@ k_fin_a(uint32_t nb, const double *partials, Scal *sc) {
             sc->rr = rr1 > 0.0 ? rr1 : 0.0;
             sc->iter = 1;
             sc->shortcut = 1;
+            sc->ypend = 1;
+            sc->kpend = 0;
             sc->done = 1;
         }
     }
 }
 
-// y += alpha p, r -= alpha q, partial r.r, with 16-byte accesses: block b (XCD-mapped) takes a
-// contiguous chunk of the 16-byte vectors; the cells past the last full vector go to logical
-// block 0.
+// r -= alpha q, partial r.r, with 16-byte accesses: block b (XCD-mapped) takes a contiguous
+// chunk of the 16-byte vectors; the cells past the last full vector go to logical block 0.
+// (y += alpha p is left to the next iteration's k_cg_a, which reads p anyway.)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_cg_b(Dims g, T *__restrict__ y, const T *__restrict__ p, T *__restrict__ r,
-         const T *__restrict__ q, Scal *sc, double *partials) {
+k_cg_b(Dims g, T *__restrict__ r, const T *__restrict__ q, Scal *sc, double *partials) {
     constexpr int W = 16 / sizeof(T);
     using V = T __attribute__((ext_vector_type(W)));
     if (sc->done) return;
-    const bool first = sc->first != 0;
     const T alpha = (T)sc->alpha;
     double acc[1] = {0.0};
     const uint32_t nv = g.N / W;
@@ -375,25 +383,18 @@ k_cg_b(Dims g, T *__restrict__ y, const T *__restrict__ p, T *__restrict__ r,
     const uint64_t v0 = (uint64_t)b * per;
     const uint32_t v1 = (uint32_t)(v0 + per < nv ? v0 + per : nv);
     for (uint32_t v = (uint32_t)(v0 < nv ? v0 : nv) + threadIdx.x; v < v1; v += kBlock) {
-        const V pv = __builtin_nontemporal_load((const V *)p + v);
         const V qv = __builtin_nontemporal_load((const V *)q + v);
         V rv = __builtin_nontemporal_load((const V *)r + v);
-        V yv = first ? (V)(T)0 : __builtin_nontemporal_load((const V *)y + v);
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            yv[k] = first ? alpha * pv[k] : yv[k] + alpha * pv[k];
             rv[k] = rv[k] - alpha * qv[k];
             acc[0] += (double)rv[k] * (double)rv[k];
         }
-        __builtin_nontemporal_store(yv, (V *)y + v);
         __builtin_nontemporal_store(rv, (V *)r + v);
     }
     if (b == 0) {
         for (uint32_t I = nv * W + threadIdx.x; I < g.N; I += kBlock) {
-            const T pi = p[I];
-            const T yi = first ? alpha * pi : y[I] + alpha * pi;
             const T ri = r[I] - alpha * q[I];
-            y[I] = yi;
             r[I] = ri;
             acc[0] += (double)ri * (double)ri;
         }
@@ -401,7 +402,35 @@ k_cg_b(Dims g, T *__restrict__ y, const T *__restrict__ p, T *__restrict__ r,
     block_partials<1>(acc, partials);
 }
 
-// beta, convergence test on ||r^|| / ||b^|| (after k_cg_b)
+// After a solve: y += alpha p_K, the term of its last iteration K >= 1 (p_K = pa for odd K, pb
+// for even); nothing when K = 0 or no term is pending (CgY).  16-byte accesses as k_cg_b.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cg_tail(Dims g, T *__restrict__ y, const T *__restrict__ pa, const T *__restrict__ pb,
+          const Scal *sc) {
+    constexpr int W = 16 / sizeof(T);
+    using V = T __attribute__((ext_vector_type(W)));
+    if (!(sc->ypend && sc->yset)) return;
+    const T *__restrict__ p = ((sc->kpend - 1) & 1) ? pb : pa;
+    const T alpha = (T)sc->alpha;
+    const uint32_t nv = g.N / W;
+    const uint32_t b = logical_block(g);
+    const uint32_t per = (nv + g.nblk - 1) / g.nblk;
+    const uint64_t v0 = (uint64_t)b * per;
+    const uint32_t v1 = (uint32_t)(v0 + per < nv ? v0 + per : nv);
+    for (uint32_t v = (uint32_t)(v0 < nv ? v0 : nv) + threadIdx.x; v < v1; v += kBlock) {
+        const V pv = __builtin_nontemporal_load((const V *)p + v);
+        V yv = __builtin_nontemporal_load((const V *)y + v);
+#pragma unroll
+        for (int k = 0; k < W; ++k) yv[k] = yv[k] + alpha * pv[k];
+        __builtin_nontemporal_store(yv, (V *)y + v);
+    }
+    if (b == 0)
+        for (uint32_t I = nv * W + threadIdx.x; I < g.N; I += kBlock) y[I] = y[I] + alpha * p[I];
+}
+
+// beta, convergence test on ||r^|| / ||b^|| (after k_cg_b or k_cg_b1_z); the term of this
+// iteration k, alpha p_k, is pending in y
 __global__ void __launch_bounds__(kBlock)
 k_fin_b(uint32_t nb, const double *partials, Scal *sc) {
     if (sc->done) return;
@@ -409,6 +438,8 @@ k_fin_b(uint32_t nb, const double *partials, Scal *sc) {
     sum_partials<1>(partials, nb, v);
     if (threadIdx.x != 0) return;
     const double rr = v[0];
+    sc->kpend = sc->iter;
+    sc->ypend = 1;
     sc->iter += 1;
     sc->beta = rr / sc->rr;
     sc->rr = rr;
@@ -424,14 +455,10 @@ template <bool FromState>
 __global__ void __launch_bounds__(kBlock)
 k_finalize(Dims g, const double *__restrict__ tp, double *__restrict__ tpp_io,
            const double *__restrict__ sv, const double *__restrict__ y,
-           const double *__restrict__ p0, Scal *sc) {
+           const double *__restrict__ r0, Scal *sc) {
     const bool tiny = sc->tiny != 0;
-    const bool have_y = sc->iter > 0;
-    // a solve that ended on its first iteration left y = alpha p0 unformed
-    const bool sh = sc->shortcut != 0;
-    const double alpha = sc->alpha;
-    const double *__restrict__ ys = sh ? p0 : y;
-    const double ysc = sh ? alpha : 1.0;
+    const CgY<double> cy = cg_y(sc, y, r0);
+    const bool have_y = cy.any;
     int32_t fl = 0;
     HF_FOR_CELLS(g, I) {
         double x0, tpi = 0.0;
@@ -442,7 +469,7 @@ k_finalize(Dims g, const double *__restrict__ tp, double *__restrict__ tpp_io,
         } else {
             x0 = ldnt(&tpp_io[I]);
         }
-        const double x = have_y ? x0 + ldnt(&sv[I]) * (ysc * ldnt(&ys[I])) : x0;
+        const double x = have_y ? x0 + ldnt(&sv[I]) * cy.at(I) : x0;
         stnt(&tpp_io[I], x);
         if (isnan(x)) fl |= FLAG_NAN_X;
         if (FromState && (x + (x - tpi)) - x < kTiny) fl |= FLAG_TINY_NEXT;
@@ -484,6 +511,9 @@ static __device__ void inner_begin(Scal *outer, Scal *inner, double inner_tol, d
     inner->beta = 0.0;
     inner->alpha = 0.0;
     inner->shortcut = 0;
+    inner->yset = 0;
+    inner->ypend = 0;
+    inner->kpend = 0;
     inner->allow_short = allow_short;
     inner->done = (rr == 0.0) || isnan(rr) || inner_itmax <= 0;
     // the refinement pass after this solve recomputes the next step's TINY test and S: only
@@ -526,6 +556,9 @@ static __device__ void carry_begin(Scal *sc, double tol, int64_t itmax, int32_t 
     sc->alpha = 0.0;
     sc->beta = 0.0;
     sc->shortcut = 0;
+    sc->yset = 0;
+    sc->ypend = 0;
+    sc->kpend = 0;
     sc->allow_short = allow_short;
     sc->done = (rr == 0.0) || isnan(rr) || itmax <= 0;
     if (isnan(rr)) sc->flags |= FLAG_BREAKDOWN;
@@ -632,22 +665,31 @@ void launch_qdens(const Dims &g, const HeatParams &hp, const int8_t *iheater, co
     hipLaunchKernelGGL(kern, dim3(1), dim3(kBlock), 0, st, nb, __VA_ARGS__)
 
 template <typename T>
-void launch_cg_b(const Dims &g, T *y, const T *p, T *r, const T *q, Scal *sc, double *partials,
-                 hipStream_t st) {
-    HF_LAUNCH(k_cg_b<T>, g, st, g, y, p, r, q, sc, partials);
+void launch_cg_b(const Dims &g, T *r, const T *q, Scal *sc, double *partials, hipStream_t st) {
+    HF_LAUNCH(k_cg_b<T>, g, st, g, r, q, sc, partials);
 }
-template void launch_cg_b<double>(const Dims &, double *, const double *, double *,
-                                  const double *, Scal *, double *, hipStream_t);
-template void launch_cg_b<float>(const Dims &, float *, const float *, float *, const float *,
-                                 Scal *, double *, hipStream_t);
+template void launch_cg_b<double>(const Dims &, double *, const double *, Scal *, double *,
+                                  hipStream_t);
+template void launch_cg_b<float>(const Dims &, float *, const float *, Scal *, double *,
+                                 hipStream_t);
+
+template <typename T>
+void launch_cg_tail(const Dims &g, T *y, const T *pa, const T *pb, const Scal *sc,
+                    hipStream_t st) {
+    HF_LAUNCH(k_cg_tail<T>, g, st, g, y, pa, pb, sc);
+}
+template void launch_cg_tail<double>(const Dims &, double *, const double *, const double *,
+                                     const Scal *, hipStream_t);
+template void launch_cg_tail<float>(const Dims &, float *, const float *, const float *,
+                                    const Scal *, hipStream_t);
 
 void launch_finalize(const Dims &g, bool from_state, const double *tp, double *tpp_io,
-                     const double *sv, const double *y, const double *p0, Scal *sc,
+                     const double *sv, const double *y, const double *r0, Scal *sc,
                      hipStream_t st) {
     if (from_state)
-        HF_LAUNCH(k_finalize<true>, g, st, g, tp, tpp_io, sv, y, p0, sc);
+        HF_LAUNCH(k_finalize<true>, g, st, g, tp, tpp_io, sv, y, r0, sc);
     else
-        HF_LAUNCH(k_finalize<false>, g, st, g, tp, tpp_io, sv, y, p0, sc);
+        HF_LAUNCH(k_finalize<false>, g, st, g, tp, tpp_io, sv, y, r0, sc);
 }
 
 // the stencil kernels run g.zblk blocks of the Dims they were launched with (their z-chunking),

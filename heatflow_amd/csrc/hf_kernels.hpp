@@ -80,25 +80,31 @@ void launch_rhs_res_solve32(const Dims &g, const RhsParams &rp, const double *x,
                             const double *b, const double *d, const double *cx,
                             const double *cy, const double *cz, float *rhat, Scal *sc,
                             double *partials, hipStream_t st);
-// first: the solve's first iteration (p = r; reductions only, nothing stored)
+// first: the solve's first iteration (p = r; reductions only, nothing stored).  Iteration
+// k >= 1 also adds the previous iteration's term to the solution: y = alpha p_old (yfirst,
+// k == 1) or y += alpha p_old, alpha being the scalars' (that iteration's) alpha.
 template <typename T>
-void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, T *q,
-                 const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
+void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, T *q, T *y,
+                 bool yfirst, const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
                  hipStream_t st);
-// the rest of the first iteration: y = alpha r, rw = r - alpha A^ r (r unchanged)
+// the rest of the first iteration: rw = r - alpha A^ r (r unchanged)
 template <typename T>
-void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *y, T *rw,
+void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *rw,
                   Scal *sc, double *partials, hipStream_t st);
+// r -= alpha q
 template <typename T>
-void launch_cg_b(const Dims &g, T *y, const T *p, T *r, const T *q, Scal *sc, double *partials,
-                 hipStream_t st);
-// y of a solve is read from `y`, or as alpha * p0 when the solve ended on its first iteration
-// (Scal::shortcut of the solve's scalars)
+void launch_cg_b(const Dims &g, T *r, const T *q, Scal *sc, double *partials, hipStream_t st);
+// after a solve: y += alpha p_K for its last iteration K >= 1 (pa, pb = p_k for odd / even k)
+template <typename T>
+void launch_cg_tail(const Dims &g, T *y, const T *pa, const T *pb, const Scal *sc,
+                    hipStream_t st);
+// The consumers of a solve's y (CgY: y after k_cg_tail, or alpha r0 when the solve ended in
+// its first iteration) and of that solve's scalars.
 void launch_finalize(const Dims &g, bool from_state, const double *tp, double *tpp_io,
-                     const double *sv, const double *y, const double *p0, Scal *sc,
+                     const double *sv, const double *y, const double *r0, Scal *sc,
                      hipStream_t st);
 void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const float *y,
-                   const float *p0, const Scal *inner, const double *sv, const double *S,
+                   const float *r0, const Scal *inner, const double *sv, const double *S,
                    const double *d, const double *cx, const double *cy, const double *cz,
                    const double *tp, double *xout, float *rhat, Scal *sc, double *partials,
                    hipStream_t st);
@@ -108,7 +114,7 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
 // 3 partials.  The pass is a function of its inputs: run again, it writes the same values.
 // qb = Qdens + B (per-step constant in this path)
 void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const double *tp,
-                    const double *tpp, const double *xb, const float *y, const float *p0,
+                    const double *tpp, const double *xb, const float *y, const float *r0,
                     const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
                     const double *qb, const double *d, const double *cx, const double *cy,
                     const double *cz, double *xout, float *rres, float *rnext, Scal *sc,

@@ -192,6 +192,53 @@ __device__ __forceinline__ void zmarch(const Dims &g, const ZCol &c, PlaneBuf<V>
     }
 }
 
+// zmarch that also hands each cell a value of its own (own(J): loads of the same addresses as
+// X's, which the compiler shares), carried in registers from the load one plane ahead.
+template <typename V, typename O, class XF, class OF, class CF>
+__device__ __forceinline__ void zmarch_own(const Dims &g, const ZCol &c, PlaneBuf<V> &sp, XF X,
+                                           OF own, CF cell) {
+    const Ring rg = ring_cell(g, c);
+    const bool per_z = g.per[2];
+    V xz_m = (V)0, xi = (V)0;
+    O oi{};
+    if (c.active) {
+        if (c.z0 > 0 || per_z) xz_m = X(zm_index(g, c.col, c.z0));
+        const uint32_t J0 = c.col + g.nxny * c.z0;
+        xi = X(J0);
+        oi = own(J0);
+    }
+    if (c.zvalid) {
+        sp.v[0][c.ly + 1][c.lx + 1] = xi;
+        if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = X(rg.col + g.nxny * c.z0);
+    }
+    int buf = 0;
+    uint32_t I = c.col + g.nxny * c.z0;
+    for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
+        const bool more = z + 1 < c.z1;
+        V xz_p = (V)0, rnext = (V)0;
+        O op{};
+        if (c.active && (z + 1 < g.nz || per_z)) {
+            const uint32_t J = zp_index(g, c.col, z);
+            xz_p = X(J);
+            op = own(J);
+        }
+        if (more && rg.col != kNoRing) rnext = X(rg.col + g.nxny * (z + 1));
+        __syncthreads();   // plane z is staged in sp.v[buf]
+        if (c.active) {
+            const InPlane<V> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
+            cell(I, z, xi, xz_m, xz_p, nb, oi);
+        }
+        if (more) {
+            sp.v[buf ^ 1][c.ly + 1][c.lx + 1] = xz_p;
+            if (rg.col != kNoRing) (&sp.v[buf ^ 1][0][0])[rg.slot] = rnext;
+        }
+        buf ^= 1;
+        xz_m = xi;
+        xi = xz_p;
+        oi = op;
+    }
+}
+
 // zmarch, software-pipelined.  The operand is formed from a raw record R = load(J) by form(R);
 // the cell's other inputs are a record C = cload(I) (I a cell of the thread's column, or 0 for
 // an inactive thread).  While plane z is computed, the loads of plane z+1's cell record and
@@ -473,26 +520,31 @@ __device__ __forceinline__ T ahat(const Dims &g, const ZCol &c, uint32_t I, uint
 // First iteration (First): p = r, and only the partials: p and q are not stored, because the
 // solve either stops here (k_fin_a's first-iteration test; y = alpha r is formed by its
 // consumer) or k_cg_b1_z recomputes q.
+// Iteration k >= 1 also adds iteration k-1's term to the solution: y = alpha p_old (yfirst,
+// k == 1) or y += alpha p_old, with alpha still that iteration's.
 template <typename T, bool First>
 __global__ void __launch_bounds__(kZTX * kZTY)
 k_cg_a_z(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
-         T *__restrict__ q, const T *__restrict__ hx, const T *__restrict__ hy,
-         const T *__restrict__ hz, Scal *sc, double *partials) {
+         T *__restrict__ q, T *__restrict__ y, int32_t yfirst, const T *__restrict__ hx,
+         const T *__restrict__ hy, const T *__restrict__ hz, Scal *sc, double *partials) {
     __shared__ PlaneBuf<T> sp;
     if (sc->done) return;
     const T beta = (T)sc->beta;
+    const T alpha = (T)sc->alpha;
     auto P = [&](uint32_t J) -> T { return First ? r[J] : (T)(r[J] + beta * pold[J]); };
+    auto own = [&](uint32_t J) -> T { return First ? (T)0 : pold[J]; };
     double acc[2] = {0.0, 0.0};
     const ZCol c = zcolumn(g);
     T hz_m = (T)0;
     if (c.active && (c.z0 > 0 || g.per[2])) hz_m = hz[zm_index(g, c.col, c.z0)];
-    zmarch<T>(g, c, sp, P, [&](uint32_t I, uint32_t z, T pi, T pz_m, T pz_p,
-                               const InPlane<T> &nb) {
+    zmarch_own<T, T>(g, c, sp, P, own, [&](uint32_t I, uint32_t z, T pi, T pz_m, T pz_p,
+                                           const InPlane<T> &nb, T po) {
         const T hzc = ldnt(&hz[I]);
         const T qi = ahat(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P);
         if (!First) {
             stnt(&pnew[I], pi);
             stnt(&q[I], qi);
+            stnt(&y[I], yfirst ? (T)(alpha * po) : (T)(ldnt(&y[I]) + alpha * po));
         }
         acc[0] += (double)pi * (double)qi;
         acc[1] += (double)qi * (double)qi;
@@ -502,13 +554,13 @@ k_cg_a_z(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restr
 }
 
 // Second half of the first CG iteration when the solve goes on: q = A^ r recomputed,
-// y = alpha r, rw = r - alpha q (into a separate vector: r is the stencil operand), partial
-// rw.rw.  The right-hand side r is left unchanged; it is p_old of iteration 1.
+// rw = r - alpha q (into a separate vector: r is the stencil operand), partial rw.rw.  The
+// right-hand side r is left unchanged; it is p_old of iteration 1 (whose k_cg_a forms
+// y = alpha r).
 template <typename T>
 __global__ void __launch_bounds__(kZTX * kZTY)
 k_cg_b1_z(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *__restrict__ hy,
-          const T *__restrict__ hz, T *__restrict__ y, T *__restrict__ rw, Scal *sc,
-          double *partials) {
+          const T *__restrict__ hz, T *__restrict__ rw, Scal *sc, double *partials) {
     __shared__ PlaneBuf<T> sp;
     if (sc->done) return;
     const T alpha = (T)sc->alpha;
@@ -522,7 +574,6 @@ k_cg_b1_z(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *__
         const T hzc = ldnt(&hz[I]);
         const T qi = ahat(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P);
         const T ri = pi - alpha * qi;
-        stnt(&y[I], alpha * pi);
         stnt(&rw[I], ri);
         acc[0] += (double)ri * (double)ri;
         hz_m = hzc;
@@ -534,18 +585,17 @@ k_cg_b1_z(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *__
 // stored as FP32 for the next inner solve, ||r^||^2, NaN and next-step TINY tests on x'.
 __global__ void __launch_bounds__(kZTX * kZTY)
 k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__restrict__ y,
-           const float *__restrict__ p0, const Scal *inner, const double *__restrict__ sv,
+           const float *__restrict__ r0, const Scal *inner, const double *__restrict__ sv,
            const double *__restrict__ S, const double *__restrict__ d,
            const double *__restrict__ cx, const double *__restrict__ cy,
            const double *__restrict__ cz, const double *__restrict__ tp,
            double *__restrict__ xout, float *__restrict__ rhat, Scal *sc, double *partials) {
     __shared__ PlaneBuf<double> sp;
-    // an inner solve that ended on its first iteration left y = alpha p0 unformed; it is
-    // formed here as k_cg_b would have (FP32 product)
-    const bool sh = inner->shortcut != 0;
-    const float *__restrict__ ys = sh ? p0 : y;
-    const float ysc = sh ? (float)inner->alpha : 1.0f;
-    auto X = [&](uint32_t J) -> double { return x[J] + sv[J] * (double)(ysc * ys[J]); };
+    // the inner solution (CgY; alpha r0 formed in FP32 as k_cg_b1_z's terms are)
+    const CgY<float> ysol = cg_y(inner, y, r0);
+    auto X = [&](uint32_t J) -> double {
+        return ysol.any ? x[J] + sv[J] * (double)ysol.at(J) : x[J];
+    };
     double acc[1] = {0.0};
     int32_t fl = 0;
     const ZCol c = zcolumn(g);
@@ -569,7 +619,8 @@ k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__res
     block_partials<1>(acc, partials);
 }
 
-// operand inputs of the fused step: T^n, T^{n-1} (or the refined x), s, y
+// operand inputs of the fused step: T^n, T^{n-1} (or the refined x), s, the inner solution
+// (y, or r0 when scaled by alpha: CgY)
 struct StepRaw {
     double tp, b, s;
     float y;
@@ -608,7 +659,7 @@ template <bool FromState, bool Transient, bool Cattaneo, bool Wrap>
 __global__ void __launch_bounds__(kZTX * kZTY) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             const double *__restrict__ tpp, const double *__restrict__ xb,
-            const float *__restrict__ y, const float *__restrict__ p0, const Scal *inner,
+            const float *__restrict__ y, const float *__restrict__ r0, const Scal *inner,
             const double *__restrict__ sv, const double *__restrict__ rhoc,
             const double *__restrict__ cvec, const double *__restrict__ qb,
             const double *__restrict__ d, const double *__restrict__ cx,
@@ -617,17 +668,17 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             Scal *sc, double *partials) {
     __shared__ PlaneBuf<D2> sp;
     const bool tiny = sc->tiny != 0;
-    // y = alpha p0 of an inner solve that ended on its first iteration, as k_cg_b forms it
-    const bool sh = inner->shortcut != 0;
-    const float *__restrict__ ys = sh ? p0 : y;
-    const float ysc = sh ? (float)inner->alpha : 1.0f;
+    // the inner solution (CgY), formed in FP32 as k_cg_b1_z forms its terms; 0 when the solve
+    // needed no iteration
+    const CgY<float> ysol = cg_y(inner, y, r0);
+    const float ysc = ysol.any ? ysol.scale : 0.0f;
     const ZCol c = zcolumn(g);
     const Ring rg = ring_cell(g, c);
     const bool per_z = Wrap && g.per[2];
     const bool xm_in = c.x > 0, xp_in = c.x + 1 < g.nx;
     const bool ym_in = c.y > 0, yp_in = c.y + 1 < g.ny;
     auto load = [&](uint32_t J) -> StepRaw {
-        return StepRaw{tp[J], FromState ? tpp[J] : xb[J], sv[J], ys[J]};
+        return StepRaw{tp[J], FromState ? tpp[J] : xb[J], sv[J], ysol.v[J]};
     };
     auto form = [&](const StepRaw &r) -> D2 {
         double base;
@@ -679,8 +730,8 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         }
         return h;
     };
-    auto pa = [](const D2 &v) { return v.a; };
-    auto pb = [](const D2 &v) { return v.b; };
+    auto pick_a = [](const D2 &v) { return v.a; };
+    auto pick_b = [](const D2 &v) { return v.b; };
     const bool guess = sp_.tiny_guess != 0;
     double acc[3] = {0.0, 0.0, 0.0};
     int32_t fl = 0, fl_next = 0;
@@ -744,8 +795,8 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 Sn1 = Sn;
             }
             if (isnan(Sn1)) fl_next |= FLAG_NAN_S;
-            const double h1 = hsum(I, z, cur, cz_m, xi, xz_m, xz_p, nbc, pa);
-            const double h2 = hsum(I, z, cur, cz_m, xi, xz_m, xz_p, nbc, pb);
+            const double h1 = hsum(I, z, cur, cz_m, xi, xz_m, xz_p, nbc, pick_a);
+            const double h2 = hsum(I, z, cur, cz_m, xi, xz_m, xz_p, nbc, pick_b);
             const double s = 1.0 / sqrt(-cur.d);
             const float rf = (float)(-s * (Sn - h1));
             double ra = -s * (Sn1 - h2);
@@ -876,17 +927,20 @@ __device__ __forceinline__ Ring ring_cell2(const Dims &g, const ZCol2 &c) {
     return r;
 }
 
-// zmarch for pairs: X(J) gives the operand pair at the even cell J.
-template <typename T, class XF, class CF>
+// zmarch for pairs: X(J) gives the operand pair at the even cell J; own(J) a pair of the
+// cell's own (loads shared with X's, carried in registers as in zmarch_own).
+template <typename T, class XF, class OF, class CF>
 __device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<vec2<T>> &sp,
-                                        XF X, CF cell) {
+                                        XF X, OF own, CF cell) {
     using T2 = vec2<T>;
     const Ring rg = ring_cell2(g, c);
     const bool per_z = g.per[2];
-    T2 xz_m = (T2)(T)0, xi = (T2)(T)0;
+    T2 xz_m = (T2)(T)0, xi = (T2)(T)0, oi = (T2)(T)0;
     if (c.active) {
         if (c.z0 > 0 || per_z) xz_m = X(zm_index(g, c.col, c.z0));
-        xi = X(c.col + g.nxny * c.z0);
+        const uint32_t J0 = c.col + g.nxny * c.z0;
+        xi = X(J0);
+        oi = own(J0);
     }
     if (c.zvalid) {
         sp.v[0][c.ly + 1][c.lx + 1] = xi;
@@ -896,13 +950,17 @@ __device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<
     uint32_t I = c.col + g.nxny * c.z0;
     for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
         const bool more = z + 1 < c.z1;
-        T2 xz_p = (T2)(T)0, rnext = (T2)(T)0;
-        if (c.active && (z + 1 < g.nz || per_z)) xz_p = X(zp_index(g, c.col, z));
+        T2 xz_p = (T2)(T)0, rnext = (T2)(T)0, op = (T2)(T)0;
+        if (c.active && (z + 1 < g.nz || per_z)) {
+            const uint32_t J = zp_index(g, c.col, z);
+            xz_p = X(J);
+            op = own(J);
+        }
         if (more && rg.col != kNoRing) rnext = X(rg.col + g.nxny * (z + 1));
         __syncthreads();   // plane z is staged in sp.v[buf]
         if (c.active) {
             const InPlane<T2> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
-            cell(I, z, xi, xz_m, xz_p, nb);
+            cell(I, z, xi, xz_m, xz_p, nb, oi);
         }
         if (more) {
             sp.v[buf ^ 1][c.ly + 1][c.lx + 1] = xz_p;
@@ -911,6 +969,7 @@ __device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<
         buf ^= 1;
         xz_m = xi;
         xi = xz_p;
+        oi = op;
     }
 }
 
@@ -971,12 +1030,14 @@ __device__ __forceinline__ vec2<T> ahat2(const Dims &g, const ZCol2 &c, uint32_t
 template <typename T, bool First>
 __global__ void __launch_bounds__(kZTX * kZTY)
 k_cg_a_z2(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
-          T *__restrict__ q, const T *__restrict__ hx, const T *__restrict__ hy,
-          const T *__restrict__ hz, Scal *sc, double *partials) {
+          T *__restrict__ q, T *__restrict__ y, int32_t yfirst, const T *__restrict__ hx,
+          const T *__restrict__ hy, const T *__restrict__ hz, Scal *sc, double *partials) {
     using T2 = vec2<T>;
     __shared__ PlaneBuf<T2> sp;
     if (sc->done) return;
     const T beta = (T)sc->beta;
+    const T alpha = (T)sc->alpha;
+    auto own = [&](uint32_t J) -> T2 { return First ? (T2)(T)0 : ld2(&pold[J]); };
     auto P1 = [&](uint32_t J) -> T { return First ? r[J] : (T)(r[J] + beta * pold[J]); };
     auto P2 = [&](uint32_t J) -> T2 {
         if (First) return ld2(&r[J]);
@@ -990,13 +1051,24 @@ k_cg_a_z2(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__rest
     const ZCol2 c = zcolumn2(g);
     T2 hz_m = (T2)(T)0;
     if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ld2(&hz[zm_index(g, c.col, c.z0)]);
-    zmarch2<T>(g, c, sp, P2, [&](uint32_t I, uint32_t z, T2 pi, T2 pz_m, T2 pz_p,
-                                 const InPlane<T2> &nb) {
+    zmarch2<T>(g, c, sp, P2, own, [&](uint32_t I, uint32_t z, T2 pi, T2 pz_m, T2 pz_p,
+                                      const InPlane<T2> &nb, T2 po) {
         const T2 hzc = ldnt2(&hz[I]);
         const T2 qi = ahat2<T>(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
         if (!First) {
             stnt2(&pnew[I], pi);
             stnt2(&q[I], qi);
+            // iteration k-1's term of the solution (alpha p_old), as k_cg_a_z
+            T2 yi;
+            if (yfirst) {
+                yi.x = alpha * po.x;
+                yi.y = alpha * po.y;
+            } else {
+                yi = ldnt2(&y[I]);
+                yi.x = yi.x + alpha * po.x;
+                yi.y = yi.y + alpha * po.y;
+            }
+            stnt2(&y[I], yi);
         }
         acc[0] += (double)pi.x * (double)qi.x + (double)pi.y * (double)qi.y;
         acc[1] += (double)qi.x * (double)qi.x + (double)qi.y * (double)qi.y;
@@ -1008,8 +1080,7 @@ k_cg_a_z2(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__rest
 template <typename T>
 __global__ void __launch_bounds__(kZTX * kZTY)
 k_cg_b1_z2(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *__restrict__ hy,
-           const T *__restrict__ hz, T *__restrict__ y, T *__restrict__ rw, Scal *sc,
-           double *partials) {
+           const T *__restrict__ hz, T *__restrict__ rw, Scal *sc, double *partials) {
     using T2 = vec2<T>;
     __shared__ PlaneBuf<T2> sp;
     if (sc->done) return;
@@ -1020,16 +1091,14 @@ k_cg_b1_z2(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *_
     const ZCol2 c = zcolumn2(g);
     T2 hz_m = (T2)(T)0;
     if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ld2(&hz[zm_index(g, c.col, c.z0)]);
-    zmarch2<T>(g, c, sp, P2, [&](uint32_t I, uint32_t z, T2 pi, T2 pz_m, T2 pz_p,
-                                 const InPlane<T2> &nb) {
+    auto own = [](uint32_t) -> T2 { return (T2)(T)0; };
+    zmarch2<T>(g, c, sp, P2, own, [&](uint32_t I, uint32_t z, T2 pi, T2 pz_m, T2 pz_p,
+                                      const InPlane<T2> &nb, T2) {
         const T2 hzc = ldnt2(&hz[I]);
         const T2 qi = ahat2<T>(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
-        T2 yi, ri;
-        yi.x = alpha * pi.x;
-        yi.y = alpha * pi.y;
+        T2 ri;
         ri.x = pi.x - alpha * qi.x;
         ri.y = pi.y - alpha * qi.y;
-        stnt2(&y[I], yi);
         stnt2(&rw[I], ri);
         acc[0] += (double)ri.x * (double)ri.x + (double)ri.y * (double)ri.y;
         hz_m = hzc;
@@ -1116,57 +1185,60 @@ void launch_rhs_res_solve32(const Dims &g, const RhsParams &rp, const double *x,
 }
 
 template <typename T>
-void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, T *q,
-                 const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
+void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, T *q, T *y,
+                 bool yfirst, const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
                  hipStream_t st) {
+    const int32_t yf = yfirst ? 1 : 0;
     if (g.pairs) {
         if (first)
-            HF_ZLAUNCH((k_cg_a_z2<T, true>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+            HF_ZLAUNCH((k_cg_a_z2<T, true>), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc,
+                       partials);
         else
-            HF_ZLAUNCH((k_cg_a_z2<T, false>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc,
+            HF_ZLAUNCH((k_cg_a_z2<T, false>), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc,
                        partials);
         return;
     }
     if (first)
-        HF_ZLAUNCH((k_cg_a_z<T, true>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+        HF_ZLAUNCH((k_cg_a_z<T, true>), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc,
+                   partials);
     else
-        HF_ZLAUNCH((k_cg_a_z<T, false>), g, st, g, r, pold, pnew, q, hx, hy, hz, sc, partials);
+        HF_ZLAUNCH((k_cg_a_z<T, false>), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc,
+                   partials);
 }
 template void launch_cg_a<double>(const Dims &, bool, const double *, const double *, double *,
-                                  double *, const double *, const double *, const double *,
-                                  Scal *, double *, hipStream_t);
+                                  double *, double *, bool, const double *, const double *,
+                                  const double *, Scal *, double *, hipStream_t);
 template void launch_cg_a<float>(const Dims &, bool, const float *, const float *, float *,
-                                 float *, const float *, const float *, const float *, Scal *,
-                                 double *, hipStream_t);
+                                 float *, float *, bool, const float *, const float *,
+                                 const float *, Scal *, double *, hipStream_t);
 
 template <typename T>
-void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *y, T *rw,
+void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *rw,
                   Scal *sc, double *partials, hipStream_t st) {
     if (g.pairs) {
-        HF_ZLAUNCH(k_cg_b1_z2<T>, g, st, g, r, hx, hy, hz, y, rw, sc, partials);
+        HF_ZLAUNCH(k_cg_b1_z2<T>, g, st, g, r, hx, hy, hz, rw, sc, partials);
         return;
     }
-    HF_ZLAUNCH(k_cg_b1_z<T>, g, st, g, r, hx, hy, hz, y, rw, sc, partials);
+    HF_ZLAUNCH(k_cg_b1_z<T>, g, st, g, r, hx, hy, hz, rw, sc, partials);
 }
 template void launch_cg_b1<double>(const Dims &, const double *, const double *, const double *,
-                                   const double *, double *, double *, Scal *, double *,
+                                   const double *, double *, Scal *, double *,
                                    hipStream_t);
 template void launch_cg_b1<float>(const Dims &, const float *, const float *, const float *,
-                                  const float *, float *, float *, Scal *, double *,
-                                  hipStream_t);
+                                  const float *, float *, Scal *, double *, hipStream_t);
 
 void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const float *y,
-                   const float *p0, const Scal *inner, const double *sv, const double *S,
+                   const float *r0, const Scal *inner, const double *sv, const double *S,
                    const double *d, const double *cx, const double *cy, const double *cz,
                    const double *tp, double *xout, float *rhat, Scal *sc, double *partials,
                    hipStream_t st) {
-    HF_ZLAUNCH(k_ir_res_z, g, st, g, ip, x, y, p0, inner, sv, S, d, cx, cy, cz, tp, xout, rhat,
+    HF_ZLAUNCH(k_ir_res_z, g, st, g, ip, x, y, r0, inner, sv, S, d, cx, cy, cz, tp, xout, rhat,
                sc, partials);
 }
 
 // (pipelined z-chunking; launch_fin_step reads its partials)
 void launch_ir_step(const Dims &g0, const StepParams &sp, bool from_state, const double *tp,
-                    const double *tpp, const double *xb, const float *y, const float *p0,
+                    const double *tpp, const double *xb, const float *y, const float *r0,
                     const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
                     const double *qb, const double *d, const double *cx, const double *cy,
                     const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
@@ -1175,10 +1247,10 @@ void launch_ir_step(const Dims &g0, const StepParams &sp, bool from_state, const
 #define HF_IR_STEP(F, T, C)                                                                   \
     do {                                                                                      \
         if (wrap)                                                                             \
-            HF_ZLAUNCH((k_ir_step_z<F, T, C, true>), g, st, g, sp, tp, tpp, xb, y, p0, inner, \
+            HF_ZLAUNCH((k_ir_step_z<F, T, C, true>), g, st, g, sp, tp, tpp, xb, y, r0, inner,  \
                        sv, rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);   \
         else                                                                                  \
-            HF_ZLAUNCH((k_ir_step_z<F, T, C, false>), g, st, g, sp, tp, tpp, xb, y, p0, inner,\
+            HF_ZLAUNCH((k_ir_step_z<F, T, C, false>), g, st, g, sp, tp, tpp, xb, y, r0, inner, \
                        sv, rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);   \
     } while (0)
     const bool tr = sp.isteady == 0, cat = sp.icattaneo == 1;

@@ -116,15 +116,17 @@ static void make_dims(Dims &g, const hf_params &p) {
     nb = (g.ntiles + g.tpb - 1) / g.tpb;
     g.nblk = (uint32_t)((nb + 7) / 8 * 8);
     // z-marching tiles of kZTX x kZTY columns: enough z-chunks for ~`zcap` blocks, one chunking
-    // per kernel class (measured at 512^3, 1 to 7% apart; HF_ZBLK, HF_ZBLK_PIPE, HF_ZBLK_COL
-    // override).  Fewer, longer chunks cost fewer prologue planes; the FP32 CG kernels, with
-    // the least work per cell, ran best at ~4096 blocks, the pipelined kernels at ~2048 and the
-    // FP64 two-cell CG kernels with whole columns (one round of resident blocks).
+    // per kernel class (measured at 512^3, 1 to 12% apart; HF_ZBLK, HF_ZBLK_PIPE, HF_ZBLK_COL
+    // override).  Fewer, longer chunks cost fewer prologue planes; the CG kernels with 8-byte
+    // accesses (FP32 pairs) ran best at ~4096 blocks, the pipelined kernels at ~2048 and the
+    // 16-byte ones (FP64 pairs, FP32 quads) with whole columns (one round of resident blocks).
     const uint32_t tx = kZTX, ty = kZTY;
     g.zty = (g.ny + ty - 1) / ty;
     g.ztx = (g.nx + tx - 1) / tx;
     g.ztx2 = (g.nx + 2 * tx - 1) / (2 * tx);
+    g.ztx4 = (g.nx + 4 * tx - 1) / (4 * tx);
     g.pairs = (g.nx % 2 == 0) && getenv("HF_NO_PAIRS") == nullptr;
+    g.quads = g.pairs && (g.nx % 4 == 0) && getenv("HF_NO_QUADS") == nullptr;
     // chunk length and tile count for ~zcap blocks, tiles of `ztx` x zty per plane
     auto chunking = [&](uint32_t zcap, uint32_t ztx, uint32_t *zchunk, uint32_t *ztiles) {
         const uint64_t per_plane = (uint64_t)ztx * g.zty;
@@ -144,7 +146,10 @@ static void make_dims(Dims &g, const hf_params &p) {
         Dims::ZLayout &l = g.zlay[k];
         chunking(caps[k], g.ztx, &l.zchunk, &l.ztiles);
         chunking(caps[k], g.ztx2, &l.zchunk2, &l.ztiles2);
-        const uint32_t zt = g.pairs ? std::max(l.ztiles, l.ztiles2) : l.ztiles;
+        chunking(caps[k], g.ztx4, &l.zchunk4, &l.ztiles4);
+        uint32_t zt = l.ztiles;
+        if (g.pairs) zt = std::max(zt, l.ztiles2);
+        if (g.quads) zt = std::max(zt, l.ztiles4);
         l.zblk = (zt + 7) / 8 * 8;
     }
     const Dims::ZLayout &l0 = g.zlay[kZDefault];
@@ -152,6 +157,8 @@ static void make_dims(Dims &g, const hf_params &p) {
     g.ztiles = l0.ztiles;
     g.zchunk2 = l0.zchunk2;
     g.ztiles2 = l0.ztiles2;
+    g.zchunk4 = l0.zchunk4;
+    g.ztiles4 = l0.ztiles4;
     g.zblk = l0.zblk;
 }
 
@@ -160,10 +167,12 @@ static uint32_t max_zblk(const Dims &g) {
     return std::max(g.zlay[0].zblk, std::max(g.zlay[1].zblk, g.zlay[2].zblk));
 }
 
-// the z-chunking of the CG kernels in precision T
+// the z-chunking of the CG kernels in precision T: whole columns for the 16-byte-access
+// kernels (FP64 pairs, FP32 quads)
 template <typename T>
 static Dims cg_dims(const Dims &g) {
-    return std::is_same<T, double>::value && g.pairs ? zdims(g, kZColumn) : g;
+    const bool wide = std::is_same<T, double>::value ? g.pairs : g.quads;
+    return wide ? zdims(g, kZColumn) : g;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -52,18 +52,22 @@ struct Dims {
     uint32_t ztx, zty, zchunk, ztiles, zblk;
     uint32_t ztx2, zchunk2, ztiles2;
     int32_t pairs;
+    // four x-neighbouring cells per thread (FP32 CG kernels, nx a multiple of 4: `quads`)
+    uint32_t ztx4, zchunk4, ztiles4;
+    int32_t quads;
     int32_t per[3];      // periodic axes (length >= 3)
     FastDiv dnx, dny;
     // z-chunkings for ~4096 (the active one above), ~2048 and ~512 blocks at 512^3; zdims()
     // selects one for a kernel and its finishing kernel
     struct ZLayout {
-        uint32_t zchunk, ztiles, zchunk2, ztiles2, zblk;
+        uint32_t zchunk, ztiles, zchunk2, ztiles2, zchunk4, ztiles4, zblk;
     } zlay[3];
 };
 
 // z-chunkings (Dims::zlay): kZDefault; kZPipe for the software-pipelined kernels
 // (k_rhs_res_z, k_ir_step_z), which favour fewer, longer chunks; kZColumn, whole columns (512
-// blocks at 512^3, all resident at once) for the FP64 two-cell CG kernels.
+// blocks at 512^3, all resident at once) for the CG kernels with 16-byte accesses (FP64 pairs,
+// FP32 quads).
 enum : int { kZDefault = 0, kZPipe = 1, kZColumn = 2 };
 inline Dims zdims(const Dims &g, int cls) {
     Dims d = g;
@@ -72,6 +76,8 @@ inline Dims zdims(const Dims &g, int cls) {
     d.ztiles = l.ztiles;
     d.zchunk2 = l.zchunk2;
     d.ztiles2 = l.ztiles2;
+    d.zchunk4 = l.zchunk4;
+    d.ztiles4 = l.ztiles4;
     d.zblk = l.zblk;
     return d;
 }

@@ -849,46 +849,63 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     block_partials<3>(acc, partials);
 }
 
-// ---- CG kernels with two x-neighbouring cells per thread (nx even) --------------------------
+// ---- CG kernels with W x-neighbouring cells per thread (nx a multiple of W) ----------------
 //
 // FP32 arrays moved one float per lane per instruction run well below the FP64 kernels'
-// bandwidth; with pairs every lane moves 8 bytes per access (16 for FP64).  A block of kZTX x kZTY threads
-// owns 2 kZTX x kZTY columns (tiles numbered as above, g.ztx2 per row, chunks of g.zchunk2
-// planes).  The LDS plane holds pairs: the x neighbours of a pair (x, x+1) are the .y of the
-// pair on its left and the .x of the pair on its right.  Sums per cell are the scalar kernels'
-// (same terms, same order).
+// bandwidth.  With W cells per thread every lane moves W values per access: 16 bytes for FP64
+// pairs (W = 2) and FP32 quads (W = 4), 8 for FP32 pairs.  A block of kZTX x kZTY threads owns
+// W kZTX x kZTY columns (tiles numbered as above, ztxW per row, chunks of zchunkW planes).  The
+// LDS plane holds W-tuples: the x neighbours of a tuple are the last cell of the tuple on its
+// left and the first of the one on its right.  Sums per cell are the scalar kernels' (same
+// terms, same order).
 
-template <typename T>
-using vec2 = T __attribute__((ext_vector_type(2)));
+template <typename T, int W>
+using vecw = T __attribute__((ext_vector_type(W)));
 
-struct ZCol2 {
-    uint32_t col, x, y, z0, z1;   // x: the even (left) cell a of the pair; b = x + 1
+// the W-cell tiling of a Dims
+template <int W>
+struct TileW {
+    static __device__ __forceinline__ uint32_t ztx(const Dims &g) {
+        return W == 2 ? g.ztx2 : g.ztx4;
+    }
+    static __device__ __forceinline__ uint32_t zchunk(const Dims &g) {
+        return W == 2 ? g.zchunk2 : g.zchunk4;
+    }
+    static __device__ __forceinline__ uint32_t ztiles(const Dims &g) {
+        return W == 2 ? g.ztiles2 : g.ztiles4;
+    }
+};
+
+struct ZColW {
+    uint32_t col, x, y, z0, z1;   // x: the first cell of the tuple
     uint32_t bx, by, lx, ly;
     bool active, zvalid;
     bool hym, hyp, wym, wyp;
     int32_t oym, oyp;
-    bool hxm_a, wxm_a;            // a's x- neighbour exists / is the periodic wrap (x == 0)
-    bool hxp_b, wxp_b;            // b's x+ neighbour exists / is the wrap (x + 2 == nx)
-    int32_t oxm_a, oxp_b;         // offsets from a's index of a's x- and b's x+ neighbours
+    bool hxm_a, wxm_a;            // the first cell's x- neighbour exists / is the wrap (x == 0)
+    bool hxp_b, wxp_b;            // the last cell's x+ neighbour exists / is the wrap
+    int32_t oxm_a, oxp_b;         // offsets from the first cell's index of those neighbours
 };
 
-__device__ __forceinline__ ZCol2 zcolumn2(const Dims &g) {
-    ZCol2 c;
+template <int W>
+__device__ __forceinline__ ZColW zcolumnw(const Dims &g) {
+    ZColW c;
     const uint32_t h = blockIdx.x;
     const uint32_t lt = (h & 7u) * (g.zblk >> 3) + (h >> 3);
-    const uint32_t per_plane = g.ztx2 * g.zty;
+    const uint32_t tx = TileW<W>::ztx(g);
+    const uint32_t per_plane = tx * g.zty;
     const uint32_t zc = lt / per_plane;
     const uint32_t rem = lt - zc * per_plane;
-    c.by = rem / g.ztx2;
-    c.bx = rem - c.by * g.ztx2;
+    c.by = rem / tx;
+    c.bx = rem - c.by * tx;
     c.lx = threadIdx.x % kZTX;
     c.ly = threadIdx.x / kZTX;
-    c.x = c.bx * (2 * kZTX) + 2 * c.lx;
+    c.x = c.bx * (W * kZTX) + W * c.lx;
     c.y = c.by * kZTY + c.ly;
-    c.z0 = zc * g.zchunk2;
-    c.z1 = c.z0 + g.zchunk2 < g.nz ? c.z0 + g.zchunk2 : g.nz;
-    c.zvalid = lt < g.ztiles2 && c.z0 < g.nz;
-    c.active = c.zvalid && c.x < g.nx && c.y < g.ny;   // nx even: x + 1 < nx as well
+    c.z0 = zc * TileW<W>::zchunk(g);
+    c.z1 = c.z0 + TileW<W>::zchunk(g) < g.nz ? c.z0 + TileW<W>::zchunk(g) : g.nz;
+    c.zvalid = lt < TileW<W>::ztiles(g) && c.z0 < g.nz;
+    c.active = c.zvalid && c.x < g.nx && c.y < g.ny;   // nx % W == 0: x + W - 1 < nx as well
     c.col = c.x + g.nx * c.y;
     c.hym = c.y > 0 || g.per[1];
     c.hyp = c.y + 1 < g.ny || g.per[1];
@@ -899,13 +916,14 @@ __device__ __forceinline__ ZCol2 zcolumn2(const Dims &g) {
     c.wxm_a = c.x == 0;
     c.hxm_a = c.x > 0 || g.per[0];
     c.oxm_a = c.wxm_a ? (int32_t)(g.nx - 1) : -1;
-    c.wxp_b = c.x + 2 == g.nx;
-    c.hxp_b = c.x + 2 < g.nx || g.per[0];
-    c.oxp_b = c.wxp_b ? -(int32_t)(g.nx - 2) : 2;
+    c.wxp_b = c.x + W == g.nx;
+    c.hxp_b = c.x + W < g.nx || g.per[0];
+    c.oxp_b = c.wxp_b ? -(int32_t)(g.nx - W) : W;
     return c;
 }
 
-__device__ __forceinline__ Ring ring_cell2(const Dims &g, const ZCol2 &c) {
+template <int W>
+__device__ __forceinline__ Ring ring_cellw(const Dims &g, const ZColW &c) {
     Ring r{0, kNoRing};
     const int t = (int)threadIdx.x;
     const int TX = (int)kZTX, TY = (int)kZTY;
@@ -922,20 +940,20 @@ __device__ __forceinline__ Ring ring_cell2(const Dims &g, const ZCol2 &c) {
     }
     const int64_t px = (int64_t)c.bx * TX + lx - 1, gy = (int64_t)c.by * TY + ly - 1;
     r.slot = ly * (TX + 2) + lx;
-    if (px >= 0 && 2 * px < (int64_t)g.nx && gy >= 0 && gy < (int64_t)g.ny)
-        r.col = (uint32_t)(2 * px) + g.nx * (uint32_t)gy;
+    if (px >= 0 && W * px < (int64_t)g.nx && gy >= 0 && gy < (int64_t)g.ny)
+        r.col = (uint32_t)(W * px) + g.nx * (uint32_t)gy;
     return r;
 }
 
-// zmarch for pairs: X(J) gives the operand pair at the even cell J; own(J) a pair of the
-// cell's own (loads shared with X's, carried in registers as in zmarch_own).
-template <typename T, class XF, class OF, class CF>
-__device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<vec2<T>> &sp,
-                                        XF X, OF own, CF cell) {
-    using T2 = vec2<T>;
-    const Ring rg = ring_cell2(g, c);
+// zmarch for W-tuples: X(J) gives the operand tuple at the first cell J; own(J) a tuple of the
+// cells' own (loads shared with X's, carried in registers as in zmarch_own).
+template <typename T, int W, class XF, class OF, class CF>
+__device__ __forceinline__ void zmarchw(const Dims &g, const ZColW &c,
+                                        PlaneBuf<vecw<T, W>> &sp, XF X, OF own, CF cell) {
+    using TW = vecw<T, W>;
+    const Ring rg = ring_cellw<W>(g, c);
     const bool per_z = g.per[2];
-    T2 xz_m = (T2)(T)0, xi = (T2)(T)0, oi = (T2)(T)0;
+    TW xz_m = (TW)(T)0, xi = (TW)(T)0, oi = (TW)(T)0;
     if (c.active) {
         if (c.z0 > 0 || per_z) xz_m = X(zm_index(g, c.col, c.z0));
         const uint32_t J0 = c.col + g.nxny * c.z0;
@@ -950,7 +968,7 @@ __device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<
     uint32_t I = c.col + g.nxny * c.z0;
     for (uint32_t z = c.z0; z < c.z1; ++z, I += g.nxny) {
         const bool more = z + 1 < c.z1;
-        T2 xz_p = (T2)(T)0, rnext = (T2)(T)0, op = (T2)(T)0;
+        TW xz_p = (TW)(T)0, rnext = (TW)(T)0, op = (TW)(T)0;
         if (c.active && (z + 1 < g.nz || per_z)) {
             const uint32_t J = zp_index(g, c.col, z);
             xz_p = X(J);
@@ -959,7 +977,7 @@ __device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<
         if (more && rg.col != kNoRing) rnext = X(rg.col + g.nxny * (z + 1));
         __syncthreads();   // plane z is staged in sp.v[buf]
         if (c.active) {
-            const InPlane<T2> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
+            const InPlane<TW> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
             cell(I, z, xi, xz_m, xz_p, nb, oi);
         }
         if (more) {
@@ -973,134 +991,129 @@ __device__ __forceinline__ void zmarch2(const Dims &g, const ZCol2 &c, PlaneBuf<
     }
 }
 
-template <typename T>
-__device__ __forceinline__ vec2<T> ld2(const T *p) { return *(const vec2<T> *)p; }
-template <typename T>
-__device__ __forceinline__ vec2<T> ldnt2(const T *p) {
-    return __builtin_nontemporal_load((const vec2<T> *)p);
+template <typename T, int W>
+__device__ __forceinline__ vecw<T, W> ldw(const T *p) { return *(const vecw<T, W> *)p; }
+template <typename T, int W>
+__device__ __forceinline__ vecw<T, W> ldntw(const T *p) {
+    return __builtin_nontemporal_load((const vecw<T, W> *)p);
 }
-template <typename T>
-__device__ __forceinline__ void stnt2(T *p, vec2<T> v) {
-    __builtin_nontemporal_store(v, (vec2<T> *)p);
+template <typename T, int W>
+__device__ __forceinline__ void stntw(T *p, vecw<T, W> v) {
+    __builtin_nontemporal_store(v, (vecw<T, W> *)p);
 }
 
-// q = A^ p for the pair (a, b) = (p.x, p.y), the scalar ahat's terms and order per cell; P1
-// reads the operand of one (wrap) cell from memory, P2 the pair at an even cell.
-template <typename T, class P1F, class P2F>
-__device__ __forceinline__ vec2<T> ahat2(const Dims &g, const ZCol2 &c, uint32_t I, uint32_t z,
-                                         vec2<T> pi, vec2<T> pz_m, vec2<T> pz_p, vec2<T> hz_m,
-                                         vec2<T> hzc, const T *__restrict__ hx,
-                                         const T *__restrict__ hy, const InPlane<vec2<T>> &nb,
-                                         P1F P1, P2F P2) {
-    using T2 = vec2<T>;
+// q = A^ p for the tuple p, the scalar ahat's terms and order per cell; P1 reads the operand of
+// one (wrap) cell from memory, PW the tuple at a first cell.
+template <typename T, int W, class P1F, class PWF>
+__device__ __forceinline__ vecw<T, W> ahatw(const Dims &g, const ZColW &c, uint32_t I, uint32_t z,
+                                            vecw<T, W> pi, vecw<T, W> pz_m, vecw<T, W> pz_p,
+                                            vecw<T, W> hz_m, vecw<T, W> hzc,
+                                            const T *__restrict__ hx, const T *__restrict__ hy,
+                                            const InPlane<vecw<T, W>> &nb, P1F P1, PWF PW) {
+    using TW = vecw<T, W>;
     const bool per_z = g.per[2];
     const bool hzm = z > 0 || per_z, hzp = z + 1 < g.nz || per_z;
-    const T2 hx2 = ld2(&hx[I]);           // a-b coupling, b-(b+1) coupling
+    const TW hxw = ldw<T, W>(&hx[I]);     // coupling of each cell to its x+ neighbour
     // absent neighbours contribute coefficient * 0 (coefficients read at I); wrap neighbours
     // are read from memory
-    T2 vym = (T2)(T)0, vyp = (T2)(T)0;
+    TW vym = (TW)(T)0, vyp = (TW)(T)0;
     T vxm = (T)0, vxp = (T)0;
-    if (c.hym) vym = c.wym ? P2(I + c.oym) : nb.ym();
-    if (c.hyp) vyp = c.wyp ? P2(I + c.oyp) : nb.yp();
-    if (c.hxm_a) vxm = c.wxm_a ? P1(I + c.oxm_a) : nb.xm().y;
-    if (c.hxp_b) vxp = c.wxp_b ? P1(I + c.oxp_b) : nb.xp().x;
-    const T2 zm = hzm ? pz_m : (T2)(T)0, zp = hzp ? pz_p : (T2)(T)0;
-    const T2 cym = ld2(&hy[c.hym ? I + c.oym : I]), cyp = ld2(&hy[I]);
+    if (c.hym) vym = c.wym ? PW(I + c.oym) : nb.ym();
+    if (c.hyp) vyp = c.wyp ? PW(I + c.oyp) : nb.yp();
+    if (c.hxm_a) vxm = c.wxm_a ? P1(I + c.oxm_a) : nb.xm()[W - 1];
+    if (c.hxp_b) vxp = c.wxp_b ? P1(I + c.oxp_b) : nb.xp()[0];
+    const TW zm = hzm ? pz_m : (TW)(T)0, zp = hzp ? pz_p : (TW)(T)0;
+    const TW cym = ldw<T, W>(&hy[c.hym ? I + c.oym : I]), cyp = ldw<T, W>(&hy[I]);
     const T cxm = hx[c.hxm_a ? I + c.oxm_a : I];
-    T sa = (T)0, sb = (T)0;
-    sa = sa + hz_m.x * zm.x;
-    sb = sb + hz_m.y * zm.y;
-    sa = sa + cym.x * vym.x;
-    sb = sb + cym.y * vym.y;
-    // x-: a's left neighbour, b's is a; x+: a's right neighbour is b, b's is the next pair's a
-    sa = sa + cxm * vxm;
-    sb = sb + hx2.x * pi.x;
-    sa = sa + hx2.x * pi.y;
-    sb = sb + hx2.y * vxp;
-    sa = sa + cyp.x * vyp.x;
-    sb = sb + cyp.y * vyp.y;
-    sa = sa + hzc.x * zp.x;
-    sb = sb + hzc.y * zp.y;
-    T2 q;
-    q.x = pi.x - sa;
-    q.y = pi.y - sb;
+    TW q;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        T s = (T)0;
+        s = s + hz_m[k] * zm[k];
+        s = s + cym[k] * vym[k];
+        // x-: the first cell's left neighbour, else the previous cell (coupling stored there);
+        // x+: the next cell, or the right neighbour of the last one
+        s = s + (k == 0 ? cxm : hxw[k > 0 ? k - 1 : 0]) * (k == 0 ? vxm : pi[k > 0 ? k - 1 : 0]);
+        s = s + hxw[k] * (k == W - 1 ? vxp : pi[k < W - 1 ? k + 1 : 0]);
+        s = s + cyp[k] * vyp[k];
+        s = s + hzc[k] * zp[k];
+        q[k] = pi[k] - s;
+    }
     return q;
 }
 
-template <typename T, bool First>
+template <typename T, int W, bool First>
 __global__ void __launch_bounds__(kZTX * kZTY)
-k_cg_a_z2(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
+k_cg_a_zw(Dims g, const T *__restrict__ r, const T *__restrict__ pold, T *__restrict__ pnew,
           T *__restrict__ q, T *__restrict__ y, int32_t yfirst, const T *__restrict__ hx,
           const T *__restrict__ hy, const T *__restrict__ hz, Scal *sc, double *partials) {
-    using T2 = vec2<T>;
-    __shared__ PlaneBuf<T2> sp;
+    using TW = vecw<T, W>;
+    __shared__ PlaneBuf<TW> sp;
     if (sc->done) return;
     const T beta = (T)sc->beta;
     const T alpha = (T)sc->alpha;
-    auto own = [&](uint32_t J) -> T2 { return First ? (T2)(T)0 : ld2(&pold[J]); };
     auto P1 = [&](uint32_t J) -> T { return First ? r[J] : (T)(r[J] + beta * pold[J]); };
-    auto P2 = [&](uint32_t J) -> T2 {
-        if (First) return ld2(&r[J]);
-        const T2 a = ld2(&r[J]), b = ld2(&pold[J]);
-        T2 v;
-        v.x = a.x + beta * b.x;
-        v.y = a.y + beta * b.y;
+    auto PW = [&](uint32_t J) -> TW {
+        if (First) return ldw<T, W>(&r[J]);
+        const TW a = ldw<T, W>(&r[J]), b = ldw<T, W>(&pold[J]);
+        TW v;
+#pragma unroll
+        for (int k = 0; k < W; ++k) v[k] = a[k] + beta * b[k];
         return v;
     };
+    auto own = [&](uint32_t J) -> TW { return First ? (TW)(T)0 : ldw<T, W>(&pold[J]); };
     double acc[2] = {0.0, 0.0};
-    const ZCol2 c = zcolumn2(g);
-    T2 hz_m = (T2)(T)0;
-    if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ld2(&hz[zm_index(g, c.col, c.z0)]);
-    zmarch2<T>(g, c, sp, P2, own, [&](uint32_t I, uint32_t z, T2 pi, T2 pz_m, T2 pz_p,
-                                      const InPlane<T2> &nb, T2 po) {
-        const T2 hzc = ldnt2(&hz[I]);
-        const T2 qi = ahat2<T>(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
+    const ZColW c = zcolumnw<W>(g);
+    TW hz_m = (TW)(T)0;
+    if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ldw<T, W>(&hz[zm_index(g, c.col, c.z0)]);
+    zmarchw<T, W>(g, c, sp, PW, own, [&](uint32_t I, uint32_t z, TW pi, TW pz_m, TW pz_p,
+                                         const InPlane<TW> &nb, TW po) {
+        const TW hzc = ldntw<T, W>(&hz[I]);
+        const TW qi = ahatw<T, W>(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, PW);
         if (!First) {
-            stnt2(&pnew[I], pi);
-            stnt2(&q[I], qi);
+            stntw<T, W>(&pnew[I], pi);
+            stntw<T, W>(&q[I], qi);
             // iteration k-1's term of the solution (alpha p_old), as k_cg_a_z
-            T2 yi;
-            if (yfirst) {
-                yi.x = alpha * po.x;
-                yi.y = alpha * po.y;
-            } else {
-                yi = ldnt2(&y[I]);
-                yi.x = yi.x + alpha * po.x;
-                yi.y = yi.y + alpha * po.y;
-            }
-            stnt2(&y[I], yi);
+            TW yi = yfirst ? (TW)(T)0 : ldntw<T, W>(&y[I]);
+#pragma unroll
+            for (int k = 0; k < W; ++k) yi[k] = yfirst ? alpha * po[k] : yi[k] + alpha * po[k];
+            stntw<T, W>(&y[I], yi);
         }
-        acc[0] += (double)pi.x * (double)qi.x + (double)pi.y * (double)qi.y;
-        acc[1] += (double)qi.x * (double)qi.x + (double)qi.y * (double)qi.y;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            acc[0] += (double)pi[k] * (double)qi[k];
+            acc[1] += (double)qi[k] * (double)qi[k];
+        }
         hz_m = hzc;
     });
     block_partials<2>(acc, partials);
 }
 
-template <typename T>
+template <typename T, int W>
 __global__ void __launch_bounds__(kZTX * kZTY)
-k_cg_b1_z2(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *__restrict__ hy,
+k_cg_b1_zw(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *__restrict__ hy,
            const T *__restrict__ hz, T *__restrict__ rw, Scal *sc, double *partials) {
-    using T2 = vec2<T>;
-    __shared__ PlaneBuf<T2> sp;
+    using TW = vecw<T, W>;
+    __shared__ PlaneBuf<TW> sp;
     if (sc->done) return;
     const T alpha = (T)sc->alpha;
     auto P1 = [&](uint32_t J) -> T { return r[J]; };
-    auto P2 = [&](uint32_t J) -> T2 { return ld2(&r[J]); };
+    auto PW = [&](uint32_t J) -> TW { return ldw<T, W>(&r[J]); };
+    auto own = [](uint32_t) -> TW { return (TW)(T)0; };
     double acc[1] = {0.0};
-    const ZCol2 c = zcolumn2(g);
-    T2 hz_m = (T2)(T)0;
-    if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ld2(&hz[zm_index(g, c.col, c.z0)]);
-    auto own = [](uint32_t) -> T2 { return (T2)(T)0; };
-    zmarch2<T>(g, c, sp, P2, own, [&](uint32_t I, uint32_t z, T2 pi, T2 pz_m, T2 pz_p,
-                                      const InPlane<T2> &nb, T2) {
-        const T2 hzc = ldnt2(&hz[I]);
-        const T2 qi = ahat2<T>(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, P2);
-        T2 ri;
-        ri.x = pi.x - alpha * qi.x;
-        ri.y = pi.y - alpha * qi.y;
-        stnt2(&rw[I], ri);
-        acc[0] += (double)ri.x * (double)ri.x + (double)ri.y * (double)ri.y;
+    const ZColW c = zcolumnw<W>(g);
+    TW hz_m = (TW)(T)0;
+    if (c.active && (c.z0 > 0 || g.per[2])) hz_m = ldw<T, W>(&hz[zm_index(g, c.col, c.z0)]);
+    zmarchw<T, W>(g, c, sp, PW, own, [&](uint32_t I, uint32_t z, TW pi, TW pz_m, TW pz_p,
+                                         const InPlane<TW> &nb, TW) {
+        const TW hzc = ldntw<T, W>(&hz[I]);
+        const TW qi = ahatw<T, W>(g, c, I, z, pi, pz_m, pz_p, hz_m, hzc, hx, hy, nb, P1, PW);
+        TW ri;
+#pragma unroll
+        for (int k = 0; k < W; ++k) ri[k] = pi[k] - alpha * qi[k];
+        stntw<T, W>(&rw[I], ri);
+#pragma unroll
+        for (int k = 0; k < W; ++k) acc[0] += (double)ri[k] * (double)ri[k];
         hz_m = hzc;
     });
     block_partials<1>(acc, partials);
@@ -1189,21 +1202,23 @@ void launch_cg_a(const Dims &g, bool first, const T *r, const T *pold, T *pnew, 
                  bool yfirst, const T *hx, const T *hy, const T *hz, Scal *sc, double *partials,
                  hipStream_t st) {
     const int32_t yf = yfirst ? 1 : 0;
+#define HF_CGA(...)                                                                          \
+    HF_ZLAUNCH((__VA_ARGS__), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc, partials)
+    if constexpr (std::is_same<T, float>::value) {
+        if (g.quads) {
+            if (first) HF_CGA(k_cg_a_zw<T, 4, true>);
+            else HF_CGA(k_cg_a_zw<T, 4, false>);
+            return;
+        }
+    }
     if (g.pairs) {
-        if (first)
-            HF_ZLAUNCH((k_cg_a_z2<T, true>), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc,
-                       partials);
-        else
-            HF_ZLAUNCH((k_cg_a_z2<T, false>), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc,
-                       partials);
+        if (first) HF_CGA(k_cg_a_zw<T, 2, true>);
+        else HF_CGA(k_cg_a_zw<T, 2, false>);
         return;
     }
-    if (first)
-        HF_ZLAUNCH((k_cg_a_z<T, true>), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc,
-                   partials);
-    else
-        HF_ZLAUNCH((k_cg_a_z<T, false>), g, st, g, r, pold, pnew, q, y, yf, hx, hy, hz, sc,
-                   partials);
+    if (first) HF_CGA(k_cg_a_z<T, true>);
+    else HF_CGA(k_cg_a_z<T, false>);
+#undef HF_CGA
 }
 template void launch_cg_a<double>(const Dims &, bool, const double *, const double *, double *,
                                   double *, double *, bool, const double *, const double *,
@@ -1215,8 +1230,14 @@ template void launch_cg_a<float>(const Dims &, bool, const float *, const float 
 template <typename T>
 void launch_cg_b1(const Dims &g, const T *r, const T *hx, const T *hy, const T *hz, T *rw,
                   Scal *sc, double *partials, hipStream_t st) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (g.quads) {
+            HF_ZLAUNCH((k_cg_b1_zw<T, 4>), g, st, g, r, hx, hy, hz, rw, sc, partials);
+            return;
+        }
+    }
     if (g.pairs) {
-        HF_ZLAUNCH(k_cg_b1_z2<T>, g, st, g, r, hx, hy, hz, rw, sc, partials);
+        HF_ZLAUNCH((k_cg_b1_zw<T, 2>), g, st, g, r, hx, hy, hz, rw, sc, partials);
         return;
     }
     HF_ZLAUNCH(k_cg_b1_z<T>, g, st, g, r, hx, hy, hz, rw, sc, partials);

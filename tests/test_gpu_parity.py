@@ -390,19 +390,25 @@ def test_first_iteration_stop_is_exact(prec):
 
 @pytest.mark.parametrize("prec", ["fp32_ir", "fp64"])
 def test_pair_kernels_match_scalar(monkeypatch, prec):
-    """With an even nx the CG kernels (FP32 inner solve, FP64 solve) take two x-neighbouring
-    cells per thread; with HF_NO_PAIRS set they take one.  The sums per cell are the same;
-    only the order of the per-block partial sums differs, so both runs meet the tolerance and
-    agree to it (log-normal kappa, Gaussian heating, 24^3 and a periodic 8 x 6 x 5 grid)."""
+    """The CG kernels take several x-neighbouring cells per thread: four in FP32 when nx is a
+    multiple of 4 (HF_NO_QUADS: two), two in FP64 and in FP32 when nx is even (HF_NO_PAIRS:
+    one).  The sums per cell are the same; only the order of the per-block partial sums
+    differs, so all runs meet the tolerance and agree to it (log-normal kappa, Gaussian
+    heating, 24^3 and a periodic 8 x 6 x 5 grid)."""
+    variants = [{}, {"HF_NO_QUADS": "1"}, {"HF_NO_PAIRS": "1"}]
+    if prec == "fp64":
+        variants = variants[1:]   # no FP64 quads
     for s in (workloads.c3(24)[0], random_system(31, 8, 6, 5, periodic=(True, False, True))):
         out = []
-        for no_pairs in (False, True):
-            if no_pairs:
-                monkeypatch.setenv("HF_NO_PAIRS", "1")
-            else:
-                monkeypatch.delenv("HF_NO_PAIRS", raising=False)
+        for env in variants:
+            for k in ("HF_NO_QUADS", "HF_NO_PAIRS"):
+                if k in env:
+                    monkeypatch.setenv(k, env[k])
+                else:
+                    monkeypatch.delenv(k, raising=False)
             with Simulation(s, precision=prec, tol=1e-14) as sim:
                 sim.initial_evolve()
                 sim.run(1, 10)
                 out.append(sim.get_state()[0])
-        assert relerr(out[0], out[1]) <= 1e-12
+        for o in out[1:]:
+            assert relerr(out[0], o) <= 1e-12

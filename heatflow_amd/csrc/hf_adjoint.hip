@@ -86,13 +86,14 @@ k_adj_step(Dims g, AdjParams q, const double *__restrict__ T, const double *__re
            const double *__restrict__ rhoc, const double *__restrict__ cvec,
            const double *__restrict__ vol, double *__restrict__ g_lo,
            double *__restrict__ g_new, double *__restrict__ gk, double *__restrict__ gp) {
+    // cell lengths are uniform per axis (checked when the adjoint starts): read once
+    const double Ls[3] = {lx[0], ly[0], lz[0]};
     HF_FOR_CELLS(g, I) {
         Nbr n = neighbours(g, I);
         const double li = lam[I], ti = T[I], ki = kappa[I];
         double acc = 0.0;
         const bool has[6] = {n.hxm, n.hxp, n.hym, n.hyp, n.hzm, n.hzp};
         const uint32_t nb[6] = {n.xm, n.xp, n.ym, n.yp, n.zm, n.zp};
-        const double Ls[3] = {lx[I], ly[I], lz[I]};
         const uint32_t pos[3] = {n.x, n.y, n.z};
         const uint32_t len[3] = {g.nx, g.ny, g.nz};
         // which boundary faces still contribute to B_I after CG assignments (mod_boundary.f90:72-150)

@@ -95,7 +95,8 @@ static void tend(hf_ctx *c, long slot) {
     if (slot >= 0) (void)hipEventRecord(c->tpool[slot].b, c->st);
 }
 
-static void make_dims(Dims &g, const hf_params &p) {
+// cus: compute units of the device (resident blocks per launch round scale with it)
+static void make_dims(Dims &g, const hf_params &p, int cus) {
     g.nx = (uint32_t)p.nx; g.ny = (uint32_t)p.ny; g.nz = (uint32_t)p.nz;
     g.nxny = g.nx * g.ny;
     g.N = g.nxny * g.nz;
@@ -127,14 +128,37 @@ static void make_dims(Dims &g, const hf_params &p) {
     g.ztx4 = (g.nx + 4 * tx - 1) / (4 * tx);
     g.pairs = (g.nx % 2 == 0) && getenv("HF_NO_PAIRS") == nullptr;
     g.quads = g.pairs && (g.nx % 4 == 0) && getenv("HF_NO_QUADS") == nullptr;
-    // chunk length and tile count for ~zcap blocks, tiles of `ztx` x zty per plane
-    auto chunking = [&](uint32_t zcap, uint32_t ztx, uint32_t *zchunk, uint32_t *ztiles) {
+    // Chunk length and tile count for about zcap blocks, tiles of `ztx` x zty per plane.  A
+    // launch runs in rounds of `slots` resident blocks, so its time goes about as rounds x
+    // (planes per chunk + 2 prologue planes).  The zcap chunking is kept unless a chunk count
+    // within a factor 2 of it costs 5% less by that measure: a tile count just past a multiple
+    // of the slots leaves a round nearly empty (at 384^3 whole columns gave 576 blocks for 512
+    // slots).
+    auto chunking = [&](uint32_t zcap, uint64_t slots, uint32_t ztx, uint32_t *zchunk,
+                        uint32_t *ztiles) {
         const uint64_t per_plane = (uint64_t)ztx * g.zty;
-        uint64_t nzc = std::max<uint64_t>(1, (zcap + per_plane - 1) / per_plane);
-        nzc = std::min<uint64_t>(nzc, g.nz);
-        *zchunk = (uint32_t)((g.nz + nzc - 1) / nzc);
-        nzc = (g.nz + *zchunk - 1) / *zchunk;
-        *ztiles = (uint32_t)(per_plane * nzc);
+        auto chunks = [&](uint64_t n) {   // chunk count actually needed for ~n chunks
+            const uint64_t zc = (g.nz + n - 1) / n;
+            return (g.nz + zc - 1) / zc;
+        };
+        auto cost = [&](uint64_t nn) {
+            const uint64_t zc = (g.nz + nn - 1) / nn;
+            return ((per_plane * nn + slots - 1) / slots) * (zc + 2);
+        };
+        const uint64_t n0 = chunks(std::min<uint64_t>(
+            g.nz, std::max<uint64_t>(1, (zcap + per_plane - 1) / per_plane)));
+        const uint64_t hi = std::max<uint64_t>(1, std::min<uint64_t>(g.nz, 2 * n0));
+        const uint64_t lo = std::min<uint64_t>(hi, std::max<uint64_t>(1, n0 / 2));
+        uint64_t best = n0, best_cost = cost(n0);
+        for (uint64_t n = lo; n <= hi; ++n) {
+            const uint64_t nn = chunks(n);
+            if (cost(nn) * 20 < best_cost * 19) {
+                best = nn;
+                best_cost = cost(nn);
+            }
+        }
+        *zchunk = (uint32_t)((g.nz + best - 1) / best);
+        *ztiles = (uint32_t)(per_plane * best);
     };
     auto env_cap = [](const char *name, uint32_t dflt) {
         const char *e = getenv(name);
@@ -142,11 +166,14 @@ static void make_dims(Dims &g, const hf_params &p) {
     };
     const uint32_t caps[3] = {env_cap("HF_ZBLK", 4096), env_cap("HF_ZBLK_PIPE", 2048),
                               env_cap("HF_ZBLK_COL", 512)};
+    // resident 512-thread blocks per CU: ~4 for the default class, 2 for the pipelined kernels
+    // (128 VGPRs) and the 16-byte CG kernels (~90 VGPRs)
+    const uint64_t slots[3] = {4ull * (uint64_t)cus, 2ull * (uint64_t)cus, 2ull * (uint64_t)cus};
     for (int k = 0; k < 3; ++k) {
         Dims::ZLayout &l = g.zlay[k];
-        chunking(caps[k], g.ztx, &l.zchunk, &l.ztiles);
-        chunking(caps[k], g.ztx2, &l.zchunk2, &l.ztiles2);
-        chunking(caps[k], g.ztx4, &l.zchunk4, &l.ztiles4);
+        chunking(caps[k], slots[k], g.ztx, &l.zchunk, &l.ztiles);
+        chunking(caps[k], slots[k], g.ztx2, &l.zchunk2, &l.ztiles2);
+        chunking(caps[k], slots[k], g.ztx4, &l.zchunk4, &l.ztiles4);
         uint32_t zt = l.ztiles;
         if (g.pairs) zt = std::max(zt, l.ztiles2);
         if (g.quads) zt = std::max(zt, l.ztiles4);
@@ -266,7 +293,13 @@ extern "C" int hf_create(hf_ctx **out, const hf_params *p) {
     hf_ctx *c = new hf_ctx();
     c->p = *p;
     hipGetDevice(&c->dev);
-    make_dims(c->g, c->p);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
+        cus < 1) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    make_dims(c->g, c->p, cus);
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
         std::string m = "hf_create: hipStreamCreate failed";

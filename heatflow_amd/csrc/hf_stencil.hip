@@ -11,8 +11,9 @@
 // The kernels with many inputs per cell (k_rhs_res_z, k_ir_step_z) are software-pipelined: the
 // next plane's loads are issued before the barrier of the current one (zmarch_pipe).
 // Tiles are numbered x fastest, then y, then z-chunk; hardware block h runs logical tile
-// (h % 8) * (zblk / 8) + h / 8, so each XCD owns a contiguous range of tiles (whole z-chunks)
-// and the ring rows of neighbouring tiles come from the same L2.
+// (h % 8) * (T / 8) + h / 8 (T: the kernel's tile count rounded up to a multiple of 8), so
+// each XCD owns a contiguous range of tiles (whole z-chunks) and the ring rows of neighbouring
+// tiles come from the same L2.
 //
 // Sums run in the ascending-column order of the reference's serial SpMV (z-, y-, x-, diag,
 // x+, y+, z+; periodic wrap terms last), the same as the per-cell formulation.
@@ -40,10 +41,18 @@ struct ZCol {
     bool wxm, wxp, wym, wyp;      // the neighbour is a periodic wrap (larger/smaller index)
 };
 
+// Logical tile of hardware block h: the kernel's own `ztiles` tiles (rounded up to a multiple
+// of 8) are spread over the 8 XCDs in contiguous ranges; blocks past them (the launch grid is
+// the largest tile count of the chunking class) get no tile.
+__device__ __forceinline__ uint32_t logical_tile(uint32_t ztiles) {
+    const uint32_t h = blockIdx.x;
+    const uint32_t tr = (ztiles + 7u) & ~7u;
+    return h < tr ? (h & 7u) * (tr >> 3) + (h >> 3) : ztiles;
+}
+
 __device__ __forceinline__ ZCol zcolumn(const Dims &g) {
     ZCol c;
-    const uint32_t h = blockIdx.x;
-    const uint32_t lt = (h & 7u) * (g.zblk >> 3) + (h >> 3);
+    const uint32_t lt = logical_tile(g.ztiles);
     const uint32_t per_plane = g.ztx * g.zty;
     const uint32_t zc = lt / per_plane;
     const uint32_t rem = lt - zc * per_plane;
@@ -890,8 +899,7 @@ struct ZColW {
 template <int W>
 __device__ __forceinline__ ZColW zcolumnw(const Dims &g) {
     ZColW c;
-    const uint32_t h = blockIdx.x;
-    const uint32_t lt = (h & 7u) * (g.zblk >> 3) + (h >> 3);
+    const uint32_t lt = logical_tile(TileW<W>::ztiles(g));
     const uint32_t tx = TileW<W>::ztx(g);
     const uint32_t per_plane = tx * g.zty;
     const uint32_t zc = lt / per_plane;

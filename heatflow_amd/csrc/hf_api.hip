@@ -826,15 +826,15 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         // vectors: neither may be the right-hand side
         float *rres = other_rvec(c, rin);
         float *rn = other_rvec(c, rin, rres);
-        // k_ir_step: read T^n, T^{n-1} (or x), y or r (FP32), s, rho c_p, tau' rho c_p,
+        // k_ir_step: read T^n, T^{n-1} (or x), y or r (FP32), rho c_p, tau' rho c_p,
         // Qdens + B, diag, 3 couplings; write x', the next FP32 residual (and this step's)
         sp.store_res = sweeps + 1 < c->last_sweeps ? 1 : 0;
         auto pass = [&]() {
             long si = tbeg(c, HF_K_IR_STEP,
-                           N * (8 * 10 + 4 + 8 + 4 + (sp.store_res ? 4 : 0)) + (xa ? N * 8 : 0));
+                           N * (8 * 9 + 4 + 8 + 4 + (sp.store_res ? 4 : 0)) + (xa ? N * 8 : 0));
             launch_ir_step(c->g, sp, xa == nullptr, c->T[0], c->T[1], xa, c->y32, rin, c->sci,
-                           c->sv, c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, rres,
-                           rn, c->sc, c->partials, c->st);
+                           c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, rres, rn,
+                           c->sc, c->partials, c->st);
             tend(c, si);
         };
         pass();

@@ -115,7 +115,7 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
 // qb = Qdens + B (per-step constant in this path)
 void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const double *tp,
                     const double *tpp, const double *xb, const float *y, const float *r0,
-                    const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
+                    const Scal *inner, const double *rhoc, const double *cvec,
                     const double *qb, const double *d, const double *cx, const double *cy,
                     const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
                     double *partials, hipStream_t st);

@@ -628,10 +628,12 @@ k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__res
     block_partials<1>(acc, partials);
 }
 
-// operand inputs of the fused step: T^n, T^{n-1} (or the refined x), s, the inner solution
-// (y, or r0 when scaled by alpha: CgY)
+// operand inputs of the fused step: T^n, T^{n-1} (or the refined x), the diagonal (s is
+// formed from it: reading s as well would cost 8 bytes a cell, the division and square root
+// are hidden under the pipelined loads) and the inner solution (y, or r0 when scaled by alpha:
+// CgY).  The cell's own record of plane z also gives it T^n, T^{n-1} and the diagonal.
 struct StepRaw {
-    double tp, b, s;
+    double tp, b, d;   // d: the diagonal, s = 1/sqrt(-d) as k_scale forms it
     float y;
 };
 
@@ -639,7 +641,7 @@ struct StepRaw {
 // tau' rho c_p, the x+ / y+ couplings and those of the x- / y- neighbours (stored at the
 // neighbour; read at the cell itself where that neighbour is absent), T^{n-1} (refined pass)
 struct StepCell {
-    double d, cz, qb, rc, cv, cx, cy, cxm, cym, tpp;
+    double cz, qb, rc, cv, cx, cy, cxm, cym, tpp;
 };
 
 // Fused step of the mixed-precision path, one pass over the grid after an inner solve:
@@ -669,8 +671,8 @@ __global__ void __launch_bounds__(kZTX * kZTY) __attribute__((amdgpu_waves_per_e
 k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             const double *__restrict__ tpp, const double *__restrict__ xb,
             const float *__restrict__ y, const float *__restrict__ r0, const Scal *inner,
-            const double *__restrict__ sv, const double *__restrict__ rhoc,
-            const double *__restrict__ cvec, const double *__restrict__ qb,
+            const double *__restrict__ rhoc, const double *__restrict__ cvec,
+            const double *__restrict__ qb,
             const double *__restrict__ d, const double *__restrict__ cx,
             const double *__restrict__ cy, const double *__restrict__ cz,
             double *__restrict__ xout, float *__restrict__ rres, float *__restrict__ rnext,
@@ -687,7 +689,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     const bool xm_in = c.x > 0, xp_in = c.x + 1 < g.nx;
     const bool ym_in = c.y > 0, yp_in = c.y + 1 < g.ny;
     auto load = [&](uint32_t J) -> StepRaw {
-        return StepRaw{tp[J], FromState ? tpp[J] : xb[J], sv[J], ysol.v[J]};
+        return StepRaw{tp[J], FromState ? tpp[J] : xb[J], d[J], ysol.v[J]};
     };
     auto form = [&](const StepRaw &r) -> D2 {
         double base;
@@ -697,7 +699,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         } else {
             base = r.b;
         }
-        const double xn = base + r.s * (double)(ysc * r.y);
+        const double xn = base + (1.0 / sqrt(-r.d)) * (double)(ysc * r.y);
         return D2(xn, xn + (xn - r.tp));
     };
     auto X = [&](uint32_t J) -> D2 { return form(load(J)); };
@@ -705,7 +707,6 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     // I must be a cell of this thread's column (active threads) or 0
     auto cload = [&](uint32_t I) -> StepCell {
         StepCell v;
-        v.d = ldnt(&d[I]);
         v.cz = ldnt(&cz[I]);
         v.qb = ldnt(&qb[I]);
         v.rc = Transient ? ldnt(&rhoc[I]) : 0.0;
@@ -718,13 +719,14 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         return v;
     };
     // H v of the cell in the reference's serial order (as hx_sum), K picks x' or x1
-    auto hsum = [&](uint32_t I, uint32_t z, const StepCell &cv, double cz_m, const D2 &xi,
-                    const D2 &xz_m, const D2 &xz_p, const D2 *nbc, auto pick) -> double {
+    auto hsum = [&](uint32_t I, uint32_t z, const StepCell &cv, double di, double cz_m,
+                    const D2 &xi, const D2 &xz_m, const D2 &xz_p, const D2 *nbc,
+                    auto pick) -> double {
         double h = 0.0;
         h = h + cz_m * (z > 0 ? pick(xz_m) : 0.0);
         h = h + cv.cym * (ym_in ? pick(nbc[-(int)(kZTX + 2)]) : 0.0);
         h = h + cv.cxm * (xm_in ? pick(nbc[-1]) : 0.0);
-        h = h + cv.d * pick(xi);
+        h = h + di * pick(xi);
         h = h + cv.cx * (xp_in ? pick(nbc[1]) : 0.0);
         h = h + cv.cy * (yp_in ? pick(nbc[(int)(kZTX + 2)]) : 0.0);
         h = h + cv.cz * (z + 1 < g.nz ? pick(xz_p) : 0.0);
@@ -804,14 +806,14 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 Sn1 = Sn;
             }
             if (isnan(Sn1)) fl_next |= FLAG_NAN_S;
-            const double h1 = hsum(I, z, cur, cz_m, xi, xz_m, xz_p, nbc, pick_a);
-            const double h2 = hsum(I, z, cur, cz_m, xi, xz_m, xz_p, nbc, pick_b);
-            const double s = 1.0 / sqrt(-cur.d);
+            const double h1 = hsum(I, z, cur, ri.d, cz_m, xi, xz_m, xz_p, nbc, pick_a);
+            const double h2 = hsum(I, z, cur, ri.d, cz_m, xi, xz_m, xz_p, nbc, pick_b);
+            const double s = 1.0 / sqrt(-ri.d);
             const float rf = (float)(-s * (Sn - h1));
             double ra = -s * (Sn1 - h2);
             if (guess) {
                 // H 1: the row sum over the same terms (absent couplings are stored as 0)
-                double h1s = cur.d;
+                double h1s = ri.d;
                 h1s = h1s + cz_m;
                 h1s = h1s + (Wrap ? cy[c.hym ? I + c.oym : I] : cur.cym) * (c.hym ? 1.0 : 0.0);
                 h1s = h1s + (Wrap ? cx[c.hxm ? I + c.oxm : I] : cur.cxm) * (c.hxm ? 1.0 : 0.0);
@@ -1268,7 +1270,7 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
 // (pipelined z-chunking; launch_fin_step reads its partials)
 void launch_ir_step(const Dims &g0, const StepParams &sp, bool from_state, const double *tp,
                     const double *tpp, const double *xb, const float *y, const float *r0,
-                    const Scal *inner, const double *sv, const double *rhoc, const double *cvec,
+                    const Scal *inner, const double *rhoc, const double *cvec,
                     const double *qb, const double *d, const double *cx, const double *cy,
                     const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
                     double *partials, hipStream_t st) {
@@ -1277,10 +1279,10 @@ void launch_ir_step(const Dims &g0, const StepParams &sp, bool from_state, const
     do {                                                                                      \
         if (wrap)                                                                             \
             HF_ZLAUNCH((k_ir_step_z<F, T, C, true>), g, st, g, sp, tp, tpp, xb, y, r0, inner,  \
-                       sv, rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);   \
+                       rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);       \
         else                                                                                  \
             HF_ZLAUNCH((k_ir_step_z<F, T, C, false>), g, st, g, sp, tp, tpp, xb, y, r0, inner, \
-                       sv, rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);   \
+                       rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);       \
     } while (0)
     const bool tr = sp.isteady == 0, cat = sp.icattaneo == 1;
     const bool wrap = g.per[0] || g.per[1] || g.per[2];

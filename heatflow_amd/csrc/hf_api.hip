@@ -254,6 +254,7 @@ static int alloc_mode(hf_ctx *c) {
         TRY(dalloc(c, &c->S, N)); TRY(dalloc(c, &c->T[2], N));
         TRY(dalloc(c, &c->rn[0], N)); TRY(dalloc(c, &c->rn[1], N));
         TRY(dalloc(c, &c->rw32, N)); TRY(dalloc(c, &c->qb, N));
+        c->qb_valid = false;
     } else {
         TRY(dalloc(c, &c->hx, N)); TRY(dalloc(c, &c->hy, N)); TRY(dalloc(c, &c->hz, N));
         TRY(dalloc(c, &c->r64, N)); TRY(dalloc(c, &c->p64[0], N)); TRY(dalloc(c, &c->p64[1], N));
@@ -496,6 +497,7 @@ extern "C" int hf_assemble(hf_ctx *c) {
     if (!c->uploaded) return set_err(c, HF_ERR_STATE, "hf_assemble: cells not uploaded");
     launch_assemble(c->g, c->ap, c->kappa, c->lx, c->ly, c->lz, c->rhoc, c->tau_s, c->d, c->cx,
                     c->cy, c->cz, c->bvec, c->sc, c->st);
+    c->qb_valid = false;
     TRY(check_launch(c, "k_assemble"));
     launch_scale(c->g, c->d, c->cx, c->cy, c->cz, c->sv, c->hx, c->hy, c->hz, c->fx, c->fy, c->fz,
                  c->st);
@@ -506,6 +508,7 @@ extern "C" int hf_assemble(hf_ctx *c) {
         hp.tb4 = 0.0;
         launch_qdens(c->g, hp, c->iheater, c->ells, c->tau_s, c->kappa /* unused: ells == 0 */,
                      c->qextra, c->qd, c->sc, c->st);
+        c->qb_valid = false;
         TRY(check_launch(c, "k_qdens"));
     }
     TRY(sync_scal(c, c->sc, c->h_sc));
@@ -712,6 +715,7 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         long sq = tbeg(c, HF_K_QDENS, N * (1 + 8 * 3 + 8 + (c->qextra ? 8 : 0)));
         launch_qdens(c->g, hp, c->iheater, c->ells, c->tau_s, c->T[0], c->qextra, c->qd, c->sc,
                      c->st);
+        c->qb_valid = false;
         tend(c, sq);
     }
     RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
@@ -803,8 +807,12 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
                                c->partials, c->st);
         tend(c, sr);
         launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
-        // Qdens and B only change through calls that end a carried run
-        launch_sum2(c->g, c->qd, c->bvec, c->qb, c->st);
+        // Qdens + B, once per change of either (they change only through calls that end a
+        // carried run)
+        if (!c->qb_valid) {
+            launch_sum2(c->g, c->qd, c->bvec, c->qb, c->st);
+            c->qb_valid = true;
+        }
         TRY(check_launch(c, "k_rhs_res"));
         rin = c->r32;
     }
@@ -900,6 +908,7 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         long sq = tbeg(c, HF_K_QDENS, N * (1 + 8 * 3 + 8 + (c->qextra ? 8 : 0)));
         launch_qdens(c->g, hp, c->iheater, c->ells, c->tau_s, c->T[0], c->qextra, c->qd, c->sc,
                      c->st);
+        c->qb_valid = false;
         tend(c, sq);
     }
     RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};

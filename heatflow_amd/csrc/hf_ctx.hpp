@@ -53,7 +53,8 @@ struct hf_ctx {
     // fused step left the next step's r^0 (and the scalars) valid for the current state;
     // `carry_r` is that r^0, formed for the TINY outcome `tiny_guess` (the last one seen)
     float *rn[2] = {nullptr, nullptr};
-    double *qb = nullptr;        // Qdens + B for the fused step (set when a carried run starts)
+    double *qb = nullptr;        // Qdens + B for the fused step
+    bool qb_valid = false;       // qb holds the current Qdens + B (cleared when either changes)
     bool carry = false;
     bool tiny_guess = true;
     int64_t last_sweeps = 1;     // refinement sweeps the last fused step needed

@@ -318,6 +318,33 @@ def test_fp32_ir_carried_residual_matches_fresh_start():
     assert relerr(ta, tb) <= 1e-12
 
 
+def test_fp32_ir_fields_changed_between_runs():
+    """The fused pass reads Qdens + B from a sum formed once per change of either.  A run, a
+    new kappa and extra heating (new B and Qdens after re-assembly), then a second run from the
+    state the first left: the second run matches the oracle built on the new fields."""
+    s, _ = workloads.c3(18)
+    n = s.params.n
+    rng = np.random.default_rng(11)
+    old = oracle.OracleSystem(s)   # keeps the first fields' arrays
+    with Simulation(s, precision="fp32_ir", tol=1e-14) as sim:
+        sim.initial_evolve()
+        sim.run(1, 6)
+        tp, tpp = sim.get_state()
+        s.cells.kappa = s.cells.kappa * rng.uniform(0.5, 2.0, n)
+        s.cells.qdens_extra = rng.uniform(0.0, 5.0e11, n)
+        sim.set_fields(s.cells.kappa, s.cells.qdens_extra)
+        sim.sparse_Hmatrix()
+        sim.run(7, 6)
+        ta, _ = sim.get_state()
+    rc, otp, _, _ = oracle.OracleSystem(s).simulate_steps(list(range(7, 13)), tp, tpp)
+    assert rc == 0
+    assert relerr(ta, otp) <= 1e-12
+    # the same steps on the first fields end far from these (a stale sum would show)
+    rc, otp_old, _, _ = old.simulate_steps(list(range(7, 13)), tp, tpp)
+    assert rc == 0
+    assert relerr(ta, otp_old) > 1e-9
+
+
 @pytest.mark.parametrize("heating", ["uniform", "gaussian"])
 def test_fp32_ir_tiny_guess(heating):
     """The fused pass forms the next step's residual for the last TINY outcome seen (first

@@ -32,6 +32,7 @@ struct AdjointState {
     double *ckpt = nullptr;   // nseg x 2 x N
     double *seg = nullptr;    // K x N
     double *g_hi = nullptr, *g_lo = nullptr, *g_new = nullptr, *lam = nullptr;
+    double *lam_prev = nullptr, *lam_next = nullptr;   // lambda of the step before; guess
     double *gk = nullptr, *gp = nullptr;
     double *probe_val = nullptr;  // device, nprobe
     int64_t *probe_idx = nullptr;
@@ -47,6 +48,7 @@ void adjoint_free(AdjointState *a) {
     if (!a) return;
     hipFree(a->ckpt); hipFree(a->seg);
     hipFree(a->g_hi); hipFree(a->g_lo); hipFree(a->g_new); hipFree(a->lam);
+    hipFree(a->lam_prev); hipFree(a->lam_next);
     hipFree(a->gk); hipFree(a->gp);
     hipFree(a->probe_val); hipFree(a->probe_idx); hipFree(a->probe_w); hipFree(a->probe_t);
     hipFree(a->fin); hipFree(a->seed);
@@ -215,8 +217,10 @@ int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_ever
     }
     if (A.N != N) {
         hipFree(A.g_hi); hipFree(A.g_lo); hipFree(A.g_new); hipFree(A.lam);
+        hipFree(A.lam_prev); hipFree(A.lam_next);
         hipFree(A.gk); hipFree(A.gp); hipFree(A.fin);
         A.g_hi = A.g_lo = A.g_new = A.lam = A.gk = A.gp = A.fin = nullptr;
+        A.lam_prev = A.lam_next = nullptr;
         A.N = N;
     }
     ATRY(ensure(c, &A.ckpt, (size_t)2 * N * A.nseg_cap));
@@ -249,6 +253,7 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
     const size_t N = c->g.N, B8 = N * 8;
     ATRY(ensure(c, &A.g_hi, N)); ATRY(ensure(c, &A.g_lo, N)); ATRY(ensure(c, &A.g_new, N));
     ATRY(ensure(c, &A.lam, N)); ATRY(ensure(c, &A.gk, N)); ATRY(ensure(c, &A.gp, N));
+    ATRY(ensure(c, &A.lam_prev, N)); ATRY(ensure(c, &A.lam_next, N));
     AHIP(c, hipMemcpyAsync(A.g_hi, seed, B8, hipMemcpyDefault, c->st));
     AHIP(c, hipMemsetAsync(A.g_lo, 0, B8, c->st));
     AHIP(c, hipMemsetAsync(A.lam, 0, B8, c->st));
@@ -269,6 +274,7 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
 
     const int64_t K = A.K, nsteps = A.nsteps, itime0 = A.itime0;
     const int64_t nseg = (nsteps + K - 1) / K;
+    int64_t solved = 0;   // adjoint solves done: lam (and lam_prev) hold their last solutions
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t b = s * K, e = std::min(nsteps, (s + 1) * K);
         ATRY(restore_state(c, A.ckpt + (size_t)2 * N * s, A.ckpt + (size_t)2 * N * s + N));
@@ -279,8 +285,23 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
         }
         for (int64_t n = e - 1; n >= b; --n) {
             int64_t it = 0;
-            // lambda^{n+1} = H^{-1} gbar^{n+1}, warm-started from lambda^{n+2}
-            ATRY(ctx_solve_dev(c, A.g_hi, A.lam, p.tol, p.itmax, &it));
+            // lambda^{n+1} = H^{-1} gbar^{n+1}.  Initial guess: 0 for the first solve,
+            // lambda^{n+2} for the second, then 2 lambda^{n+2} - lambda^{n+3} (the adjoint
+            // recurrence has the forward step's form, whose guess is 2 T^n - T^{n-1}).
+            if (solved == 0) {
+                ATRY(ctx_solve_dev(c, A.g_hi, A.lam, p.tol, p.itmax, &it));
+            } else {
+                if (solved == 1)
+                    AHIP(c, hipMemcpyAsync(A.lam_next, A.lam, B8, hipMemcpyDeviceToDevice, c->st));
+                else
+                    launch_extrap(c->g, A.lam, A.lam_prev, A.lam_next, c->st);
+                ATRY(ctx_solve_dev(c, A.g_hi, A.lam_next, p.tol, p.itmax, &it));
+                double *old_prev = A.lam_prev;
+                A.lam_prev = A.lam;
+                A.lam = A.lam_next;
+                A.lam_next = old_prev;
+            }
+            solved += 1;
             const double *Tn1 = A.seg + (size_t)N * (n - b);
             hipLaunchKernelGGL(k_adj_step, dim3(c->g.nblk), dim3(kBlock), 0, c->st, c->g, q, Tn1,
                                A.lam, c->kappa, c->lx, c->ly, c->lz, c->rhoc, c->cvec, c->vol,

@@ -601,6 +601,17 @@ k_sum2(Dims g, const double *__restrict__ a, const double *__restrict__ b,
     HF_FOR_CELLS(g, I) { stnt(&out[I], ldnt(&a[I]) + ldnt(&b[I])); }
 }
 
+// out = a + (a - b): linear extrapolation from the last two solutions (the adjoint solve's
+// initial guess; the forward step forms its guess the same way inside k_rhs_res_z)
+__global__ void __launch_bounds__(kBlock)
+k_extrap(Dims g, const double *__restrict__ a, const double *__restrict__ b,
+         double *__restrict__ out) {
+    HF_FOR_CELLS(g, I) {
+        const double ai = ldnt(&a[I]);
+        stnt(&out[I], ai + (ai - ldnt(&b[I])));
+    }
+}
+
 // TINY rule from an arbitrary state (first step after hf_set_state)
 __global__ void __launch_bounds__(kBlock)
 k_tiny_from_state(Dims g, const double *__restrict__ tp, const double *__restrict__ tpp,
@@ -735,6 +746,9 @@ void launch_step_begin(Scal *sc, hipStream_t st) {
 
 void launch_sum2(const Dims &g, const double *a, const double *b, double *out, hipStream_t st) {
     HF_LAUNCH(k_sum2, g, st, g, a, b, out);
+}
+void launch_extrap(const Dims &g, const double *a, const double *b, double *out, hipStream_t st) {
+    HF_LAUNCH(k_extrap, g, st, g, a, b, out);
 }
 
 void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,

@@ -142,5 +142,6 @@ void launch_step_begin(Scal *sc, hipStream_t st);
 void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,
                             hipStream_t st);
 void launch_sum2(const Dims &g, const double *a, const double *b, double *out, hipStream_t st);
+void launch_extrap(const Dims &g, const double *a, const double *b, double *out, hipStream_t st);
 
 }  // namespace hf

@@ -17,7 +17,7 @@ HF_ERR_ARG, HF_ERR_ALLOC, HF_ERR_HIP, HF_ERR_STATE, HF_ERR_NO_DEVICE = 10, 11, 1
 HF_PREC_FP64, HF_PREC_FP32_IR = 0, 1
 # kernel ids of hf_kernel_times
 KERNELS = ("k_qdens", "k_rhs_res<fp64>", "k_rhs_res<fp32>", "k_cg_a<fp64>", "k_cg_b<fp64>",
-           "k_cg_a<fp32>", "k_cg_b<fp32>", "k_finalize", "k_ir_res", "k_ir_step")
+           "k_cg_a<fp32>", "k_cg_b<fp32>", "k_finalize", "k_ir_res", "k_ir_step", "k_step64")
 
 # every symbol include/heatflow_hip.h declares
 EXPORTS = (

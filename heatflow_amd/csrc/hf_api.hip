@@ -260,6 +260,7 @@ static int alloc_mode(hf_ctx *c) {
         TRY(dalloc(c, &c->r64, N)); TRY(dalloc(c, &c->p64[0], N)); TRY(dalloc(c, &c->p64[1], N));
         TRY(dalloc(c, &c->rw64, N));
         TRY(dalloc(c, &c->q64, N)); TRY(dalloc(c, &c->y64, N));
+        TRY(dalloc(c, &c->r64b, N)); TRY(dalloc(c, &c->T[2], N));
     }
     return HF_OK;
 }
@@ -706,7 +707,78 @@ static CgVecs<float> cg32(hf_ctx *c, const float *rin) {
     return CgVecs<float>{rin, c->rw32, {c->p32[0], c->p32[1]}, c->q32, c->y32, c->fx, c->fy, c->fz};
 }
 
+// FP64 step with k_finalize fused into the next step's right-hand side (k_step64_z): per step
+// one solve and one fused pass; the first step after a change of state starts from k_rhs_res.
+// Results are bit-identical to step_fp64's unfused kernels.  Needs a per-step constant Qdens
+// (no time-dependent heater modes or radiation: those recompute Qdens from T^n first).  The
+// next step's residual depends on its TINY test, an OR over the grid that the pass itself
+// decides: the pass forms it for the outcome of the last test (c->tiny_guess), and a wrong guess
+// makes the next step start from k_rhs_res.
+static int step_fp64_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
+    (void)itime;
+    const hf_params &p = c->p;
+    const double N = (double)c->g.N;
+    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
+    const double *rin;
+    if (c->carry) {
+        // step start and the scalars of the carried residual in one launch
+        launch_carry_start(c->sc, nullptr, rp.tol, rp.itmax, allow_short(c), 0.0, 0, 0.0, c->st);
+        rin = c->carry_r64;
+    } else {
+        launch_step_begin(c->sc, c->st);
+        // k_rhs_res: read Tp, Tpp, rho c_p, tau' rho c_p, Qdens, B, diag, 3 couplings; write r^
+        long sr = tbeg(c, HF_K_RHS64, N * 8 * 11);
+        launch_rhs_res_state64(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec,
+                               c->d, c->cx, c->cy, c->cz, c->r64, nullptr, c->sc, c->partials,
+                               c->st);
+        tend(c, sr);
+        launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
+        TRY(check_launch(c, "k_rhs_res"));
+        rin = c->r64;
+    }
+    c->carry = false;
+    int64_t it = 0;
+    TRY(run_cg<double>(c, c->sc, c->h_sc, cg64(c, rin), &c->pred_iters, &it, HF_K_CG_A64,
+                       HF_K_CG_B64));
+    int32_t fl = c->h_sc->flags | c->setup_flags;
+    if (fl & (FLAG_NAN_B | FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S))
+        return flags_to_rc(c, fl);
+    const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bnorm2) : 0.0;
+    // the next step's right-hand side goes to the vector this solve did not read
+    double *rn = rin == c->r64 ? c->r64b : c->r64;
+    // k_step64: read Tp, Tpp, diag, y, rho c_p, tau' rho c_p, Qdens, B, 3 couplings; write x and
+    // the next r^
+    long sf = tbeg(c, HF_K_STEP64, N * 8 * (11 + 2));
+    launch_step64(c->g, rp, c->tiny_guess, c->T[0], c->T[1], c->y64, rin, c->rhoc, c->cvec,
+                  c->qd, c->bvec, c->d, c->cx, c->cy, c->cz, c->T[2], rn, c->sc, c->partials,
+                  c->st);
+    tend(c, sf);
+    launch_fin_next(c->g, c->partials, c->sc, c->tiny_guess, c->st);
+    TRY(check_launch(c, "k_step64"));
+    TRY(sync_scal(c, c->sc, c->h_sc));
+    // rotation: Temp_pp <- Temp_p, Temp_p <- x (mod_evolve.f90:169-170); the old Temp_pp buffer
+    // becomes the spare
+    double *old_tpp = c->T[1];
+    c->T[1] = c->T[0];
+    c->T[0] = c->T[2];
+    c->T[2] = old_tpp;
+    // the next step starts from the residual the pass left if its TINY case was guessed
+    const bool tiny_next = (c->h_sc->flags & FLAG_TINY_NEXT) != 0;
+    c->carry = tiny_next == c->tiny_guess;
+    if (!c->carry) c->stats.tiny_misses += 1;
+    c->tiny_guess = tiny_next;
+    c->carry_r64 = rn;
+    c->stats.steps += 1;
+    c->stats.outer_iters += it;
+    c->stats.last_err = e;
+    if (iter) *iter = it;
+    if (err) *err = e;
+    return HF_OK;
+}
+
 static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
+    if (!c->dyn_heat && !(c->debug & 1)) return step_fp64_fused(c, itime, iter, err);
+    c->carry = false;
     const hf_params &p = c->p;
     launch_step_begin(c->sc, c->st);
     const double N = (double)c->g.N;

@@ -39,12 +39,15 @@ struct hf_ctx {
     double *sv = nullptr, *hx = nullptr, *hy = nullptr, *hz = nullptr;
     float *fx = nullptr, *fy = nullptr, *fz = nullptr;
     double *qd = nullptr;        // Qdens (constant, or of the last step)
-    // state: T[0] = Temp_p, T[1] = Temp_pp, T[2] = spare (mixed-mode refined x)
+    // state: T[0] = Temp_p, T[1] = Temp_pp, T[2] = spare (the fused steps' x, the refined x)
     double *T[3] = {nullptr, nullptr, nullptr};
     double *S = nullptr;
     // FP64 CG (r64: right-hand side of a solve, rw64: its residual from iteration 1 on)
     double *r64 = nullptr, *rw64 = nullptr, *p64[2] = {nullptr, nullptr}, *q64 = nullptr,
            *y64 = nullptr;
+    // the fused FP64 step writes the next step's right-hand side into the vector of r64, r64b
+    // that the current solve does not read; carry_r64 is the one the next step starts from
+    double *r64b = nullptr, *carry_r64 = nullptr;
     // FP32 inner CG
     float *r32 = nullptr, *rw32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr,
           *y32 = nullptr;

@@ -5,9 +5,12 @@
 //   k_scale        Jacobi scaling s = 1/sqrt(-diag), scaled couplings, FP32 copies
 //   k_coo_*        export in the reference COO order                  (mod_setup.f90:89-152)
 //   k_qdens        per-step heater power density                      (mod_heating.f90:27-169)
-//   k_cg_b         y += alpha p, r -= alpha q, r.r                     (mod_Sparse.f90:412-420)
+//   k_cg_b         r -= alpha q, r.r                                   (mod_Sparse.f90:412-420)
+//   k_cg_tail      y += alpha p of the last iteration
 //   k_finalize     x = x0 + s*y, NaN guard, TINY flag of the next step (mod_evolve.f90:140-170)
 //   k_fin_step     refinement test and next-step norms after the fused mixed step
+//   k_fin_next     next-step norms after the fused FP64 step
+//   k_extrap       a + (a - b): the adjoint solve's initial guess
 //   k_fin_*        one-block finishing kernels of the reductions (scalars, convergence)
 // The stencil kernels (fused RHS/residual, CG SpMV, refinement residual) are in hf_stencil.hip.
 //
@@ -572,7 +575,7 @@ __global__ void k_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax,
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     step_begin(sc);
     carry_begin(sc, tol, itmax, allow_short);
-    inner_begin(sc, inner, inner_tol, tol, inner_itmax, safety, allow_short);
+    if (inner) inner_begin(sc, inner, inner_tol, tol, inner_itmax, safety, allow_short);
 }
 
 // After the fused mixed step: FP64 residual of this step (refinement test as k_fin_ir) and
@@ -591,6 +594,18 @@ k_fin_step(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max, in
     if (isnan(v[0])) sc->flags |= FLAG_BREAKDOWN;
     sc->next_bnorm2 = v[1];
     sc->next_rr[guess ? 1 : 0] = v[2];
+    sc->next_rr[guess ? 0 : 1] = __builtin_nan("");
+}
+
+// After the fused FP64 step: the next step's norms ||b^_next||^2, ||r^0_next||^2 (TINY case
+// `guess`; the other case's is NaN, never carried)
+__global__ void __launch_bounds__(kBlock)
+k_fin_next(uint32_t nb, const double *partials, Scal *sc, int32_t guess) {
+    double v[2];
+    sum_partials<2>(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    sc->next_bnorm2 = v[0];
+    sc->next_rr[guess ? 1 : 0] = v[1];
     sc->next_rr[guess ? 0 : 1] = __builtin_nan("");
 }
 
@@ -727,6 +742,11 @@ void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sw
                      bool tiny_guess, hipStream_t st) {
     // after k_ir_step_z (pipelined z-chunking)
     HF_FIN(k_fin_step, g.zlay[kZPipe].zblk, st, partials, sc, sweeps_max, (int32_t)tiny_guess);
+}
+void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
+                     hipStream_t st) {
+    // after k_step64_z (pipelined z-chunking)
+    HF_FIN(k_fin_next, g.zlay[kZPipe].zblk, st, partials, sc, (int32_t)tiny_guess);
 }
 void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
                         double inner_tol, int64_t inner_itmax, double safety, hipStream_t st) {

@@ -119,6 +119,14 @@ void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const 
                     const double *qb, const double *d, const double *cx, const double *cy,
                     const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
                     double *partials, hipStream_t st);
+// Fused FP64 step: k_finalize of this step (x -> xout) and k_rhs_res_z of the next from
+// (x, T^n), with the next step's TINY shift if `guess` (r^0 -> rnext); bit-identical to the two
+// kernels run apart.  2 partials (as k_rhs_res_z); NaN flags of the next S go to flags_next.
+void launch_step64(const Dims &g, const RhsParams &rp, bool guess, const double *tp,
+                   const double *tpp, const double *y, const double *r0, const double *rhoc,
+                   const double *cvec, const double *qd, const double *bvec, const double *d,
+                   const double *cx, const double *cy, const double *cz, double *xout,
+                   double *rnext, Scal *sc, double *partials, hipStream_t st);
 // one-block finishing kernels: sum the per-block partials in block order, update the scalars
 // allow_short: the solve may end after its first iteration (k_fin_a)
 void launch_fin_start(const Dims &g, const double *partials, Scal *sc, double tol, int64_t itmax,
@@ -131,9 +139,13 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
 // tiny_guess: the TINY case the step's rnext was formed for
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                      bool tiny_guess, hipStream_t st);
-// start of a carried mixed-precision step (k_step_begin; the scalars from the norms the last
-// fused pass left, as launch_fin_start for a residual from the state) and the first inner
-// solve's scalars (as launch_inner_begin(sc, inner, ..., outer_tol = tol)), in one launch
+// after launch_step64: the next step's norms for the TINY case `tiny_guess`
+void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
+                     hipStream_t st);
+// start of a carried step (k_step_begin; the scalars from the norms the last fused pass left,
+// as launch_fin_start for a residual from the state) and, for a mixed-precision step (inner not
+// null), the first inner solve's scalars (as launch_inner_begin(sc, inner, ..., outer_tol =
+// tol)), in one launch
 void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
                         double inner_tol, int64_t inner_itmax, double safety, hipStream_t st);
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,

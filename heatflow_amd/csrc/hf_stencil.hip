@@ -23,6 +23,7 @@
 //   k_cg_b1_z    the first iteration's y, r updates with q recomputed
 //   k_ir_res_z   x' = x + s y, r^ = -s (S - H x'), ||r^||^2  (mixed-precision refinement)
 //   k_ir_step_z  the refinement residual fused with the next step's right-hand side
+//   k_step64_z   k_finalize fused with the next step's k_rhs_res_z (FP64 path)
 #include <type_traits>
 
 #include "hf_device.hpp"
@@ -860,6 +861,103 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     block_partials<3>(acc, partials);
 }
 
+// operand inputs of the fused FP64 step: T^n, T^{n-1}, the diagonal (s is formed from it as
+// k_scale forms it) and the solution (y, or r0 scaled by alpha: CgY)
+struct Step64Raw {
+    double tp, tpp, d, y;
+};
+
+// Fused step of the FP64 path, one pass over the grid after the solve: k_finalize of this step
+// followed by k_rhs_res_z of the next, with the same operations in the same order, so results
+// are bit-identical to the two kernels run apart:
+//   x = x0 + s y                  x0 = 2 T^n - T^{n-1} (+TINY) (mod_evolve.f90:140-141) -> xout
+//   x1 = x + (x - T^n) (+TINY)    the next step's initial guess, staged in LDS
+//   r^0 = -s (S_{n+1} - H x1)     S_{n+1} from (x, T^n) (mod_evolve.f90:112-119) -> rnext
+// The next step's TINY case (any x1 - x < TINY) is an OR over the grid, known only after the
+// pass: x1 is formed for the guessed outcome `guess`, and the flag the pass raises tells the
+// host whether the guess held (else the next step starts from k_rhs_res_z).  NaN flags of
+// S_{n+1} go to flags_next, which the next step's start makes current.
+template <int Mode, bool Wrap>
+__global__ void __launch_bounds__(kZTX * kZTY)
+k_step64_z(Dims g, RhsParams rp, int32_t guess, const double *__restrict__ tp,
+           const double *__restrict__ tpp, const double *__restrict__ y,
+           const double *__restrict__ r0, const double *__restrict__ rhoc,
+           const double *__restrict__ cvec, const double *__restrict__ qd,
+           const double *__restrict__ bvec, const double *__restrict__ d,
+           const double *__restrict__ cx, const double *__restrict__ cy,
+           const double *__restrict__ cz, double *__restrict__ xout,
+           double *__restrict__ rnext, Scal *sc, double *partials) {
+    __shared__ PlaneBuf<double> sp;
+    const bool tiny = sc->tiny != 0;
+    const CgY<double> ysol = cg_y(sc, y, r0);
+    auto load = [&](uint32_t J) -> Step64Raw { return Step64Raw{tp[J], tpp[J], d[J], ysol.v[J]}; };
+    // x^{n+1} of a cell as k_finalize forms it
+    auto xnew = [&](const Step64Raw &v) -> double {
+        double x0 = v.tp + (v.tp - v.tpp);
+        if (tiny) x0 = x0 + kTiny;
+        return ysol.any ? x0 + (1.0 / sqrt(-v.d)) * (ysol.scale * v.y) : x0;
+    };
+    // the next step's x0 as k_rhs_res_z forms it from the state (x, T^n)
+    auto form = [&](const Step64Raw &v) -> double {
+        const double x = xnew(v);
+        double x1 = x + (x - v.tp);
+        if (guess) x1 = x1 + kTiny;
+        return x1;
+    };
+    auto X = [&](uint32_t J) -> double { return form(load(J)); };
+    double acc[2] = {0.0, 0.0};
+    int32_t fl = 0, fl_next = 0;
+    const ZCol c = zcolumn(g);
+    const NbOff o = nb_offsets_in(g, c);
+    // the diagonal comes with the cell's own operand record (ri.d), not with these
+    auto cload = [&](uint32_t I) -> RhsCell {
+        RhsCell v;
+        v.rc = Mode != kSteady ? ldnt(&rhoc[I]) : 0.0;
+        v.cv = Mode == kCattaneo ? ldnt(&cvec[I]) : 0.0;
+        v.qd = ldnt(&qd[I]);
+        v.bv = ldnt(&bvec[I]);
+        v.h = HCell{0.0, cx[I], cy[I], ldnt(&cz[I]), cx[I + o.xm], cy[I + o.ym]};
+        return v;
+    };
+    double cz_m = 0.0;
+    if (c.active && (c.z0 > 0 || (Wrap && g.per[2]))) cz_m = cz[zm_index(g, c.col, c.z0)];
+    zmarch_pipe<double, Step64Raw, RhsCell, Wrap>(
+        g, c, sp, load, form, cload,
+        [&](uint32_t I, uint32_t z, double xi, double xz_m, double xz_p,
+            const InPlane<double> &nb, const RhsCell &cv, const Step64Raw &ri) {
+            const double x = xnew(ri);
+            stnt(&xout[I], x);
+            if (isnan(x)) fl |= FLAG_NAN_X;                                  // mod_evolve.f90:153
+            if ((x + (x - ri.tp)) - x < kTiny) fl |= FLAG_TINY_NEXT;          // :141, next step
+            double S;
+            if (Mode != kSteady) {
+                S = -(rp.inverse_time * x * cv.rc) - cv.qd - cv.bv;
+                if (Mode == kCattaneo) {
+                    const double scat = (ri.tp - 2.0 * x) * cv.cv;
+                    if (isnan(scat)) fl_next |= FLAG_NAN_SCAT;
+                    S = S + scat;
+                }
+            } else {
+                S = -cv.qd - cv.bv;
+            }
+            if (isnan(S)) fl_next |= FLAG_NAN_S;
+            HCell h = cv.h;
+            h.d = ri.d;
+            const double hxv = hx_sum_v<Wrap>(g, c, I, z, h, cx, cy, xi, xz_m, xz_p, cz_m, nb, X);
+            const double r = S - hxv;
+            const double s = 1.0 / sqrt(-ri.d);
+            const double rh = -s * r;
+            stnt(&rnext[I], rh);
+            const double bh = s * S;
+            acc[0] += bh * bh;
+            acc[1] += rh * rh;
+            cz_m = cv.h.cz;
+        });
+    block_flags(fl, &sc->flags);
+    block_flags(fl_next, &sc->flags_next);
+    block_partials<2>(acc, partials);
+}
+
 // ---- CG kernels with W x-neighbouring cells per thread (nx a multiple of W) ----------------
 //
 // FP32 arrays moved one float per lane per instruction run well below the FP64 kernels'
@@ -1265,6 +1363,29 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
                    hipStream_t st) {
     HF_ZLAUNCH(k_ir_res_z, g, st, g, ip, x, y, r0, inner, sv, S, d, cx, cy, cz, tp, xout, rhat,
                sc, partials);
+}
+
+// (pipelined z-chunking; launch_fin_next reads its partials)
+void launch_step64(const Dims &g0, const RhsParams &rp, bool guess, const double *tp,
+                   const double *tpp, const double *y, const double *r0, const double *rhoc,
+                   const double *cvec, const double *qd, const double *bvec, const double *d,
+                   const double *cx, const double *cy, const double *cz, double *xout,
+                   double *rnext, Scal *sc, double *partials, hipStream_t st) {
+    const Dims g = zdims(g0, kZPipe);
+    const int32_t gs = guess ? 1 : 0;
+#define HF_STEP64(M, W)                                                                       \
+    HF_ZLAUNCH((k_step64_z<M, W>), g, st, g, rp, gs, tp, tpp, y, r0, rhoc, cvec, qd, bvec, d, \
+               cx, cy, cz, xout, rnext, sc, partials)
+    const int mode = s_mode(rp.isteady, rp.icattaneo);
+    const bool wrap = g.per[0] || g.per[1] || g.per[2];
+    if (mode == kCattaneo) {
+        if (wrap) HF_STEP64(kCattaneo, true); else HF_STEP64(kCattaneo, false);
+    } else if (mode == kFourier) {
+        if (wrap) HF_STEP64(kFourier, true); else HF_STEP64(kFourier, false);
+    } else {
+        if (wrap) HF_STEP64(kSteady, true); else HF_STEP64(kSteady, false);
+    }
+#undef HF_STEP64
 }
 
 // (pipelined z-chunking; launch_fin_step reads its partials)

@@ -151,9 +151,10 @@ int hf_solve(hf_ctx *ctx, const double *b, double *x, int32_t itol, double tol, 
 int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
 
 /* Debug switches (no reference counterpart): bit 0 keeps S of every step for hf_get_vector(2)
- * (costs one extra N-vector write per step; mixed mode then runs the unfused step); bit 1
- * turns off the first-iteration stop of CG (a solve whose first iteration already meets the
- * tolerance ends there, with the next residual norm predicted from p.q and q.q). */
+ * (costs one extra N-vector write per step; both precision modes then run the unfused
+ * step kernels); bit 1 turns off the first-iteration stop of CG (a solve whose first
+ * iteration already meets the tolerance ends there, with the next residual norm predicted
+ * from p.q and q.q). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */
@@ -179,7 +180,7 @@ int hf_reset_stats(hf_ctx *ctx);
  * arrays each kernel touches, every array element counted once).  Reading resets the pool. */
 enum {
     HF_K_QDENS = 0, HF_K_RHS64, HF_K_RHS32, HF_K_CG_A64, HF_K_CG_B64, HF_K_CG_A32, HF_K_CG_B32,
-    HF_K_FINAL, HF_K_IR_RES, HF_K_IR_STEP, HF_K_COUNT
+    HF_K_FINAL, HF_K_IR_RES, HF_K_IR_STEP, HF_K_STEP64, HF_K_COUNT
 };
 int hf_set_timing(hf_ctx *ctx, int32_t on);
 int hf_kernel_times(hf_ctx *ctx, double *ms, int64_t *launches, double *bytes);

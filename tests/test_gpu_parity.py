@@ -153,6 +153,55 @@ def test_steps_match_oracle(gi):
     assert all(i > 0 for i in its)
 
 
+FUSED64_CASES = [dict(GEOMETRIES[i]) for i in (0, 1, 2, 3, 5, 6, 7, 8, 9)] + [
+    dict(nx=5, ny=4, nz=6, isteady=1)]
+
+
+def _fp64_runs(s, tp0, tpp0, nsteps):
+    """(T^n, T^{n-1}, stats) of the fused FP64 step and of the unfused one (debug bit 0)."""
+    out = []
+    for dbg in (0, 1):
+        with Simulation(s) as sim:
+            sim.set_debug(dbg)
+            sim.set_state(tp0, tpp0)
+            sim.run(1, nsteps)
+            tp, tpp = sim.get_state()
+            out.append((tp, tpp, sim.stats()))
+    return out
+
+
+@pytest.mark.parametrize("ci", range(len(FUSED64_CASES)))
+def test_fp64_fused_step_bit_exact(ci):
+    """The FP64 step fuses k_finalize with the next step's right-hand side (k_step64_z) and
+    carries that residual to the next step; debug bit 0 runs the kernels apart.  Same bits over
+    6 steps on every geometry without radiation (periodic axes, 1-cell axes, Fourier, steady,
+    near-zero conductivities), and both match the oracle."""
+    s = random_system(700 + ci, **FUSED64_CASES[ci])
+    tp0, tpp0 = random_state(ci, s.params.n)
+    (ta, tpa, sta), (tb, tpb, _) = _fp64_runs(s, tp0, tpp0, 6)
+    assert np.array_equal(bits(ta), bits(tb)) and np.array_equal(bits(tpa), bits(tpb))
+    rc, otp, _, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 7)), tp0, tpp0)
+    assert rc == 0
+    assert relerr(ta, otp) <= RTOL_FP64
+
+
+@pytest.mark.parametrize("heating", ["uniform", "gaussian"])
+def test_fp64_fused_step_tiny_guess(heating):
+    """The fused FP64 pass forms the next step's residual for the last TINY outcome seen (first
+    guess: TINY).  Uniform heating of an insulated grid makes the first guess miss once (the
+    next step starts from the state); Gaussian heating with 300 K faces keeps TINY and the
+    guess never misses.  Bit-identical to the unfused kernels in both cases."""
+    s, _ = workloads.c3(16)
+    p = s.params
+    if heating == "uniform":
+        p.kappa_bound = (0.0,) * 6
+        s.cells.qdens_extra = np.full(p.n, 1.0e12)
+    t0 = np.full(p.n, p.t_system)
+    (ta, tpa, sta), (tb, tpb, _) = _fp64_runs(s, t0, t0, 10)
+    assert sta.tiny_misses == (1 if heating == "uniform" else 0)
+    assert np.array_equal(bits(ta), bits(tb)) and np.array_equal(bits(tpa), bits(tpb))
+
+
 def test_c1_hundred_steps():
     """16^3 Cattaneo, one centre heater, 100 steps at the reference tolerance."""
     s, meta = workloads.c1(16)

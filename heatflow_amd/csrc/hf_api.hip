@@ -722,7 +722,8 @@ static int step_fp64_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err)
     const double *rin;
     if (c->carry) {
         // step start and the scalars of the carried residual in one launch
-        launch_carry_start(c->sc, nullptr, rp.tol, rp.itmax, allow_short(c), 0.0, 0, 0.0, c->st);
+        launch_carry_start(c->sc, nullptr, rp.tol, rp.itmax, allow_short(c), 0.0, 0, 0.0, false,
+                           c->st);
         rin = c->carry_r64;
     } else {
         launch_step_begin(c->sc, c->st);
@@ -822,6 +823,8 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
 
 // inner CG target floored at this fraction of the outer tolerance
 constexpr double kInnerSafety = 0.25;
+// steps without alpha reuse after a step whose reused alpha missed the inner target
+constexpr int64_t kReusePause = 16;
 
 // Queues the inner FP32 CG of one refinement sweep on A^ y = r (r unchanged); *batch_out is
 // the number of iterations queued.  begin: set up the inner scalars (else already done).
@@ -857,17 +860,25 @@ static float *other_rvec(hf_ctx *c, const float *a, const float *b = nullptr) {
 // guess makes the next step start from k_rhs_res.  The pass stores this step's residual only
 // for a sweep that the last step followed with another (c->last_sweeps); when a sweep then
 // needs one after all, the same pass runs again with the store.
+// A carried step after one whose inner solve ended on its first iteration (y = alpha r^0) and
+// needed no second sweep reuses that alpha instead of solving: A^ is close to the identity
+// here, so alpha changes little from step to step.  The sweep's FP64 residual must then meet
+// half the inner solve's own target (in norm); if it does not, the step goes on with ordinary
+// sweeps, and reuse pauses for kReusePause steps, twice as long after each further miss in a
+// row.
 static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     (void)itime;
     const hf_params &p = c->p;
     const double N = (double)c->g.N;
     RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
     const float *rin;
-    bool inner_started = false;
+    bool inner_started = false, reuse = false;
     if (c->carry) {
-        // step, carried scalars and the first inner solve's scalars in one launch
+        reuse = c->reuse_ok && c->reuse_pause == 0 && allow_short(c) && (c->debug & 4) == 0;
+        // step, carried scalars and the first inner solve's scalars (or the reused alpha) in
+        // one launch
         launch_carry_start(c->sc, c->sci, rp.tol, rp.itmax, allow_short(c), p.inner_tol,
-                           p.inner_itmax, kInnerSafety, c->st);
+                           p.inner_itmax, kInnerSafety, reuse, c->st);
         inner_started = true;
         rin = c->carry_r;
     } else {
@@ -896,10 +907,12 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     double *xa = nullptr;
     int nb = 0;
     int64_t inner_total = 0, sweeps = 0;
+    bool short_stop = false;   // the last sweep's inner solve ended on its first iteration
     for (;;) {
         std::vector<SlotRec> slots;
         int64_t batch = 0;
-        TRY(inner_solve(c, rin, &batch, slots, !inner_started));
+        const bool reused = reuse && sweeps == 0;
+        if (!reused) TRY(inner_solve(c, rin, &batch, slots, !inner_started));
         inner_started = false;
         double *xo = xbuf[nb];
         // the pass reads y (or the right-hand side, for y = alpha r) and writes two residual
@@ -918,16 +931,20 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
             tend(c, si);
         };
         pass();
-        launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->tiny_guess, c->st);
+        launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->tiny_guess, reused, c->st);
         TRY(check_launch(c, "k_ir_step"));
         TRY(sync_scal(c, c->sc, c->h_sc, 2));   // outer and inner scalars
-        const int64_t in_it = c->h_sci->iter;
-        drop_idle_slots(c, slots, in_it, c->h_sci->shortcut != 0);
-        if (c->h_sci->shortcut) c->stats.first_iter_stops += 1;
-        inner_total += in_it;
         sweeps += 1;
-        // next batch: what this sweep needed + 1 (a batch cut short costs a refinement sweep)
-        c->pred_inner = c->h_sci->done ? std::max<int64_t>(1, in_it + 1) : batch * 2;
+        if (!reused) {
+            const int64_t in_it = c->h_sci->iter;
+            drop_idle_slots(c, slots, in_it, c->h_sci->shortcut != 0);
+            short_stop = c->h_sci->shortcut != 0;
+            if (short_stop) c->stats.first_iter_stops += 1;
+            inner_total += in_it;
+            // next batch: what this sweep needed + 1 (a batch cut short costs a refinement
+            // sweep)
+            c->pred_inner = c->h_sci->done ? std::max<int64_t>(1, in_it + 1) : batch * 2;
+        }
         if (!c->h_sc->done && !sp.store_res) {
             // a further sweep needs this step's residual: the same pass again, with the store
             // (its other outputs, partials and flags come out the same; the scalars stay)
@@ -959,6 +976,21 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     if (!c->carry) c->stats.tiny_misses += 1;
     c->tiny_guess = tiny_next;
     c->last_sweeps = sweeps;
+    // alpha reuse: after a step done in one sweep whose alpha was reused or came from a solve
+    // that ended on its first iteration
+    if (reuse) {
+        c->stats.alpha_reuses += 1;
+        if (sweeps > 1) {
+            c->stats.reuse_misses += 1;
+            c->reuse_pause = kReusePause << std::min<int64_t>(c->reuse_miss_run, 6);
+            c->reuse_miss_run += 1;
+        } else {
+            c->reuse_miss_run = 0;
+        }
+    } else if (c->reuse_pause > 0) {
+        c->reuse_pause -= 1;
+    }
+    c->reuse_ok = sweeps == 1 && (reuse || short_stop);
     const double e = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->outer_rr / c->h_sc->bnorm2) : 0.0;
     c->stats.steps += 1;
     c->stats.outer_iters += sweeps;

@@ -61,6 +61,11 @@ struct hf_ctx {
     bool carry = false;
     bool tiny_guess = true;
     int64_t last_sweeps = 1;     // refinement sweeps the last fused step needed
+    // the next carried step may reuse the last inner solve's alpha (it ended on its first
+    // iteration, or reused alpha itself, and the step needed one sweep); steps left before
+    // reuse is tried again after a miss
+    bool reuse_ok = false;
+    int64_t reuse_pause = 0, reuse_miss_run = 0;
     float *carry_r = nullptr;
     // reductions / scalars
     Scal *sc = nullptr, *sci = nullptr;

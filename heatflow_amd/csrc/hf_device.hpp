@@ -96,6 +96,7 @@ struct Scal {
     double thresh2;   // stop when rr <= thresh2
     double pq, alpha, beta;
     double outer_rr;  // mixed mode: FP64 residual^2 after the last refinement sweep
+    double reuse_thresh2;  // mixed mode: the inner target a step that reuses alpha must meet
     int64_t iter;     // iterations of this solve
     int64_t itmax;
     int32_t done;

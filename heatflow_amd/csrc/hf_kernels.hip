@@ -499,15 +499,19 @@ k_fin_ir(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max) {
 // Starts an inner FP32 solve from the outer scalars: thresholds relative to this sweep's
 // residual, floored by the outer target (no point solving further than the outer test needs).
 // Run by one thread.
+static __device__ double inner_target(const Scal *outer, double inner_tol, double outer_tol,
+                                      double safety) {
+    const double t_rel = inner_tol * inner_tol * outer->outer_rr;
+    const double t_abs = safety * safety * outer_tol * outer_tol * outer->bnorm2;
+    return t_rel > t_abs ? t_rel : t_abs;
+}
 static __device__ void inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
                                    int64_t inner_itmax, double safety, int32_t allow_short) {
     const double rr = outer->outer_rr;
-    double t_rel = inner_tol * inner_tol * rr;
-    double t_abs = safety * safety * outer_tol * outer_tol * outer->bnorm2;
     inner->rr = rr;
     inner->rr_start = rr;
     inner->bnorm2 = outer->bnorm2;
-    inner->thresh2 = t_rel > t_abs ? t_rel : t_abs;
+    inner->thresh2 = inner_target(outer, inner_tol, outer_tol, safety);
     inner->iter = 0;
     inner->itmax = inner_itmax;
     inner->first = 1;
@@ -569,28 +573,50 @@ static __device__ void carry_begin(Scal *sc, double tol, int64_t itmax, int32_t 
 
 // Start of a carried mixed-precision step and of its first inner solve in one launch:
 // step_begin, carry_begin, inner_begin.
+// With `reuse`, the first sweep takes y = alpha r^0 with the step length alpha of the last
+// inner solve (kept in inner->alpha) instead of solving: the inner scalars describe a solve
+// that ended on its first iteration, and the sweep's FP64 residual must meet half the target
+// (in norm) the solve would have had (reuse_thresh2, k_fin_step), so that a kept sweep is no
+// less accurate than a solve's.
 __global__ void k_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax,
                               int32_t allow_short, double inner_tol, int64_t inner_itmax,
-                              double safety) {
+                              double safety, int32_t reuse) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     step_begin(sc);
     carry_begin(sc, tol, itmax, allow_short);
-    if (inner) inner_begin(sc, inner, inner_tol, tol, inner_itmax, safety, allow_short);
+    if (!inner) return;
+    if (reuse) {
+        sc->reuse_thresh2 = 0.25 * inner_target(sc, inner_tol, tol, safety);
+        inner->iter = 0;
+        inner->shortcut = 1;
+        inner->yset = 0;
+        inner->ypend = 1;
+        inner->kpend = 0;
+        inner->done = 1;
+        // as inner_begin: only the last sweep of a step sets the next step's flags
+        sc->flags &= ~FLAG_TINY_NEXT;
+        sc->flags_next = 0;
+    } else {
+        inner_begin(sc, inner, inner_tol, tol, inner_itmax, safety, allow_short);
+    }
 }
 
 // After the fused mixed step: FP64 residual of this step (refinement test as k_fin_ir) and
 // the next step's norms.  partials: ||r^||^2, ||b^_next||^2, ||r^0_next||^2 (TINY case
 // `guess`).
 __global__ void __launch_bounds__(kBlock)
-k_fin_step(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max, int32_t guess) {
+k_fin_step(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max, int32_t guess,
+           int32_t reused) {
     double v[3];
     sum_partials<3>(partials, nb, v);
     if (threadIdx.x != 0) return;
     const double prev = sc->outer_rr;
+    // a sweep that reused alpha passes only within the inner target (reuse_thresh2)
+    const double thr = reused ? sc->reuse_thresh2 : sc->thresh2;
     sc->outer_rr = v[0];
     sc->iter += 1;
     const bool stalled = sc->iter > 1 && v[0] > 0.25 * prev;
-    sc->done = (v[0] <= sc->thresh2) || sc->iter >= sweeps_max || stalled || isnan(v[0]);
+    sc->done = (v[0] <= thr) || sc->iter >= sweeps_max || stalled || isnan(v[0]);
     if (isnan(v[0])) sc->flags |= FLAG_BREAKDOWN;
     sc->next_bnorm2 = v[1];
     sc->next_rr[guess ? 1 : 0] = v[2];
@@ -739,9 +765,10 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
     HF_FIN(k_fin_ir, g.zblk, st, partials, sc, sweeps_max);
 }
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
-                     bool tiny_guess, hipStream_t st) {
+                     bool tiny_guess, bool reused, hipStream_t st) {
     // after k_ir_step_z (pipelined z-chunking)
-    HF_FIN(k_fin_step, g.zlay[kZPipe].zblk, st, partials, sc, sweeps_max, (int32_t)tiny_guess);
+    HF_FIN(k_fin_step, g.zlay[kZPipe].zblk, st, partials, sc, sweeps_max, (int32_t)tiny_guess,
+           (int32_t)reused);
 }
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st) {
@@ -749,9 +776,10 @@ void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_
     HF_FIN(k_fin_next, g.zlay[kZPipe].zblk, st, partials, sc, (int32_t)tiny_guess);
 }
 void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
-                        double inner_tol, int64_t inner_itmax, double safety, hipStream_t st) {
+                        double inner_tol, int64_t inner_itmax, double safety, bool reuse,
+                        hipStream_t st) {
     hipLaunchKernelGGL(k_carry_start, dim3(1), dim3(64), 0, st, sc, inner, tol, itmax,
-                       (int32_t)allow_short, inner_tol, inner_itmax, safety);
+                       (int32_t)allow_short, inner_tol, inner_itmax, safety, (int32_t)reuse);
 }
 
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,

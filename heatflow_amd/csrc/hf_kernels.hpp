@@ -136,18 +136,21 @@ void launch_fin_b(const Dims &g, const double *partials, Scal *sc, hipStream_t s
 void launch_fin_b1(const Dims &g, const double *partials, Scal *sc, hipStream_t st);  // after k_cg_b1
 void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                    hipStream_t st);
-// tiny_guess: the TINY case the step's rnext was formed for
+// tiny_guess: the TINY case the step's rnext was formed for; reused: the sweep took the last
+// inner solve's alpha (launch_carry_start) and passes only at the inner target
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
-                     bool tiny_guess, hipStream_t st);
+                     bool tiny_guess, bool reused, hipStream_t st);
 // after launch_step64: the next step's norms for the TINY case `tiny_guess`
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st);
 // start of a carried step (k_step_begin; the scalars from the norms the last fused pass left,
 // as launch_fin_start for a residual from the state) and, for a mixed-precision step (inner not
 // null), the first inner solve's scalars (as launch_inner_begin(sc, inner, ..., outer_tol =
-// tol)), in one launch
+// tol)), in one launch.  reuse (mixed): no first inner solve; the first sweep takes
+// y = alpha r^0 with the last solve's alpha and must meet the inner target
 void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
-                        double inner_tol, int64_t inner_itmax, double safety, hipStream_t st);
+                        double inner_tol, int64_t inner_itmax, double safety, bool reuse,
+                        hipStream_t st);
 void launch_inner_begin(Scal *outer, Scal *inner, double inner_tol, double outer_tol,
                         int64_t inner_itmax, double safety, bool allow_short, hipStream_t st);
 void launch_step_begin(Scal *sc, hipStream_t st);

@@ -154,7 +154,10 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * (costs one extra N-vector write per step; both precision modes then run the unfused
  * step kernels); bit 1 turns off the first-iteration stop of CG (a solve whose first
  * iteration already meets the tolerance ends there, with the next residual norm predicted
- * from p.q and q.q). */
+ * from p.q and q.q), and with it the reuse of its step length; bit 2 turns off that reuse
+ * alone (mixed mode: a step after one whose inner solve ended on its first iteration takes
+ * y = alpha r^0 with that solve's alpha, and solves only if the result misses the inner
+ * target). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */
@@ -168,6 +171,10 @@ typedef struct hf_stats {
                                  wrong (that successor started from the state)          */
     int64_t residual_reruns;  /* mixed-mode passes run again to store the residual a
                                  further refinement sweep needed                        */
+    int64_t alpha_reuses;     /* mixed-mode steps whose first sweep reused the last inner
+                                 solve's step length instead of solving                 */
+    int64_t reuse_misses;     /* of those, steps whose residual missed the inner target
+                                 (ordinary sweeps followed)                             */
 } hf_stats;
 int hf_get_stats(hf_ctx *ctx, hf_stats *st);
 int hf_reset_stats(hf_ctx *ctx);

@@ -440,6 +440,51 @@ def test_fp32_ir_refinement_sweeps():
     assert relerr(tp, otp) <= 1e-12
 
 
+def test_fp32_ir_alpha_reuse():
+    """After a step whose inner solve ended on its first iteration (y = alpha r^0) in one sweep,
+    the next carried step takes y = alpha r^0 with that alpha and no inner solve, and keeps it
+    only if its FP64 residual meets half the inner solve's target.  Over 30 steps of the
+    log-normal workload at 256^3 most steps reuse alpha; with reuse and without it (debug bit 2)
+    the temperatures meet the 1e-10 gate against the FP64 path at the reference tolerance."""
+    s, _ = workloads.c3(256)
+    with Simulation(s, tol=1e-32) as ref:
+        ref.initial_evolve()
+        ref.run(1, 30)
+        otp, _ = ref.get_state()
+    res = {}
+    for dbg in (0, 4):
+        with Simulation(s, precision="fp32_ir", tol=1e-12) as sim:
+            sim.set_debug(dbg)
+            sim.initial_evolve()
+            sim.run(1, 30)
+            res[dbg] = (sim.get_state()[0], sim.stats())
+    (ta, sa), (tb, sb) = res[0], res[4]
+    assert sa.alpha_reuses >= 15, (sa.alpha_reuses, sa.reuse_misses)
+    assert sb.alpha_reuses == 0 and sa.inner_iters < sb.inner_iters
+    assert relerr(ta, otp) <= 1e-10 and relerr(tb, otp) <= 1e-10
+
+
+def test_fp32_ir_alpha_reuse_miss():
+    """A reused alpha whose sweep misses the target (40^3, where the later steps' one-iteration
+    residuals sit near the inner target) is followed by ordinary sweeps in the same step and a
+    pause of reuse; the step still meets the tolerance and the run the 1e-10 gate."""
+    s, _ = workloads.c3(40)
+    t0 = np.full(s.params.n, s.params.t_system)
+    with Simulation(s, tol=1e-32) as ref:
+        ref.initial_evolve()
+        ref.run(1, 20)
+        t64, _ = ref.get_state()
+    with Simulation(s, precision="fp32_ir", tol=1e-12) as sim:
+        sim.initial_evolve()
+        errs = [sim.simulate(k)[1] for k in range(1, 21)]
+        t32, _ = sim.get_state()
+        st = sim.stats()
+    assert st.reuse_misses >= 1 and st.alpha_reuses == st.reuse_misses, (st.alpha_reuses,)
+    assert st.outer_iters > st.steps   # the missed step took a further sweep
+    assert max(errs) <= 1e-12
+    assert relerr(t32, t64) <= 1e-10
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
 def test_first_iteration_stop_is_exact(prec):
     """A CG solve whose first iteration already meets the tolerance stops there: the next
@@ -447,18 +492,19 @@ def test_first_iteration_stop_is_exact(prec):
     which forms it with k_cg_b's own product.  With that stop turned off (debug bit 1) the
     same iteration runs in full, so the temperatures must agree bit for bit; over 20 steps
     at least 5 solves take the stop (inner tolerance 0.5 for the mixed inner solves, so that
-    nearly every inner solve stops after one iteration and the refinement does the rest)."""
+    nearly every inner solve stops after one iteration and the refinement does the rest).
+    Debug bit 2 keeps every mixed step solving (no reuse of the last step length)."""
     s, _ = workloads.c3(24)
     out = {}
-    for dbg in (0, 2):
+    for dbg in (4, 6):
         with Simulation(s, precision=prec, tol=1e-12,
                         inner_tol=0.5 if prec == "fp32_ir" else 1e-6) as sim:
             sim.set_debug(dbg)
             sim.initial_evolve()
             its = [sim.simulate(k)[0] for k in range(1, 21)]
             out[dbg] = (sim.get_state()[0], its, sim.stats().first_iter_stops)
-    t0, its0, n0 = out[0]
-    t2, its2, n2 = out[2]
+    t0, its0, n0 = out[4]
+    t2, its2, n2 = out[6]
     assert its0 == its2
     assert n0 >= 5 and n2 == 0, (n0, n2, its0)
     assert np.array_equal(bits(t0), bits(t2))

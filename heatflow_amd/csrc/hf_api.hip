@@ -861,11 +861,13 @@ static float *other_rvec(hf_ctx *c, const float *a, const float *b = nullptr) {
 // for a sweep that the last step followed with another (c->last_sweeps); when a sweep then
 // needs one after all, the same pass runs again with the store.
 // A carried step after one whose inner solve ended on its first iteration (y = alpha r^0) and
-// needed no second sweep reuses that alpha instead of solving: A^ is close to the identity
-// here, so alpha changes little from step to step.  The sweep's FP64 residual must then meet
-// half the inner solve's own target (in norm); if it does not, the step goes on with ordinary
-// sweeps, and reuse pauses for kReusePause steps, twice as long after each further miss in a
-// row.
+// needed no second sweep takes y = alpha r^0 with that alpha instead of solving: A^ is close to
+// the identity here, so alpha changes little from step to step.  The sweep's FP64 residual
+// must then meet half the inner solve's own target (in norm); if it does not, the step goes on
+// with ordinary sweeps, and reuse pauses for kReusePause steps, twice as long after each
+// further miss in a row.  A reusing sweep also yields its own step's alpha (k_fin_step), which
+// the next step reuses: the lag is one step, so the small residual component along r^0 that a
+// lagged alpha leaves does not build up over long runs of reuse.
 static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     (void)itime;
     const hf_params &p = c->p;
@@ -931,7 +933,8 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
             tend(c, si);
         };
         pass();
-        launch_fin_step(c->g, c->partials, c->sc, sweeps_max, c->tiny_guess, reused, c->st);
+        launch_fin_step(c->g, c->partials, c->sc, c->sci, sweeps_max, c->tiny_guess, reused,
+                        c->st);
         TRY(check_launch(c, "k_ir_step"));
         TRY(sync_scal(c, c->sc, c->h_sc, 2));   // outer and inner scalars
         sweeps += 1;

@@ -603,13 +603,19 @@ __global__ void k_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax,
 
 // After the fused mixed step: FP64 residual of this step (refinement test as k_fin_ir) and
 // the next step's norms.  partials: ||r^||^2, ||b^_next||^2, ||r^0_next||^2 (TINY case
-// `guess`).
+// `guess`), r^0 . r'.  A sweep that reused alpha (y = alpha r^0, r' = r^0 - alpha A^ r^0)
+// leaves the step length a solve would have taken, alpha r^0.r^0 / (r^0.r^0 - r^0.r'), in
+// inner->alpha for the next reuse, so the reused step length is never more than one step old.
 __global__ void __launch_bounds__(kBlock)
-k_fin_step(uint32_t nb, const double *partials, Scal *sc, int64_t sweeps_max, int32_t guess,
-           int32_t reused) {
-    double v[3];
-    sum_partials<3>(partials, nb, v);
+k_fin_step(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int64_t sweeps_max,
+           int32_t guess, int32_t reused) {
+    double v[4];
+    sum_partials<4>(partials, nb, v);
     if (threadIdx.x != 0) return;
+    if (reused) {
+        const double r0r0 = sc->rr_start, den = r0r0 - v[3];
+        if (den > 0.0 && isfinite(den)) inner->alpha = inner->alpha * (r0r0 / den);
+    }
     const double prev = sc->outer_rr;
     // a sweep that reused alpha passes only within the inner target (reuse_thresh2)
     const double thr = reused ? sc->reuse_thresh2 : sc->thresh2;
@@ -764,11 +770,11 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
                    hipStream_t st) {
     HF_FIN(k_fin_ir, g.zblk, st, partials, sc, sweeps_max);
 }
-void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
-                     bool tiny_guess, bool reused, hipStream_t st) {
+void launch_fin_step(const Dims &g, const double *partials, Scal *sc, Scal *inner,
+                     int64_t sweeps_max, bool tiny_guess, bool reused, hipStream_t st) {
     // after k_ir_step_z (pipelined z-chunking)
-    HF_FIN(k_fin_step, g.zlay[kZPipe].zblk, st, partials, sc, sweeps_max, (int32_t)tiny_guess,
-           (int32_t)reused);
+    HF_FIN(k_fin_step, g.zlay[kZPipe].zblk, st, partials, sc, inner, sweeps_max,
+           (int32_t)tiny_guess, (int32_t)reused);
 }
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st) {

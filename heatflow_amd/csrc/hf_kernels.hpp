@@ -111,7 +111,7 @@ void launch_ir_res(const Dims &g, const IrParams &ip, const double *x, const flo
 // Fused mixed-precision step: x' = base + s y (base = the state's x0, or xb), this step's
 // FP64 residual at x' (FP32 -> rres if sp.store_res; its norm always), x' -> xout, and the next
 // step's scaled r^0 from (x', T^n), with the TINY shift if sp.tiny_guess (FP32 -> rnext);
-// 3 partials.  The pass is a function of its inputs: run again, it writes the same values.
+// 4 partials.  The pass is a function of its inputs: run again, it writes the same values.
 // qb = Qdens + B (per-step constant in this path)
 void launch_ir_step(const Dims &g, const StepParams &sp, bool from_state, const double *tp,
                     const double *tpp, const double *xb, const float *y, const float *r0,
@@ -137,9 +137,10 @@ void launch_fin_b1(const Dims &g, const double *partials, Scal *sc, hipStream_t 
 void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
                    hipStream_t st);
 // tiny_guess: the TINY case the step's rnext was formed for; reused: the sweep took the last
-// inner solve's alpha (launch_carry_start) and passes only at the inner target
-void launch_fin_step(const Dims &g, const double *partials, Scal *sc, int64_t sweeps_max,
-                     bool tiny_guess, bool reused, hipStream_t st);
+// inner solve's alpha (launch_carry_start), passes only within the inner target, and updates
+// inner->alpha to this step's own step length
+void launch_fin_step(const Dims &g, const double *partials, Scal *sc, Scal *inner,
+                     int64_t sweeps_max, bool tiny_guess, bool reused, hipStream_t st);
 // after launch_step64: the next step's norms for the TINY case `tiny_guess`
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st);

@@ -745,8 +745,14 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     auto pick_a = [](const D2 &v) { return v.a; };
     auto pick_b = [](const D2 &v) { return v.b; };
     const bool guess = sp_.tiny_guess != 0;
-    double acc[3] = {0.0, 0.0, 0.0};
-    int32_t fl = 0, fl_next = 0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // r^0 . r' of this thread's cells, kept in its own LDS slot rather than a register (the
+    // pipelined trip uses all 128 VGPRs that 4 waves per SIMD allow)
+    __shared__ double r0r[kZTX * kZTY];
+    r0r[threadIdx.x] = 0.0;
+    // flags of this step (NAN_X, TINY_NEXT) and of the next step's S (NAN_SCAT, NAN_S): distinct
+    // bits in one word
+    int32_t fl = 0;
     double cz_m = 0.0;
     D2 xz_m = 0.0, xi = 0.0;
     StepRaw ri{}, r1{};   // own operand inputs of planes z and z+1
@@ -799,14 +805,14 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 if (Cattaneo) {
                     Sn = Sn + (tppi - 2.0 * tpi) * cur.cv;
                     const double scat = (tpi - 2.0 * xi.a) * cur.cv;
-                    if (isnan(scat)) fl_next |= FLAG_NAN_SCAT;
+                    if (isnan(scat)) fl |= FLAG_NAN_SCAT;
                     Sn1 = Sn1 + scat;
                 }
             } else {
                 Sn = -cur.qb;
                 Sn1 = Sn;
             }
-            if (isnan(Sn1)) fl_next |= FLAG_NAN_S;
+            if (isnan(Sn1)) fl |= FLAG_NAN_S;
             const double h1 = hsum(I, z, cur, ri.d, cz_m, xi, xz_m, xz_p, nbc, pick_a);
             const double h2 = hsum(I, z, cur, ri.d, cz_m, xi, xz_m, xz_p, nbc, pick_b);
             const double s = 1.0 / sqrt(-ri.d);
@@ -816,8 +822,10 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 // H 1: the row sum over the same terms (absent couplings are stored as 0)
                 double h1s = ri.d;
                 h1s = h1s + cz_m;
-                h1s = h1s + (Wrap ? cy[c.hym ? I + c.oym : I] : cur.cym) * (c.hym ? 1.0 : 0.0);
-                h1s = h1s + (Wrap ? cx[c.hxm ? I + c.oxm : I] : cur.cxm) * (c.hxm ? 1.0 : 0.0);
+                // (an absent neighbour's term is selected away: h1s starts at the nonzero
+                // diagonal, so adding 0 or selecting it out gives the same sum)
+                h1s = h1s + (c.hym ? (Wrap ? cy[I + c.oym] : cur.cym) : 0.0);
+                h1s = h1s + (c.hxm ? (Wrap ? cx[I + c.oxm] : cur.cxm) : 0.0);
                 h1s = h1s + cur.cx;
                 h1s = h1s + cur.cy;
                 h1s = h1s + cur.cz;
@@ -831,6 +839,9 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             acc[0] += (double)rf * (double)rf;
             acc[1] += bh * bh;
             acc[2] += (double)fn * (double)fn;
+            // r^0 . r' when y = alpha r^0 (the sweep reused alpha): gives r^0 . A^ r^0
+            __hip_atomic_fetch_add(&r0r[threadIdx.x], (double)ri.y * (double)rf, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
             if (isnan(xi.a)) fl |= FLAG_NAN_X;
             if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;   // mod_evolve.f90:141, next step
             cz_m = cur.cz;
@@ -856,9 +867,10 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         if (z + 1 >= c.z1) break;
         trip(z + 1, I + g.nxny, rB, cB, r1, cur);
     }
-    block_flags(fl, &sc->flags);
-    block_flags(fl_next, &sc->flags_next);
-    block_partials<3>(acc, partials);
+    block_flags(fl & (FLAG_NAN_X | FLAG_TINY_NEXT), &sc->flags);
+    block_flags(fl & (FLAG_NAN_SCAT | FLAG_NAN_S), &sc->flags_next);
+    acc[3] = r0r[threadIdx.x];
+    block_partials<4>(acc, partials);
 }
 
 // operand inputs of the fused FP64 step: T^n, T^{n-1}, the diagonal (s is formed from it as

@@ -156,6 +156,31 @@ def cpu_baseline(n_full: int, tol: float, sample_n: int, sample_steps: int) -> d
                 sample_seconds=round(dtb, 3), builds=out)
 
 
+def per_cell_rc_bench(s, steps: int, warmup: int, tol: float) -> dict:
+    """The FP32-IR step on the same grid with rho varied by up to 0.1% per cell.  The workload's
+    rho c_p and tau' rho c_p are uniform, which the step kernels use (one value read for every
+    cell, 16 B/cell less); this leg times the per-cell reads of a general system."""
+    from heatflow_amd import Simulation
+    c = s.cells
+    rho0 = c.rho
+    rng = np.random.default_rng(7)
+    c.rho = rho0 * (1.0 + 1.0e-3 * rng.uniform(-1.0, 1.0, rho0.shape[0]))
+    try:
+        sim = Simulation(s, precision="fp32_ir", tol=tol)
+        try:
+            dt, st, kt, _ = timed_run(sim, steps, warmup)
+        finally:
+            sim.close()
+    finally:
+        c.rho = rho0
+    ks = kernel_summary(kt, s.params.n)
+    out = dict(timesteps_per_s=steps / dt, ms_per_step=1000.0 * dt / steps,
+               cg_iters=int(st.inner_iters), k_ir_step=ks.get("k_ir_step"))
+    log(f"[bench] per-cell rho c_p fp32_ir@{tol:g}: {steps / dt:.2f} steps/s, "
+        f"k_ir_step {out['k_ir_step']}")
+    return out
+
+
 def stiff_bench(s, steps: int, tol: float, dt_factor: float = 1000.0) -> dict:
     """The 512^3 workload at a 1000x longer time step (5e-8 s): about 60 CG iterations per step
     instead of one, so the solve, not the per-step passes, dominates.  Both precisions at
@@ -481,6 +506,7 @@ def main():
 
     head = results.get(f"fp32_ir@{args.tol:g}") or next(iter(results.values()))
     fp64 = results.get(f"fp64@{args.tol:g}")
+    percell = None if args.only else per_cell_rc_bench(s, K, W, args.tol)
     stiff = None if (args.no_stiff or args.only) else stiff_bench(s, args.stiff_steps, args.tol)
     grad = None if (args.no_grad or args.only) else grad_bench(args.grad_n, 200, 20, fd_cells=12)
     grad5 = None if (args.no_grad or args.no_c5 or args.only) else grad_c5_bench(args.c5_n, 500)
@@ -503,13 +529,15 @@ def main():
         "config": {"workload": f"c3: {args.n}^3 Cattaneo, log-normal kappa exp(N(ln50,1)) seed 3, "
                                f"Gaussian heating sigma=32 cells peak 1e12 W/m^3, Dirichlet 300 K "
                                f"on 6 faces, dt=5e-11, {K} steps, CG tol {args.tol:g}",
-                   "cells": N},
+                   "cells": N,
+                   "uniform_rho_cp": True},
         "fp32_ir_over_fp64": (round(head["timesteps_per_s"] / fp64["timesteps_per_s"], 3)
                               if fp64 and head is not fp64 else None),
         "modes": results,
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "grad_128": grad,
+        "per_cell_rho_cp": percell,
         "stiff_dt_x1000": stiff,
         "grad_c5_384": grad5,
     }

@@ -69,6 +69,13 @@ static int check_launch(hf_ctx *c, const char *what) {
 
 // CG solves may stop after their first iteration unless debug bit 1 turns that off
 static bool allow_short(const hf_ctx *c) { return (c->debug & 2) == 0; }
+// index mask of the rho c_p / tau' rho c_p reads: 0 when both are uniform (every cell reads
+// element 0; debug bit 3 keeps the per-cell reads)
+static uint32_t rc_mask(const hf_ctx *c) {
+    return c->uniform_rc && (c->debug & 8) == 0 ? 0u : ~0u;
+}
+// bytes per cell of the rho c_p and tau' rho c_p reads of a step kernel
+static double rc_bytes(const hf_ctx *c) { return rc_mask(c) ? 16.0 : 0.0; }
 
 // ---- kernel timing -----------------------------------------------------------------------
 
@@ -367,6 +374,18 @@ __global__ void k_derived(uint32_t N, const double *__restrict__ rho,
     ells[I] = em ? em[I] * lx[I] * ly[I] * kStefBoltz : 0.0;
 }
 
+// Sets sc->varying when rho c_p or tau' rho c_p differs from its value at cell 0 somewhere
+// (sc->varying cleared before).  Uniform fields let the step kernels read element 0 only.
+__global__ void __launch_bounds__(kBlock)
+k_uniform_check(uint32_t N, const double *__restrict__ rhoc, const double *__restrict__ cvec,
+                Scal *sc) {
+    const double r0 = rhoc[0], c0 = cvec[0];
+    int32_t diff = 0;
+    for (uint32_t I = blockIdx.x * kBlock + threadIdx.x; I < N; I += gridDim.x * kBlock)
+        if (!(ldnt(&rhoc[I]) == r0) || !(ldnt(&cvec[I]) == c0)) diff = 1;
+    block_flags(diff, &sc->varying);
+}
+
 extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho,
                                const double *heat_capacity, const double *tau_scaled,
                                const double *em, const double *len_x, const double *len_y,
@@ -397,6 +416,18 @@ extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho
                        (uint32_t)N, rho_d, hc_d, c->tau_s, em_d, c->lx, c->ly, c->rhoc, c->cvec,
                        c->ells);
     TRY(check_launch(c, "k_derived"));
+    {
+        int32_t varying = 0;
+        HIPCHK(c, hipMemsetAsync(&c->sc->varying, 0, sizeof(int32_t), c->st));
+        const uint32_t nb = (uint32_t)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
+        hipLaunchKernelGGL(k_uniform_check, dim3(nb), dim3(kBlock), 0, c->st, (uint32_t)N,
+                           c->rhoc, c->cvec, c->sc);
+        TRY(check_launch(c, "k_uniform_check"));
+        HIPCHK(c, hipMemcpyAsync(&varying, &c->sc->varying, sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        c->uniform_rc = varying == 0;
+    }
     if (qdens_extra) {
         TRY(dalloc(c, &c->qextra, N));
         HIPCHK(c, hipMemcpyAsync(c->qextra, qdens_extra, B8, hipMemcpyDefault, c->st));
@@ -718,7 +749,7 @@ static int step_fp64_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err)
     (void)itime;
     const hf_params &p = c->p;
     const double N = (double)c->g.N;
-    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
+    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo, rc_mask(c)};
     const double *rin;
     if (c->carry) {
         // step start and the scalars of the carried residual in one launch
@@ -728,7 +759,7 @@ static int step_fp64_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err)
     } else {
         launch_step_begin(c->sc, c->st);
         // k_rhs_res: read Tp, Tpp, rho c_p, tau' rho c_p, Qdens, B, diag, 3 couplings; write r^
-        long sr = tbeg(c, HF_K_RHS64, N * 8 * 11);
+        long sr = tbeg(c, HF_K_RHS64, N * (8 * 9 + rc_bytes(c)));
         launch_rhs_res_state64(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec,
                                c->d, c->cx, c->cy, c->cz, c->r64, nullptr, c->sc, c->partials,
                                c->st);
@@ -749,7 +780,7 @@ static int step_fp64_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err)
     double *rn = rin == c->r64 ? c->r64b : c->r64;
     // k_step64: read Tp, Tpp, diag, y, rho c_p, tau' rho c_p, Qdens, B, 3 couplings; write x and
     // the next r^
-    long sf = tbeg(c, HF_K_STEP64, N * 8 * (11 + 2));
+    long sf = tbeg(c, HF_K_STEP64, N * (8 * 11 + rc_bytes(c)));
     launch_step64(c->g, rp, c->tiny_guess, c->T[0], c->T[1], c->y64, rin, c->rhoc, c->cvec,
                   c->qd, c->bvec, c->d, c->cx, c->cy, c->cz, c->T[2], rn, c->sc, c->partials,
                   c->st);
@@ -791,9 +822,9 @@ static int step_fp64(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         c->qb_valid = false;
         tend(c, sq);
     }
-    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
+    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo, rc_mask(c)};
     // k_rhs_res: read Tp, Tpp, rho c_p, tau' rho c_p, Qdens, B, diag, 3 couplings; write r^
-    long sr = tbeg(c, HF_K_RHS64, N * 8 * (10 + 1 + ((c->debug & 1) ? 1 : 0)));
+    long sr = tbeg(c, HF_K_RHS64, N * (8 * (9 + ((c->debug & 1) ? 1 : 0)) + rc_bytes(c)));
     launch_rhs_res_state64(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec, c->d,
                            c->cx, c->cy, c->cz, c->r64, (c->debug & 1) ? c->S : nullptr, c->sc,
                            c->partials, c->st);
@@ -872,7 +903,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     (void)itime;
     const hf_params &p = c->p;
     const double N = (double)c->g.N;
-    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
+    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo, rc_mask(c)};
     const float *rin;
     bool inner_started = false, reuse = false;
     if (c->carry) {
@@ -886,7 +917,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     } else {
         launch_step_begin(c->sc, c->st);
         // k_rhs_res: read 10 FP64 arrays; write r^ (FP32)
-        long sr = tbeg(c, HF_K_RHS32, N * (8 * 10 + 4));
+        long sr = tbeg(c, HF_K_RHS32, N * (8 * 8 + rc_bytes(c) + 4));
         launch_rhs_res_state32(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec,
                                c->d, c->cx, c->cy, c->cz, c->r32, nullptr, nullptr, c->sc,
                                c->partials, c->st);
@@ -902,7 +933,8 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         rin = c->r32;
     }
     c->carry = false;
-    StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo, c->tiny_guess ? 1 : 0, 0};
+    StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo, c->tiny_guess ? 1 : 0, 0,
+                  rc_mask(c)};
     const int64_t sweeps_max = std::min<int64_t>(p.itmax, 64);
     // refined x ping-pongs between the spare state buffer and S (unused by this path)
     double *xbuf[2] = {c->T[2], c->S};
@@ -926,7 +958,8 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         sp.store_res = sweeps + 1 < c->last_sweeps ? 1 : 0;
         auto pass = [&]() {
             long si = tbeg(c, HF_K_IR_STEP,
-                           N * (8 * 9 + 4 + 8 + 4 + (sp.store_res ? 4 : 0)) + (xa ? N * 8 : 0));
+                           N * (8 * 7 + rc_bytes(c) + 4 + 8 + 4 + (sp.store_res ? 4 : 0)) +
+                               (xa ? N * 8 : 0));
             launch_ir_step(c->g, sp, xa == nullptr, c->T[0], c->T[1], xa, c->y32, rin, c->sci,
                            c->rhoc, c->cvec, c->qb, c->d, c->cx, c->cy, c->cz, xo, rres, rn,
                            c->sc, c->partials, c->st);
@@ -1018,10 +1051,10 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         c->qb_valid = false;
         tend(c, sq);
     }
-    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo};
+    RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo, rc_mask(c)};
     // x0 goes to T[2]; S is kept for the refinement residuals.
     // k_rhs_res: read 10 FP64 arrays; write r^ (FP32), S and x0 (FP64)
-    long sr = tbeg(c, HF_K_RHS32, N * (8 * 10 + 4 + 8 + 8));
+    long sr = tbeg(c, HF_K_RHS32, N * (8 * 8 + rc_bytes(c) + 4 + 8 + 8));
     launch_rhs_res_state32(c->g, rp, c->T[0], c->T[1], c->rhoc, c->cvec, c->qd, c->bvec, c->d,
                            c->cx, c->cy, c->cz, c->r32, c->S, c->T[2], c->sc, c->partials, c->st);
     tend(c, sr);

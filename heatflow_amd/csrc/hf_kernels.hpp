@@ -27,11 +27,14 @@ struct HeatParams {
     int32_t any_heater;
 };
 
+// rc_mask: the index mask of the rho c_p and tau' rho c_p reads, ~0 (per cell) or 0 (uniform
+// fields: every cell reads element 0)
 struct RhsParams {
     double inverse_time;
     double tol;
     int64_t itmax;
     int32_t isteady, icattaneo;
+    uint32_t rc_mask = ~0u;
 };
 
 struct IrParams {
@@ -45,6 +48,7 @@ struct StepParams {
     int32_t isteady, icattaneo;
     int32_t tiny_guess;        // form the next residual for this outcome of its TINY test
     int32_t store_res;         // store this step's residual (needed only for a further sweep)
+    uint32_t rc_mask = ~0u;    // as RhsParams::rc_mask
 };
 
 void launch_assemble(const Dims &g, const AsmParams &ap, const double *kappa, const double *lx,

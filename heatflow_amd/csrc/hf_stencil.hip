@@ -453,8 +453,8 @@ k_rhs_res_z(Dims g, RhsParams rp, const double *__restrict__ tp, const double *_
     const NbOff o = nb_offsets_in(g, c);
     auto cload = [&](uint32_t I) -> RhsCell {
         RhsCell v;
-        v.rc = (FromState && Mode != kSteady) ? ldnt(&rhoc[I]) : 0.0;
-        v.cv = (FromState && Mode == kCattaneo) ? ldnt(&cvec[I]) : 0.0;
+        v.rc = (FromState && Mode != kSteady) ? ldnt(&rhoc[I & rp.rc_mask]) : 0.0;
+        v.cv = (FromState && Mode == kCattaneo) ? ldnt(&cvec[I & rp.rc_mask]) : 0.0;
         v.qd = FromState ? ldnt(&qd[I]) : ldnt(&bin[I]);
         v.bv = FromState ? ldnt(&bvec[I]) : 0.0;
         v.h = hcell_load(I, o, d, cx, cy, cz);
@@ -710,8 +710,8 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         StepCell v;
         v.cz = ldnt(&cz[I]);
         v.qb = ldnt(&qb[I]);
-        v.rc = Transient ? ldnt(&rhoc[I]) : 0.0;
-        v.cv = (Transient && Cattaneo) ? ldnt(&cvec[I]) : 0.0;
+        v.rc = Transient ? ldnt(&rhoc[I & sp_.rc_mask]) : 0.0;
+        v.cv = (Transient && Cattaneo) ? ldnt(&cvec[I & sp_.rc_mask]) : 0.0;
         v.cx = cx[I];
         v.cy = cy[I];
         v.cxm = cx[I + o.xm];
@@ -924,8 +924,8 @@ k_step64_z(Dims g, RhsParams rp, int32_t guess, const double *__restrict__ tp,
     // the diagonal comes with the cell's own operand record (ri.d), not with these
     auto cload = [&](uint32_t I) -> RhsCell {
         RhsCell v;
-        v.rc = Mode != kSteady ? ldnt(&rhoc[I]) : 0.0;
-        v.cv = Mode == kCattaneo ? ldnt(&cvec[I]) : 0.0;
+        v.rc = Mode != kSteady ? ldnt(&rhoc[I & rp.rc_mask]) : 0.0;
+        v.cv = Mode == kCattaneo ? ldnt(&cvec[I & rp.rc_mask]) : 0.0;
         v.qd = ldnt(&qd[I]);
         v.bv = ldnt(&bvec[I]);
         v.h = HCell{0.0, cx[I], cy[I], ldnt(&cz[I]), cx[I + o.xm], cy[I + o.ym]};

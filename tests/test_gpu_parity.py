@@ -486,6 +486,24 @@ def test_fp32_ir_alpha_reuse_miss():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
+def test_uniform_heat_capacity_reads_are_exact(prec):
+    """Where rho c_p and tau' rho c_p are the same in every cell (found at upload), the step
+    kernels read one value for all cells; debug bit 3 keeps the per-cell reads.  The values
+    are the same, so 12 steps of the log-normal workload (uniform rho, c_p, tau) give the same
+    bits either way."""
+    s, _ = workloads.c3(24)
+    out = []
+    for dbg in (0, 8):
+        with Simulation(s, precision=prec, tol=1e-14) as sim:
+            sim.set_debug(dbg)
+            sim.initial_evolve()
+            sim.run(1, 12)
+            out.append(sim.get_state())
+    assert np.array_equal(bits(out[0][0]), bits(out[1][0]))
+    assert np.array_equal(bits(out[0][1]), bits(out[1][1]))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
 def test_first_iteration_stop_is_exact(prec):
     """A CG solve whose first iteration already meets the tolerance stops there: the next
     residual norm is predicted as alpha^2 q.q - r.r and y = alpha p is left to its consumer,

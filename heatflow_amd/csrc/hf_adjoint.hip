@@ -65,6 +65,9 @@ struct AdjParams {
     int32_t isteady, icattaneo;
     int32_t bface[6];   // face is a domain boundary that enters the diagonal (non-periodic)
     int32_t cg[6];
+    // index masks of the rho c_p / tau' rho c_p and volume reads: 0 where the field is uniform
+    // (every cell reads element 0), else ~0
+    uint32_t rc_mask, vol_mask;
 };
 
 __device__ __forceinline__ double dG_int(double ki, double kj, double L, double br) {
@@ -123,11 +126,11 @@ k_adj_step(Dims g, AdjParams q, const double *__restrict__ T, const double *__re
             }
         }
         gk[I] += acc;
-        gp[I] += -li / vol[I];
+        gp[I] += -li / vol[I & q.vol_mask];
         double a = 0.0, cc = 0.0;
         if (q.isteady == 0) {
-            a = q.inverse_time * rhoc[I];
-            cc = q.icattaneo == 1 ? cvec[I] : 0.0;
+            a = q.inverse_time * rhoc[I & q.rc_mask];
+            cc = q.icattaneo == 1 ? cvec[I & q.rc_mask] : 0.0;
         }
         g_lo[I] += -(a + 2.0 * cc) * li;
         g_new[I] = cc * li;
@@ -271,6 +274,8 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
     q.br = p.br;
     q.isteady = p.isteady;
     q.icattaneo = p.icattaneo;
+    q.rc_mask = c->uniform_rc && (c->debug & 8) == 0 ? 0u : ~0u;
+    q.vol_mask = c->uniform_vol && (c->debug & 8) == 0 ? 0u : ~0u;
 
     const int64_t K = A.K, nsteps = A.nsteps, itime0 = A.itime0;
     const int64_t nseg = (nsteps + K - 1) / K;

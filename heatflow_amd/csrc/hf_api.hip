@@ -374,15 +374,18 @@ __global__ void k_derived(uint32_t N, const double *__restrict__ rho,
     ells[I] = em ? em[I] * lx[I] * ly[I] * kStefBoltz : 0.0;
 }
 
-// Sets sc->varying when rho c_p or tau' rho c_p differs from its value at cell 0 somewhere
-// (sc->varying cleared before).  Uniform fields let the step kernels read element 0 only.
+// Sets bit 0 of sc->varying when rho c_p or tau' rho c_p differs from its value at cell 0
+// somewhere, bit 1 when the volume does (sc->varying cleared before).  Uniform fields let the
+// step and adjoint kernels read element 0 only.
 __global__ void __launch_bounds__(kBlock)
 k_uniform_check(uint32_t N, const double *__restrict__ rhoc, const double *__restrict__ cvec,
-                Scal *sc) {
-    const double r0 = rhoc[0], c0 = cvec[0];
+                const double *__restrict__ vol, Scal *sc) {
+    const double r0 = rhoc[0], c0 = cvec[0], v0 = vol[0];
     int32_t diff = 0;
-    for (uint32_t I = blockIdx.x * kBlock + threadIdx.x; I < N; I += gridDim.x * kBlock)
-        if (!(ldnt(&rhoc[I]) == r0) || !(ldnt(&cvec[I]) == c0)) diff = 1;
+    for (uint32_t I = blockIdx.x * kBlock + threadIdx.x; I < N; I += gridDim.x * kBlock) {
+        if (!(ldnt(&rhoc[I]) == r0) || !(ldnt(&cvec[I]) == c0)) diff |= 1;
+        if (!(ldnt(&vol[I]) == v0)) diff |= 2;
+    }
     block_flags(diff, &sc->varying);
 }
 
@@ -421,12 +424,13 @@ extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho
         HIPCHK(c, hipMemsetAsync(&c->sc->varying, 0, sizeof(int32_t), c->st));
         const uint32_t nb = (uint32_t)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
         hipLaunchKernelGGL(k_uniform_check, dim3(nb), dim3(kBlock), 0, c->st, (uint32_t)N,
-                           c->rhoc, c->cvec, c->sc);
+                           c->rhoc, c->cvec, c->vol, c->sc);
         TRY(check_launch(c, "k_uniform_check"));
         HIPCHK(c, hipMemcpyAsync(&varying, &c->sc->varying, sizeof(int32_t),
                                  hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
-        c->uniform_rc = varying == 0;
+        c->uniform_rc = (varying & 1) == 0;
+        c->uniform_vol = (varying & 2) == 0;
     }
     if (qdens_extra) {
         TRY(dalloc(c, &c->qextra, N));

@@ -76,6 +76,7 @@ struct hf_ctx {
     bool uploaded = false, assembled = false, has_state = false;
     bool any_heater = false, dyn_heat = false;
     bool uniform_rc = false;     // rho c_p and tau' rho c_p are the same in every cell
+    bool uniform_vol = false;    // the cell volume is the same in every cell
     double heated_volume = 0.0;
     int32_t setup_flags = 0;
     int64_t pred_iters = 4, pred_inner = 8, pred_adj = 8;

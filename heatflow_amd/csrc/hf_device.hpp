@@ -106,7 +106,8 @@ struct Scal {
     int32_t shortcut; // the solve stopped after its first iteration: y = alpha p (p = r0)
     int32_t flags_next;  // NaN bits of the next step's S, computed ahead by the fused step
     int32_t allow_short; // k_fin_a may end a solve after its first iteration
-    int32_t varying;     // set at upload when rho c_p or tau' rho c_p differs between cells
+    int32_t varying;     // set at upload: bit 0 rho c_p or tau' rho c_p, bit 1 the volume
+                         // differs between cells
     // next step's ||b^||^2 and ||r^0||^2 without / with the TINY shift (fused mixed step)
     double next_bnorm2, next_rr[2];
     // the solution y (CgY below): y holds at least one term (yset); the term alpha p_kpend

@@ -157,8 +157,9 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * from p.q and q.q), and with it the reuse of its step length; bit 2 turns off that reuse
  * alone (mixed mode: a step after one whose inner solve ended on its first iteration takes
  * y = alpha r^0 with that solve's alpha, and solves only if the result misses the inner
- * target); bit 3 keeps per-cell reads of rho c_p and tau' rho c_p where both are uniform
- * (found at upload; the step kernels then read one value for every cell). */
+ * target); bit 3 keeps per-cell reads of rho c_p and tau' rho c_p where both are uniform,
+ * and of the cell volume in the adjoint (found at upload; the kernels then read one value for
+ * every cell). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */

@@ -183,3 +183,22 @@ def test_adjoint_mixed_precision_matches_fp64():
     assert np.abs(gk32 - gk64).max() <= 1e-6 * np.abs(gk64).max()
     h = m["heaters"]
     assert np.abs(gp32[h] - gp64[h]).max() <= 1e-6 * np.abs(gp64[h]).max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
+def test_adjoint_uniform_field_reads_are_exact(prec):
+    """The 128^3 gradient workload has uniform rho c_p, tau' rho c_p and cell volume; the
+    adjoint kernel (and the forward step kernels) then read one value for every cell, and debug
+    bit 3 keeps the per-cell reads.  Loss and both gradients come out bit-identical."""
+    s, m = workloads.c4(16, nprobe=16)
+    targets = np.full(len(m["probes"]), 300.0)
+    out = []
+    for dbg in (0, 8):
+        with Simulation(s, precision=prec, tol=1e-13) as sim:
+            sim.set_debug(dbg)
+            sim.initial_evolve()
+            out.append(sim.grad(1, 12, 4, m["probes"], targets))
+    (la, ka, pa), (lb, kb, pb) = out
+    assert la == lb
+    assert np.array_equal(ka.view(np.int64), kb.view(np.int64))
+    assert np.array_equal(pa.view(np.int64), pb.view(np.int64))

@@ -83,14 +83,17 @@ def roofline(kt: dict) -> dict:
 
 
 def measured_traffic(kernel: str, n: int):
-    """HBM bytes per launch of `kernel` at n^3 from the committed rocprofv3 counter summary
+    """HBM bytes per launch of `kernel` at n^3 from the committed rocprofv3 counter summaries
     (FETCH_SIZE / WRITE_SIZE passes, scaled by a copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_c3-{n}_fp32-ir.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        d = json.load(f)
-    return d["bytes_per_launch"] if d.get("kernel") == kernel and d.get("n") == n else None
+    for mode in ("fp32-ir", "fp64"):
+        path = os.path.join(ROOT, "profiles", f"hbm_traffic_c3-{n}_{mode}.json")
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("kernel") == kernel and d.get("n") == n:
+            return d["bytes_per_launch"]
+    return None
 
 
 def rel_errors(a, ref):
@@ -512,7 +515,8 @@ def main():
     grad5 = None if (args.no_grad or args.no_c5 or args.only) else grad_c5_bench(args.c5_n, 500)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, args.tol, args.cpu_n,
                                                                 args.cpu_steps)
-    head["roofline"]["traffic"] = measured_traffic(head["roofline"]["kernel"], args.n)
+    for r in results.values():
+        r["roofline"]["traffic"] = measured_traffic(r["roofline"]["kernel"], args.n)
     line = {
         "metric": "timesteps/s at 512^3 (FP32-refined vs FP64, 1e-10 rel); fwd+adjoint grad time",
         "value": round(head["timesteps_per_s"], 4),

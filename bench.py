@@ -319,8 +319,9 @@ def grad_bench(n: int, steps: int, ckpt: int, fd_cells: int = 16) -> dict:
 
 def grad_c5_bench(n: int, steps: int) -> dict:
     """The 384^3 layered workload: forward + adjoint with sqrt(steps) checkpoints, FP32-inner /
-    FP64-refined and plain FP64 solves both stopping at a scaled residual of 1e-13, each against
-    the FP64 gradient at the reference tolerance 1e-32; device memory held by each context."""
+    FP64-refined and plain FP64 solves both stopping at a scaled residual of 1e-13 (and the
+    FP32-IR run once more with the adjoint solves at 1e-10), each against the FP64 gradient at
+    the reference tolerance 1e-32; device memory held by each context."""
     import torch
     from heatflow_amd import Simulation, workloads
     s, m = workloads.c5(n)
@@ -328,12 +329,14 @@ def grad_c5_bench(n: int, steps: int) -> dict:
     targets = np.full(len(m["probes"]), 300.0)
     res = {}
     grads = {}
-    for prec, tol in (("fp64", 1e-32), ("fp32_ir", 1e-13), ("fp64", 1e-13)):
-        key = f"{prec}@{tol:g}"
+    for prec, tol, atol in (("fp64", 1e-32, 0.0), ("fp32_ir", 1e-13, 0.0), ("fp64", 1e-13, 0.0),
+                            ("fp32_ir", 1e-13, 1e-10)):
+        key = f"{prec}@{tol:g}" + (f"/adjoint@{atol:g}" if atol else "")
         torch.cuda.synchronize()
         free0, total = torch.cuda.mem_get_info()
         sim = Simulation(s, precision=prec, tol=tol)
         try:
+            sim.set_adjoint_tol(atol)
             sim.initial_evolve()
             sync()
             t0 = time.perf_counter()
@@ -347,7 +350,7 @@ def grad_c5_bench(n: int, steps: int) -> dict:
         res[key] = dict(seconds=round(secs, 3), loss=loss, device_bytes=int(free0 - free1))
         log(f"[bench] grad c5 {n}^3 {key}: {res[key]}")
     L0, gk0, gp0 = grads["fp64@1e-32"]
-    for key in ("fp32_ir@1e-13", "fp64@1e-13"):
+    for key in ("fp32_ir@1e-13", "fp64@1e-13", "fp32_ir@1e-13/adjoint@1e-10"):
         L, gk, gp = grads[key]
         res[key]["err_vs_fp64_reftol"] = dict(
             loss=abs(L - L0) / abs(L0),

@@ -25,7 +25,7 @@ EXPORTS = (
     "hf_set_state", "hf_get_state", "hf_step", "hf_run", "hf_solve", "hf_get_vector",
     "hf_set_debug", "hf_get_stats", "hf_reset_stats", "hf_set_solver", "hf_grad",
     "hf_set_timing", "hf_kernel_times", "hf_forward_checkpointed", "hf_adjoint",
-    "hf_set_fields",
+    "hf_set_fields", "hf_set_adjoint_tol",
     "hf_last_error", "hf_destroy", "hf_version",
 )
 
@@ -106,6 +106,7 @@ def load() -> C.CDLL:
         "hf_forward_checkpointed": (C.c_int, [P, C.c_int64, C.c_int64, C.c_int64]),
         "hf_adjoint": (C.c_int, [P, P, P, P]),
         "hf_set_fields": (C.c_int, [P, P, P]),
+        "hf_set_adjoint_tol": (C.c_int, [P, C.c_double]),
         "hf_kernel_times": (C.c_int, [P, P, P, P]),
         "hf_grad": (C.c_int, [P, C.c_int64, C.c_int64, C.c_int64, C.c_int64, P, P, P, D, P, P]),
         "hf_last_error": (C.c_char_p, [P]),

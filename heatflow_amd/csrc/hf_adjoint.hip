@@ -279,6 +279,7 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
 
     const int64_t K = A.K, nsteps = A.nsteps, itime0 = A.itime0;
     const int64_t nseg = (nsteps + K - 1) / K;
+    const double atol = c->adj_tol > 0.0 ? c->adj_tol : p.tol;   // hf_set_adjoint_tol
     int64_t solved = 0;   // adjoint solves done: lam (and lam_prev) hold their last solutions
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t b = s * K, e = std::min(nsteps, (s + 1) * K);
@@ -294,13 +295,13 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
             // lambda^{n+2} for the second, then 2 lambda^{n+2} - lambda^{n+3} (the adjoint
             // recurrence has the forward step's form, whose guess is 2 T^n - T^{n-1}).
             if (solved == 0) {
-                ATRY(ctx_solve_dev(c, A.g_hi, A.lam, p.tol, p.itmax, &it));
+                ATRY(ctx_solve_dev(c, A.g_hi, A.lam, atol, p.itmax, &it));
             } else {
                 if (solved == 1)
                     AHIP(c, hipMemcpyAsync(A.lam_next, A.lam, B8, hipMemcpyDeviceToDevice, c->st));
                 else
                     launch_extrap(c->g, A.lam, A.lam_prev, A.lam_next, c->st);
-                ATRY(ctx_solve_dev(c, A.g_hi, A.lam_next, p.tol, p.itmax, &it));
+                ATRY(ctx_solve_dev(c, A.g_hi, A.lam_next, atol, p.itmax, &it));
                 double *old_prev = A.lam_prev;
                 A.lam_prev = A.lam;
                 A.lam = A.lam_next;

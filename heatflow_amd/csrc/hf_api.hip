@@ -1368,6 +1368,13 @@ extern "C" int hf_set_fields(hf_ctx *c, const double *kappa, const double *qdens
     return HF_OK;
 }
 
+extern "C" int hf_set_adjoint_tol(hf_ctx *c, double tol) {
+    if (!c) return HF_ERR_ARG;
+    if (!(tol >= 0.0)) return set_err(c, HF_ERR_ARG, "hf_set_adjoint_tol: tol must be >= 0");
+    c->adj_tol = tol;
+    return HF_OK;
+}
+
 extern "C" int hf_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every,
                        int64_t nprobe, const int64_t *probe_idx, const double *target,
                        const double *weight, double *loss, double *dl_dkappa,

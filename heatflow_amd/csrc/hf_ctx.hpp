@@ -80,6 +80,7 @@ struct hf_ctx {
     double heated_volume = 0.0;
     int32_t setup_flags = 0;
     int64_t pred_iters = 4, pred_inner = 8, pred_adj = 8;
+    double adj_tol = 0.0;        // tolerance of the adjoint solves (0: p.tol)
     hf_stats stats{};
     std::vector<void *> allocs;
     hf::AdjointState *adj = nullptr;

@@ -220,6 +220,10 @@ class Simulation:
         pq, keep_q = _anyptr(qdens_extra)
         check(self.lib.hf_set_fields(self.ctx, pk, pq), self.ctx)
 
+    def set_adjoint_tol(self, tol: float) -> None:
+        """Scaled-residual tolerance of the adjoint solves (0: the forward tolerance)."""
+        check(self.lib.hf_set_adjoint_tol(self.ctx, float(tol)), self.ctx)
+
     def forward_checkpointed(self, itime0: int, nsteps: int, ckpt_every: int) -> None:
         check(self.lib.hf_forward_checkpointed(self.ctx, int(itime0), int(nsteps),
                                                int(ckpt_every)), self.ctx)

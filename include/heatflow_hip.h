@@ -205,11 +205,18 @@ int hf_set_solver(hf_ctx *ctx, double tol, int64_t itmax, int32_t precision, dou
  * checkpoint) seeded with dL/dT^{final} (length N), and returns the gradients with respect to
  * the per-cell conductivity kappa (dl_dkappa, length N) and to a per-cell power P_I that enters
  * as Qdens_I += P_I / volume_I (dl_dpower, length N).  The adjoint solves use the context's
- * precision mode (FP64, or FP32 inner CG inside FP64 refinement) and tolerance.  Requires
+ * precision mode (FP64, or FP32 inner CG inside FP64 refinement) and tolerance (or the one
+ * hf_set_adjoint_tol gives).  Requires
  * per-axis uniform cell lengths; the BR switch (x BR where neighbouring kappas differ) is
  * treated as locally constant.  The context is left at the end state of the forward run. */
 int hf_forward_checkpointed(hf_ctx *ctx, int64_t itime0, int64_t nsteps, int64_t ckpt_every);
 int hf_adjoint(hf_ctx *ctx, const double *dl_dt_final, double *dl_dkappa, double *dl_dpower);
+
+/* Scaled-residual tolerance of the adjoint solves, apart from the forward's (0: the forward's,
+ * the default).  Their right-hand side carries no offset like the forward's 300 K, so the same
+ * relative tolerance asks more of them; the gradient's error is set by the forward solves down
+ * to a much looser adjoint tolerance (at 384^3, forward 1e-13: unchanged up to 1e-10). */
+int hf_set_adjoint_tol(hf_ctx *ctx, double tol);
 
 /* Convenience: forward + reverse for L = sum_k w_k (T^{final}[probe_k] - target_k)^2
  * (weight may be NULL for w = 1); *loss gets L. */

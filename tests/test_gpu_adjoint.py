@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 import torch
 
-from heatflow_amd import Simulation, workloads
+from heatflow_amd import HeatFlowError, Simulation, workloads
 from heatflow_amd.autograd import heatflow_run
 
 from dense_ref import dense_run
@@ -202,3 +202,27 @@ def test_adjoint_uniform_field_reads_are_exact(prec):
     assert la == lb
     assert np.array_equal(ka.view(np.int64), kb.view(np.int64))
     assert np.array_equal(pa.view(np.int64), pb.view(np.int64))
+
+
+def test_adjoint_tolerance_apart():
+    """hf_set_adjoint_tol gives the adjoint solves their own tolerance.  With the forward at the
+    reference tolerance and the adjoint at 1e-10 (scaled residual), the loss is unchanged (it
+    comes from the forward alone) and the gradients stay within 1e-6 of those with the adjoint
+    at the reference tolerance; a negative tolerance is refused."""
+    s, m = workloads.c4(16, nprobe=16)
+    targets = np.full(len(m["probes"]), 300.0)
+    out = []
+    for atol in (0.0, 1e-10):
+        with Simulation(s, tol=1e-32) as sim:
+            sim.set_adjoint_tol(atol)
+            sim.initial_evolve()
+            out.append(sim.grad(1, 12, 4, m["probes"], targets))
+    (la, ka, pa), (lb, kb, pb) = out
+    assert la == lb
+    dk = np.abs(ka - kb).max() / np.abs(ka).max()
+    h = m["heaters"]
+    dp = np.abs(pa[h] - pb[h]).max() / np.abs(pa[h]).max()
+    assert 0.0 < dk <= 1e-6 and dp <= 1e-6, (dk, dp)
+    with Simulation(s) as sim:
+        with pytest.raises(HeatFlowError):
+            sim.set_adjoint_tol(-1.0)

@@ -59,28 +59,6 @@ void adjoint_invalidate(AdjointState *a) {
     if (a) a->valid = false;
 }
 
-struct AdjParams {
-    double inverse_time;
-    double kb[6], tb[6], br;
-    int32_t isteady, icattaneo;
-    int32_t bface[6];   // face is a domain boundary that enters the diagonal (non-periodic)
-    int32_t cg[6];
-    // index masks of the rho c_p / tau' rho c_p and volume reads: 0 where the field is uniform
-    // (every cell reads element 0), else ~0
-    uint32_t rc_mask, vol_mask;
-};
-
-__device__ __forceinline__ double dG_int(double ki, double kj, double L, double br) {
-    double s = ki + kj;
-    double v = 2.0 * kj * kj / (s * s * L * L);
-    return ki != kj ? v * br : v;
-}
-
-__device__ __forceinline__ double dG_bnd(double kb, double k, double L, double br) {
-    double s = kb + k;
-    double v = 2.0 * kb * kb / (s * s * L * L);
-    return k != kb ? v * br : v;
-}
 
 // One reverse step: gradient contributions of step n (T = T^{n+1}, lam = lambda^{n+1}) and the
 // adjoint history update.  g_lo (partial gbar^n) += -(a+2c) lam; g_new = c lam (gbar^{n-1}).
@@ -280,6 +258,9 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
     const int64_t K = A.K, nsteps = A.nsteps, itime0 = A.itime0;
     const int64_t nseg = (nsteps + K - 1) / K;
     const double atol = c->adj_tol > 0.0 ? c->adj_tol : p.tol;   // hf_set_adjoint_tol
+    // the z-marching kernel for grids without periodic axes; debug bit 4 keeps the flat one
+    const bool zmarch_adj =
+        !(p.periodic[0] || p.periodic[1] || p.periodic[2]) && (c->debug & 16) == 0;
     int64_t solved = 0;   // adjoint solves done: lam (and lam_prev) hold their last solutions
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t b = s * K, e = std::min(nsteps, (s + 1) * K);
@@ -309,9 +290,13 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
             }
             solved += 1;
             const double *Tn1 = A.seg + (size_t)N * (n - b);
-            hipLaunchKernelGGL(k_adj_step, dim3(c->g.nblk), dim3(kBlock), 0, c->st, c->g, q, Tn1,
-                               A.lam, c->kappa, c->lx, c->ly, c->lz, c->rhoc, c->cvec, c->vol,
-                               A.g_lo, A.g_new, A.gk, A.gp);
+            if (zmarch_adj)
+                launch_adj_step_z(c->g, q, Tn1, A.lam, c->kappa, c->lx, c->ly, c->lz, c->rhoc,
+                                  c->cvec, c->vol, A.g_lo, A.g_new, A.gk, A.gp, c->st);
+            else
+                hipLaunchKernelGGL(k_adj_step, dim3(c->g.nblk), dim3(kBlock), 0, c->st, c->g, q,
+                                   Tn1, A.lam, c->kappa, c->lx, c->ly, c->lz, c->rhoc, c->cvec,
+                                   c->vol, A.g_lo, A.g_new, A.gk, A.gp);
             AHIP(c, hipGetLastError());
             // shift: gbar^n (complete) becomes the next right-hand side
             double *old_hi = A.g_hi;

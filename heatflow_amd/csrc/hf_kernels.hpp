@@ -51,6 +51,39 @@ struct StepParams {
     uint32_t rc_mask = ~0u;    // as RhsParams::rc_mask
 };
 
+// adjoint step (hf_adjoint.hip): parameters and the derivatives of hmatrixfunc's face
+// conductance G = 2 k_I k_J / ((k_I + k_J) L^2) [x BR if k_I != k_J] with respect to k_I, for
+// an interior face and a bath face (src/heatflow/mod_hmatrix.f90:288-314, 357-385)
+struct AdjParams {
+    double inverse_time;
+    double kb[6], tb[6], br;
+    int32_t isteady, icattaneo;
+    int32_t bface[6];   // face is a domain boundary that enters the diagonal (non-periodic)
+    int32_t cg[6];
+    // index masks of the rho c_p / tau' rho c_p and volume reads: 0 where the field is uniform
+    // (every cell reads element 0), else ~0
+    uint32_t rc_mask, vol_mask;
+};
+
+__device__ __forceinline__ double dG_int(double ki, double kj, double L, double br) {
+    double s = ki + kj;
+    double v = 2.0 * kj * kj / (s * s * L * L);
+    return ki != kj ? v * br : v;
+}
+
+__device__ __forceinline__ double dG_bnd(double kb, double k, double L, double br) {
+    double s = kb + k;
+    double v = 2.0 * kb * kb / (s * s * L * L);
+    return k != kb ? v * br : v;
+}
+
+// z-marching adjoint step on a grid without periodic axes (the same per-cell sums as the flat
+// k_adj_step of hf_adjoint.hip, in the same order)
+void launch_adj_step_z(const Dims &g, const AdjParams &q, const double *T, const double *lam,
+                       const double *kappa, const double *lx, const double *ly, const double *lz,
+                       const double *rhoc, const double *cvec, const double *vol, double *g_lo,
+                       double *g_new, double *gk, double *gp, hipStream_t st);
+
 void launch_assemble(const Dims &g, const AsmParams &ap, const double *kappa, const double *lx,
                      const double *ly, const double *lz, const double *rhoc,
                      const double *tau_s, double *d, double *cx, double *cy, double *cz,

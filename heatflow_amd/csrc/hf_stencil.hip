@@ -1246,6 +1246,84 @@ k_cg_b1_zw(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *_
 
 // k_rhs_res_z for the run-time S mode and periodicity, with the pipelined kernels' z-chunking
 // (launch_fin_start reads its partials)
+// ---- adjoint step ----------------------------------------------------------------------------
+
+// the adjoint step's stencil operand of a cell: lambda^{n+1}, T^{n+1}, kappa
+struct AdjV {
+    double l, t, k;
+    __device__ AdjV() = default;
+    __device__ constexpr AdjV(double v) : l(v), t(v), k(v) {}
+    __device__ constexpr AdjV(double a, double b, double c) : l(a), t(b), k(c) {}
+};
+
+// k_adj_step (hf_adjoint.hip) as a z-march: each cell's face terms over x-, x+, y-, y+, z-, z+
+// (the same terms in the same order), with the neighbours' (lambda, T, kappa) from the staged
+// LDS plane and registers instead of global memory.  No periodic axis (those grids run the
+// flat kernel).
+__global__ void __launch_bounds__(kZTX * kZTY)
+k_adj_step_z(Dims g, AdjParams q, const double *__restrict__ T, const double *__restrict__ lam,
+             const double *__restrict__ kappa, const double *__restrict__ lx,
+             const double *__restrict__ ly, const double *__restrict__ lz,
+             const double *__restrict__ rhoc, const double *__restrict__ cvec,
+             const double *__restrict__ vol, double *__restrict__ g_lo,
+             double *__restrict__ g_new, double *__restrict__ gk, double *__restrict__ gp) {
+    __shared__ PlaneBuf<AdjV> sp;
+    // cell lengths are uniform per axis (checked when the adjoint starts): read once
+    const double Ls[3] = {lx[0], ly[0], lz[0]};
+    const ZCol c = zcolumn(g);
+    auto X = [&](uint32_t J) -> AdjV { return AdjV(lam[J], T[J], kappa[J]); };
+    zmarch<AdjV>(g, c, sp, X, [&](uint32_t I, uint32_t z, AdjV xi, AdjV xz_m, AdjV xz_p,
+                                  const InPlane<AdjV> &nb) {
+        const double li = xi.l, ti = xi.t, ki = xi.k;
+        const uint32_t pos[3] = {c.x, c.y, z};
+        const uint32_t len[3] = {g.nx, g.ny, g.nz};
+        const bool has[6] = {c.x > 0, c.x + 1 < g.nx, c.y > 0, c.y + 1 < g.ny, z > 0,
+                             z + 1 < g.nz};
+        // which boundary faces still contribute to B_I after CG assignments
+        // (mod_boundary.f90:72-150)
+        int survive = 0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int ax = f >> 1;
+            const bool at_face = pos[ax] == ((f & 1) ? len[ax] - 1 : 0u);
+            if (!q.bface[f] || !at_face) continue;
+            if (q.cg[f]) survive = 0;
+            else survive |= (1 << f);
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int ax = f >> 1;
+            if (has[f]) {
+                const AdjV v = f == 0 ? nb.xm() : f == 1 ? nb.xp() : f == 2 ? nb.ym()
+                             : f == 3 ? nb.yp() : f == 4 ? xz_m : xz_p;
+                acc += dG_int(ki, v.k, Ls[ax], q.br) * (li - v.l) * (ti - v.t);
+            } else if (q.bface[f]) {
+                const double tb = (survive >> f) & 1 ? q.tb[f] : 0.0;
+                acc += dG_bnd(q.kb[f], ki, Ls[ax], q.br) * li * (ti - tb);
+            }
+        }
+        gk[I] += acc;
+        gp[I] += -li / vol[I & q.vol_mask];
+        double a = 0.0, cc = 0.0;
+        if (q.isteady == 0) {
+            a = q.inverse_time * rhoc[I & q.rc_mask];
+            cc = q.icattaneo == 1 ? cvec[I & q.rc_mask] : 0.0;
+        }
+        g_lo[I] += -(a + 2.0 * cc) * li;
+        g_new[I] = cc * li;
+    });
+}
+
+void launch_adj_step_z(const Dims &g0, const AdjParams &q, const double *T, const double *lam,
+                       const double *kappa, const double *lx, const double *ly, const double *lz,
+                       const double *rhoc, const double *cvec, const double *vol, double *g_lo,
+                       double *g_new, double *gk, double *gp, hipStream_t st) {
+    const Dims g = zdims(g0, kZDefault);
+    HF_ZLAUNCH(k_adj_step_z, g, st, g, q, T, lam, kappa, lx, ly, lz, rhoc, cvec, vol, g_lo, g_new,
+               gk, gp);
+}
+
 template <typename OutT, bool FS, bool SS, bool SX>
 static void launch_rhs(const Dims &g0, const RhsParams &rp, const double *tp, const double *tpp,
                        const double *xin, const double *bin, const double *rhoc,

@@ -159,7 +159,8 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * y = alpha r^0 with that solve's alpha, and solves only if the result misses the inner
  * target); bit 3 keeps per-cell reads of rho c_p and tau' rho c_p where both are uniform,
  * and of the cell volume in the adjoint (found at upload; the kernels then read one value for
- * every cell). */
+ * every cell); bit 4 runs the adjoint step with the flat per-cell kernel instead of the
+ * z-marching one (grids with a periodic axis always use the flat kernel). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */

@@ -261,18 +261,23 @@ __device__ __forceinline__ void zmarch_own(const Dims &g, const ZCol &c, PlaneBu
 // per loop iteration alternate the register sets that carry loads between trips (a copy at
 // the end of a trip would wait for the loads just issued; the compiler does not unroll a loop
 // with a barrier by itself).  Wrap = false: no axis is periodic.
-template <typename V, typename R, typename C, bool Wrap, class XL, class XF, class CL, class CF>
-__device__ __forceinline__ void zmarch_pipe(const Dims &g, const ZCol &c, PlaneBuf<V> &sp,
-                                            XL load, XF form, CL cload, CF cell) {
+// The own cell's operand is formed by fown(R, E &), which may also hand the cell values it
+// computed on the way (E: the Jacobi scale, say), so the cell need not compute them again;
+// cell(..., ri, ei) gets them with its raw record.
+template <typename V, typename R, typename C, typename E, bool Wrap, class XL, class XF,
+          class XO, class CL, class CF>
+__device__ __forceinline__ void zmarch_pipe_own(const Dims &g, const ZCol &c, PlaneBuf<V> &sp,
+                                                XL load, XF form, XO fown, CL cload, CF cell) {
     const Ring rg = ring_cell(g, c);
     const bool per_z = Wrap && g.per[2];
     V xz_m = (V)0, xi = (V)0;
     R rA{}, rB{}, ri{};
     C cA{}, cB{};
+    E ei{};
     if (c.active) {
         if (c.z0 > 0 || per_z) xz_m = form(load(zm_index(g, c.col, c.z0)));
         ri = load(c.col + g.nxny * c.z0);
-        xi = form(ri);
+        xi = fown(ri, ei);
         if (c.z0 + 1 < g.nz || per_z) rA = load(zp_index(g, c.col, c.z0));
         cA = cload(c.col + g.nxny * c.z0);
     }
@@ -292,10 +297,11 @@ __device__ __forceinline__ void zmarch_pipe(const Dims &g, const ZCol &c, PlaneB
         cn = cload(c.active && more ? I + g.nxny : own);
         __syncthreads();   // plane z is staged in sp.v[buf]
         V xz_p = (V)0;
+        E en{};
         if (c.active) {
-            xz_p = has_p ? form(r1) : (V)0;
+            xz_p = has_p ? fown(r1, en) : (V)0;
             const InPlane<V> nb{&sp.v[buf][c.ly + 1][c.lx + 1]};
-            cell(I, z, xi, xz_m, xz_p, nb, cur, ri);
+            cell(I, z, xi, xz_m, xz_p, nb, cur, ri, ei);
         }
         if (more) {
             // sp.v[buf ^ 1] was last read before this trip's barrier
@@ -306,6 +312,7 @@ __device__ __forceinline__ void zmarch_pipe(const Dims &g, const ZCol &c, PlaneB
         xz_m = xi;
         xi = xz_p;
         ri = r1;
+        ei = en;
     };
     uint32_t I = c.col + g.nxny * c.z0;
     for (uint32_t z = c.z0; z < c.z1; z += 2, I += 2 * g.nxny) {
@@ -313,6 +320,17 @@ __device__ __forceinline__ void zmarch_pipe(const Dims &g, const ZCol &c, PlaneB
         if (z + 1 >= c.z1) break;
         trip(z + 1, I + g.nxny, rB, cB, rA, cA);
     }
+}
+
+struct NoOwn {};
+
+template <typename V, typename R, typename C, bool Wrap, class XL, class XF, class CL, class CF>
+__device__ __forceinline__ void zmarch_pipe(const Dims &g, const ZCol &c, PlaneBuf<V> &sp,
+                                            XL load, XF form, CL cload, CF cell) {
+    zmarch_pipe_own<V, R, C, NoOwn, Wrap>(
+        g, c, sp, load, form, [&](const R &r, NoOwn &) -> V { return form(r); }, cload,
+        [&](uint32_t I, uint32_t z, V xi, V xz_m, V xz_p, const InPlane<V> &nb, const C &cur,
+            const R &ri, const NoOwn &) { cell(I, z, xi, xz_m, xz_p, nb, cur, ri); });
 }
 
 // Offsets from a cell to its in-grid x- / y- neighbours' couplings (stored at the neighbour;
@@ -703,6 +721,19 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         const double xn = base + (1.0 / sqrt(-r.d)) * (double)(ysc * r.y);
         return D2(xn, xn + (xn - r.tp));
     };
+    // form for the thread's own cells, also handing back s (same operations)
+    auto form_s = [&](const StepRaw &r, double &si) -> D2 {
+        double base;
+        if (FromState) {
+            base = r.tp + (r.tp - r.b);
+            if (tiny) base = base + kTiny;
+        } else {
+            base = r.b;
+        }
+        si = 1.0 / sqrt(-r.d);
+        const double xn = base + si * (double)(ysc * r.y);
+        return D2(xn, xn + (xn - r.tp));
+    };
     auto X = [&](uint32_t J) -> D2 { return form(load(J)); };
     const NbOff o = nb_offsets_in(g, c);
     // I must be a cell of this thread's column (active threads) or 0
@@ -755,6 +786,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     int32_t fl = 0;
     double cz_m = 0.0;
     D2 xz_m = 0.0, xi = 0.0;
+    double s_i = 0.0;     // s of the own cell of plane z, formed with xi
     StepRaw ri{}, r1{};   // own operand inputs of planes z and z+1
     StepCell cur{};       // cell inputs of plane z
     if (c.active) {
@@ -764,7 +796,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             cz_m = cz[J];
         }
         ri = load(c.col + g.nxny * c.z0);
-        xi = form(ri);
+        xi = form_s(ri, s_i);
         if (c.z0 + 1 < g.nz || per_z) r1 = load(zp_index(g, c.col, c.z0));
         cur = cload(c.col + g.nxny * c.z0);
     }
@@ -791,8 +823,9 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         cn = cload(c.active && more ? I + g.nxny : own);
         __syncthreads();   // plane z is staged in sp.v[buf]
         D2 xz_p = 0.0;
+        double s_p = 0.0;
         if (c.active) {
-            xz_p = has_p ? form(r1) : D2(0.0);
+            xz_p = has_p ? form_s(r1, s_p) : D2(0.0);
             const D2 *nbc = &sp.v[buf][c.ly + 1][c.lx + 1];
             const double tpi = ri.tp;
             const double tppi = FromState ? ri.b : cur.tpp;
@@ -815,7 +848,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             if (isnan(Sn1)) fl |= FLAG_NAN_S;
             const double h1 = hsum(I, z, cur, ri.d, cz_m, xi, xz_m, xz_p, nbc, pick_a);
             const double h2 = hsum(I, z, cur, ri.d, cz_m, xi, xz_m, xz_p, nbc, pick_b);
-            const double s = 1.0 / sqrt(-ri.d);
+            const double s = s_i;
             const float rf = (float)(-s * (Sn - h1));
             double ra = -s * (Sn1 - h2);
             if (guess) {
@@ -854,6 +887,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         buf ^= 1;
         xz_m = xi;
         xi = xz_p;
+        s_i = s_p;
         ri = r1;
     };
     // Two trips per iteration, alternating the register sets (A, B) that carry the loads
@@ -877,6 +911,11 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
 // k_scale forms it) and the solution (y, or r0 scaled by alpha: CgY)
 struct Step64Raw {
     double tp, tpp, d, y;
+};
+
+// what forming a cell's own operand yields on the way: x^{n+1} and s = 1/sqrt(-d)
+struct Own64 {
+    double x, s;
 };
 
 // Fused step of the FP64 path, one pass over the grid after the solve: k_finalize of this step
@@ -916,6 +955,16 @@ k_step64_z(Dims g, RhsParams rp, int32_t guess, const double *__restrict__ tp,
         if (guess) x1 = x1 + kTiny;
         return x1;
     };
+    // the same for the thread's own cells, handing the cell its x and s (same operations)
+    auto fown = [&](const Step64Raw &v, Own64 &e) -> double {
+        e.s = 1.0 / sqrt(-v.d);
+        double x0 = v.tp + (v.tp - v.tpp);
+        if (tiny) x0 = x0 + kTiny;
+        e.x = ysol.any ? x0 + e.s * (ysol.scale * v.y) : x0;
+        double x1 = e.x + (e.x - v.tp);
+        if (guess) x1 = x1 + kTiny;
+        return x1;
+    };
     auto X = [&](uint32_t J) -> double { return form(load(J)); };
     double acc[2] = {0.0, 0.0};
     int32_t fl = 0, fl_next = 0;
@@ -933,11 +982,11 @@ k_step64_z(Dims g, RhsParams rp, int32_t guess, const double *__restrict__ tp,
     };
     double cz_m = 0.0;
     if (c.active && (c.z0 > 0 || (Wrap && g.per[2]))) cz_m = cz[zm_index(g, c.col, c.z0)];
-    zmarch_pipe<double, Step64Raw, RhsCell, Wrap>(
-        g, c, sp, load, form, cload,
+    zmarch_pipe_own<double, Step64Raw, RhsCell, Own64, Wrap>(
+        g, c, sp, load, form, fown, cload,
         [&](uint32_t I, uint32_t z, double xi, double xz_m, double xz_p,
-            const InPlane<double> &nb, const RhsCell &cv, const Step64Raw &ri) {
-            const double x = xnew(ri);
+            const InPlane<double> &nb, const RhsCell &cv, const Step64Raw &ri, const Own64 &e) {
+            const double x = e.x;
             stnt(&xout[I], x);
             if (isnan(x)) fl |= FLAG_NAN_X;                                  // mod_evolve.f90:153
             if ((x + (x - ri.tp)) - x < kTiny) fl |= FLAG_TINY_NEXT;          // :141, next step
@@ -957,7 +1006,7 @@ k_step64_z(Dims g, RhsParams rp, int32_t guess, const double *__restrict__ tp,
             h.d = ri.d;
             const double hxv = hx_sum_v<Wrap>(g, c, I, z, h, cx, cy, xi, xz_m, xz_p, cz_m, nb, X);
             const double r = S - hxv;
-            const double s = 1.0 / sqrt(-ri.d);
+            const double s = e.s;
             const double rh = -s * r;
             stnt(&rnext[I], rh);
             const double bh = s * S;

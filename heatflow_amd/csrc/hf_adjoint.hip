@@ -33,6 +33,7 @@ struct AdjointState {
     double *seg = nullptr;    // K x N
     double *g_hi = nullptr, *g_lo = nullptr, *g_new = nullptr, *lam = nullptr;
     double *lam_prev = nullptr, *lam_next = nullptr;   // lambda of the step before; guess
+    double *lam_pp = nullptr;                          // lambda two steps before
     double *gk = nullptr, *gp = nullptr;
     double *probe_val = nullptr;  // device, nprobe
     int64_t *probe_idx = nullptr;
@@ -48,7 +49,7 @@ void adjoint_free(AdjointState *a) {
     if (!a) return;
     hipFree(a->ckpt); hipFree(a->seg);
     hipFree(a->g_hi); hipFree(a->g_lo); hipFree(a->g_new); hipFree(a->lam);
-    hipFree(a->lam_prev); hipFree(a->lam_next);
+    hipFree(a->lam_prev); hipFree(a->lam_next); hipFree(a->lam_pp);
     hipFree(a->gk); hipFree(a->gp);
     hipFree(a->probe_val); hipFree(a->probe_idx); hipFree(a->probe_w); hipFree(a->probe_t);
     hipFree(a->fin); hipFree(a->seed);
@@ -234,7 +235,7 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
     const size_t N = c->g.N, B8 = N * 8;
     ATRY(ensure(c, &A.g_hi, N)); ATRY(ensure(c, &A.g_lo, N)); ATRY(ensure(c, &A.g_new, N));
     ATRY(ensure(c, &A.lam, N)); ATRY(ensure(c, &A.gk, N)); ATRY(ensure(c, &A.gp, N));
-    ATRY(ensure(c, &A.lam_prev, N)); ATRY(ensure(c, &A.lam_next, N));
+    ATRY(ensure(c, &A.lam_prev, N)); ATRY(ensure(c, &A.lam_next, N)); ATRY(ensure(c, &A.lam_pp, N));
     AHIP(c, hipMemcpyAsync(A.g_hi, seed, B8, hipMemcpyDefault, c->st));
     AHIP(c, hipMemsetAsync(A.g_lo, 0, B8, c->st));
     AHIP(c, hipMemsetAsync(A.lam, 0, B8, c->st));
@@ -273,20 +274,25 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
         for (int64_t n = e - 1; n >= b; --n) {
             int64_t it = 0;
             // lambda^{n+1} = H^{-1} gbar^{n+1}.  Initial guess: 0 for the first solve,
-            // lambda^{n+2} for the second, then 2 lambda^{n+2} - lambda^{n+3} (the adjoint
-            // recurrence has the forward step's form, whose guess is 2 T^n - T^{n-1}).
+            // lambda^{n+2} for the second, 2 lambda^{n+2} - lambda^{n+3} for the third (the
+            // adjoint recurrence has the forward step's form, whose guess is 2 T^n - T^{n-1}),
+            // then the quadratic 3 lambda^{n+2} - 3 lambda^{n+3} + lambda^{n+4} (debug bit 5:
+            // linear throughout).
             if (solved == 0) {
                 ATRY(ctx_solve_dev(c, A.g_hi, A.lam, atol, p.itmax, &it));
             } else {
                 if (solved == 1)
                     AHIP(c, hipMemcpyAsync(A.lam_next, A.lam, B8, hipMemcpyDeviceToDevice, c->st));
-                else
+                else if (solved == 2 || (c->debug & 32))
                     launch_extrap(c->g, A.lam, A.lam_prev, A.lam_next, c->st);
+                else
+                    launch_extrap3(c->g, A.lam, A.lam_prev, A.lam_pp, A.lam_next, c->st);
                 ATRY(ctx_solve_dev(c, A.g_hi, A.lam_next, atol, p.itmax, &it));
-                double *old_prev = A.lam_prev;
+                double *old_pp = A.lam_pp;
+                A.lam_pp = A.lam_prev;
                 A.lam_prev = A.lam;
                 A.lam = A.lam_next;
-                A.lam_next = old_prev;
+                A.lam_next = old_pp;
             }
             solved += 1;
             const double *Tn1 = A.seg + (size_t)N * (n - b);

@@ -11,6 +11,7 @@
 //   k_fin_step     refinement test and next-step norms after the fused mixed step
 //   k_fin_next     next-step norms after the fused FP64 step
 //   k_extrap       a + (a - b): the adjoint solve's initial guess
+//   k_extrap3      c + 3 (a - b): the same from three solutions
 //   k_fin_*        one-block finishing kernels of the reductions (scalars, convergence)
 // The stencil kernels (fused RHS/residual, CG SpMV, refinement residual) are in hf_stencil.hip.
 //
@@ -659,6 +660,16 @@ k_extrap(Dims g, const double *__restrict__ a, const double *__restrict__ b,
     }
 }
 
+// out = c + 3 (a - b): quadratic extrapolation from the last three solutions (the adjoint
+// solve's initial guess once three are known)
+__global__ void __launch_bounds__(kBlock)
+k_extrap3(Dims g, const double *__restrict__ a, const double *__restrict__ b,
+          const double *__restrict__ c, double *__restrict__ out) {
+    HF_FOR_CELLS(g, I) {
+        stnt(&out[I], ldnt(&c[I]) + 3.0 * (ldnt(&a[I]) - ldnt(&b[I])));
+    }
+}
+
 // TINY rule from an arbitrary state (first step after hf_set_state)
 __global__ void __launch_bounds__(kBlock)
 k_tiny_from_state(Dims g, const double *__restrict__ tp, const double *__restrict__ tpp,
@@ -803,6 +814,10 @@ void launch_sum2(const Dims &g, const double *a, const double *b, double *out, h
 }
 void launch_extrap(const Dims &g, const double *a, const double *b, double *out, hipStream_t st) {
     HF_LAUNCH(k_extrap, g, st, g, a, b, out);
+}
+void launch_extrap3(const Dims &g, const double *a, const double *b, const double *c,
+                    double *out, hipStream_t st) {
+    HF_LAUNCH(k_extrap3, g, st, g, a, b, c, out);
 }
 
 void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,

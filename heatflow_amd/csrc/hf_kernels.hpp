@@ -196,5 +196,7 @@ void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, 
                             hipStream_t st);
 void launch_sum2(const Dims &g, const double *a, const double *b, double *out, hipStream_t st);
 void launch_extrap(const Dims &g, const double *a, const double *b, double *out, hipStream_t st);
+void launch_extrap3(const Dims &g, const double *a, const double *b, const double *c,
+                    double *out, hipStream_t st);
 
 }  // namespace hf

@@ -260,3 +260,24 @@ def test_adjoint_step_kernels_agree(ci):
     assert np.array_equal(ka.view(np.int64), kb.view(np.int64))
     assert np.array_equal(pa.view(np.int64), pb.view(np.int64))
     assert np.any(ka != 0.0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
+def test_adjoint_guess_orders_agree(prec):
+    """From the fourth adjoint solve on, the initial guess is the quadratic extrapolation
+    3 lambda^{n+2} - 3 lambda^{n+3} + lambda^{n+4}; debug bit 5 keeps the linear one.  The guess
+    changes where each solve starts, not what it converges to: loss and gradients agree to the
+    solve tolerance (here 1e-13 on the scaled residual; gradients within 1e-9 of max |g|)."""
+    s, m = workloads.c4(32, nprobe=32)
+    targets = np.full(len(m["probes"]), 300.0)
+    out = []
+    for dbg in (0, 32):
+        with Simulation(s, precision=prec, tol=1e-13) as sim:
+            sim.set_debug(dbg)
+            sim.initial_evolve()
+            out.append(sim.grad(1, 40, 8, m["probes"], targets))
+    (la, ka, pa), (lb, kb, pb) = out
+    assert abs(la - lb) <= 1e-9 * abs(lb)
+    assert np.max(np.abs(ka - kb)) <= 1e-9 * np.max(np.abs(kb))
+    assert np.max(np.abs(pa - pb)) <= 1e-9 * np.max(np.abs(pb))
+    assert not np.array_equal(ka, kb)   # the two guesses did run

@@ -10,8 +10,7 @@
 //   k_finalize     x = x0 + s*y, NaN guard, TINY flag of the next step (mod_evolve.f90:140-170)
 //   k_fin_step     refinement test and next-step norms after the fused mixed step
 //   k_fin_next     next-step norms after the fused FP64 step
-//   k_extrap       a + (a - b): the adjoint solve's initial guess
-//   k_extrap3      c + 3 (a - b): the same from three solutions
+//   k_extrap       polynomial extrapolation of the last solutions: the adjoint solve's guess
 //   k_fin_*        one-block finishing kernels of the reductions (scalars, convergence)
 // The stencil kernels (fused RHS/residual, CG SpMV, refinement residual) are in hf_stencil.hip.
 //
@@ -649,24 +648,16 @@ k_sum2(Dims g, const double *__restrict__ a, const double *__restrict__ b,
     HF_FOR_CELLS(g, I) { stnt(&out[I], ldnt(&a[I]) + ldnt(&b[I])); }
 }
 
-// out = a + (a - b): linear extrapolation from the last two solutions (the adjoint solve's
-// initial guess; the forward step forms its guess the same way inside k_rhs_res_z)
+// out = sum_j w_j v_j over the last m solutions, newest first: the polynomial extrapolation
+// through them (the adjoint solve's initial guess; weights from the host)
 __global__ void __launch_bounds__(kBlock)
-k_extrap(Dims g, const double *__restrict__ a, const double *__restrict__ b,
-         double *__restrict__ out) {
+k_extrap(Dims g, ExtrapArgs a, double *__restrict__ out) {
     HF_FOR_CELLS(g, I) {
-        const double ai = ldnt(&a[I]);
-        stnt(&out[I], ai + (ai - ldnt(&b[I])));
-    }
-}
-
-// out = c + 3 (a - b): quadratic extrapolation from the last three solutions (the adjoint
-// solve's initial guess once three are known)
-__global__ void __launch_bounds__(kBlock)
-k_extrap3(Dims g, const double *__restrict__ a, const double *__restrict__ b,
-          const double *__restrict__ c, double *__restrict__ out) {
-    HF_FOR_CELLS(g, I) {
-        stnt(&out[I], ldnt(&c[I]) + 3.0 * (ldnt(&a[I]) - ldnt(&b[I])));
+        double v = a.w[0] * ldnt(&a.v[0][I]);
+#pragma unroll
+        for (int j = 1; j < kExtrapMax; ++j)
+            if (j < a.m) v = v + a.w[j] * ldnt(&a.v[j][I]);
+        stnt(&out[I], v);
     }
 }
 
@@ -812,14 +803,9 @@ void launch_step_begin(Scal *sc, hipStream_t st) {
 void launch_sum2(const Dims &g, const double *a, const double *b, double *out, hipStream_t st) {
     HF_LAUNCH(k_sum2, g, st, g, a, b, out);
 }
-void launch_extrap(const Dims &g, const double *a, const double *b, double *out, hipStream_t st) {
-    HF_LAUNCH(k_extrap, g, st, g, a, b, out);
+void launch_extrap(const Dims &g, const ExtrapArgs &a, double *out, hipStream_t st) {
+    HF_LAUNCH(k_extrap, g, st, g, a, out);
 }
-void launch_extrap3(const Dims &g, const double *a, const double *b, const double *c,
-                    double *out, hipStream_t st) {
-    HF_LAUNCH(k_extrap3, g, st, g, a, b, c, out);
-}
-
 void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,
                             hipStream_t st) {
     HF_LAUNCH(k_tiny_from_state, g, st, g, tp, tpp, sc);

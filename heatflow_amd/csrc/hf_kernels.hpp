@@ -195,8 +195,13 @@ void launch_step_begin(Scal *sc, hipStream_t st);
 void launch_tiny_from_state(const Dims &g, const double *tp, const double *tpp, Scal *sc,
                             hipStream_t st);
 void launch_sum2(const Dims &g, const double *a, const double *b, double *out, hipStream_t st);
-void launch_extrap(const Dims &g, const double *a, const double *b, double *out, hipStream_t st);
-void launch_extrap3(const Dims &g, const double *a, const double *b, const double *c,
-                    double *out, hipStream_t st);
+// out = sum_{j < m} w[j] v[j] (v[0] the newest of the last m solutions)
+constexpr int kExtrapMax = 8;
+struct ExtrapArgs {
+    const double *v[kExtrapMax];
+    double w[kExtrapMax];
+    int32_t m;
+};
+void launch_extrap(const Dims &g, const ExtrapArgs &a, double *out, hipStream_t st);
 
 }  // namespace hf

@@ -162,7 +162,7 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * every cell); bit 4 runs the adjoint step with the flat per-cell kernel instead of the
  * z-marching one (grids with a periodic axis always use the flat kernel); bit 5 starts every
  * adjoint solve from the linear extrapolation 2 lambda^{n+2} - lambda^{n+3} instead of the
- * quadratic one. */
+ * polynomial through the last six solutions. */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */

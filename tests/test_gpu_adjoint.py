@@ -264,10 +264,10 @@ def test_adjoint_step_kernels_agree(ci):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
 def test_adjoint_guess_orders_agree(prec):
-    """From the fourth adjoint solve on, the initial guess is the quadratic extrapolation
-    3 lambda^{n+2} - 3 lambda^{n+3} + lambda^{n+4}; debug bit 5 keeps the linear one.  The guess
-    changes where each solve starts, not what it converges to: loss and gradients agree to the
-    solve tolerance (here 1e-13 on the scaled residual; gradients within 1e-9 of max |g|)."""
+    """The adjoint solves start from the polynomial extrapolation through the last six
+    solutions (fewer at first); debug bit 5 keeps the linear 2 lambda^{n+2} - lambda^{n+3}.  The
+    guess changes where each solve starts, not what it converges to: loss and gradients agree to
+    the solve tolerance (here 1e-13 on the scaled residual; gradients within 1e-9 of max |g|)."""
     s, m = workloads.c4(32, nprobe=32)
     targets = np.full(len(m["probes"]), 300.0)
     out = []

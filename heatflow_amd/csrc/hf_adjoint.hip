@@ -275,7 +275,8 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
     // the z-marching kernel for grids without periodic axes; debug bit 4 keeps the flat one
     const bool zmarch_adj =
         !(p.periodic[0] || p.periodic[1] || p.periodic[2]) && (c->debug & 16) == 0;
-    int64_t solved = 0;   // adjoint solves done: hist[0 .. min(solved, hmax) - 1] hold the last
+    // adjoint solves done; hist[0 .. min(solved, hmax) - 1] hold the last solutions, newest first
+    int64_t solved = 0;
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t b = s * K, e = std::min(nsteps, (s + 1) * K);
         ATRY(restore_state(c, A.ckpt + (size_t)2 * N * s, A.ckpt + (size_t)2 * N * s + N));

@@ -9,7 +9,7 @@ from heatflow_amd import HeatFlowError, Simulation, workloads
 from heatflow_amd.autograd import heatflow_run
 
 from _cases import random_system
-from dense_ref import dense_run
+from dense_ref import DENSE_BOUNDARIES, dense_run, with_boundary
 
 pytestmark = pytest.mark.gpu
 
@@ -18,11 +18,15 @@ def _loss_np(T, probes, targets):
     return float(np.sum((T[probes] - targets) ** 2))
 
 
+@pytest.mark.parametrize("bi", range(len(DENSE_BOUNDARIES)))
 @pytest.mark.parametrize("icat", [0, 1])
-def test_adjoint_matches_dense_autograd(icat):
+def test_adjoint_matches_dense_autograd(icat, bi):
     """12^3 copy of the 128^3 gradient workload, 20 steps, checkpoints every 6 (a partial last
-    segment): dL/dkappa over all cells and dL/dP at the heaters."""
+    segment): dL/dkappa over all cells and dL/dP at the heaters.  Boundary variants: Dirichlet
+    baths, periodic axes (the flat adjoint kernel with its wrap terms) and CG faces (bath terms
+    that a CG face's assignment removes from B carry no dB/dkappa)."""
     s, m = workloads.c4(12, icattaneo=icat, nheat=4, nprobe=24)
+    s = with_boundary(s, bi)
     steps, ckpt = 20, 6
     # targets from the ground-truth kappa* (dense forward)
     with torch.no_grad():

@@ -225,14 +225,17 @@ def test_linbcg_matches_dense_solve():
 
 # ---- the dense torch restatement used to check the adjoint ----------------------------------
 
+@pytest.mark.parametrize("bi", range(5))
 @pytest.mark.parametrize("icat", [0, 1])
-def test_dense_reference_matches_oracle(icat):
+def test_dense_reference_matches_oracle(icat, bi):
     """tests/dense_ref.py (the autograd reference of the adjoint tests) steps the same physics
-    as the oracle: per-cell power P enters as Qdens = P / V."""
+    as the oracle: per-cell power P enters as Qdens = P / V; periodic axes and CG faces as the
+    oracle has them."""
     import torch
-    from dense_ref import dense_run
+    from dense_ref import dense_run, with_boundary
     from heatflow_amd import workloads
     s, m = workloads.c4(6, icattaneo=icat, nheat=3, nprobe=5)
+    s = with_boundary(s, bi)
     rc, tp, _, _ = oracle.OracleSystem(s).run(7)
     assert rc == 0
     k = torch.as_tensor(s.cells.kappa)

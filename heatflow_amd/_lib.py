@@ -58,6 +58,7 @@ class HfStats(C.Structure):
         ("steps", C.c_int64), ("outer_iters", C.c_int64), ("inner_iters", C.c_int64),
         ("last_err", C.c_double), ("first_iter_stops", C.c_int64), ("tiny_misses", C.c_int64),
         ("residual_reruns", C.c_int64), ("alpha_reuses", C.c_int64), ("reuse_misses", C.c_int64),
+        ("adjoint_solves", C.c_int64), ("adjoint_iters", C.c_int64),
     ]
 
 

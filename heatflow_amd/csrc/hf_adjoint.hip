@@ -313,6 +313,8 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
                 A.lam_next = oldest;
             }
             solved += 1;
+            c->stats.adjoint_solves += 1;
+            c->stats.adjoint_iters += it;
             const double *Tn1 = A.seg + (size_t)N * (n - b);
             if (zmarch_adj)
                 launch_adj_step_z(c->g, q, Tn1, A.hist[0], c->kappa, c->lx, c->ly, c->lz, c->rhoc,

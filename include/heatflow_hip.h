@@ -180,6 +180,8 @@ typedef struct hf_stats {
                                  solve's step length instead of solving                 */
     int64_t reuse_misses;     /* of those, steps whose residual missed the inner target
                                  (ordinary sweeps followed)                             */
+    int64_t adjoint_solves;   /* adjoint solves (hf_adjoint, hf_grad)                   */
+    int64_t adjoint_iters;    /* their CG iterations (FP32 inner ones in mixed mode)    */
 } hf_stats;
 int hf_get_stats(hf_ctx *ctx, hf_stats *st);
 int hf_reset_stats(hf_ctx *ctx);

@@ -274,13 +274,17 @@ def test_adjoint_guess_orders_agree(prec):
     the solve tolerance (here 1e-13 on the scaled residual; gradients within 1e-9 of max |g|)."""
     s, m = workloads.c4(32, nprobe=32)
     targets = np.full(len(m["probes"]), 300.0)
-    out = []
+    out, iters = [], []
     for dbg in (0, 32):
         with Simulation(s, precision=prec, tol=1e-13) as sim:
             sim.set_debug(dbg)
             sim.initial_evolve()
             out.append(sim.grad(1, 40, 8, m["probes"], targets))
+            st = sim.stats()
+            assert st.adjoint_solves == 40
+            iters.append(st.adjoint_iters)
     (la, ka, pa), (lb, kb, pb) = out
+    assert iters[0] < iters[1]   # the six-point guess starts the solves closer
     assert abs(la - lb) <= 1e-9 * abs(lb)
     assert np.max(np.abs(ka - kb)) <= 1e-9 * np.max(np.abs(kb))
     assert np.max(np.abs(pa - pb)) <= 1e-9 * np.max(np.abs(pb))

@@ -649,15 +649,30 @@ k_sum2(Dims g, const double *__restrict__ a, const double *__restrict__ b,
 }
 
 // out = sum_j w_j v_j over the last m solutions, newest first: the polynomial extrapolation
-// through them (the adjoint solve's initial guess; weights from the host)
+// through them (the adjoint solve's initial guess; weights from the host).  16-byte accesses
+// as k_cg_b: block b (XCD-mapped) takes a contiguous chunk of the cell pairs; an odd last cell
+// goes to logical block 0.
 __global__ void __launch_bounds__(kBlock)
 k_extrap(Dims g, ExtrapArgs a, double *__restrict__ out) {
-    HF_FOR_CELLS(g, I) {
-        double v = a.w[0] * ldnt(&a.v[0][I]);
+    using V = double __attribute__((ext_vector_type(2)));
+    const uint32_t nv = g.N / 2;
+    const uint32_t b = logical_block(g);
+    const uint32_t per = (nv + g.nblk - 1) / g.nblk;
+    const uint64_t v0 = (uint64_t)b * per;
+    const uint32_t v1 = (uint32_t)(v0 + per < nv ? v0 + per : nv);
+    for (uint32_t i = (uint32_t)(v0 < nv ? v0 : nv) + threadIdx.x; i < v1; i += kBlock) {
+        V v = a.w[0] * __builtin_nontemporal_load((const V *)a.v[0] + i);
 #pragma unroll
         for (int j = 1; j < kExtrapMax; ++j)
-            if (j < a.m) v = v + a.w[j] * ldnt(&a.v[j][I]);
-        stnt(&out[I], v);
+            if (j < a.m) v = v + a.w[j] * __builtin_nontemporal_load((const V *)a.v[j] + i);
+        __builtin_nontemporal_store(v, (V *)out + i);
+    }
+    if (b == 0) {
+        for (uint32_t I = nv * 2 + threadIdx.x; I < g.N; I += kBlock) {
+            double v = a.w[0] * a.v[0][I];
+            for (int j = 1; j < a.m; ++j) v = v + a.w[j] * a.v[j][I];
+            out[I] = v;
+        }
     }
 }
 

@@ -279,8 +279,12 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
     int64_t solved = 0;
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t b = s * K, e = std::min(nsteps, (s + 1) * K);
-        ATRY(restore_state(c, A.ckpt + (size_t)2 * N * s, A.ckpt + (size_t)2 * N * s + N));
-        for (int64_t n = b; n < e; ++n) {
+        // T^e, the state after the segment's last step, is the next checkpoint's (or the final
+        // state): the recomputation stops one step short (none at all for ckpt_every = 1)
+        const double *Te = s + 1 < nseg ? A.ckpt + (size_t)2 * N * (s + 1) : A.fin;
+        if (e - 1 > b)
+            ATRY(restore_state(c, A.ckpt + (size_t)2 * N * s, A.ckpt + (size_t)2 * N * s + N));
+        for (int64_t n = b; n < e - 1; ++n) {
             ATRY(ctx_step(c, itime0 + n, false));
             AHIP(c, hipMemcpyAsync(A.seg + (size_t)N * (n - b), c->T[0], B8,
                                    hipMemcpyDeviceToDevice, c->st));
@@ -315,7 +319,7 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
             solved += 1;
             c->stats.adjoint_solves += 1;
             c->stats.adjoint_iters += it;
-            const double *Tn1 = A.seg + (size_t)N * (n - b);
+            const double *Tn1 = n == e - 1 ? Te : A.seg + (size_t)N * (n - b);
             if (zmarch_adj)
                 launch_adj_step_z(c->g, q, Tn1, A.hist[0], c->kappa, c->lx, c->ly, c->lz, c->rhoc,
                                   c->cvec, c->vol, A.g_lo, A.g_new, A.gk, A.gp, c->st);

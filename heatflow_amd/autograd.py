@@ -6,10 +6,15 @@
 
 kappa: per-cell conductivity (W/(m K)), power: per-cell heater power P (W) entering as a power
 density P / volume (the superset per-cell input of the C-ABI), both float64 tensors of length N
-(on the GPU or the host).  The forward runs `nsteps` FP64 time steps from T^0 = T^{-1} = t_init
-on the library's device context and keeps (T^n, T^{n-1}) checkpoints every `ckpt_every` steps;
-backward re-runs each segment from its checkpoint and sweeps the adjoint backwards
-(hf_forward_checkpointed / hf_adjoint in include/heatflow_hip.h).
+(on the GPU or the host).  The forward runs `nsteps` time steps from T^0 = T^{-1} = t_init on
+the library's device context, in the context's precision mode (FP64, or FP32-inner / FP64-refined)
+and tolerance, and keeps (T^n, T^{n-1}) checkpoints every `ckpt_every` steps; backward re-runs
+each segment from its checkpoint and sweeps the adjoint backwards (hf_forward_checkpointed /
+hf_adjoint in include/heatflow_hip.h).
+
+The checkpoints live in the context, so one `Simulation` holds one pending graph: backward of a
+graph whose forward was followed by another forward on the same context raises RuntimeError
+(and the library itself refuses after the fields, the operator or the solver changed).
 """
 from __future__ import annotations
 
@@ -29,6 +34,7 @@ class HeatFlowRun(torch.autograd.Function):
         sim.sparse_Hmatrix()
         sim.initial_evolve(t_init)
         sim.forward_checkpointed(itime0, nsteps, ckpt_every)
+        ctx.generation = sim.forward_generation
         out = torch.empty(sim.n, dtype=torch.float64, device=kappa.device)
         if out.is_cuda:
             torch.cuda.synchronize(out.device)   # the library writes on its own stream
@@ -40,6 +46,10 @@ class HeatFlowRun(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g_t: torch.Tensor):
         sim = ctx.sim
+        if sim.forward_generation != ctx.generation:
+            raise RuntimeError("heatflow_run: another forward ran on this Simulation since this "
+                               "graph's forward; its checkpoints are gone (one pending graph "
+                               "per Simulation)")
         gk = torch.empty(sim.n, dtype=torch.float64, device=ctx.dev)
         gp = torch.empty(sim.n, dtype=torch.float64, device=ctx.dev)
         if gk.is_cuda:

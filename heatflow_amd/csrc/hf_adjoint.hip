@@ -6,6 +6,9 @@
 // the adjoint solve is the forward solver on the same operator:
 //   lambda^{n+1} = H^{-1} gbar^{n+1},
 //   gbar^n     += -(a + 2c) lambda^{n+1},   gbar^{n-1} += c lambda^{n+1},
+//   gbar^n_I   += 4 em_I L1 L2 sigma (T^n_I)^3 / V_heated lambda^{n+1}_I   (radiative source:
+//                 with heaters present Qdens_I = (Q_I - em_I L1 L2 sigma (T^n_I^4 - T_bath^4))
+//                 / V_heated, src/heatflow/mod_heating.f90:121-123, 164-167),
 //   dL/dkappa_I += sum_faces dG/dkappa_I (lambda_I - lambda_J)(T_I - T_J)
 //                + sum_bfaces dGb/dkappa_I lambda_I (T_I - [T_bath surviving in B_I]),
 //   dL/dP_I     += -lambda_I / V_I          (P_I enters as Qdens_I += P_I / V_I),
@@ -30,7 +33,7 @@ struct AdjointState {
     size_t N = 0;
     int64_t nseg_cap = 0, k_cap = 0;
     double *ckpt = nullptr;   // nseg x 2 x N
-    double *seg = nullptr;    // K x N
+    double *seg = nullptr;    // max(K - 1, 1) x N: the recomputed states of one segment
     double *g_hi = nullptr, *g_lo = nullptr, *g_new = nullptr;
     // the last adjoint solutions, newest first (hist[0] is lambda^{n+1}), and the next guess
     double *hist[kExtrapMax] = {};
@@ -113,7 +116,12 @@ k_adj_step(Dims g, AdjParams q, const double *__restrict__ T, const double *__re
             a = q.inverse_time * rhoc[I & q.rc_mask];
             cc = q.icattaneo == 1 ? cvec[I & q.rc_mask] : 0.0;
         }
-        g_lo[I] += -(a + 2.0 * cc) * li;
+        double gl = g_lo[I] + -(a + 2.0 * cc) * li;
+        if (q.rad_scale != 0.0) {
+            const double tn = q.tn[I];
+            gl = gl + q.rad_scale * q.ells[I] * (tn * tn * tn) * li;
+        }
+        g_lo[I] = gl;
         g_new[I] = cc * li;
     }
 }
@@ -164,7 +172,7 @@ static int ensure(hf_ctx *c, double **p, size_t n) {
 // restore (T^n, T^{n-1}) into the context state and recompute the TINY flag
 static int restore_state(hf_ctx *c, const double *tp, const double *tpp) {
     const size_t B8 = (size_t)c->g.N * 8;
-    c->carry = false;
+    reset_step_heuristics(c);
     AHIP(c, hipMemcpyAsync(c->T[0], tp, B8, hipMemcpyDeviceToDevice, c->st));
     AHIP(c, hipMemcpyAsync(c->T[1], tpp, B8, hipMemcpyDeviceToDevice, c->st));
     launch_step_begin(c->sc, c->st);
@@ -176,16 +184,10 @@ static int restore_state(hf_ctx *c, const double *tp, const double *tpp) {
 static int validate(hf_ctx *c) {
     if (!c->assembled || !c->has_state)
         return ctx_set_err(c, HF_ERR_STATE, "adjoint: needs hf_assemble and hf_set_state first");
-    const size_t N = c->g.N, B8 = N * 8;
-    // per-axis uniform lengths (the reference's system.in gives Lx/nx etc., mod_inputs.f90:619-622)
-    std::vector<double> h(N);
-    const double *arrs[3] = {c->lx, c->ly, c->lz};
-    for (int a = 0; a < 3; ++a) {
-        AHIP(c, hipMemcpy(h.data(), arrs[a], B8, hipMemcpyDeviceToHost));
-        for (size_t i = 1; i < N; ++i)
-            if (h[i] != h[0])
-                return ctx_set_err(c, HF_ERR_ARG, "adjoint: cell lengths must be uniform per axis");
-    }
+    // per-axis uniform lengths (the reference's system.in gives Lx/nx etc., mod_inputs.f90:619-622),
+    // found by k_uniform_check at upload
+    if (!c->uniform_len)
+        return ctx_set_err(c, HF_ERR_ARG, "adjoint: cell lengths must be uniform per axis");
     return HF_OK;
 }
 
@@ -197,13 +199,18 @@ int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_ever
     const size_t N = c->g.N, B8 = N * 8;
     if (!c->adj) c->adj = new AdjointState();
     AdjointState &A = *c->adj;
+    // the checkpoints are rewritten below: valid again only once the run has completed
+    A.valid = false;
     const int64_t K = ckpt_every;
     const int64_t nseg = (nsteps + K - 1) / K;
-    if (A.N != N || A.nseg_cap < nseg || A.k_cap < K) {
+    // a segment of K steps is recomputed up to its second-to-last state (the last one is the
+    // next checkpoint's): K - 1 states
+    const int64_t kseg = std::max<int64_t>(K - 1, 1);
+    if (A.N != N || A.nseg_cap < nseg || A.k_cap < kseg) {
         hipFree(A.ckpt); hipFree(A.seg);
         A.ckpt = A.seg = nullptr;
         A.nseg_cap = std::max(nseg, A.N == N ? A.nseg_cap : 0);
-        A.k_cap = std::max(K, A.N == N ? A.k_cap : 0);
+        A.k_cap = std::max(kseg, A.N == N ? A.k_cap : 0);
     }
     if (A.N != N) {
         hipFree(A.g_hi); hipFree(A.g_lo); hipFree(A.g_new);
@@ -268,6 +275,10 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
     q.icattaneo = p.icattaneo;
     q.rc_mask = c->uniform_rc && (c->debug & 8) == 0 ? 0u : ~0u;
     q.vol_mask = c->uniform_vol && (c->debug & 8) == 0 ? 0u : ~0u;
+    // the radiative term enters S only with heaters present (mod_evolve.f90:79-80)
+    q.rad_scale = (c->any_heater && c->any_em && c->heated_volume > 0.0)
+                      ? 4.0 / c->heated_volume : 0.0;
+    q.ells = c->ells;
 
     const int64_t K = A.K, nsteps = A.nsteps, itime0 = A.itime0;
     const int64_t nseg = (nsteps + K - 1) / K;
@@ -320,6 +331,8 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
             c->stats.adjoint_solves += 1;
             c->stats.adjoint_iters += it;
             const double *Tn1 = n == e - 1 ? Te : A.seg + (size_t)N * (n - b);
+            // T^n, the state step n started from (the radiative term's derivative)
+            q.tn = n == b ? A.ckpt + (size_t)2 * N * s : A.seg + (size_t)N * (n - 1 - b);
             if (zmarch_adj)
                 launch_adj_step_z(c->g, q, Tn1, A.hist[0], c->kappa, c->lx, c->ly, c->lz, c->rhoc,
                                   c->cvec, c->vol, A.g_lo, A.g_new, A.gk, A.gp, c->st);

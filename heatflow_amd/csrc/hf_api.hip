@@ -375,16 +375,19 @@ __global__ void k_derived(uint32_t N, const double *__restrict__ rho,
 }
 
 // Sets bit 0 of sc->varying when rho c_p or tau' rho c_p differs from its value at cell 0
-// somewhere, bit 1 when the volume does (sc->varying cleared before).  Uniform fields let the
-// step and adjoint kernels read element 0 only.
+// somewhere, bit 1 when the volume does, bit 2 when a cell length along some axis does
+// (sc->varying cleared before).  Uniform fields let the step and adjoint kernels read element 0
+// only; the adjoint needs uniform lengths per axis.
 __global__ void __launch_bounds__(kBlock)
 k_uniform_check(uint32_t N, const double *__restrict__ rhoc, const double *__restrict__ cvec,
-                const double *__restrict__ vol, Scal *sc) {
-    const double r0 = rhoc[0], c0 = cvec[0], v0 = vol[0];
+                const double *__restrict__ vol, const double *__restrict__ lx,
+                const double *__restrict__ ly, const double *__restrict__ lz, Scal *sc) {
+    const double r0 = rhoc[0], c0 = cvec[0], v0 = vol[0], x0 = lx[0], y0 = ly[0], z0 = lz[0];
     int32_t diff = 0;
     for (uint32_t I = blockIdx.x * kBlock + threadIdx.x; I < N; I += gridDim.x * kBlock) {
         if (!(ldnt(&rhoc[I]) == r0) || !(ldnt(&cvec[I]) == c0)) diff |= 1;
         if (!(ldnt(&vol[I]) == v0)) diff |= 2;
+        if (!(ldnt(&lx[I]) == x0) || !(ldnt(&ly[I]) == y0) || !(ldnt(&lz[I]) == z0)) diff |= 4;
     }
     block_flags(diff, &sc->varying);
 }
@@ -396,6 +399,7 @@ extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho
                                const int64_t *iheater, const double *qdens_extra) {
     if (!c) return HF_ERR_ARG;
     c->carry = false;   // device state or scratch changes: no carried residual
+    hf::adjoint_invalidate(c->adj);   // new fields: the checkpoints no longer belong to them
     if (!kappa || !rho || !heat_capacity || !tau_scaled || !len_x || !len_y || !len_z || !volume)
         return set_err(c, HF_ERR_ARG, "hf_upload_cells: required array is NULL");
     const size_t N = c->g.N, B8 = N * sizeof(double);
@@ -424,13 +428,14 @@ extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho
         HIPCHK(c, hipMemsetAsync(&c->sc->varying, 0, sizeof(int32_t), c->st));
         const uint32_t nb = (uint32_t)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
         hipLaunchKernelGGL(k_uniform_check, dim3(nb), dim3(kBlock), 0, c->st, (uint32_t)N,
-                           c->rhoc, c->cvec, c->vol, c->sc);
+                           c->rhoc, c->cvec, c->vol, c->lx, c->ly, c->lz, c->sc);
         TRY(check_launch(c, "k_uniform_check"));
         HIPCHK(c, hipMemcpyAsync(&varying, &c->sc->varying, sizeof(int32_t),
                                  hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         c->uniform_rc = (varying & 1) == 0;
         c->uniform_vol = (varying & 2) == 0;
+        c->uniform_len = (varying & 4) == 0;
     }
     if (qdens_extra) {
         TRY(dalloc(c, &c->qextra, N));
@@ -443,6 +448,7 @@ extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho
     if (iheater) HIPCHK(c, hipMemcpy(ih.data(), iheater, N * 8, hipMemcpyDefault));
     c->any_heater = false;
     c->dyn_heat = false;
+    c->any_em = false;
     c->heated_volume = 0.0;
     std::vector<int8_t> ih8(N, 0);
     if (iheater) {
@@ -466,6 +472,7 @@ extern "C" int hf_upload_cells(hf_ctx *c, const double *kappa, const double *rho
                 if (emh[i] != 0.0) { any_em = true; break; }
         }
         c->dyn_heat = c->any_heater && (other_modes || any_em);
+        c->any_em = any_em;
     }
     HIPCHK(c, hipMemcpyAsync(c->iheater, ih8.data(), N, hipMemcpyHostToDevice, c->st));
     // CG_dir power densities use grid(1,1,1), (1,1,nz), (1,ny,1), (nx,1,1) volumes
@@ -530,6 +537,7 @@ static int sync_scal(hf_ctx *c, Scal *dev, Scal *host, int count = 1) {
 extern "C" int hf_assemble(hf_ctx *c) {
     if (!c) return HF_ERR_ARG;
     c->carry = false;   // device state or scratch changes: no carried residual
+    hf::adjoint_invalidate(c->adj);   // a new operator: the checkpoints no longer belong to it
     if (!c->uploaded) return set_err(c, HF_ERR_STATE, "hf_assemble: cells not uploaded");
     launch_assemble(c->g, c->ap, c->kappa, c->lx, c->ly, c->lz, c->rhoc, c->tau_s, c->d, c->cx,
                     c->cy, c->cz, c->bvec, c->sc, c->st);
@@ -604,9 +612,25 @@ extern "C" int hf_export_coo(hf_ctx *c, int64_t *n, int64_t *len, int64_t *count
     return HF_OK;
 }
 
+namespace hf {
+void reset_step_heuristics(hf_ctx *c) {
+    const hf_ctx fresh{};
+    c->carry = false;
+    c->tiny_guess = fresh.tiny_guess;
+    c->last_sweeps = fresh.last_sweeps;
+    c->reuse_ok = fresh.reuse_ok;
+    c->reuse_pause = fresh.reuse_pause;
+    c->reuse_miss_run = fresh.reuse_miss_run;
+    c->pred_iters = fresh.pred_iters;
+    c->pred_inner = fresh.pred_inner;
+}
+}  // namespace hf
+
 extern "C" int hf_set_state(hf_ctx *c, const double *tp, const double *tpp) {
     if (!c || !tp || !tpp) return set_err(c, HF_ERR_ARG, "hf_set_state: null argument");
-    c->carry = false;   // device state or scratch changes: no carried residual
+    // a new state: no carried residual, and the heuristics start as in a new context
+    hf::reset_step_heuristics(c);
+    c->pred_adj = hf_ctx{}.pred_adj;
     const size_t B8 = (size_t)c->g.N * sizeof(double);
     HIPCHK(c, hipMemcpyAsync(c->T[0], tp, B8, hipMemcpyDefault, c->st));
     HIPCHK(c, hipMemcpyAsync(c->T[1], tpp, B8, hipMemcpyDefault, c->st));
@@ -662,13 +686,22 @@ struct CgVecs {
     const T *hx, *hy, *hz;     // scaled couplings
 };
 
+// linbcg's own stop rule for an FP64 solve (hf_solve): mode 1 (itol 1) stops on ||r|| / ||b||,
+// mode 2 (itol 2) on ||r / diag H|| / ||b|| (src/heatflow/mod_Sparse.f90:418-425), both checked
+// after every iteration in place of the scaled-residual test
+struct WStop {
+    int32_t mode;
+    double tol;
+};
+
 // Queues CG iterations k0 .. k0+count-1 (launches past convergence return at once).
 // Iteration 0: k_cg_a (first: p = rin, only p.q and q.q), k_fin_a (which may end the solve:
 // y = alpha rin is then formed by the consumer), k_cg_b1 (q recomputed; y, rw), k_fin_b.
 // Iteration k >= 1: k_cg_a (p = rw + beta p_old, p_old = rin for k = 1), k_cg_b on rw.
+// With a WStop, each iteration ends with the weighted norm of its residual (rw) and the test.
 template <typename T>
 static void queue_cg(hf_ctx *c, Scal *sc, const CgVecs<T> &v, int64_t k0, int64_t count,
-                     std::vector<SlotRec> *slots, int ida, int idb) {
+                     std::vector<SlotRec> *slots, int ida, int idb, const WStop *ws = nullptr) {
     const double N = (double)c->g.N, w = (double)sizeof(T);
     const Dims gc = cg_dims<T>(c->g);   // z-chunking of the stencil CG kernels
     for (int64_t k = k0; k < k0 + count; ++k) {
@@ -701,6 +734,11 @@ static void queue_cg(hf_ctx *c, Scal *sc, const CgVecs<T> &v, int64_t k0, int64_
             tend(c, sb);
             launch_fin_b(c->g, c->partials, sc, c->st);
         }
+        if constexpr (std::is_same<T, double>::value) {
+            if (ws)
+                launch_wstop(c->g, v.rw, c->sv, ws->mode, k + 1, ws->tol, sc, c->partials,
+                             c->st);
+        }
         if (slots && sa >= 0) slots->push_back({sa, k, false});
         if (slots && sb >= 0) slots->push_back({sb, k, true});
     }
@@ -709,12 +747,12 @@ static void queue_cg(hf_ctx *c, Scal *sc, const CgVecs<T> &v, int64_t k0, int64_
 // CG on A^ y = rin until sc->done, in batches of `pred` iterations per host synchronisation.
 template <typename T>
 static int run_cg(hf_ctx *c, Scal *sc, Scal *h, const CgVecs<T> &v, int64_t *pred,
-                  int64_t *iters, int ida, int idb) {
+                  int64_t *iters, int ida, int idb, const WStop *ws = nullptr) {
     int64_t batch = std::max<int64_t>(1, *pred);
     int64_t queued = 0;
     std::vector<SlotRec> slots;
     for (;;) {
-        queue_cg<T>(c, sc, v, queued, batch, &slots, ida, idb);
+        queue_cg<T>(c, sc, v, queued, batch, &slots, ida, idb, ws);
         queued += batch;
         TRY(check_launch(c, "k_cg"));
         TRY(sync_scal(c, sc, h));
@@ -1153,8 +1191,9 @@ extern "C" int hf_run(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t *iters_
 
 extern "C" int hf_solve(hf_ctx *c, const double *b, double *x, int32_t itol, double tol,
                         int64_t itmax, int64_t *iter, double *err) {
-    (void)itol;
     if (!c || !b || !x) return set_err(c, HF_ERR_ARG, "hf_solve: null argument");
+    if (itol != 1 && itol != 2)
+        return set_err(c, HF_ERR_ARG, "hf_solve: itol must be 1 or 2 (mod_Sparse.f90:418-423)");
     c->carry = false;   // device state or scratch changes: no carried residual
     if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_solve: not assembled");
     if (c->p.precision != HF_PREC_FP64 || !c->hx)
@@ -1171,16 +1210,19 @@ extern "C" int hf_solve(hf_ctx *c, const double *b, double *x, int32_t itol, dou
     launch_rhs_res_solve64(c->g, rp, xd, bd, c->d, c->cx, c->cy, c->cz, c->r64, c->sc,
                            c->partials, c->st);
     launch_fin_start(c->g, c->partials, c->sc, rp.tol, rp.itmax, allow_short(c), c->st);
+    // linbcg's stop rule: ||b|| unscaled, then the itol norm of every iteration's residual
+    const WStop ws{itol, tol};
+    launch_wstop(c->g, bd, nullptr, 0, -1, tol, c->sc, c->partials, c->st);
     TRY(check_launch(c, "k_rhs_res(solve)"));
     int64_t it = 0, pred = std::max<int64_t>(4, c->pred_iters);
     TRY(run_cg<double>(c, c->sc, c->h_sc, cg64(c, c->r64),
-                       &pred, &it, HF_K_CG_A64, HF_K_CG_B64));
+                       &pred, &it, HF_K_CG_A64, HF_K_CG_B64, &ws));
     launch_finalize(c->g, false, nullptr, xd, c->sv, c->y64, c->r64, c->sc, c->st);
     TRY(check_launch(c, "k_finalize(solve)"));
     HIPCHK(c, hipMemcpyAsync(x, xd, B8, hipMemcpyDefault, c->st));
     TRY(sync_scal(c, c->sc, c->h_sc));
     if (iter) *iter = it;
-    if (err) *err = c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bnorm2) : 0.0;
+    if (err) *err = c->h_sc->wb2 > 0 ? std::sqrt(c->h_sc->wrr / c->h_sc->wb2) : 0.0;
     return HF_OK;
 }
 
@@ -1258,6 +1300,9 @@ extern "C" int hf_set_solver(hf_ctx *c, double tol, int64_t itmax, int32_t preci
     c->carry = false;   // device state or scratch changes: no carried residual
     if (precision != HF_PREC_FP64 && precision != HF_PREC_FP32_IR)
         return set_err(c, HF_ERR_ARG, "hf_set_solver: unknown precision");
+    // the reverse sweep recomputes segments with the current solver: checkpoints of a forward
+    // run under other controls would not match
+    hf::adjoint_invalidate(c->adj);
     const int32_t old = c->p.precision;
     c->p.tol = tol;
     c->p.itmax = itmax;

@@ -75,8 +75,10 @@ struct hf_ctx {
 
     bool uploaded = false, assembled = false, has_state = false;
     bool any_heater = false, dyn_heat = false;
+    bool any_em = false;         // some cell has a nonzero emissivity (with heaters: radiates)
     bool uniform_rc = false;     // rho c_p and tau' rho c_p are the same in every cell
     bool uniform_vol = false;    // the cell volume is the same in every cell
+    bool uniform_len = false;    // each axis has one cell length (the adjoint's precondition)
     double heated_volume = 0.0;
     int32_t setup_flags = 0;
     int64_t pred_iters = 4, pred_inner = 8, pred_adj = 8;
@@ -99,6 +101,10 @@ struct hf_ctx {
 
 
 namespace hf {
+// Returns the step-to-step heuristics of both precision modes (carried residual, TINY guess,
+// sweep and step-length-reuse history, CG batch predictions) to their values in a new context,
+// so that steps from a state set by the caller run the same way whatever ran before (hf_api.hip)
+void reset_step_heuristics(hf_ctx *c);
 // one time step without the final host synchronisation (hf_api.hip)
 int ctx_step(hf_ctx *c, int64_t itime, bool final_sync);
 // FP64 solve of H x = b on device vectors; x holds the initial guess (hf_api.hip)

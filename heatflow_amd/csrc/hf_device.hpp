@@ -114,6 +114,9 @@ struct Scal {
     // (alpha = this Scal's alpha) is still to be added (ypend)
     int32_t yset, ypend;
     int64_t kpend;
+    // linbcg's own stop rule (hf_solve): ||b||^2 unscaled and the last iteration's weighted
+    // residual norm^2 (itol 1: ||r||^2, itol 2: ||r / diag H||^2)
+    double wb2, wrr;
 };
 
 // The solution y of a CG solve on A^ y = r0, as its consumers read it.  The SpMV pass of

@@ -405,6 +405,48 @@ k_cg_b(Dims g, T *__restrict__ r, const T *__restrict__ q, Scal *sc, double *par
     block_partials<1>(acc, partials);
 }
 
+// Norms of linbcg's own stop rule (src/heatflow/mod_Sparse.f90:395, 418-425) for hf_solve.
+// The CG here runs on the scaled system, whose residual is r^ = -s r with s = 1/sqrt(-diag H),
+// so r = -r^ / s and the Jacobi-preconditioned residual z = r / diag H = s r^.  Sum of w_I^2 v_I^2
+// over the cells: mode 0 w = 1 (||b||^2), mode 1 w = 1/s (||r||^2 from r^, itol 1), mode 2
+// w = s (||z||^2, itol 2).  expect >= 0: only when iteration expect - 1 has just completed
+// (sc->iter == expect), else the launch returns at once (its finishing kernel skips as well).
+__global__ void __launch_bounds__(kBlock)
+k_wsum2(Dims g, const double *__restrict__ v, const double *__restrict__ s, int32_t mode,
+        int64_t expect, const Scal *sc, double *partials) {
+    if (expect >= 0 && sc->iter != expect) return;
+    double acc[1] = {0.0};
+    HF_FOR_CELLS(g, I) {
+        double t = v[I];
+        if (mode == 1) t = t / s[I];
+        else if (mode == 2) t = t * s[I];
+        acc[0] += t * t;
+    }
+    block_partials<1>(acc, partials);
+}
+
+// mode 0: ||b||^2 -> wb2, and wrr = 0 (a solve that needs no iteration reports err 0).
+// Else, when iteration expect - 1 has just completed: wrr = its weighted norm^2, and the solve
+// is done if wrr <= tol^2 ||b||^2 (linbcg: err <= tol).
+__global__ void __launch_bounds__(kBlock)
+k_fin_wstop(uint32_t nb, const double *partials, Scal *sc, double tol, int32_t mode,
+            int64_t expect) {
+    if (mode != 0 && sc->iter != expect) return;
+    double v[1];
+    sum_partials<1>(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        sc->wb2 = v[0];
+        sc->wrr = 0.0;
+        // the solve's own test on the scaled residual is switched off (thresh2 < 0 never
+        // holds; the first-iteration prediction cannot meet it either)
+        sc->thresh2 = -1.0;
+        return;
+    }
+    sc->wrr = v[0];
+    if (v[0] <= tol * tol * sc->wb2) sc->done = 1;
+}
+
 // After a solve: y += alpha p_K, the term of its last iteration K >= 1 (p_K = pa for odd K, pb
 // for even); nothing when K = 0 or no term is pending (CgY).  16-byte accesses as k_cg_b.
 template <typename T>
@@ -757,6 +799,13 @@ template void launch_cg_tail<double>(const Dims &, double *, const double *, con
                                      const Scal *, hipStream_t);
 template void launch_cg_tail<float>(const Dims &, float *, const float *, const float *,
                                     const Scal *, hipStream_t);
+
+void launch_wstop(const Dims &g, const double *v, const double *s, int32_t mode, int64_t expect,
+                  double tol, Scal *sc, double *partials, hipStream_t st) {
+    HF_LAUNCH(k_wsum2, g, st, g, v, s, mode, expect, sc, partials);
+    hipLaunchKernelGGL(k_fin_wstop, dim3(1), dim3(kBlock), 0, st, g.nblk, partials, sc, tol, mode,
+                       expect);
+}
 
 void launch_finalize(const Dims &g, bool from_state, const double *tp, double *tpp_io,
                      const double *sv, const double *y, const double *r0, Scal *sc,

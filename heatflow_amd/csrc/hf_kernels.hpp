@@ -63,6 +63,13 @@ struct AdjParams {
     // index masks of the rho c_p / tau' rho c_p and volume reads: 0 where the field is uniform
     // (every cell reads element 0), else ~0
     uint32_t rc_mask, vol_mask;
+    // radiative source (src/heatflow/mod_heating.f90:121-123, 164-167): with heaters present,
+    // Qdens_I holds -em_I L1 L2 sigma (T_I^4 - T_bath^4) / V_heated, so dS^n_I/dT^n_I gains
+    // rad_scale * ells_I * (T^n_I)^3 with rad_scale = 4 / V_heated and ells = em L1 L2 sigma;
+    // rad_scale = 0 when no cell radiates
+    double rad_scale;
+    const double *ells;   // em L1 L2 sigma per cell
+    const double *tn;     // T^n, the state the step started from
 };
 
 __device__ __forceinline__ double dG_int(double ki, double kj, double L, double br) {
@@ -137,6 +144,11 @@ void launch_cg_tail(const Dims &g, T *y, const T *pa, const T *pb, const Scal *s
                     hipStream_t st);
 // The consumers of a solve's y (CgY: y after k_cg_tail, or alpha r0 when the solve ended in
 // its first iteration) and of that solve's scalars.
+// linbcg's stop norms for hf_solve (k_wsum2 + k_fin_wstop): mode 0 records ||b||^2 of v and
+// switches the scaled-residual test off; mode 1 / 2 (itol 1 / 2) the weighted norm^2 of the
+// scaled residual v after iteration expect - 1, ending the solve when it meets tol
+void launch_wstop(const Dims &g, const double *v, const double *s, int32_t mode, int64_t expect,
+                  double tol, Scal *sc, double *partials, hipStream_t st);
 void launch_finalize(const Dims &g, bool from_state, const double *tp, double *tpp_io,
                      const double *sv, const double *y, const double *r0, Scal *sc,
                      hipStream_t st);

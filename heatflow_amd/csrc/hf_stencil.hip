@@ -1359,7 +1359,12 @@ k_adj_step_z(Dims g, AdjParams q, const double *__restrict__ T, const double *__
             a = q.inverse_time * rhoc[I & q.rc_mask];
             cc = q.icattaneo == 1 ? cvec[I & q.rc_mask] : 0.0;
         }
-        g_lo[I] += -(a + 2.0 * cc) * li;
+        double gl = g_lo[I] + -(a + 2.0 * cc) * li;
+        if (q.rad_scale != 0.0) {
+            const double tn = q.tn[I];
+            gl = gl + q.rad_scale * q.ells[I] * (tn * tn * tn) * li;
+        }
+        g_lo[I] = gl;
         g_new[I] = cc * li;
     });
 }

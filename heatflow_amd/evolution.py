@@ -87,6 +87,7 @@ class Simulation:
                  itmax: int = 50000, inner_tol: float = 1.0e-6, inner_itmax: int = 200,
                  assemble: bool = True):
         self.lib = _lib.load()
+        self.forward_generation = 0
         self.system = system
         self.n = system.params.n
         self.precision = precision
@@ -225,6 +226,9 @@ class Simulation:
         check(self.lib.hf_set_adjoint_tol(self.ctx, float(tol)), self.ctx)
 
     def forward_checkpointed(self, itime0: int, nsteps: int, ckpt_every: int) -> None:
+        # every forward replaces the context's checkpoints: the count tells an autograd graph
+        # whether its own are still there
+        self.forward_generation += 1
         check(self.lib.hf_forward_checkpointed(self.ctx, int(itime0), int(nsteps),
                                                int(ckpt_every)), self.ctx)
 

@@ -137,9 +137,11 @@ int hf_step(hf_ctx *ctx, int64_t itime, int64_t *iter, double *err);
 int hf_run(hf_ctx *ctx, int64_t itime0, int64_t nsteps, int64_t *iters_total);
 
 /* linbcg-compatible solve of H x = b on the assembled H (src/heatflow/mod_Sparse.f90:372-434):
- * x is the initial guess on entry and the solution on exit.  itol = 1 or 2 picks the
- * reference's error norm label; the device solver stops on the Jacobi-scaled relative
- * residual.  *iter and *err as in the reference (unused by its caller). */
+ * x is the initial guess on entry and the solution on exit.  The stop rule is linbcg's,
+ * checked after every iteration on the recursively updated residual r: itol = 1 stops when
+ * ||r|| / ||b|| <= tol, itol = 2 when ||r / diag H|| / ||b|| <= tol (:395, 418-425); other itol
+ * values are HF_ERR_ARG.  The iterates are Jacobi-preconditioned CG's (linbcg's BiCG on the
+ * symmetric H).  *iter and *err as in the reference (unused by its caller); an FP64 context. */
 int hf_solve(hf_ctx *ctx, const double *b, double *x, int32_t itol, double tol, int64_t itmax,
              int64_t *iter, double *err);
 
@@ -211,9 +213,13 @@ int hf_set_solver(hf_ctx *ctx, double tol, int64_t itmax, int32_t precision, dou
  * the per-cell conductivity kappa (dl_dkappa, length N) and to a per-cell power P_I that enters
  * as Qdens_I += P_I / volume_I (dl_dpower, length N).  The adjoint solves use the context's
  * precision mode (FP64, or FP32 inner CG inside FP64 refinement) and tolerance (or the one
- * hf_set_adjoint_tol gives).  Requires
- * per-axis uniform cell lengths; the BR switch (x BR where neighbouring kappas differ) is
- * treated as locally constant.  The context is left at the end state of the forward run. */
+ * hf_set_adjoint_tol gives).  The temperature-dependent radiative source of heated systems
+ * (src/heatflow/mod_heating.f90:121-123) enters through its derivative
+ * 4 em L1 L2 sigma (T^n)^3 / V_heated; heater modes 2-6 depend on time only.  Requires
+ * per-axis uniform cell lengths (HF_ERR_ARG otherwise); the BR switch (x BR where neighbouring
+ * kappas differ) is treated as locally constant.  hf_upload_cells, hf_assemble, hf_set_fields
+ * and hf_set_solver discard the checkpoints (hf_adjoint then returns HF_ERR_STATE).  The context
+ * is left at the end state of the forward run. */
 int hf_forward_checkpointed(hf_ctx *ctx, int64_t itime0, int64_t nsteps, int64_t ckpt_every);
 int hf_adjoint(hf_ctx *ctx, const double *dl_dt_final, double *dl_dkappa, double *dl_dpower);
 

@@ -79,6 +79,9 @@ def olib() -> C.CDLL:
         _olib.hfo_coo_len.restype = C.c_int64
         _olib.hfo_sparse_hmatrix.restype = C.c_int64
         _olib.hfo_heater.restype = C.c_double
+        _olib.hfo_coo_hash.restype = C.c_int64
+        _olib.hfo_coo_entry_hash.restype = C.c_uint64
+        _olib.hfo_coo_entry_hash.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double]
     return _olib
 
 
@@ -125,6 +128,14 @@ class OracleSystem:
         olib().hfo_sparse_hmatrix(C.byref(self.s), _p(val), _p(irow), _p(jcol), C.byref(cnt))
         return self.n, L, cnt.value, val, irow, jcol
 
+    def coo_hash(self):
+        """(len, count, hash) of the COO sparse_Hmatrix would write, streamed (nothing stored):
+        sum over positions k of mix(k, irow, jcol, value bits) mod 2^64 (hf_oracle.h)."""
+        h = C.c_uint64()
+        cnt = C.c_int64()
+        L = olib().hfo_coo_hash(C.byref(self.s), C.byref(h), C.byref(cnt))
+        return L, cnt.value, h.value
+
     def boundary(self) -> np.ndarray:
         B = np.empty(self.n)
         olib().hfo_boundary(C.byref(self.s), _p(B))
@@ -157,6 +168,17 @@ class OracleSystem:
         rc = olib().hfo_run(C.byref(self.s), C.byref(sv), C.c_int64(ntime), C.c_double(ts),
                             C.c_int(1 if init else 0), _p(tp), _p(tpp), C.byref(tot))
         return rc, tp, tpp, tot.value
+
+    def linbcg(self, b, x, itol: int = 1, tol: float = 1e-32, itmax: int = 50000):
+        """linbcg (mod_Sparse.f90:372-434) on this system's COO H; returns (x, iter, err)."""
+        n, L, cnt, val, irow, jcol = self.coo()
+        bb = np.ascontiguousarray(b, np.float64)
+        xx = np.array(x, np.float64)
+        it, err = C.c_int64(), C.c_double()
+        olib().hfo_linbcg(C.c_int64(n), C.c_int64(L), _p(val), _p(irow), _p(jcol), _p(bb),
+                          _p(xx), C.c_int64(itol), C.c_double(tol), C.c_int64(itmax),
+                          C.byref(it), C.byref(err))
+        return xx, it.value, err.value
 
     def simulate_steps(self, itimes, temp_p, temp_pp, tol: float = 1e-32, itmax: int = 50000):
         """simulate(itime) for each itime in order, from an explicit state; returns

@@ -168,12 +168,56 @@ int64_t hfo_coo_len(const hfo_system *s) {
     return len;
 }
 
+/* Where sparse_Hmatrix's entries go: the COO arrays, or (arrays NULL) an order-sensitive hash
+ * of (position, irow, jcol, value bits) summed over the entries, sentinels included.  The mix is
+ * the splitmix64 output function (S. Vigna, public domain). */
+typedef struct {
+    double *val;
+    int64_t *irow, *jcol;
+    uint64_t hash;
+} coo_sink;
+
+static uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+uint64_t hfo_coo_entry_hash(int64_t k, int64_t i, int64_t j, double v) {
+    uint64_t vb;
+    memcpy(&vb, &v, sizeof vb);
+    return mix64(mix64(mix64(mix64((uint64_t)k) + (uint64_t)i) + (uint64_t)j) + vb);
+}
+
+static void put(coo_sink *o, int64_t k, double v, int64_t i, int64_t j) {
+    if (o->val) { o->val[k] = v; o->irow[k] = i; o->jcol[k] = j; }
+    else o->hash += hfo_coo_entry_hash(k, i, j, v);
+}
+
+static int64_t emit_coo(const hfo_system *s, coo_sink *o, int64_t *count_out);
+
 /* sparse_Hmatrix, src/heatflow/mod_setup.f90:89-152 */
 int64_t hfo_sparse_hmatrix(const hfo_system *s, double *val, int64_t *irow, int64_t *jcol,
                            int64_t *count_out) {
+    coo_sink o = {val, irow, jcol, 0};
+    return emit_coo(s, &o, count_out);
+}
+
+/* The hash of the COO that hfo_sparse_hmatrix would write (entries past `count` are the
+ * sentinels (0, -2, -1)), without storing it: for grids whose COO does not fit in memory. */
+int64_t hfo_coo_hash(const hfo_system *s, uint64_t *hash, int64_t *count_out) {
+    coo_sink o = {NULL, NULL, NULL, 0};
+    int64_t len = emit_coo(s, &o, count_out);
+    *hash = o.hash;
+    return len;
+}
+
+static int64_t emit_coo(const hfo_system *s, coo_sink *o, int64_t *count_out) {
     const int64_t nx = s->nx, ny = s->ny, nz = s->nz, NA = nx * ny * nz;
     int64_t len = hfo_coo_len(s);
-    for (int64_t k = 0; k < len; ++k) { val[k] = 0.0; irow[k] = -2; jcol[k] = -1; }
+    if (o->val)                                         /* :104-106 */
+        for (int64_t k = 0; k < len; ++k) { o->val[k] = 0.0; o->irow[k] = -2; o->jcol[k] = -1; }
     int64_t addit[6], na = 0;                          /* :107-112 */
     addit[na++] = 1;
     if (s->periodic[0]) addit[na++] = nx - 1;
@@ -187,19 +231,21 @@ int64_t hfo_sparse_hmatrix(const hfo_system *s, double *val, int64_t *irow, int6
         int64_t i = j;
         double H0 = hfo_hmatrixfunc(s, i, j);
         /* the reference writes past `len` only if the formula under-counts; guard it */
-        if (count < len) { val[count] = H0; irow[count] = i; jcol[count] = j; }
+        if (count < len) put(o, count, H0, i, j);
         count++;
         for (int64_t k = 0; k < na; ++k) {
             i = j + addit[k];
             if (i > NA) break;                          /* cycle parent_loop */
             H0 = hfo_hmatrixfunc(s, i, j);
             if (fabs(H0) < TINY) continue;              /* cycle neighbour_loop */
-            if (count < len) { val[count] = H0; irow[count] = i; jcol[count] = j; }
+            if (count < len) put(o, count, H0, i, j);
             count++;
-            if (count < len) { val[count] = H0; irow[count] = j; jcol[count] = i; }
+            if (count < len) put(o, count, H0, j, i);
             count++;
         }
     }
+    if (!o->val)
+        for (int64_t k = count; k < len; ++k) put(o, k, 0.0, -2, -1);
     *count_out = count;
     return len;
 }

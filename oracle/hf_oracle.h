@@ -84,6 +84,11 @@ void hfo_scat(const hfo_system *s, const double *temp_p, const double *temp_pp, 
 
 /* solver::linbcg (src/heatflow/mod_Sparse.f90:372-434) over a COO matrix in the
  * reference layout (SRSax/SRStx/SRSdiag, :293-366).  x is in/out. */
+/* Order-sensitive hash of the COO hfo_sparse_hmatrix would write (positions 0..len-1, the
+ * sentinel tail included), computed without storing it; and the hash of one entry (the sum
+ * over the entries, mod 2^64, is the COO's hash). */
+int64_t hfo_coo_hash(const hfo_system *s, uint64_t *hash, int64_t *count_out);
+uint64_t hfo_coo_entry_hash(int64_t k, int64_t i, int64_t j, double v);
 void hfo_linbcg(int64_t n, int64_t len, const double *val, const int64_t *irow,
                 const int64_t *jcol, const double *b, double *x, int64_t itol, double tol,
                 int64_t itmax, int64_t *iter, double *err);

@@ -77,3 +77,29 @@ GEOMETRIES = [
     dict(nx=5, ny=3, nz=3, periodic=(False, True, False)),
     dict(nx=4, ny=5, nz=3, icattaneo=0, qextra=True),
 ]
+
+
+_M1, _M2, _M3 = np.uint64(0x9e3779b97f4a7c15), np.uint64(0xbf58476d1ce4e5b9), np.uint64(0x94d049bb133111eb)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = z + _M1
+    z = (z ^ (z >> np.uint64(30))) * _M2
+    z = (z ^ (z >> np.uint64(27))) * _M3
+    return z ^ (z >> np.uint64(31))
+
+
+def coo_hash(val: np.ndarray, irow: np.ndarray, jcol: np.ndarray, chunk: int = 1 << 25,
+             k0: int = 0) -> int:
+    """The order-sensitive COO hash of the oracle (hfo_coo_hash, hf_oracle.h): the sum over
+    positions k of mix(mix(mix(mix(k) + irow) + jcol) + value bits), mod 2^64, in chunks; the
+    arrays hold positions k0, k0 + 1, ..."""
+    total = np.uint64(0)
+    with np.errstate(over="ignore"):
+        for a in range(0, len(val), chunk):
+            b = min(a + chunk, len(val))
+            k = np.arange(k0 + a, k0 + b, dtype=np.uint64)
+            h = _mix64(_mix64(_mix64(_mix64(k) + irow[a:b].view(np.uint64))
+                              + jcol[a:b].view(np.uint64)) + val[a:b].view(np.uint64))
+            total = total + np.sum(h, dtype=np.uint64)
+    return int(total)

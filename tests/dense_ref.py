@@ -8,6 +8,11 @@ the last cell of a line to the first (mod_hmatrix.f90:196-246) and have no bath 
 keeps its bath conductance on the diagonal and assigns B_I = T_BathCG / (V n_a n_b) in the face
 order x1, xN, y1, yN, z1, zN (mod_boundary.f90:43-66, 81-82), so a later face's bath term adds
 to it and a later CG face replaces it.
+
+Heaters of constant mode 1 (iheater = 1) and the radiative source follow Heating::heater
+(src/heatflow/mod_heating.f90:27-169): with at least one heater, every cell gets
+Q_I = power_in [iheater_I = 1] - em_I L1 L2 sigma (T^n_I^4 - T_Bath^4) and Qdens = Q / V_heated,
+recomputed from T^n each step, so autograd differentiates the T^4 term too.
 """
 from __future__ import annotations
 
@@ -15,6 +20,8 @@ import dataclasses
 
 import numpy as np
 import torch
+
+STEF_BOLTZ = 5.670373e-8   # constants::StefBoltz, src/heatflow/mod_constants.f90:16
 
 
 # boundary variants of the adjoint checks: periodic axes (wrap couplings, no bath faces) and CG
@@ -29,6 +36,28 @@ DENSE_BOUNDARIES = [
 ]
 
 
+def radiative_system(n: int = 8, icattaneo: int = 1, em: float = 1.0, seed: int = 60):
+    """A small heated system in which the radiative source's derivative weighs in the step:
+    1 mm cells (L1 L2 = 1e-6 m^2), em = `em`, rho c_p = 1e4, dt = 1 s, kappa ~ 1e-3 W/(m K),
+    two mode-1 heaters (V_heated = 2e-9 m^3), T_Bath = 290 K for the radiative term, Dirichlet
+    x faces at 300 K.  4 em L1 L2 sigma T^3 / V_heated is about 3e3 W/(m^3 K), against
+    rho c_p / dt = 1e4 and face conductances of about 1e3: a gradient that leaves the term out
+    is off by tens of percent.  Returns (system, probes)."""
+    from heatflow_amd.system import Params, System
+    rng = np.random.default_rng(seed)
+    N = n ** 3
+    ih = np.zeros(N, np.int64)
+    ih[rng.choice(N, 2, replace=False)] = 1
+    p = Params(nx=n, ny=n, nz=n, time_step=1.0, icattaneo=icattaneo, isteady=0,
+               kappa_bound=(1.0e-3, 1.0e-3, 0.0, 0.0, 0.0, 0.0), t_bath=(300.0,) * 6,
+               power_in=1.0e-5, t_bath_rad=290.0, t_system=300.0)
+    L = n * 1.0e-3
+    s = System.uniform(p, (L, L, L), kappa=1.0e-3 * rng.uniform(0.5, 1.5, N), rho=10.0,
+                       heat_capacity=1.0e3, tau=0.2 if icattaneo else 0.0, em=em, iheater=ih)
+    probes = rng.choice(N, 24, replace=False)
+    return s, probes
+
+
 def with_boundary(s, bi):
     """The system s with the boundary variant DENSE_BOUNDARIES[bi]."""
     from heatflow_amd.system import System
@@ -36,7 +65,9 @@ def with_boundary(s, bi):
 
 
 def dense_run(system, kappa: torch.Tensor, power: torch.Tensor, nsteps: int,
-              t_init: float | None = None) -> torch.Tensor:
+              t_init: float | None = None, radiative_grad: bool = True) -> torch.Tensor:
+    """T after `nsteps` steps.  radiative_grad=False holds the radiative term's T^n constant
+    for autograd (same values): the gradient an adjoint without that term's derivative gives."""
     p, c = system.params, system.cells
     nx, ny, nz = p.nx, p.ny, p.nz
     N = p.n
@@ -91,12 +122,25 @@ def dense_run(system, kappa: torch.Tensor, power: torch.Tensor, nsteps: int,
     alpha = cvec + rc / dt
     H = H - torch.diag(alpha)
     qd = power / vol
+    ih = np.asarray(c.iheater)
+    heated = ih > 0
+    if heated.any():
+        assert set(np.unique(ih[heated])) == {1}, "dense reference: constant heaters only"
+        hv = float(np.sum(np.asarray(c.volume)[heated]))
+        q_heat = torch.as_tensor(np.where(ih == 1, p.power_in, 0.0), dtype=torch.float64)
+        ells = torch.as_tensor(np.asarray(c.em) * np.asarray(c.len_x) * np.asarray(c.len_y)
+                               * STEF_BOLTZ, dtype=torch.float64)
     lu, piv = torch.linalg.lu_factor(H)
     t0 = p.t_system if t_init is None else t_init
     tp = torch.full((N,), t0, dtype=torch.float64)
     tpp = tp.clone()
     for _ in range(nsteps):
-        S = -(tp * rc / dt) - qd - B + cvec * (tpp - 2.0 * tp)
+        qdn = qd
+        if heated.any():
+            tr = tp if radiative_grad else tp.detach()
+            q = q_heat - ells * (tr ** 4 - p.t_bath_rad ** 4)
+            qdn = q / hv + qd
+        S = -(tp * rc / dt) - qdn - B + cvec * (tpp - 2.0 * tp)
         tn = torch.linalg.lu_solve(lu, piv, S.unsqueeze(1)).squeeze(1)
         tpp, tp = tp, tn
     return tp

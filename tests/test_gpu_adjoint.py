@@ -9,7 +9,7 @@ from heatflow_amd import HeatFlowError, Simulation, workloads
 from heatflow_amd.autograd import heatflow_run
 
 from _cases import random_system
-from dense_ref import DENSE_BOUNDARIES, dense_run, with_boundary
+from dense_ref import DENSE_BOUNDARIES, dense_run, radiative_system, with_boundary
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +75,22 @@ def test_autograd_function_on_device():
                                    atol=1e-12 * np.abs(gp).max())
     finally:
         sim.close()
+
+
+def test_autograd_refuses_a_stale_graph():
+    """A second heatflow_run on the same Simulation replaces the checkpoints the first graph's
+    backward would need: that backward raises; the second graph's still works."""
+    s, m = workloads.c4(12, nprobe=8)
+    with Simulation(s, tol=1e-32) as sim:
+        k = torch.tensor(s.cells.kappa, requires_grad=True)
+        pw = torch.tensor(m["power"], requires_grad=True)
+        pr = torch.as_tensor(m["probes"])
+        T1 = heatflow_run(k, pw, sim, 6, 3)
+        T2 = heatflow_run(k, pw, sim, 6, 3)
+        with pytest.raises(RuntimeError):
+            T1[pr].sum().backward()
+        T2[pr].sum().backward()
+        assert k.grad is not None and torch.any(k.grad != 0)
 
 
 @pytest.mark.parametrize("icat", [0, 1])
@@ -316,3 +332,68 @@ def test_adjoint_checkpoint_spacing_is_exact(prec):
             assert abs(la - lb) <= 1e-9 * abs(lb)
             assert np.max(np.abs(ka - kb)) <= 1e-9 * np.max(np.abs(kb))
             assert np.max(np.abs(pa - pb)) <= 1e-9 * np.max(np.abs(pb))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("icat", [0, 1])
+def test_adjoint_radiative_source_matches_dense(icat, prec):
+    """A heated system whose radiative source -em L1 L2 sigma (T^n^4 - T_Bath^4) / V_heated
+    (src/heatflow/mod_heating.f90:121-123, 164-167) weighs in every step: the adjoint carries its
+    derivative 4 em L1 L2 sigma (T^n)^3 / V_heated into the adjoint history.  Loss and gradients
+    against dense FP64 autograd (which differentiates T^4 itself); the same autograd with the
+    term's T held constant is 60-90% away, so a missing term cannot pass."""
+    s, probes = radiative_system(8, icattaneo=icat)
+    steps, ckpt = 10, 3
+    k = torch.tensor(s.cells.kappa, requires_grad=True)
+    pw = torch.zeros(s.params.n, dtype=torch.float64, requires_grad=True)
+    T = dense_run(s, k, pw, steps)
+    targets = T.detach().numpy()[probes] - 0.01
+    loss = ((T[torch.as_tensor(probes)] - torch.as_tensor(targets)) ** 2).sum()
+    loss.backward()
+    gk_ref, gp_ref = k.grad.numpy(), pw.grad.numpy()
+    k2 = torch.tensor(s.cells.kappa, requires_grad=True)
+    pw2 = torch.zeros(s.params.n, dtype=torch.float64, requires_grad=True)
+    T2 = dense_run(s, k2, pw2, steps, radiative_grad=False)
+    ((T2[torch.as_tensor(probes)] - torch.as_tensor(targets)) ** 2).sum().backward()
+    tol = 1e-32 if prec == "fp64" else 1e-13
+    with Simulation(s, precision=prec, tol=tol) as sim:
+        sim.initial_evolve()
+        L, gk, gp = sim.grad(1, steps, ckpt, probes, targets)
+    bar = 1e-7 if prec == "fp64" else 1e-6
+    assert abs(L - loss.item()) <= bar * abs(loss.item())
+    ek = np.abs(gk - gk_ref).max() / np.abs(gk_ref).max()
+    ep = np.abs(gp - gp_ref).max() / np.abs(gp_ref).max()
+    print(f"radiative adjoint {prec} icat={icat}: dL/dkappa {ek:.3e}, dL/dP {ep:.3e}", flush=True)
+    assert ek <= bar and ep <= bar
+    miss = np.abs(k2.grad.numpy() - gk_ref).max() / np.abs(gk_ref).max()
+    assert miss > 0.1
+
+
+def test_adjoint_refuses_mixed_uses():
+    """The checkpoints belong to the fields, operator and solver controls of their forward run:
+    re-uploading, re-assembling or changing the solver in between makes hf_adjoint fail with
+    HF_ERR_STATE instead of reversing another system.  Non-uniform cell lengths are refused."""
+    s, m = workloads.c4(12, nprobe=8)
+    seed = np.zeros(s.params.n)
+    seed[m["probes"]] = 1.0
+    for change in ("assemble", "solver", "upload"):
+        with Simulation(s, tol=1e-32) as sim:
+            sim.initial_evolve()
+            sim.forward_checkpointed(1, 6, 3)
+            sim.adjoint(seed)   # valid
+            if change == "assemble":
+                sim.sparse_Hmatrix()
+            elif change == "solver":
+                sim.set_solver(1e-13, 50000, "fp64")
+            else:
+                sim.upload(s)
+            with pytest.raises(HeatFlowError) as ei:
+                sim.adjoint(seed)
+            assert ei.value.code == 13
+    s2 = random_system(5, 6, 5, 4)
+    s2.cells.len_x[7] *= 1.5
+    with Simulation(s2, tol=1e-32) as sim:
+        sim.initial_evolve()
+        with pytest.raises(HeatFlowError) as ei:
+            sim.forward_checkpointed(1, 4, 2)
+        assert ei.value.code == 10

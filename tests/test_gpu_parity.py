@@ -255,6 +255,50 @@ def test_linbcg_dense():
     assert it > 0
 
 
+def _dense_h_solution(o, b):
+    n, L, cnt, val, irow, jcol = o.coo()
+    H = np.zeros((n, n))
+    m = irow > 0
+    np.add.at(H, (irow[m] - 1, jcol[m] - 1), val[m])
+    return H, np.linalg.solve(H, b)
+
+
+@pytest.mark.parametrize("itol,tol", [(1, 1e-4), (1, 1e-8), (1, 1e-12), (2, 1e-17), (2, 1e-19),
+                                      (2, 1e-21)])
+def test_linbcg_stop_rule_matches_oracle(itol, tol):
+    """hf_solve stops on linbcg's own test (mod_Sparse.f90:418-425): itol 1 on ||r|| / ||b||,
+    itol 2 on ||r / diag H|| / ||b||, after the first iteration that meets it.  On a steady
+    system (no mass term; 14 to 36 iterations) the iteration count is the oracle's (BiCG on the
+    COO H, which the reference build reproduces) within one, err <= tol, and the same solve
+    capped one iteration earlier has err > tol."""
+    s = random_system(21, 6, 5, 4, isteady=1)
+    o = oracle.OracleSystem(s)
+    b = np.random.default_rng(0).uniform(-1e3, 1e3, s.params.n)
+    x0 = np.zeros(s.params.n)
+    xo, ito, erro = o.linbcg(b, x0, itol=itol, tol=tol)
+    with Simulation(s) as sim:
+        x, it, err = sim.linbcg(b, x0, itol=itol, tol=tol)
+        _, it_prev, err_prev = sim.linbcg(b, x0, itol=itol, tol=tol, itmax=it - 1)
+    assert err <= tol < err_prev
+    assert it_prev == it - 1
+    assert abs(it - ito) <= 1, (it, ito)
+    # the solution's error is what the stop rule allows: ||x - x*|| <= ||H^-1 W|| tol ||b||
+    # (W = I for itol 1, diag H for itol 2; the recursive residual is the true one to round-off)
+    H, xd = _dense_h_solution(o, b)
+    W = np.eye(len(b)) if itol == 1 else np.diag(np.diag(H))
+    bound = np.linalg.norm(np.linalg.solve(H, W), 2) * tol * np.linalg.norm(b)
+    assert np.linalg.norm(x - xd) <= 2 * bound + 1e-12 * np.linalg.norm(xd)
+    assert np.linalg.norm(xo - xd) <= 2 * bound + 1e-12 * np.linalg.norm(xd)
+
+
+def test_linbcg_rejects_unknown_itol():
+    s = random_system(21, 5, 4, 4)
+    with Simulation(s) as sim:
+        with pytest.raises(HeatFlowError) as ei:
+            sim.linbcg(np.ones(s.params.n), np.zeros(s.params.n), itol=3)
+        assert ei.value.code == 10
+
+
 # ---- failure handling (the reference's fatal NaN guards) -----------------------------------
 
 def _nan_system(kind):
@@ -552,3 +596,30 @@ def test_pair_kernels_match_scalar(monkeypatch, prec):
                 out.append(sim.get_state()[0])
         for o in out[1:]:
             assert relerr(out[0], o) <= 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp32_ir", "fp64"])
+def test_timed_steps_do_not_depend_on_earlier_calls(prec):
+    """hf_set_state starts the step heuristics afresh (carried residual, TINY guess, sweep and
+    step-length-reuse history, CG batch sizes): 20 steps from the same state after 0, 2 or 5
+    earlier steps give identical statistics and bit-identical temperatures (the 512^3
+    mixed-precision workload's physics at 32^3, in the step-length-reuse regime)."""
+    s, _ = workloads.c3(32)
+    out = []
+    for warm in (0, 2, 5):
+        with Simulation(s, precision=prec, tol=1e-12) as sim:
+            sim.initial_evolve()
+            if warm:
+                sim.run(1, warm)
+            sim.initial_evolve()
+            sim.reset_stats()
+            sim.run(1, 20)
+            st = sim.stats()
+            stats = (st.steps, st.outer_iters, st.inner_iters, st.first_iter_stops,
+                     st.tiny_misses, st.residual_reruns, st.alpha_reuses, st.reuse_misses)
+            out.append((stats, sim.get_state()[0]))
+    for stats, tp in out[1:]:
+        assert stats == out[0][0]
+        assert np.array_equal(tp.view(np.int64), out[0][1].view(np.int64))
+    if prec == "fp32_ir":
+        assert out[0][0][6] > 0   # the run reused the step length

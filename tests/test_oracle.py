@@ -242,3 +242,47 @@ def test_dense_reference_matches_oracle(icat, bi):
     pw = torch.as_tensor(m["power"])
     td = dense_run(s, k, pw, 7).numpy()
     np.testing.assert_allclose(td, tp, rtol=1e-12)
+
+
+@pytest.mark.parametrize("icat", [0, 1])
+def test_dense_reference_radiative_matches_oracle(icat):
+    """The dense reference's heaters and radiative source (Q = power_in - em L1 L2 sigma
+    (T^4 - T_Bath^4), Qdens = Q / V_heated, mod_heating.f90:121-123, 164-167) step the same
+    temperatures as the oracle on a system where radiation cools it by ~9 K in 10 steps."""
+    import torch
+    from dense_ref import dense_run, radiative_system
+    s, _ = radiative_system(6, icattaneo=icat)
+    rc, tp, _, _ = oracle.OracleSystem(s).run(10)
+    assert rc == 0
+    td = dense_run(s, torch.as_tensor(s.cells.kappa), torch.zeros(s.params.n,
+                   dtype=torch.float64), 10).numpy()
+    assert tp.min() < 295.0
+    np.testing.assert_allclose(td, tp, rtol=1e-12)
+
+
+def test_coo_hash_streamed_matches_stored():
+    """The oracle's streamed COO hash (nothing stored; used where the COO does not fit in
+    memory) equals the numpy hash of its stored COO, entry hash by entry hash and in total,
+    on periodic, CG-face and TINY-dropping geometries; swapping two entries or flipping one
+    value bit changes it."""
+    from _cases import GEOMETRIES, coo_hash, random_system
+    for gi in (0, 1, 3, 7):
+        s = random_system(300 + gi, **GEOMETRIES[gi])
+        o = oracle.OracleSystem(s)
+        n, L, cnt, val, irow, jcol = o.coo()
+        L2, cnt2, h = o.coo_hash()
+        assert (L2, cnt2) == (L, cnt)
+        assert coo_hash(val, irow, jcol) == h
+        for k in (0, cnt // 2, L - 1):
+            e = oracle.olib().hfo_coo_entry_hash(k, int(irow[k]), int(jcol[k]), float(val[k]))
+            assert coo_hash(val[k:k + 1], irow[k:k + 1], jcol[k:k + 1], k0=k) == e
+        v2 = val.copy()
+        v2[[1, 2]] = v2[[2, 1]]
+        i2 = irow.copy()
+        i2[[1, 2]] = i2[[2, 1]]
+        j2 = jcol.copy()
+        j2[[1, 2]] = j2[[2, 1]]
+        assert coo_hash(v2, i2, j2) != h
+        v3 = val.copy()
+        v3.view(np.uint64)[5] ^= np.uint64(1)
+        assert coo_hash(v3, irow, jcol) != h

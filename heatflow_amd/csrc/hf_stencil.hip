@@ -685,7 +685,7 @@ struct StepCell {
 // the same pass with each trip's loads issued after its barrier.  Transient is isteady == 0,
 // Cattaneo icattaneo == 1, Wrap some axis periodic (compile-time, so the trip is straight-line
 // code the scheduler can order freely).
-template <bool FromState, bool Transient, bool Cattaneo, bool Wrap>
+template <bool FromState, bool Transient, bool Cattaneo, bool Wrap, bool UniformRc>
 __global__ void __launch_bounds__(kZTX * kZTY) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             const double *__restrict__ tpp, const double *__restrict__ xb,
@@ -698,6 +698,10 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             Scal *sc, double *partials) {
     __shared__ PlaneBuf<D2> sp;
     const bool tiny = sc->tiny != 0;
+    // rho c_p and tau' rho c_p of every cell when both are uniform (UniformRc): one scalar
+    // read instead of a load per cell, and two fewer doubles carried per cell record
+    const double rc_u = UniformRc ? rhoc[0] : 0.0;
+    const double cv_u = UniformRc && Cattaneo ? cvec[0] : 0.0;
     // the inner solution (CgY), formed in FP32 as k_cg_b1_z forms its terms; 0 when the solve
     // needed no iteration
     const CgY<float> ysol = cg_y(inner, y, r0);
@@ -741,8 +745,8 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         StepCell v;
         v.cz = ldnt(&cz[I]);
         v.qb = ldnt(&qb[I]);
-        v.rc = Transient ? ldnt(&rhoc[I & sp_.rc_mask]) : 0.0;
-        v.cv = (Transient && Cattaneo) ? ldnt(&cvec[I & sp_.rc_mask]) : 0.0;
+        v.rc = Transient && !UniformRc ? ldnt(&rhoc[I & sp_.rc_mask]) : 0.0;
+        v.cv = Transient && Cattaneo && !UniformRc ? ldnt(&cvec[I & sp_.rc_mask]) : 0.0;
         v.cx = cx[I];
         v.cy = cy[I];
         v.cxm = cx[I + o.xm];
@@ -776,11 +780,16 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     auto pick_a = [](const D2 &v) { return v.a; };
     auto pick_b = [](const D2 &v) { return v.b; };
     const bool guess = sp_.tiny_guess != 0;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    // r^0 . r' of this thread's cells, kept in its own LDS slot rather than a register (the
-    // pipelined trip uses all 128 VGPRs that 4 waves per SIMD allow)
-    __shared__ double r0r[kZTX * kZTY];
-    r0r[threadIdx.x] = 0.0;
+    // This thread's sums over its cells (|r^|^2, |b^|^2, |r^0|^2, r^0 . r'), each kept in its
+    // own LDS slot and added to in cell order, rather than in registers: the pipelined trip
+    // uses nearly all 128 VGPRs that 4 waves per SIMD allow
+    __shared__ double part[4][kZTX * kZTY];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) part[k][threadIdx.x] = 0.0;
+    auto add_part = [&](int k, double v) {
+        __hip_atomic_fetch_add(&part[k][threadIdx.x], v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
     // flags of this step (NAN_X, TINY_NEXT) and of the next step's S (NAN_SCAT, NAN_S): distinct
     // bits in one word
     int32_t fl = 0;
@@ -805,10 +814,12 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = X(rg.col + g.nxny * c.z0);
     }
     int buf = 0;
-    // One trip: plane z (cell I of the column) with its own operand inputs of plane z+1 (r1)
-    // and its cell inputs (cur), loaded one trip earlier; loads r2 (z+2) and cn (z+1).
-    auto trip = [&](uint32_t z, uint32_t I, const StepRaw &r1, const StepCell &cur, StepRaw &r2,
-                    StepCell &cn) {
+    // One trip: plane z (cell I of the column) with its own operand inputs of planes z (ri)
+    // and z+1 (r1) and its cell inputs (cur), loaded one and two trips earlier; loads r2 (z+2),
+    // cn (z+1) and, on a ring thread, the ring cell's inputs of plane z+1 (formed into the LDS
+    // plane at the end of the trip).
+    auto trip = [&](uint32_t z, uint32_t I, const StepRaw &ri, const StepRaw &r1,
+                    const StepCell &cur, StepRaw &r2, StepCell &cn) {
         const bool more = z + 1 < c.z1;
         const bool has_p = c.active && (z + 1 < g.nz || per_z);
         const bool has_p2 = c.active && more && (z + 2 < g.nz || per_z);
@@ -819,7 +830,12 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         // thread read recently (cache hits): its own cell, or cell 0 (inactive threads).
         const uint32_t own = c.active ? I : 0u;
         r2 = load(has_p2 ? zp_index(g, c.col, z + 1) : own);
-        const StepRaw rr = load(rg.col != kNoRing ? rg.col + g.nxny * (more ? z + 1 : z) : 0u);
+        // (the address is opaque to the compiler, which would otherwise load under a branch on
+        // the ring test and zero the record elsewhere: that write waits for the record's
+        // previous load)
+        uint32_t jr = rg.col != kNoRing ? rg.col + g.nxny * (more ? z + 1 : z) : 0u;
+        asm volatile("" : "+v"(jr));
+        const StepRaw rr = load(jr);
         cn = cload(c.active && more ? I + g.nxny : own);
         __syncthreads();   // plane z is staged in sp.v[buf]
         D2 xz_p = 0.0;
@@ -833,11 +849,13 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             // the next, with Qdens + B summed once per run (qb)
             double Sn, Sn1;
             if (Transient) {
-                Sn = -(sp_.inverse_time * tpi * cur.rc) - cur.qb;
-                Sn1 = -(sp_.inverse_time * xi.a * cur.rc) - cur.qb;
+                const double rci = UniformRc ? rc_u : cur.rc;
+                const double cvi = UniformRc ? cv_u : cur.cv;
+                Sn = -(sp_.inverse_time * tpi * rci) - cur.qb;
+                Sn1 = -(sp_.inverse_time * xi.a * rci) - cur.qb;
                 if (Cattaneo) {
-                    Sn = Sn + (tppi - 2.0 * tpi) * cur.cv;
-                    const double scat = (tpi - 2.0 * xi.a) * cur.cv;
+                    Sn = Sn + (tppi - 2.0 * tpi) * cvi;
+                    const double scat = (tpi - 2.0 * xi.a) * cvi;
                     if (isnan(scat)) fl |= FLAG_NAN_SCAT;
                     Sn1 = Sn1 + scat;
                 }
@@ -869,12 +887,11 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             if (sp_.store_res) stnt(&rres[I], rf);
             stnt(&rnext[I], fn);
             const double bh = s * Sn1;
-            acc[0] += (double)rf * (double)rf;
-            acc[1] += bh * bh;
-            acc[2] += (double)fn * (double)fn;
+            add_part(0, (double)rf * (double)rf);
+            add_part(1, bh * bh);
+            add_part(2, (double)fn * (double)fn);
             // r^0 . r' when y = alpha r^0 (the sweep reused alpha): gives r^0 . A^ r^0
-            __hip_atomic_fetch_add(&r0r[threadIdx.x], (double)ri.y * (double)rf, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            add_part(3, (double)ri.y * (double)rf);
             if (isnan(xi.a)) fl |= FLAG_NAN_X;
             if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;   // mod_evolve.f90:141, next step
             cz_m = cur.cz;
@@ -888,22 +905,34 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         xz_m = xi;
         xi = xz_p;
         s_i = s_p;
-        ri = r1;
     };
-    // Two trips per iteration, alternating the register sets (A, B) that carry the loads
-    // between trips: a copy from one set to the other at the end of each trip would wait for
+    // Six trips per iteration: the own operand records of planes z, z+1, z+2 rotate through
+    // three register sets and the cell records of planes z, z+1 through two, so every loaded
+    // record keeps its registers from its load to its last use.  A record that changed
+    // registers between trips would be copied at the loop's back edge, and the copy waits for
     // the loads just issued.  (The compiler does not unroll a loop with a barrier by itself.)
-    StepRaw rB{};
-    StepCell cB{};
-    uint32_t I = c.col + g.nxny * c.z0;
-    for (uint32_t z = c.z0; z < c.z1; z += 2, I += 2 * g.nxny) {
-        trip(z, I, r1, cur, rB, cB);
+    StepRaw r2{};
+    StepCell cn{};
+    const uint32_t P = g.nxny;
+    uint32_t I = c.col + P * c.z0;
+    for (uint32_t z = c.z0; z < c.z1; z += 6, I += 6 * P) {
+        trip(z, I, ri, r1, cur, r2, cn);
         if (z + 1 >= c.z1) break;
-        trip(z + 1, I + g.nxny, rB, cB, r1, cur);
+        trip(z + 1, I + P, r1, r2, cn, ri, cur);
+        if (z + 2 >= c.z1) break;
+        trip(z + 2, I + 2 * P, r2, ri, cur, r1, cn);
+        if (z + 3 >= c.z1) break;
+        trip(z + 3, I + 3 * P, ri, r1, cn, r2, cur);
+        if (z + 4 >= c.z1) break;
+        trip(z + 4, I + 4 * P, r1, r2, cur, ri, cn);
+        if (z + 5 >= c.z1) break;
+        trip(z + 5, I + 5 * P, r2, ri, cn, r1, cur);
     }
     block_flags(fl & (FLAG_NAN_X | FLAG_TINY_NEXT), &sc->flags);
     block_flags(fl & (FLAG_NAN_SCAT | FLAG_NAN_S), &sc->flags_next);
-    acc[3] = r0r[threadIdx.x];
+    double acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] = part[k][threadIdx.x];
     block_partials<4>(acc, partials);
 }
 
@@ -1540,27 +1569,33 @@ void launch_ir_step(const Dims &g0, const StepParams &sp, bool from_state, const
                     const double *cz, double *xout, float *rres, float *rnext, Scal *sc,
                     double *partials, hipStream_t st) {
     const Dims g = zdims(g0, kZPipe);
+#define HF_IR_STEP_W(F, T, C, U)                                                               \
+    do {                                                                                       \
+        if (wrap)                                                                              \
+            HF_ZLAUNCH((k_ir_step_z<F, T, C, true, U>), g, st, g, sp, tp, tpp, xb, y, r0, inner, \
+                       rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);        \
+        else                                                                                   \
+            HF_ZLAUNCH((k_ir_step_z<F, T, C, false, U>), g, st, g, sp, tp, tpp, xb, y, r0,      \
+                       inner, rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);  \
+    } while (0)
 #define HF_IR_STEP(F, T, C)                                                                   \
     do {                                                                                      \
-        if (wrap)                                                                             \
-            HF_ZLAUNCH((k_ir_step_z<F, T, C, true>), g, st, g, sp, tp, tpp, xb, y, r0, inner,  \
-                       rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);       \
-        else                                                                                  \
-            HF_ZLAUNCH((k_ir_step_z<F, T, C, false>), g, st, g, sp, tp, tpp, xb, y, r0, inner, \
-                       rhoc, cvec, qb, d, cx, cy, cz, xout, rres, rnext, sc, partials);       \
+        if (T && sp.rc_mask == 0) HF_IR_STEP_W(F, T, C, true);                                \
+        else HF_IR_STEP_W(F, T, C, false);                                                    \
     } while (0)
     const bool tr = sp.isteady == 0, cat = sp.icattaneo == 1;
     const bool wrap = g.per[0] || g.per[1] || g.per[2];
     if (from_state) {
-        if (!tr) HF_IR_STEP(true, false, false);
+        if (!tr) HF_IR_STEP_W(true, false, false, false);
         else if (cat) HF_IR_STEP(true, true, true);
         else HF_IR_STEP(true, true, false);
     } else {
-        if (!tr) HF_IR_STEP(false, false, false);
+        if (!tr) HF_IR_STEP_W(false, false, false, false);
         else if (cat) HF_IR_STEP(false, true, true);
         else HF_IR_STEP(false, true, false);
     }
 #undef HF_IR_STEP
+#undef HF_IR_STEP_W
 }
 
 }  // namespace hf

@@ -28,35 +28,45 @@ def errors(t, ref, t_bath=300.0):
     return float(np.max(d / np.abs(ref))), float(d.max() / np.abs(ref - t_bath).max())
 
 
+_ORACLE = {}   # oracle runs shared by the variants of a test
+
+
 # ---- layered slabs + spherical inclusions (kappa 150 / 1.4 / 30, inclusions 0.2) ----------
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
-def test_layered_inclusions_forward_matches_oracle(prec):
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32_ir", 1e-12), ("fp32_ir", 1e-13)])
+def test_layered_inclusions_forward_matches_oracle(prec, tol):
     """The 384^3 forward+adjoint workload's geometry at 32^3 (8 slabs of 4 planes, 0.2 W/(m K)
-    spheres filling 5%, 8 heaters), 40 steps, against the oracle at the reference tolerance:
-    temperatures to 1e-10 relative and the heating signal T - 300 K to 1e-7 of its peak."""
+    spheres filling 8%, 8 heaters), 40 steps, against the oracle at the reference tolerance:
+    temperatures to 1e-10 relative and the heating signal T - 300 K to 1e-6 of its peak.
+    The stop test bounds the 2-norm of the scaled residual, which lets the error gather in a few
+    cells and grow over the steps: FP64 at 1e-12 ends 1.75e-10 away here, so the FP64 gate runs
+    at 1e-13; the refined solver stops on the FP64 residual of its iterate."""
     s, m = workloads.c5(32, nheat=8, nprobe=32)
-    assert 0.04 < m["inclusion_fraction"] < 0.08
+    assert 0.04 < m["inclusion_fraction"] < 0.1
     assert set(np.unique(s.cells.kappa)) == {0.2, 1.4, 30.0, 150.0}
-    rc, otp, _, _ = oracle.OracleSystem(s).run(40)
+    if "c5-32" not in _ORACLE:
+        _ORACLE["c5-32"] = oracle.OracleSystem(s).run(40)
+    rc, otp, _, _ = _ORACLE["c5-32"]
     assert rc == 0
-    with Simulation(s, precision=prec, tol=1e-12) as sim:
+    with Simulation(s, precision=prec, tol=tol) as sim:
         sim.initial_evolve()
         sim.run(1, 40)
         tp, _ = sim.get_state()
     rel, sig = errors(tp, otp)
-    print(f"layered 32^3 {prec}: max rel {rel:.3e}, signal {sig:.3e}, "
+    print(f"layered 32^3 {prec}@{tol:g}: max rel {rel:.3e}, signal {sig:.3e}, "
           f"peak signal {np.abs(otp - 300).max():.3e} K", flush=True)
     assert np.abs(otp - 300.0).max() > 1e-2
-    assert rel <= 1e-10 and sig <= 1e-7
+    assert rel <= 1e-10 and sig <= 1e-6
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
 def test_layered_inclusions_gradient_matches_dense(prec):
-    """The same geometry at 12^3 (one inclusion sphere, the kappa contrasts 150 : 1.4 : 30 :
-    0.2 across slab and sphere faces), 20 steps, checkpoints every 6: loss, dL/dkappa and dL/dP
-    of the adjoint against dense FP64 torch autograd (itself pinned to the oracle)."""
+    """The same builder at 12^3 (the inclusion spheres, radius 4 cells and up, then cover 3/4
+    of the grid; every contrast of 150 : 1.4 : 30 : 0.2 occurs across slab and sphere faces),
+    20 steps, checkpoints every 6: loss, dL/dkappa and dL/dP of the adjoint against dense FP64
+    torch autograd (itself pinned to the oracle)."""
     s, m = workloads.c5(12, nheat=4, nprobe=24)
+    assert set(np.unique(s.cells.kappa)) == {0.2, 1.4, 30.0, 150.0}
     steps, ckpt = 20, 6
     k = torch.tensor(s.cells.kappa, requires_grad=True)
     pw = torch.tensor(m["power"], requires_grad=True)
@@ -86,7 +96,9 @@ def test_two_phase_64_matches_oracle(prec):
     """The 256^3 two-phase workload's physics at 64^3 (Bernoulli(0.3) voxels of kappa 0.2 in
     kappa 150, 32 heaters), 20 steps, against the oracle at the reference tolerance."""
     s, _ = workloads.c2(64)
-    rc, otp, _, _ = oracle.OracleSystem(s).run(20)
+    if "c2-64" not in _ORACLE:
+        _ORACLE["c2-64"] = oracle.OracleSystem(s).run(20)
+    rc, otp, _, _ = _ORACLE["c2-64"]
     assert rc == 0
     with Simulation(s, precision=prec, tol=1e-12) as sim:
         sim.initial_evolve()
@@ -94,17 +106,20 @@ def test_two_phase_64_matches_oracle(prec):
         tp, _ = sim.get_state()
     rel, sig = errors(tp, otp)
     print(f"two-phase 64^3 {prec}: max rel {rel:.3e}, signal {sig:.3e}", flush=True)
-    assert rel <= 1e-10 and sig <= 1e-7
+    assert rel <= 1e-10 and sig <= 1e-6
 
 
 def test_two_phase_full_size_fp64():
-    """The 256^3 two-phase workload as configured (16.8M cells, 50 steps, FP64): at tol 1e-12
+    """The 256^3 two-phase workload as configured (16.8M cells, 50 steps, FP64): at tol 1e-14
     against the same run at the reference tolerance 1e-32 (1e-10 relative, 1e-6 of the signal),
     and the last step's true residual |S^49 - H T^50| / |S^49| from the device's own H, B,
-    Qdens and the state (the operator itself is pinned by the COO hash below)."""
+    Qdens and the state (the operator itself is pinned by the COO hash below).
+    The stop test bounds the 2-norm of the residual over all cells, so the largest cell error
+    it allows grows with sqrt(N) and with the number of steps: at 1e-12 this grid ends 5.8e-9
+    away from the 1e-32 run (6.3e-11 at 64^3 over 20 steps)."""
     s, _ = workloads.c2(256)
     out = {}
-    for tol in (1e-12, 1e-32):
+    for tol in (1e-14, 1e-32):
         with Simulation(s, tol=tol) as sim:
             sim.initial_evolve()
             sim.run(1, 49)
@@ -113,8 +128,8 @@ def test_two_phase_full_size_fp64():
             t50, _ = sim.get_state()
             vec = {k: sim.vector(k) for k in ("diag", "cx", "cy", "cz", "B", "Qdens")}
         out[tol] = (t48, t49, t50, vec)
-    rel, sig = errors(out[1e-12][2], out[1e-32][2])
-    print(f"two-phase 256^3: tol 1e-12 vs 1e-32 max rel {rel:.3e}, signal {sig:.3e}", flush=True)
+    rel, sig = errors(out[1e-14][2], out[1e-32][2])
+    print(f"two-phase 256^3: tol 1e-14 vs 1e-32 max rel {rel:.3e}, signal {sig:.3e}", flush=True)
     assert rel <= 1e-10 and sig <= 1e-6
     t48, t49, t50, v = out[1e-32]
     n = 256

@@ -25,7 +25,7 @@ EXPORTS = (
     "hf_set_state", "hf_get_state", "hf_step", "hf_run", "hf_solve", "hf_get_vector",
     "hf_set_debug", "hf_get_stats", "hf_reset_stats", "hf_set_solver", "hf_grad",
     "hf_set_timing", "hf_kernel_times", "hf_forward_checkpointed", "hf_adjoint",
-    "hf_set_fields", "hf_set_adjoint_tol",
+    "hf_set_fields", "hf_set_adjoint_tol", "hf_export_coo_noalloc",
     "hf_last_error", "hf_destroy", "hf_version",
 )
 
@@ -93,6 +93,7 @@ def load() -> C.CDLL:
         "hf_upload_cells": (C.c_int, [P, P, P, P, P, P, P, P, P, P, P, P]),
         "hf_assemble": (C.c_int, [P]),
         "hf_export_coo": (C.c_int, [P, I64, I64, I64, P, P, P]),
+        "hf_export_coo_noalloc": (C.c_int, [P, I64, I64, I64, P, P, P]),
         "hf_set_state": (C.c_int, [P, P, P]),
         "hf_get_state": (C.c_int, [P, P, P]),
         "hf_step": (C.c_int, [P, C.c_int64, I64, D]),

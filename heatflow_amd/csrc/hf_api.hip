@@ -302,6 +302,16 @@ extern "C" int hf_create(hf_ctx **out, const hf_params *p) {
     hf_ctx *c = new hf_ctx();
     c->p = *p;
     hipGetDevice(&c->dev);
+    {
+        // the static scratch of hf_export_coo_noalloc on this device (once per device)
+        const hipError_t e = hf::noalloc_setup(c->dev);
+        if (e != hipSuccess) {
+            delete c;
+            return set_err(nullptr, HF_ERR_HIP,
+                           std::string("hf_create: hipGetSymbolAddress(scratch): ") +
+                               hipGetErrorString(e));
+        }
+    }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
         cus < 1) {

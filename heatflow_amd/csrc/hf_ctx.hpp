@@ -115,6 +115,9 @@ int ctx_alloc(hf_ctx *c, double **ptr, size_t n);
 // reverse-time gradient (hf_adjoint.hip)
 int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every);
 int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *dl_dpower);
+// takes the device address of the no-allocation export's static scratch array on the current
+// device, once per device (hf_noalloc.hip)
+hipError_t noalloc_setup(int dev);
 int adjoint_grad(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every, int64_t nprobe,
                  const int64_t *probe_idx, const double *target, const double *weight,
                  double *loss, double *dl_dkappa, double *dl_dpower);

@@ -293,4 +293,32 @@ __device__ __forceinline__ void block_flags(int32_t f, int32_t *dst) {
     if (threadIdx.x == 0 && sf) atomicOr(dst, sf);
 }
 
+// ---- COO emission (sparse_Hmatrix, src/heatflow/mod_setup.f90:89-152) ------------------------
+
+// a coupling sparse_Hmatrix emits: |H0| >= TINY (mod_setup.f90:130)
+__device__ __forceinline__ bool emitted(double c) { return !(fabs(c) < kTiny); }
+
+// emitted couplings of column j in addit order: x+1, x wrap, y+nx, y wrap, z+nxny, z wrap
+__device__ __forceinline__ int coo_slots(const Dims &g, const Nbr &n, uint32_t j,
+                                         const double *cx, const double *cy, const double *cz,
+                                         uint32_t *rows, double *vals) {
+    int k = 0;
+    if (n.x + 1 < g.nx && emitted(cx[j])) { rows[k] = j + 1; vals[k] = cx[j]; ++k; }
+    if (g.per[0] && n.x == 0) {
+        uint32_t i = j + g.nx - 1;
+        if (emitted(cx[i])) { rows[k] = i; vals[k] = cx[i]; ++k; }
+    }
+    if (g.ny > 1 && n.y + 1 < g.ny && emitted(cy[j])) { rows[k] = j + g.nx; vals[k] = cy[j]; ++k; }
+    if (g.per[1] && n.y == 0) {
+        uint32_t i = j + (g.ny - 1) * g.nx;
+        if (emitted(cy[i])) { rows[k] = i; vals[k] = cy[i]; ++k; }
+    }
+    if (g.nz > 1 && n.z + 1 < g.nz && emitted(cz[j])) { rows[k] = j + g.nxny; vals[k] = cz[j]; ++k; }
+    if (g.per[2] && n.z == 0) {
+        uint32_t i = j + (g.nz - 1) * g.nxny;
+        if (emitted(cz[i])) { rows[k] = i; vals[k] = cz[i]; ++k; }
+    }
+    return k;
+}
+
 }  // namespace hf

@@ -120,6 +120,16 @@ int hf_assemble(hf_ctx *ctx);
 int hf_export_coo(hf_ctx *ctx, int64_t *n, int64_t *len, int64_t *count, double *val,
                   int64_t *irow, int64_t *jcol);
 
+/* hf_export_coo for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); no counterpart in the reference.
+ * Same outputs as hf_export_coo, but val, irow and jcol must be device memory (`len` entries
+ * each; all three, or all NULL for n/len/count only) and the call allocates nothing: its scan
+ * scratch is a fixed 2 MiB device array that each GPU holds for the life of the process,
+ * which covers grids of up to 2^30 cells (HF_ERR_ARG beyond).  Calls that overlap in time on
+ * one GPU share that array and race. */
+int hf_export_coo_noalloc(hf_ctx *ctx, int64_t *n, int64_t *len, int64_t *count, double *val,
+                          int64_t *irow, int64_t *jcol);
+
 /* Sets / reads globe_data::Temp_p and Temp_pp (src/heatflow/mod_global.f90:10), i.e. T^n and
  * T^{n-1}.  hf_set_state replaces initial_evolve's assignment
  * (src/heatflow/mod_init_evolve.f90:34-51). */

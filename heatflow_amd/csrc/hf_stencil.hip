@@ -286,14 +286,18 @@ __device__ __forceinline__ void zmarch_pipe_own(const Dims &g, const ZCol &c, Pl
         if (rg.col != kNoRing) (&sp.v[0][0][0])[rg.slot] = form(load(rg.col + g.nxny * c.z0));
     }
     int buf = 0;
-    auto trip = [&](uint32_t z, uint32_t I, const R &r1, const C &cur, R &r2, C &cn) {
+    auto trip = [&](uint32_t z, uint32_t I, const R &ri, const R &r1, const C &cur, R &r2,
+                    C &cn) {
         const bool more = z + 1 < c.z1;
         const bool has_p = c.active && (z + 1 < g.nz || per_z);
         const bool has_p2 = c.active && more && (z + 2 < g.nz || per_z);
         const bool has_r = more && rg.col != kNoRing;
         const uint32_t own = c.active ? I : 0u;
         r2 = load(has_p2 ? zp_index(g, c.col, z + 1) : own);
-        const R rr = load(rg.col != kNoRing ? rg.col + g.nxny * (more ? z + 1 : z) : 0u);
+        // (opaque to the compiler, which would otherwise load under a branch on the ring test)
+        uint32_t jr = rg.col != kNoRing ? rg.col + g.nxny * (more ? z + 1 : z) : 0u;
+        asm volatile("" : "+v"(jr));
+        const R rr = load(jr);
         cn = cload(c.active && more ? I + g.nxny : own);
         __syncthreads();   // plane z is staged in sp.v[buf]
         V xz_p = (V)0;
@@ -311,14 +315,26 @@ __device__ __forceinline__ void zmarch_pipe_own(const Dims &g, const ZCol &c, Pl
         buf ^= 1;
         xz_m = xi;
         xi = xz_p;
-        ri = r1;
         ei = en;
     };
-    uint32_t I = c.col + g.nxny * c.z0;
-    for (uint32_t z = c.z0; z < c.z1; z += 2, I += 2 * g.nxny) {
-        trip(z, I, rA, cA, rB, cB);
+    // Six trips per iteration: the own operand records of planes z, z+1, z+2 rotate through
+    // three register sets and the cell records of planes z, z+1 through two, so every loaded
+    // record keeps its registers from its load to its last use (a record that changed
+    // registers would be copied at the loop's back edge, waiting for the loads just issued).
+    const uint32_t P = g.nxny;
+    uint32_t I = c.col + P * c.z0;
+    for (uint32_t z = c.z0; z < c.z1; z += 6, I += 6 * P) {
+        trip(z, I, ri, rA, cA, rB, cB);
         if (z + 1 >= c.z1) break;
-        trip(z + 1, I + g.nxny, rB, cB, rA, cA);
+        trip(z + 1, I + P, rA, rB, cB, ri, cA);
+        if (z + 2 >= c.z1) break;
+        trip(z + 2, I + 2 * P, rB, ri, cA, rA, cB);
+        if (z + 3 >= c.z1) break;
+        trip(z + 3, I + 3 * P, ri, rA, cB, rB, cA);
+        if (z + 4 >= c.z1) break;
+        trip(z + 4, I + 4 * P, rA, rB, cA, ri, cB);
+        if (z + 5 >= c.z1) break;
+        trip(z + 5, I + 5 * P, rB, ri, cB, rA, cA);
     }
 }
 

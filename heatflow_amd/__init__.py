@@ -13,4 +13,5 @@ the C-ABI in include/heatflow_hip.h.  Python entry points:
 """
 from ._lib import HeatFlowError, load as load_library  # noqa: F401
 from .evolution import Simulation  # noqa: F401
-from .system import Cells, Params, System, read_inputs  # noqa: F401
+from .system import (Cells, Geometry, Params, System, read_inputs, read_mat,  # noqa: F401
+                     read_params, read_system)

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libheatflow_hip.so")
 HF_OK = 0
 HF_ERR_NAN_B, HF_ERR_NAN_Q, HF_ERR_NAN_QDENS, HF_ERR_NAN_SCAT, HF_ERR_NAN_S, HF_ERR_NAN_X = range(1, 7)
 HF_ERR_ARG, HF_ERR_ALLOC, HF_ERR_HIP, HF_ERR_STATE, HF_ERR_NO_DEVICE = 10, 11, 12, 13, 14
+HF_ERR_UNSTABLE = 15
 HF_PREC_FP64, HF_PREC_FP32_IR = 0, 1
 # kernel ids of hf_kernel_times
 KERNELS = ("k_qdens", "k_rhs_res<fp64>", "k_rhs_res<fp32>", "k_cg_a<fp64>", "k_cg_b<fp64>",
@@ -26,6 +27,7 @@ EXPORTS = (
     "hf_set_debug", "hf_get_stats", "hf_reset_stats", "hf_set_solver", "hf_grad",
     "hf_set_timing", "hf_kernel_times", "hf_forward_checkpointed", "hf_adjoint",
     "hf_set_fields", "hf_set_adjoint_tol", "hf_export_coo_noalloc",
+    "hf_builtin_material", "hf_upload_materials", "hf_check_stability",
     "hf_last_error", "hf_destroy", "hf_version",
 )
 
@@ -62,6 +64,11 @@ class HfStats(C.Structure):
     ]
 
 
+class HfMaterial(C.Structure):
+    _fields_ = [("index", C.c_int64), ("kappa", C.c_double), ("rho", C.c_double),
+                ("heat_capacity", C.c_double), ("tau", C.c_double), ("em", C.c_double)]
+
+
 class HeatFlowError(RuntimeError):
     """Raised for a non-zero HF_* return code; ``code`` holds it."""
 
@@ -86,6 +93,13 @@ def load() -> C.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -m heatflow_amd.build` "
             "(there is no CPU fallback)")
+    # torch ships its own HIP and HSA runtimes.  Loaded first, they are the ones this library
+    # binds to (same sonames), so the process has one runtime and torch tensors are device
+    # memory of it; loaded after this library, a second HSA instance finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     sig = {
         "hf_params_default": (None, [C.POINTER(HfParams)]),
@@ -94,6 +108,11 @@ def load() -> C.CDLL:
         "hf_assemble": (C.c_int, [P]),
         "hf_export_coo": (C.c_int, [P, I64, I64, I64, P, P, P]),
         "hf_export_coo_noalloc": (C.c_int, [P, I64, I64, I64, P, P, P]),
+        "hf_builtin_material": (C.c_int, [C.c_int64, C.POINTER(HfMaterial), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32)]),
+        "hf_upload_materials": (C.c_int, [P, P, P, C.c_int32, P, C.c_double, C.c_double,
+                                          C.c_double]),
+        "hf_check_stability": (C.c_int, [P, I64, D]),
         "hf_set_state": (C.c_int, [P, P, P]),
         "hf_get_state": (C.c_int, [P, P, P]),
         "hf_step": (C.c_int, [P, C.c_int64, I64, D]),

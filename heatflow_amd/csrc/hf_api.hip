@@ -1247,10 +1247,16 @@ extern "C" int hf_get_vector(hf_ctx *c, int32_t which, double *out) {
         case 4: src = c->cx; break;
         case 5: src = c->cy; break;
         case 6: src = c->cz; break;
+        case 7: src = c->kappa; break;
+        case 8: src = c->rhoc; break;
+        case 9: src = c->tau_s; break;
+        case 10: src = c->ells; break;
+        case 11: src = c->vol; break;
         default: return set_err(c, HF_ERR_ARG, "hf_get_vector: unknown vector id");
     }
     if (!src) return set_err(c, HF_ERR_STATE, "hf_get_vector: vector not available");
-    if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_get_vector: not assembled");
+    if (which >= 7 ? !c->uploaded : !c->assembled)
+        return set_err(c, HF_ERR_STATE, "hf_get_vector: not uploaded / assembled");
     HIPCHK(c, hipMemcpyAsync(out, src, (size_t)c->g.N * 8, hipMemcpyDefault, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return HF_OK;

@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import HF_PREC_FP32_IR, HF_PREC_FP64, HeatFlowError, HfParams, HfStats, check
-from .system import System
+from .system import Geometry, Params, System
 
 PRECISIONS = {"fp64": HF_PREC_FP64, "fp32_ir": HF_PREC_FP32_IR}
 
@@ -95,9 +95,33 @@ class Simulation:
         ctx = C.c_void_p()
         check(self.lib.hf_create(C.byref(ctx), C.byref(p)), None)
         self.ctx = ctx
-        self.upload(system)
-        if assemble:
-            self.sparse_Hmatrix()
+        if system.cells is not None:
+            self.upload(system)
+            if assemble:
+                self.sparse_Hmatrix()
+
+    @classmethod
+    def from_geometry(cls, params: Params, geometry: Geometry, file_materials: dict = None,
+                      check_stability: bool = False, assemble: bool = True, **solver):
+        """set_global_variables on the device (src/heatflow/mod_setup.f90:34-68): the per-cell
+        kappa, rho, heat_capacity, tau/dt^2 and em come from material() over the geometry's
+        material ids (built-in materials, index > 99, and file materials {index: {key: value}}
+        as read_mat returns them), with stability() over every cell when check_stability
+        (the reference's _Check_Stability), then sparse_Hmatrix unless assemble is False.
+        `system.cells` stays None: the fields live on the device (Simulation.vector)."""
+        if (geometry.nx, geometry.ny, geometry.nz) != (params.nx, params.ny, params.nz):
+            raise ValueError("geometry and params disagree on the grid")
+        sim = cls(System(params, None), assemble=False, **solver)
+        try:
+            sim.upload_materials(geometry, file_materials or {})
+            if check_stability:
+                sim.check_stability()
+            if assemble:
+                sim.sparse_Hmatrix()
+        except BaseException:
+            sim.close()
+            raise
+        return sim
 
     # -- setup --------------------------------------------------------------------------------
     def upload(self, system: System, qdens_extra: Optional[np.ndarray] = None) -> None:
@@ -112,6 +136,36 @@ class Simulation:
         qe = None if qe is None else _f64(qe)
         check(self.lib.hf_upload_cells(self.ctx, *[_ptr(a) for a in self._keep], _ptr(qe)),
               self.ctx)
+
+    def upload_materials(self, geometry: Geometry, file_materials: dict) -> None:
+        """Per-cell fields from material ids on the device (hf_upload_materials)."""
+        keys = ("kappa", "rho", "heat_capacity", "tau", "em")
+        fm = (_lib.HfMaterial * max(len(file_materials), 1))()
+        for k, (idx, m) in enumerate(sorted(file_materials.items())):
+            fm[k].index = int(idx)
+            for key in keys:
+                setattr(fm[k], key, float(m[key]))
+        ids = np.ascontiguousarray(geometry.imaterial_type, np.int64)
+        heat = np.ascontiguousarray(geometry.iheater, np.int64)
+        if ids.size != self.n or heat.size != self.n:
+            raise ValueError("geometry arrays must hold nx*ny*nz cells")
+        lx, ly, lz = geometry.cell_lengths
+        check(self.lib.hf_upload_materials(self.ctx, _ptr(ids), _ptr(heat), len(file_materials),
+                                           fm, lx, ly, lz), self.ctx)
+
+    def stability(self):
+        """setup::stability over every cell (mod_setup.f90:158-199): (first failing cell in
+        loop order or -1, its failing var_stability or 0)."""
+        cell, value = C.c_int64(-1), C.c_double(0.0)
+        rc = self.lib.hf_check_stability(self.ctx, C.byref(cell), C.byref(value))
+        if rc not in (_lib.HF_OK, _lib.HF_ERR_UNSTABLE):
+            check(rc, self.ctx)
+        return cell.value, value.value
+
+    def check_stability(self) -> None:
+        """As the reference's _Check_Stability does: raises HeatFlowError (HF_ERR_UNSTABLE)
+        naming the first failing cell, where the reference prints and stops."""
+        check(self.lib.hf_check_stability(self.ctx, None, None), self.ctx)
 
     def sparse_Hmatrix(self) -> None:
         check(self.lib.hf_assemble(self.ctx), self.ctx)
@@ -166,7 +220,8 @@ class Simulation:
         return xx, it.value, err.value
 
     def vector(self, which: str) -> np.ndarray:
-        ids = {"B": 0, "Qdens": 1, "S": 2, "diag": 3, "cx": 4, "cy": 5, "cz": 6}
+        ids = {"B": 0, "Qdens": 1, "S": 2, "diag": 3, "cx": 4, "cy": 5, "cz": 6, "kappa": 7,
+               "rhoc": 8, "tau_s": 9, "ells": 10, "volume": 11}
         out = np.empty(self.n)
         check(self.lib.hf_get_vector(self.ctx, ids[which], _ptr(out)), self.ctx)
         return out

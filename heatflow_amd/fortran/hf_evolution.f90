@@ -9,12 +9,17 @@
 !!
 !! On the first call the module gathers the per-cell fields of grid(:,:,:) into
 !! structure-of-arrays, creates a device context, uploads and assembles H on the device.  Each
-!! call runs one step on the device and leaves the same Temp_p/Temp_pp on the host as the CPU
-!! solver, so data_write and final_print see the same arrays: the old Temp_p becomes Temp_pp by
-!! moving the allocation, and only the new Temp_p is copied back.  The host history goes to the
-!! device on the first call and at step 1 (after initial_evolve, which may read a restart);
-!! the driver does not change it between steps otherwise.  The reference's fatal NaN guards are
-!! reproduced: the message of mod_evolve.f90:70-158 and `stop 1`.
+!! call runs one step on the device.  The state stays there: it comes back to Temp_p/Temp_pp
+!! only on the steps whose output reads it -- data_write's log records and steady-state check
+!! (src/heatflow/mod_output.f90:69-111, 117-131) and final_print at the last step or, with
+!! IVERB >= 3, at every step (heatflow.f90:86-87, mod_output.f90:139-142, 207-238) -- so those
+!! see the same arrays as with the CPU solver, and the other steps copy nothing.  The host
+!! history goes to the device on the first call and at step 1 (after initial_evolve, which may
+!! read a restart); the driver does not change it between steps otherwise.  With TempDepProp = 1
+!! the reference rebuilds H after every step (ChangeProp, mod_tempdep.f90:38-65, 64); here H is
+!! re-assembled on the device from the fields already there.  heated_volume is set as heater
+!! sets it (mod_heating.f90:137-141).  The reference's fatal NaN guards are reproduced: the
+!! message of mod_evolve.f90:70-158 and `stop 1`.
 module evolution
   use iso_c_binding
   use constants, only: real12, int12
@@ -22,8 +27,9 @@ module evolution
        heated_steps, kappaBoundx1, kappaBoundy1, kappaBoundz1, kappaBoundNx, kappaBoundNy, &
        kappaBoundNz, T_Bathx1, T_Bathx2, T_Bathy1, T_Bathy2, T_Bathz1, T_Bathz2, T_Bath, &
        T_BathCG, BR, Periodicx, Periodicy, Periodicz, CG_x_m, CG_x_p, CG_y_m, CG_y_p, &
-       CG_z_m, CG_z_p, TempDepProp
-  use globe_data, only: Temp_p, Temp_pp
+       CG_z_m, CG_z_p, TempDepProp, ntime, write_every, WriteToTxt, Test_Run, &
+       Check_Steady_State, IVERB
+  use globe_data, only: Temp_p, Temp_pp, heated_volume
   implicit none
   private
   public :: simulate
@@ -94,6 +100,8 @@ module evolution
   type(c_ptr), save :: ctx = c_null_ptr
   !! the device holds Temp_p / Temp_pp as the host last received them
   logical, save :: device_current = .false.
+  !! heater's heated_volume: the volume of the cells with iheater > 0, in loop order
+  real(real12), save :: volume_heated = 0.0_real12
 
 contains
 
@@ -104,14 +112,9 @@ contains
     integer(c_int64_t) :: iter
     real(c_double) :: err
 
-    real(c_double), allocatable :: moved(:)
-
     if (.not. c_associated(ctx)) then
        call device_setup()
        device_current = .false.
-    else if (TempDepProp .eq. 1) then
-       ! ChangeProp (mod_tempdep.f90:38-65) re-runs sparse_Hmatrix after every step
-       call upload_cells()
     end if
     ! The driver changes the history only through initial_evolve, before step 1
     ! (heatflow.f90:79-80, mod_init_evolve.f90:34-51); otherwise the device copy is current.
@@ -122,15 +125,36 @@ contains
     end if
     rc = hf_step(ctx, int(itime, c_int64_t), iter, err)
     if (rc /= 0) call fail(rc)
-    ! Temp_pp = Temp_p; Temp_p = x (mod_evolve.f90:169-170): the old Temp_p moves to Temp_pp
-    ! without a copy and only the new Temp_p comes back from the device
-    call move_alloc(Temp_pp, moved)
-    call move_alloc(Temp_p, Temp_pp)
-    call move_alloc(moved, Temp_p)
-    rc = hf_get_state(ctx, Temp_p)
-    if (rc /= 0) call fail(rc)
     device_current = .true.
+    heated_volume = volume_heated
+    ! Temp_pp = Temp_p; Temp_p = x (mod_evolve.f90:169-170), on the host when output reads them
+    if (host_reads_state(itime)) then
+       rc = hf_get_state(ctx, Temp_p, Temp_pp)
+       if (rc /= 0) call fail(rc)
+    end if
+    ! ChangeProp (mod_tempdep.f90:38-65) rebuilds H from the grid after every step
+    ! (mod_evolve.f90:172-174)
+    if (TempDepProp .eq. 1) then
+       rc = hf_assemble(ctx)
+       if (rc /= 0) call fail(rc)
+    end if
   end subroutine simulate
+
+  !! whether the driver reads Temp_p / Temp_pp after step itime: data_write builds its log
+  !! record from Temp_pp at step 1 and every write_every steps when writing to text, and
+  !! compares it in the steady-state check (mod_output.f90:69-131); final_print writes both at
+  !! the last step, and at every step with IVERB >= 3 (heatflow.f90:86-87)
+  logical function host_reads_state(itime)
+    integer(int12), intent(in) :: itime
+    host_reads_state = itime == ntime .or. IVERB >= 3 .or. &
+         (Check_Steady_State .and. itime > 1)
+    if (.not. Test_Run) then
+       if (itime == 1) host_reads_state = .true.
+       if (WriteToTxt .and. write_every > 0) then
+          if (mod(itime, write_every) == 0) host_reads_state = .true.
+       end if
+    end if
+  end function host_reads_state
 
   subroutine device_setup()
     type(hf_params_t) :: p
@@ -163,6 +187,7 @@ contains
          vol(:)
     integer(c_int64_t), allocatable :: ih(:)
     integer(c_int) :: rc
+    integer(int12) :: i
     kappa = reshape(grid%kappa, [NA])
     rho = reshape(grid%rho, [NA])
     hc = reshape(grid%heat_capacity, [NA])
@@ -177,6 +202,10 @@ contains
     if (rc /= 0) call fail(rc)
     rc = hf_assemble(ctx)
     if (rc /= 0) call fail(rc)
+    volume_heated = 0.0_real12
+    do i = 1, NA
+       if (ih(i) > 0) volume_heated = volume_heated + vol(i)
+    end do
   end subroutine upload_cells
 
   !! the reference's fatal guards (mod_evolve.f90:70-158): message, then stop 1

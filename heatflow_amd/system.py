@@ -125,12 +125,41 @@ def _logical(v: str) -> bool:
     return out
 
 
-def read_inputs(dirname: str) -> System:
-    """Reads <dirname>/param.in, mat.in and system.in like read_all_files
-    (src/heatflow/mod_inputs.f90:116-193) for file materials, then applies
-    set_global_variables' per-cell material assignment (mod_setup.f90:53-68)."""
-    # system.in (read_system, :602-649)
-    with open(os.path.join(dirname, "system.in")) as f:
+@dataclasses.dataclass
+class Geometry:
+    """What system.in holds (read_system, src/heatflow/mod_inputs.f90:602-649): the grid,
+    the box lengths and, per cell (x fastest), grid%imaterial_type and grid%iheater.  `save` and
+    `load` keep it as a binary .npz file: a 512^3 grid is 134M "id:heater" tokens as text."""
+    nx: int
+    ny: int
+    nz: int
+    lengths: Sequence[float]          # Lx, Ly, Lz of the box
+    imaterial_type: np.ndarray        # int64, N
+    iheater: np.ndarray               # int64, N
+
+    @property
+    def cell_lengths(self):
+        """grid%Length(1:3) = Lx/nx, Ly/ny, Lz/nz (mod_inputs.f90:619-621)."""
+        Lx, Ly, Lz = self.lengths
+        return Lx / float(self.nx), Ly / float(self.ny), Lz / float(self.nz)
+
+    def save(self, path: str) -> None:
+        np.savez(path, dims=np.array([self.nx, self.ny, self.nz], np.int64),
+                 lengths=np.asarray(self.lengths, np.float64),
+                 imaterial_type=np.ascontiguousarray(self.imaterial_type, np.int64),
+                 iheater=np.ascontiguousarray(self.iheater, np.int64))
+
+    @staticmethod
+    def load(path: str) -> "Geometry":
+        with np.load(path, allow_pickle=False) as d:
+            nx, ny, nz = (int(v) for v in d["dims"])
+            return Geometry(nx, ny, nz, tuple(float(v) for v in d["lengths"]),
+                            d["imaterial_type"].astype(np.int64), d["iheater"].astype(np.int64))
+
+
+def read_system(path: str) -> Geometry:
+    """system.in as read_system reads it (mod_inputs.f90:602-649)."""
+    with open(path) as f:
         lines = [ln for ln in f.read().splitlines()]
     pos = 0
 
@@ -152,10 +181,14 @@ def read_inputs(dirname: str) -> System:
                 m, h = t.split(":")
                 mat[iz, iy, ix] = int(m)
                 heat[iz, iy, ix] = int(h)
+    return Geometry(nx, ny, nz, (Lx, Ly, Lz), mat.reshape(-1), heat.reshape(-1))
 
-    # param.in (read_param, :204-360)
+
+def read_params(path: str, nx: int, ny: int, nz: int) -> Params:
+    """param.in as read_param and check_param read it (mod_inputs.f90:204-470), for the keys
+    the time step uses."""
     kv = {}
-    for k, v in _kv_lines(os.path.join(dirname, "param.in")):
+    for k, v in _kv_lines(path):
         kv[k] = v
     D = lambda k, d: float(kv[k].replace("d", "e").replace("D", "e")) if k in kv else d
     I = lambda k, d: int(kv[k]) if k in kv else d
@@ -181,11 +214,14 @@ def read_inputs(dirname: str) -> System:
         br=D("BR", 1.0), cg=cg, t_bath_cg=tcg, power_in=D("power_in", 0.0),
         freq=D("freq", 1.0), heated_steps=I("heattime", 0), t_bath_rad=t_bath,
         t_system=D("T_System", 300.0), ntime=I("ntime", 10))
+    return params
 
-    # mat.in (read_mat, :654-721): blocks "index / key = value ... / 0"
+
+def read_mat(path: str) -> dict:
+    """mat.in as read_mat reads it (mod_inputs.f90:654-721): {index: {key: value}}."""
     mats = {}
     cur = None
-    with open(os.path.join(dirname, "mat.in")) as f:
+    with open(path) as f:
         for raw in f:
             line = raw.split("!", 1)[0].strip()
             if not line:
@@ -198,12 +234,23 @@ def read_inputs(dirname: str) -> System:
             if "=" in line and cur is not None:
                 k, v = line.split("=", 1)
                 cur[k.strip()] = float(v.strip().replace("d", "e").replace("D", "e"))
-    n = nx * ny * nz
-    flat_mat = mat.reshape(n)
-    get = lambda key: np.array([mats[int(m)][key] for m in flat_mat], np.float64)
+    return mats
+
+
+def read_inputs(dirname: str) -> System:
+    """Reads <dirname>/param.in, mat.in and system.in like read_all_files
+    (src/heatflow/mod_inputs.f90:116-193) for file materials, then applies
+    set_global_variables' per-cell material assignment (mod_setup.f90:53-68) on the host.
+    (Simulation.from_geometry does that assignment on the device, built-in materials
+    included.)"""
+    geo = read_system(os.path.join(dirname, "system.in"))
+    params = read_params(os.path.join(dirname, "param.in"), geo.nx, geo.ny, geo.nz)
+    mats = read_mat(os.path.join(dirname, "mat.in"))
+    n = geo.nx * geo.ny * geo.nz
+    get = lambda key: np.array([mats[int(m)][key] for m in geo.imaterial_type], np.float64)
+    lx, ly, lz = geo.cell_lengths
     cells = Cells(
         kappa=get("kappa"), rho=get("rho"), heat_capacity=get("heat_capacity"), tau=get("tau"),
-        em=get("em"),
-        len_x=np.full(n, Lx / float(nx)), len_y=np.full(n, Ly / float(ny)),
-        len_z=np.full(n, Lz / float(nz)), iheater=heat.reshape(n).copy())
+        em=get("em"), len_x=np.full(n, lx), len_y=np.full(n, ly), len_z=np.full(n, lz),
+        iheater=geo.iheater.copy())
     return System(params, cells)

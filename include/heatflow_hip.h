@@ -42,7 +42,9 @@ enum {
     HF_ERR_ALLOC = 11,      /* device allocation failed                               */
     HF_ERR_HIP = 12,        /* HIP runtime error                                      */
     HF_ERR_STATE = 13,      /* call order: e.g. hf_step before hf_assemble            */
-    HF_ERR_NO_DEVICE = 14   /* no HIP device                                          */
+    HF_ERR_NO_DEVICE = 14,  /* no HIP device                                          */
+    HF_ERR_UNSTABLE = 15    /* hf_check_stability: the stability condition fails
+                               (setup::stability prints and stops, mod_setup.f90:175-197) */
 };
 
 /* Solver precision modes */
@@ -106,6 +108,42 @@ int hf_upload_cells(hf_ctx *ctx, const double *kappa, const double *rho,
                     const double *len_x, const double *len_y, const double *len_z,
                     const double *volume, const int64_t *iheater, const double *qdens_extra);
 
+/* One material of material()'s table (src/heatflow/mod_material.f90:54-267): the fields the
+ * time step uses.  File materials (index 1..99) are what the caller read from mat.in
+ * (mod_inputs.f90:655-720); the built-in ones (index > 99) are the library's. */
+typedef struct hf_material {
+    int64_t index;
+    double kappa, rho, heat_capacity;
+    double tau;                /* physical relaxation time (s)                          */
+    double em;                 /* emissivity                                            */
+} hf_material;
+
+/* The library's copy of built-in material `index` (> 99; mod_material.f90:116-265): *m as
+ * material() sets it (its single-precision literals, widened), and whether material() assigns
+ * tau and em for it (*sets_tau, *sets_em; otherwise the value of the previous cell in the
+ * setup loop stays).  HF_ERR_ARG for an unknown index.  Needs no device. */
+int hf_builtin_material(int64_t index, hf_material *m, int32_t *sets_tau, int32_t *sets_em);
+
+/* Setup on the device from material ids: the per-cell fields that set_global_variables
+ * derives with material() (src/heatflow/mod_setup.f90:34-68) -- kappa, rho, heat_capacity,
+ * grid%tau = tau / dt^2, em -- then as hf_upload_cells.  imaterial_type (N ids) and iheater
+ * (N flags, may be NULL) are grid%imaterial_type and grid%iheater of read_system, host or
+ * device memory; len_x/y/z are its uniform cell lengths Lx/nx, Ly/ny, Lz/nz and the volume
+ * their product (mod_inputs.f90:619-622).  Where a built-in material does not assign tau or em,
+ * a cell takes the value of the last earlier cell (loop order) whose material did, 0 before
+ * the first (the reference leaves that local uninitialised).  An unknown id is HF_ERR_ARG
+ * (material() prints and exits, mod_material.f90:105-107, 262-264). */
+int hf_upload_materials(hf_ctx *ctx, const int64_t *imaterial_type, const int64_t *iheater,
+                        int32_t nfile, const hf_material *file_materials, double len_x,
+                        double len_y, double len_z);
+
+/* setup::stability over the uploaded cells (src/heatflow/mod_setup.f90:158-199, run when
+ * _Check_Stability): dt kappa/(rho c_p) (1/L1^2 + 1/L2^2 + 1/L3^2) must not exceed 1/12 (the
+ * reference's single-precision constant), nor the same with kappaBoundx1 on cells of the x1,
+ * y1 or z1 faces.  HF_OK, or HF_ERR_UNSTABLE with *cell the first failing cell in loop order
+ * (-1 if none) and *value its failing value (either may be NULL). */
+int hf_check_stability(hf_ctx *ctx, int64_t *cell, double *value);
+
 /* Assembles H on the device as a 7-point stencil (diagonal + 3 forward couplings) and the
  * boundary vector B.  Replaces setup::sparse_Hmatrix (src/heatflow/mod_setup.f90:89-152).
  * Values follow hmatrixfunc's operand order (src/heatflow/mod_hmatrix.f90:62-391). */
@@ -159,7 +197,9 @@ int hf_solve(hf_ctx *ctx, const double *b, double *x, int32_t itol, double tol, 
  * which = 0: B (boundary vector, mod_boundary.f90:24-152)
  *         1: Qdens of the last step (mod_heating.f90:164-167)
  *         2: S of the last step (mod_evolve.f90:112-119)
- *         3: diagonal of H, 4/5/6: forward couplings in x/y/z (0 where absent) */
+ *         3: diagonal of H, 4/5/6: forward couplings in x/y/z (0 where absent)
+ * and the uploaded per-cell fields: 7 kappa, 8 lin_rhoc = rho c_p, 9 grid%tau = tau/dt^2,
+ * 10 em L1 L2 sigma (the radiative coefficient), 11 volume. */
 int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
 
 /* Debug switches (no reference counterpart): bit 0 keeps S of every step for hf_get_vector(2)

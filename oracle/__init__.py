@@ -80,6 +80,7 @@ def olib() -> C.CDLL:
         _olib.hfo_sparse_hmatrix.restype = C.c_int64
         _olib.hfo_heater.restype = C.c_double
         _olib.hfo_coo_hash.restype = C.c_int64
+        _olib.hfo_stability.restype = C.c_int64
         _olib.hfo_coo_entry_hash.restype = C.c_uint64
         _olib.hfo_coo_entry_hash.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double]
     return _olib
@@ -135,6 +136,14 @@ class OracleSystem:
         cnt = C.c_int64()
         L = olib().hfo_coo_hash(C.byref(self.s), C.byref(h), C.byref(cnt))
         return L, cnt.value, h.value
+
+    def stability(self):
+        """setup::stability over the cells: (first failing cell or -1, its value, whether the
+        boundary test failed)."""
+        v = C.c_double()
+        b = C.c_int32()
+        cell = olib().hfo_stability(C.byref(self.s), C.byref(v), C.byref(b))
+        return cell, v.value, bool(b.value)
 
     def boundary(self) -> np.ndarray:
         B = np.empty(self.n)
@@ -250,6 +259,77 @@ def rlib(omp: bool = False) -> C.CDLL:
 
 def ref_omp_available() -> bool:
     return os.path.exists(REF_OMP_SO)
+
+
+# material()'s built-in materials (src/heatflow/mod_material.f90:116-265): index ->
+# (kappa, rho, heat_capacity, tau or None where the case leaves tau as it was).  The literals
+# there are default REAL, so each value is the single-precision rounding of the decimal.
+_BUILTIN_MATERIALS = {
+    100: (1.0, 1.0, 1.0, 0.0),            # :118-126
+    140: (130.0, 2328.0, 4200.0, None),   # :136-144
+    3001: (0.19, 941.0, 840.0, None),     # :146-153
+    3002: (1.42, 1700.0, 1100.0, None),   # :155-161
+    3003: (1.59, 1800.0, 1400.0, None),   # :163-169
+    3004: (1.25, 2300.0, 2300.0, None),   # :171-177
+    9001: (0.6, 1090.0, 3421.0, None),    # :179-185
+    9002: (0.209, 1109.0, 3391.0, None),  # :187-193
+    9003: (0.317, 911.0, 2348.0, None),   # :195-201
+    310: (130.0, 2329.0, 700.0, None),    # :203-209
+    320: (1.4, 2400.0, 720.0, 1.0),       # :211-218
+    330: (427.0, 10524.0, 236.0, None),   # :220-226
+    340: (0.2, 1153.0, 1500.0, 1.0),      # :228-235
+    350: (10.2, 7140.0, 361.0, 1.0),      # :237-244
+}
+
+
+def materials(ids, file_materials=None):
+    """set_global_variables' per-cell loop over material() (src/heatflow/mod_setup.f90:55-68,
+    src/heatflow/mod_material.f90:54-265), cell by cell in loop order: (kappa, rho,
+    heat_capacity, tau, em) per cell, tau unscaled.  file_materials: {index: {key: value}} as
+    read_mat returns mat.in.  material() assigns tau and em for file materials, tau for the
+    built-ins that set it and em for none of them; a value it leaves alone is the one the
+    previous call left in set_global_variables' locals (0 before any call: the reference
+    leaves them uninitialised).  Unknown or non-positive ids raise ValueError (material()
+    prints and exits, :76-80, :104-107, :261-264)."""
+    f32 = lambda v: float(np.float32(v))
+    fm = file_materials or {}
+    ids = np.asarray(ids, np.int64).reshape(-1)
+    out = np.zeros((5, ids.size))
+    tau = em = 0.0
+    for i, mid in enumerate(ids.tolist()):
+        if mid <= 0:
+            raise ValueError(f"imaterial_type not recognized: {mid}")
+        if mid <= 99:                                          # :86-108
+            if mid not in fm:
+                raise ValueError(f"imaterial_type not recognized: {mid}")
+            m = fm[mid]
+            kappa, rho, hc, tau, em = (float(m[k]) for k in
+                                       ("kappa", "rho", "heat_capacity", "tau", "em"))
+        else:                                                  # :115-265
+            if mid not in _BUILTIN_MATERIALS:
+                raise ValueError(f"imaterial_type not recognized: {mid}")
+            k, r, h, t = _BUILTIN_MATERIALS[mid]
+            kappa, rho, hc = f32(k), f32(r), f32(h)
+            if t is not None:
+                tau = f32(t)
+        out[:, i] = (kappa, rho, hc, tau, em)
+    return tuple(out)
+
+
+def ref_materials(ids, file_materials=()):
+    """The reference's own material() (src/heatflow/mod_material.f90:40-265) over the ids in
+    order, as set_global_variables' loop calls it (mod_setup.f90:55-68): per id (kappa, rho,
+    heat_capacity, tau, em) with tau unscaled.  file_materials: dicts with index, kappa, rho,
+    heat_capacity, tau, em (mat.in's entries)."""
+    ids = np.ascontiguousarray(ids, np.int64)
+    fm = list(file_materials)
+    col = lambda k, dt=np.float64: np.ascontiguousarray([m[k] for m in fm] or [0], dt)
+    fidx, fk, fr, fh, ft, fe = (col("index", np.int64), col("kappa"), col("rho"),
+                                col("heat_capacity"), col("tau"), col("em"))
+    out = [np.empty(len(ids)) for _ in range(5)]
+    _big_stack(rlib().hfr_materials, C.c_int64(len(ids)), _p(ids), C.c_int64(len(fm)),
+               _p(fidx), _p(fk), _p(fr), _p(fh), _p(ft), _p(fe), *[_p(a) for a in out])
+    return tuple(out)
 
 
 class RefSystem:

@@ -497,3 +497,34 @@ int hfo_run(const hfo_system *s, const hfo_solver *sv, int64_t ntime, double t_s
     free(val); free(irow); free(jcol);
     return rc;
 }
+
+/* setup::stability, src/heatflow/mod_setup.f90:158-199.  The threshold 1.0/12.0 is a
+ * default-real (single precision) expression in the reference. */
+int64_t hfo_stability(const hfo_system *s, double *value, int32_t *boundary_test) {
+    const float thresh = 1.0f / 12.0f;
+    int64_t I = 0;
+    for (int64_t iz = 1; iz <= s->nz; ++iz)
+        for (int64_t iy = 1; iy <= s->ny; ++iy)
+            for (int64_t ix = 1; ix <= s->nx; ++ix, ++I) {
+                const double l1 = s->len_x[I], l2 = s->len_y[I], l3 = s->len_z[I];
+                const double rhoc = s->rho[I] * s->heat_capacity[I];
+                double alpha = s->kappa[I] / rhoc;                                  /* :170 */
+                double var = s->time_step * alpha *
+                             (1 / (l1 * l1) + 1 / (l2 * l2) + 1 / (l3 * l3));        /* :171-173 */
+                if (var > (double)thresh) {                                         /* :176 */
+                    if (value) *value = var;
+                    if (boundary_test) *boundary_test = 0;
+                    return I;
+                }
+                if (ix == 1 || iy == 1 || iz == 1) {                                /* :187 */
+                    alpha = s->kappa_bound[0] / rhoc;
+                    var = s->time_step * alpha * (1 / (l1 * l1) + 1 / (l2 * l2) + 1 / (l3 * l3));
+                    if (var > (double)thresh) {
+                        if (value) *value = var;
+                        if (boundary_test) *boundary_test = 1;
+                        return I;
+                    }
+                }
+            }
+    return -1;
+}

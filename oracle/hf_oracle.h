@@ -82,16 +82,23 @@ double hfo_heater(const hfo_system *s, int64_t itime, const double *temp_p, doub
 /* cattaneo::S_catS (src/heatflow/mod_cattaneo.f90:23-42) */
 void hfo_scat(const hfo_system *s, const double *temp_p, const double *temp_pp, double *scat);
 
-/* solver::linbcg (src/heatflow/mod_Sparse.f90:372-434) over a COO matrix in the
- * reference layout (SRSax/SRStx/SRSdiag, :293-366).  x is in/out. */
 /* Order-sensitive hash of the COO hfo_sparse_hmatrix would write (positions 0..len-1, the
  * sentinel tail included), computed without storing it; and the hash of one entry (the sum
  * over the entries, mod 2^64, is the COO's hash). */
 int64_t hfo_coo_hash(const hfo_system *s, uint64_t *hash, int64_t *count_out);
 uint64_t hfo_coo_entry_hash(int64_t k, int64_t i, int64_t j, double v);
+
+/* solver::linbcg (src/heatflow/mod_Sparse.f90:372-434) over a COO matrix in the
+ * reference layout (SRSax/SRStx/SRSdiag, :293-366).  x is in/out. */
 void hfo_linbcg(int64_t n, int64_t len, const double *val, const int64_t *irow,
                 const int64_t *jcol, const double *b, double *x, int64_t itol, double tol,
                 int64_t itmax, int64_t *iter, double *err);
+
+/* setup::stability (src/heatflow/mod_setup.f90:158-199) over the cells in loop order, as
+ * set_global_variables calls it per cell (:65): returns the first cell index (0-based) whose
+ * test fails, or -1; *value gets that cell's failing var_stability.  *boundary_test is 1 when
+ * the boundary test (kappaBoundx1 on the x1, y1, z1 faces, :182-194) failed, else 0. */
+int64_t hfo_stability(const hfo_system *s, double *value, int32_t *boundary_test);
 
 /* evolution::simulate(itime) (src/heatflow/mod_evolve.f90:45-177) on an assembled COO
  * H; updates temp_p / temp_pp in place (history rotation :169-170).  Returns an HFO_*

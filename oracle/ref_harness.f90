@@ -19,6 +19,7 @@ module hf_ref_harness
   use heating, only: heater
   use cattaneo, only: S_catS
   use solver, only: linbcg
+  use materials, only: ref_material => material
   implicit none
 
 contains
@@ -227,5 +228,45 @@ contains
     Temp_pp = Temp_p
     Temp_p = x
   end subroutine hfr_simulate_qextra
+
+  !! The reference's material() (src/heatflow/mod_material.f90:54-267) over n cells in order,
+  !! as set_global_variables calls it (src/heatflow/mod_setup.f90:53-68): the output locals
+  !! persist from cell to cell, so a built-in material that does not assign tau or em keeps the
+  !! previous cell's value.  The locals start at 0 here (uninitialised in the reference).
+  !! File materials (index <= 99) come from the nfile entries given (input_materials, read
+  !! from mat.in in the reference).  Every id must be known (material() calls exit otherwise).
+  subroutine hfr_materials(n, ids, nfile, findex, fkappa, frho, fhc, ftau, fem, &
+       kappa_o, rho_o, hc_o, tau_o, em_o) bind(C, name="hfr_materials")
+    integer(c_int64_t), value :: n, nfile
+    integer(c_int64_t), intent(in) :: ids(*), findex(*)
+    real(c_double), intent(in) :: fkappa(*), frho(*), fhc(*), ftau(*), fem(*)
+    real(c_double), intent(out) :: kappa_o(*), rho_o(*), hc_o(*), tau_o(*), em_o(*)
+    real(real12) :: kappa, kappa3D, h_conv, heat_capacity, rho, sound_speed, tau, em
+    integer(int12) :: i
+    if (allocated(input_materials)) deallocate(input_materials)
+    allocate(input_materials(max(nfile, 1_c_int64_t)))
+    do i = 1, nfile
+       input_materials(i)%index = findex(i)
+       input_materials(i)%heat_capacity = fhc(i)
+       input_materials(i)%h_conv = 0.0_real12
+       input_materials(i)%kappa = fkappa(i)
+       input_materials(i)%kappa3D = fkappa(i)
+       input_materials(i)%rho = frho(i)
+       input_materials(i)%sound_speed = 0.0_real12
+       input_materials(i)%tau = ftau(i)
+       input_materials(i)%em = fem(i)
+    end do
+    if (nfile == 0) input_materials(1)%index = -1
+    kappa = 0; kappa3D = 0; h_conv = 0; heat_capacity = 0; rho = 0; sound_speed = 0
+    tau = 0; em = 0
+    do i = 1, n
+       call ref_material(ids(i), kappa, kappa3D, h_conv, heat_capacity, rho, sound_speed, tau, em)
+       kappa_o(i) = kappa
+       rho_o(i) = rho
+       hc_o(i) = heat_capacity
+       tau_o(i) = tau
+       em_o(i) = em
+    end do
+  end subroutine hfr_materials
 
 end module hf_ref_harness

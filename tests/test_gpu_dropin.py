@@ -108,3 +108,53 @@ def test_dropin_matches_reference_program_c1(tmp_path):
     assert Tref.max() > 300.001          # the heater heated
     # both print ES16.8 (9 significant digits): agreement to the print resolution
     assert np.abs(Thip - Tref).max() <= 1e-6
+
+
+def _outputs(workdir):
+    out = os.path.join(workdir, "outputs")
+    return (read_tempdis(os.path.join(out, "TempDis.dat")),
+            read_tempdis(os.path.join(out, "TempDisTPD.dat")))
+
+
+def _set_param(path, key, value):
+    with open(path) as f:
+        lines = f.readlines()
+    write_param(lines, path, {key: value})
+
+
+@pytest.mark.skipif(not os.path.exists(REFEXE), reason="reference program not built")
+def test_dropin_copies_state_only_for_output(tmp_path):
+    """The drop-in keeps the state on the device and copies Temp_p / Temp_pp back only on the
+    steps whose output reads them.  With IVERB = 3 final_print writes TempDis.dat and
+    TempDisTPD.dat after every step (every step copies); with IVERB = 0 only after the last one
+    (one copy in 30 steps).  Both give the same files, equal to the reference program's to its
+    print resolution: T^30 and, in TempDisTPD.dat, T^29."""
+    outs = {}
+    for name, exe, iverb in (("ref", REFEXE, "0"), ("hip0", DROPIN, "0"), ("hip3", DROPIN, "3")):
+        inp = str(tmp_path / name / "inputs")
+        write_c1_inputs(inp, ntime=30)
+        _set_param(os.path.join(inp, "param.in"), "IVERB", iverb)
+        run_program(exe, str(tmp_path / name))
+        outs[name] = _outputs(str(tmp_path / name))
+    for a, b in zip(outs["hip0"], outs["hip3"]):
+        assert np.array_equal(a, b)
+    for a, b in zip(outs["hip0"], outs["ref"]):
+        assert np.abs(a - b).max() <= 1e-6
+    assert not np.array_equal(outs["hip0"][0], outs["hip0"][1])   # T^30 differs from T^29
+
+
+@pytest.mark.skipif(not os.path.exists(REFEXE), reason="reference program not built")
+def test_dropin_temperature_dependent_properties(tmp_path):
+    """TempDepProp = 1: the reference rebuilds H from the grid after every step (ChangeProp,
+    mod_tempdep.f90:38-65, whose table update is commented out), so its results equal
+    TempDepProp = 0.  The drop-in re-assembles H on the device after every step and gives the
+    reference program's TempDis.dat, and the same as its own TempDepProp = 0 run."""
+    outs = {}
+    for name, exe, tdp in (("ref", REFEXE, "1"), ("hip1", DROPIN, "1"), ("hip0", DROPIN, "0")):
+        inp = str(tmp_path / name / "inputs")
+        write_c1_inputs(inp, ntime=20)
+        _set_param(os.path.join(inp, "param.in"), "TempDepProp", tdp)
+        run_program(exe, str(tmp_path / name))
+        outs[name] = _outputs(str(tmp_path / name))
+    assert np.array_equal(outs["hip1"][0], outs["hip0"][0])
+    assert np.abs(outs["hip1"][0] - outs["ref"][0]).max() <= 1e-6

@@ -96,3 +96,53 @@ def test_workload_builders_small():
     s, m = workloads.c4(12, nheat=3, nprobe=7)
     assert len(m["probes"]) == 7 and np.count_nonzero(m["power"]) == 3
     assert np.all(np.abs(s.cells.kappa / 100.0 - 1.0) <= 0.1)
+
+
+def test_builtin_material_table_matches_oracle():
+    """hf_builtin_material (a host table, no device) against the oracle's material() over
+    every built-in id: the same single-precision roundings, tau assigned by the same ids, em by
+    none; unknown ids are HF_ERR_ARG."""
+    import oracle
+    lib = _lib.load()
+    ids = [100, 140, 3001, 3002, 3003, 3004, 9001, 9002, 9003, 310, 320, 330, 340, 350]
+    for mid in ids:
+        m, st, se = _lib.HfMaterial(), C.c_int32(), C.c_int32()
+        assert lib.hf_builtin_material(mid, C.byref(m), C.byref(st), C.byref(se)) == _lib.HF_OK
+        # a file material sets tau = 3e-9 and em = 0.5 first: what the built-in leaves stays
+        fm = {7: dict(kappa=1.0, rho=1.0, heat_capacity=1.0, tau=3e-9, em=0.5)}
+        k, r, h, t, e = oracle.materials([7, mid], fm)
+        assert (m.index, m.kappa, m.rho, m.heat_capacity) == (mid, k[1], r[1], h[1])
+        assert bool(st.value) == (t[1] != 3e-9) and se.value == 0 and e[1] == 0.5
+        if st.value:
+            assert m.tau == t[1]
+    m = _lib.HfMaterial()
+    for bad in (0, 1, 99, 141, 3005):
+        assert lib.hf_builtin_material(bad, C.byref(m), None, None) == _lib.HF_ERR_ARG
+
+
+def test_geometry_text_and_binary_forms(tmp_path, golden_dir):
+    """system.in read as read_system reads it (mod_inputs.f90:602-649) and the binary .npz
+    form: the same ids, heater flags and cell lengths."""
+    from heatflow_amd.system import Geometry, read_system
+    g = read_system(os.path.join(golden_dir, "system.in"))
+    assert (g.nx, g.ny, g.nz) == (25, 25, 1)
+    assert np.all(g.imaterial_type == 1) and g.iheater.sum() == 0
+    assert g.cell_lengths == (0.002 / 25.0, 0.002 / 25.0, 0.002)
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 7, 5, 3
+    ids = rng.choice([1, 140, 320], size=nx * ny * nz)
+    heat = rng.integers(0, 2, size=nx * ny * nz)
+    lines = [f"{nx} {ny} {nz}", "0.7 0.5 0.3"]
+    for iz in range(nz):
+        lines.append("")
+        for iy in range(ny):
+            row = range((iz * ny + iy) * nx, (iz * ny + iy + 1) * nx)
+            lines.append(" ".join(f"{ids[i]}:{heat[i]}" for i in row))
+    (tmp_path / "system.in").write_text("\n".join(lines) + "\n")
+    g = read_system(str(tmp_path / "system.in"))
+    assert np.array_equal(g.imaterial_type, ids) and np.array_equal(g.iheater, heat)
+    g.save(str(tmp_path / "geo.npz"))
+    h = Geometry.load(str(tmp_path / "geo.npz"))
+    assert (h.nx, h.ny, h.nz, tuple(h.lengths)) == (nx, ny, nz, (0.7, 0.5, 0.3))
+    assert np.array_equal(h.imaterial_type, ids) and np.array_equal(h.iheater, heat)
+    assert h.cell_lengths == (0.7 / 7.0, 0.5 / 5.0, 0.3 / 3.0)

@@ -286,3 +286,88 @@ def test_coo_hash_streamed_matches_stored():
         v3 = val.copy()
         v3.view(np.uint64)[5] ^= np.uint64(1)
         assert coo_hash(v3, irow, jcol) != h
+
+
+# ---- setup: material() over the grid and the stability check ------------------------------
+
+BUILTIN_IDS = [100, 140, 3001, 3002, 3003, 3004, 9001, 9002, 9003, 310, 320, 330, 340, 350]
+FILE_MATS = {1: dict(kappa=0.541, rho=997.0, heat_capacity=4200.0, tau=1e-12, em=0.0),
+             7: dict(kappa=2.5, rho=1000.0, heat_capacity=900.0, tau=3e-9, em=0.7),
+             42: dict(kappa=0.1, rho=10.0, heat_capacity=5.0, tau=0.0, em=0.25)}
+
+
+def test_materials_known_answers():
+    """The reference's own material test (test/test_mod_material.f90:22-35): silicon (140) has
+    kappa 130, heat_capacity 4200, rho 2328 to 1e-6; the built-ins are the single-precision
+    roundings of their literals (0.19 is not 0.19 in double); tau carries from the last material
+    that set it, em only from file materials."""
+    k, r, h, t, e = oracle.materials([140], {})
+    assert abs(k[0] - 130) < 1e-6 and abs(h[0] - 4200) < 1e-6 and abs(r[0] - 2328) < 1e-6
+    k, *_ = oracle.materials([3001, 9002], {})
+    assert k[0] == float(np.float32(0.19)) != 0.19 and k[1] == float(np.float32(0.209))
+    _, _, _, t, e = oracle.materials([140, 320, 140, 7, 330, 350, 100, 3001], FILE_MATS)
+    assert list(t) == [0.0, 1.0, 1.0, 3e-9, 3e-9, 1.0, 0.0, 0.0]
+    assert list(e) == [0.0, 0.0, 0.0, 0.7, 0.7, 0.7, 0.7, 0.7]
+    for bad in ([0], [-1], [5], [141]):
+        with pytest.raises(ValueError):
+            oracle.materials(bad, FILE_MATS)
+
+
+@needs_ref
+def test_materials_match_reference_build():
+    """The oracle's material loop against the reference's own material() (ref harness), over
+    5000 ids drawn from every built-in and three file materials: bit-exact."""
+    rng = np.random.default_rng(11)
+    ids = rng.choice(BUILTIN_IDS + list(FILE_MATS), size=5000)
+    ids[:3] = [3001, 140, 9003]          # cells before the first tau / em assignment
+    got = oracle.materials(ids, FILE_MATS)
+    ref = oracle.ref_materials(ids, [dict(index=i, **m) for i, m in FILE_MATS.items()])
+    for a, b in zip(got, ref):
+        assert np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+def _stability_system(n=(6, 5, 4), dt=1.0, kappa=None, kb=0.0, rhoc=(1.0, 1.0)):
+    from heatflow_amd.system import Cells, Params, System
+    nx, ny, nz = n
+    N = nx * ny * nz
+    p = Params(nx=nx, ny=ny, nz=nz, time_step=dt, kappa_bound=(kb,) * 6)
+    k = np.full(N, 1.0) if kappa is None else np.asarray(kappa, np.float64)
+    c = Cells(kappa=k, rho=np.full(N, rhoc[0]), heat_capacity=np.full(N, rhoc[1]),
+              tau=np.zeros(N), em=np.zeros(N), len_x=np.full(N, 1.0), len_y=np.full(N, 0.5),
+              len_z=np.full(N, 2.0), iheater=np.zeros(N, np.int64))
+    return System(p, c)
+
+
+def test_stability_closed_forms():
+    """setup::stability (mod_setup.f90:158-199): var = dt kappa/(rho c_p) (1 + 4 + 1/4) for
+    cells 1 x 0.5 x 2 against the single-precision 1/12.  A var between the double 1/12 and
+    float(1/12) passes (pins the constant); the first failing cell is the smallest index in
+    loop order; kappaBoundx1 is tested on the x1, y1 and z1 faces only."""
+    s = 5.25
+    thr32 = float(np.float32(1.0) / np.float32(12.0))
+    assert thr32 > 1.0 / 12.0
+    dt_mid = (1.0 / 12.0 + thr32) / 2 / s            # var strictly between the two constants
+    assert oracle.OracleSystem(_stability_system(dt=dt_mid)).stability()[0] == -1
+    dt_fail = thr32 * 1.0000001 / s
+    cell, v, b = oracle.OracleSystem(_stability_system(dt=dt_fail)).stability()
+    assert cell == 0 and not b and v > thr32
+    # one interior cell 40 times more conductive fails alone, at its own index
+    N = 120
+    kap = np.full(N, 1.0)
+    hot = 2 + 6 * (3 + 5 * 2)                          # (x, y, z) = (2, 3, 2), 0-based
+    kap[hot] = 40.0
+    dt = 0.05 / s
+    cell, v, b = oracle.OracleSystem(_stability_system(dt=dt, kappa=kap)).stability()
+    assert cell == hot and not b and abs(v - dt * 40.0 * s) <= 1e-15
+    # kappaBoundx1 failing: cell 0 (on every low face) is the first
+    cell, v, b = oracle.OracleSystem(_stability_system(dt=dt, kb=40.0)).stability()
+    assert cell == 0 and b and abs(v - dt * 40.0 * s) <= 1e-15
+    # the reference's own stability test case (test/test_mod_setup.f90:117-142: unit cell,
+    # kappa = rho = c_p = 1, dt = 1e4) fails the condition: var = 3e4 (the reference stops)
+    from heatflow_amd.system import Cells, Params, System
+    one = np.ones(8)
+    sysr = System(Params(nx=2, ny=2, nz=2, time_step=1e4),
+                  Cells(kappa=one, rho=one, heat_capacity=one, tau=0 * one, em=0 * one,
+                        len_x=one, len_y=one, len_z=one, iheater=np.zeros(8, np.int64)))
+    cell, v, b = oracle.OracleSystem(sysr).stability()
+    assert cell == 0 and v == 3e4 and not b

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libheatflow_hip.so")
 SOURCES = ["hf_kernels.hip", "hf_stencil.hip", "hf_api.hip", "hf_adjoint.hip", "hf_noalloc.hip",
-           "hf_setup.hip"]
+           "hf_setup.hip", "hf_persist.hip"]
 
 # -ffp-contract=off: products and sums round exactly as written (bit-exact assembly, and the
 # same arithmetic as the oracle); packed FP32 instructions are disabled for all device code.

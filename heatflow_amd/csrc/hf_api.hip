@@ -1164,10 +1164,114 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     return HF_OK;
 }
 
+// Steps of an FP64 context on a grid of at most kSmallMaxCells cells run in one persistent
+// workgroup (k_run_small), unless debug bit 0 (keep S) or bit 6 asks for the multi-kernel path.
+static bool small_path(const hf_ctx *c) {
+    return c->p.precision == HF_PREC_FP64 && c->hx != nullptr && c->g.N <= kSmallMaxCells &&
+           (c->debug & (1 | 64)) == 0;
+}
+
+// nsteps FP64 steps from itime0 on the persistent small-grid path, kSmallMaxSteps per launch
+// and one host synchronisation per launch.  The scalars of the multi-kernel path are left as
+// its next step expects them (TINY outcome in sc->flags, no carried residual).
+static int run_small(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t *iters_total,
+                     int64_t *last_iter, double *last_err) {
+    if (!c->small_out) TRY(dalloc(c, &c->small_out, 1));
+    if (c->dyn_heat && !c->small_hp) TRY(dalloc(c, &c->small_hp, kSmallMaxSteps));
+    c->carry = false;
+    const hf_params &p = c->p;
+    const double N = (double)c->g.N;
+    int64_t tot = 0, done = 0;
+    while (done < nsteps) {
+        const int32_t ns = (int32_t)std::min<int64_t>(nsteps - done, kSmallMaxSteps);
+        if (c->dyn_heat) {
+            c->small_hp_host.resize(ns);
+            for (int32_t k = 0; k < ns; ++k) c->small_hp_host[k] = heat_params(c, itime0 + done + k);
+            HIPCHK(c, hipMemcpyAsync(c->small_hp, c->small_hp_host.data(),
+                                     ns * sizeof(HeatParams), hipMemcpyHostToDevice, c->st));
+            c->qb_valid = false;
+        }
+        SmallRun a{};
+        a.g = c->g;
+        a.inverse_time = 1.0 / p.time_step;
+        a.tol = p.tol;
+        a.itmax = p.itmax;
+        a.isteady = p.isteady;
+        a.icattaneo = p.icattaneo;
+        a.allow_short = allow_short(c) ? 1 : 0;
+        a.nsteps = ns;
+        a.dyn = c->dyn_heat ? 1 : 0;
+        a.rc_mask = rc_mask(c);
+        a.t[0] = c->T[0];
+        a.t[1] = c->T[1];
+        a.t[2] = c->T[2];
+        a.rhoc = c->rhoc;
+        a.cvec = c->cvec;
+        a.bvec = c->bvec;
+        a.d = c->d;
+        a.cx = c->cx;
+        a.cy = c->cy;
+        a.cz = c->cz;
+        a.hx = c->hx;
+        a.hy = c->hy;
+        a.hz = c->hz;
+        a.sv = c->sv;
+        a.qd = c->qd;
+        a.hp = c->small_hp;
+        a.iheater = c->iheater;
+        a.ells = c->ells;
+        a.tau_s = c->tau_s;
+        a.qextra = c->qextra;
+        a.r = c->r64;
+        a.p0 = c->p64[0];
+        a.p1 = c->p64[1];
+        a.q = c->q64;
+        a.y = c->y64;
+        a.sc = c->sc;
+        a.out = c->small_out;
+        long slot = tbeg(c, HF_K_PERSIST, 0.0);
+        launch_run_small(a, c->st);
+        tend(c, slot);
+        TRY(check_launch(c, "k_run_small"));
+        SmallOut o{};
+        HIPCHK(c, hipMemcpyAsync(&o, c->small_out, sizeof(SmallOut), hipMemcpyDeviceToHost,
+                                 c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        double *t3[3] = {c->T[0], c->T[1], c->T[2]};
+        c->T[0] = t3[o.rot % 3];
+        c->T[1] = t3[(o.rot + 1) % 3];
+        c->T[2] = t3[(o.rot + 2) % 3];
+        // algorithmic bytes as the multi-kernel FP64 step counts them: right-hand side, 120 B a
+        // CG iteration, x; plus Qdens when it is recomputed
+        if (slot >= 0)
+            c->tpool[slot].bytes = N * ((double)o.steps * (8 * 9 + rc_bytes(c) + 8 * 5 +
+                                                           (c->dyn_heat ? 33.0 : 0.0)) +
+                                        120.0 * (double)o.iters);
+        c->tiny_guess = (o.flags & FLAG_TINY_NEXT) != 0;
+        c->h_sc->flags = o.flags;
+        c->stats.steps += o.steps;
+        c->stats.outer_iters += o.iters;
+        c->stats.first_iter_stops += o.first_stops;
+        if (o.steps > 0) c->stats.last_err = o.last_err;
+        tot += o.iters;
+        done += o.steps;
+        if (last_iter) *last_iter = o.last_iter;
+        if (last_err) *last_err = o.last_err;
+        const int32_t nan = o.flags & (FLAG_NAN_Q | FLAG_NAN_QDENS | FLAG_NAN_SCAT | FLAG_NAN_S |
+                                       FLAG_NAN_X);
+        if (nan) return flags_to_rc(c, nan);
+        if (o.steps != ns)
+            return set_err(c, HF_ERR_HIP, "k_run_small: the run ended early without a NaN flag");
+    }
+    if (iters_total) *iters_total = tot;
+    return HF_OK;
+}
+
 static int step_common(hf_ctx *c, int64_t itime, int64_t *iter, double *err, bool final_sync) {
     if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_step: not assembled");
     if (!c->has_state) return set_err(c, HF_ERR_STATE, "hf_step: no state (hf_set_state)");
     if (c->setup_flags) return flags_to_rc(c, c->setup_flags);
+    if (small_path(c)) return run_small(c, itime, 1, nullptr, iter, err);
     int rc = c->p.precision == HF_PREC_FP32_IR ? step_ir(c, itime, iter, err)
                                                : step_fp64(c, itime, iter, err);
     if (rc != HF_OK) return rc;
@@ -1185,6 +1289,12 @@ extern "C" int hf_step(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
 
 extern "C" int hf_run(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t *iters_total) {
     if (!c) return HF_ERR_ARG;
+    if (small_path(c) && nsteps > 0) {
+        if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_run: not assembled");
+        if (!c->has_state) return set_err(c, HF_ERR_STATE, "hf_run: no state (hf_set_state)");
+        if (c->setup_flags) return flags_to_rc(c, c->setup_flags);
+        return run_small(c, itime0, nsteps, iters_total, nullptr, nullptr);
+    }
     int64_t tot = 0;
     for (int64_t k = 0; k < nsteps; ++k) {
         int64_t it = 0;

@@ -84,6 +84,10 @@ struct hf_ctx {
     int64_t pred_iters = 4, pred_inner = 8, pred_adj = 8;
     double adj_tol = 0.0;        // tolerance of the adjoint solves (0: p.tol)
     hf_stats stats{};
+    // persistent small-grid steps (hf_persist.hip): run summary and per-step heat parameters
+    hf::SmallOut *small_out = nullptr;
+    hf::HeatParams *small_hp = nullptr;
+    std::vector<hf::HeatParams> small_hp_host;
     std::vector<void *> allocs;
     hf::AdjointState *adj = nullptr;
 

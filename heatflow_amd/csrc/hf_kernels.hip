@@ -244,19 +244,7 @@ k_scan_apply(const int64_t *__restrict__ in, int64_t n, const int64_t *__restric
 // ---- right-hand side -----------------------------------------------------------------------
 
 // Heater power density Qdens of one step (src/heatflow/mod_heating.f90:59-167) plus the
-// superset per-cell density.  qmode[m] is the mode-m power of this step computed on the host;
-// mode 5 adds (q5b * tau) * q5c with tau = tau_s * dt^2 (:56, :104-105).
-__device__ __forceinline__ double heater_q(const HeatParams &hp, int8_t mode, double tau_s,
-                                           double ells, double tp) {
-    double q;
-    switch (mode) {
-        case 0: q = 0.0; break;
-        case 5: q = hp.qmode[5] + hp.q5b * (tau_s * hp.dt2) * hp.q5c; break;
-        default: q = (mode >= 1 && mode <= 6) ? hp.qmode[mode] : 0.0; break;
-    }
-    return q - ells * (pow(tp, 4.0) - hp.tb4);
-}
-
+// superset per-cell density (heater_q: hf_kernels.hpp).
 __global__ void __launch_bounds__(kBlock)
 k_qdens(Dims g, HeatParams hp, const int8_t *__restrict__ iheater,
         const double *__restrict__ ells, const double *__restrict__ tau_s,

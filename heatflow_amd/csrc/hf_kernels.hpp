@@ -27,6 +27,28 @@ struct HeatParams {
     int32_t any_heater;
 };
 
+// Heater source of one cell and step (src/heatflow/mod_heating.f90:59-123): qmode[m] is the
+// mode-m power of this step computed on the host; mode 5 adds (q5b * tau) * q5c with
+// tau = tau_s * dt^2 (:56, :104-105); the radiative term -em L1 L2 sigma (T^4 - T_Bath^4) on
+// every cell (:121-123).
+__device__ __forceinline__ double heater_q(const HeatParams &hp, int8_t mode, double tau_s,
+                                           double ells, double tp) {
+    double q;
+    switch (mode) {
+        case 0: q = 0.0; break;
+        case 5: q = hp.qmode[5] + hp.q5b * (tau_s * hp.dt2) * hp.q5c; break;
+        default: q = (mode >= 1 && mode <= 6) ? hp.qmode[mode] : 0.0; break;
+    }
+    return q - ells * (pow(tp, 4.0) - hp.tb4);
+}
+
+// Mode of S (compile-time in the step kernels): kSteady (isteady /= 0), kFourier (transient,
+// icattaneo /= 1), kCattaneo (transient, icattaneo == 1)
+enum : int { kSteady = 0, kFourier = 1, kCattaneo = 2 };
+__host__ __device__ constexpr int s_mode(int32_t isteady, int32_t icattaneo) {
+    return isteady != 0 ? kSteady : (icattaneo == 1 ? kCattaneo : kFourier);
+}
+
 // rc_mask: the index mask of the rho c_p and tau' rho c_p reads, ~0 (per cell) or 0 (uniform
 // fields: every cell reads element 0)
 struct RhsParams {
@@ -215,5 +237,41 @@ struct ExtrapArgs {
     int32_t m;
 };
 void launch_extrap(const Dims &g, const ExtrapArgs &a, double *out, hipStream_t st);
+
+// ---- FP64 time steps of a small grid in one persistent workgroup (hf_persist.hip) ----------
+
+constexpr int kSmallThreads = 1024;             // the one workgroup
+constexpr uint32_t kSmallMaxCells = 32768;      // grids up to this many cells take the path
+constexpr int32_t kSmallMaxSteps = 256;         // steps per launch (per-step heat parameters)
+
+// what a run of k_run_small leaves (written by thread 0 at its end)
+struct SmallOut {
+    int64_t steps;          // steps completed (a step stopped by a NaN guard is not counted)
+    int64_t iters;          // CG iterations over them
+    int64_t last_iter;      // of the last completed step
+    int64_t first_stops;    // solves that ended on their first iteration
+    double last_err;        // ||r^|| / ||b^|| of the last completed step
+    int32_t flags;          // NaN bits of the step that stopped the run, TINY_NEXT of the last
+    int32_t rot;            // (T^n, T^{n-1}, spare) = t[rot], t[(rot + 1) % 3], t[(rot + 2) % 3]
+};
+
+struct SmallRun {
+    Dims g;
+    double inverse_time, tol;
+    int64_t itmax;
+    int32_t isteady, icattaneo, allow_short, nsteps;
+    int32_t dyn;            // Qdens from T^n every step (hp[step]): time-dependent heating
+    uint32_t rc_mask;       // as RhsParams::rc_mask
+    double *t[3];           // T^n, T^{n-1}, spare
+    const double *rhoc, *cvec, *bvec, *d, *cx, *cy, *cz, *hx, *hy, *hz, *sv;
+    double *qd;             // Qdens (read; written per step when dyn)
+    const HeatParams *hp;   // per step (dyn)
+    const int8_t *iheater;
+    const double *ells, *tau_s, *qextra;
+    double *r, *p0, *p1, *q, *y;   // CG vectors
+    Scal *sc;               // flags in (TINY_NEXT of the last step) and out
+    SmallOut *out;
+};
+void launch_run_small(const SmallRun &a, hipStream_t st);
 
 }  // namespace hf

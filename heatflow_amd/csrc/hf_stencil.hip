@@ -437,12 +437,8 @@ __device__ __forceinline__ double hx_sum(const Dims &g, const ZCol &c, uint32_t 
     return h;
 }
 
-// Mode of S (compile-time): kSteady (isteady /= 0), kFourier (transient, icattaneo /= 1),
-// kCattaneo (transient, icattaneo == 1).  Kernels built for a solve (not FromState) use kSteady.
-enum : int { kSteady = 0, kFourier = 1, kCattaneo = 2 };
-__host__ __device__ constexpr int s_mode(int32_t isteady, int32_t icattaneo) {
-    return isteady != 0 ? kSteady : (icattaneo == 1 ? kCattaneo : kFourier);
-}
+// Mode of S: kSteady, kFourier, kCattaneo (hf_kernels.hpp).  Kernels built for a solve (not
+// FromState) use kSteady.
 
 // operand inputs of k_rhs_res_z: T^n, T^{n-1} (state) or x (solve)
 struct RhsRaw {

@@ -237,6 +237,93 @@ def read_mat(path: str) -> dict:
     return mats
 
 
+def write_inputs(system: System, dirname: str, ntime: Optional[int] = None,
+                 write_every: int = 10, **param_overrides) -> None:
+    """Writes a System in the reference's input format: <dirname>/param.in, mat.in and
+    system.in, as read_param, read_mat and read_system read them (mod_inputs.f90:204-360,
+    602-721) -- the inverse of read_inputs.  Each distinct (kappa, rho, heat_capacity, tau, em)
+    becomes a file material (at most 99); cell lengths must be uniform per axis and there is no
+    per-cell power density (the reference cannot express either).  Output is off (IVERB 0,
+    no text output, _Test_Run) unless overridden: param_overrides replace or add param.in keys."""
+    p, c = system.params, system.cells
+    n = p.n
+    if c.qdens_extra is not None:
+        raise ValueError("a per-cell power density has no reference input form")
+    lens = []
+    for a in (c.len_x, c.len_y, c.len_z):
+        a = np.asarray(a, np.float64)
+        if np.any(a != a[0]):
+            raise ValueError("cell lengths must be uniform per axis")
+        lens.append(float(a[0]))
+    # materials: distinct rows of the five fields, found column by column (a row-wise unique
+    # over 16.8M cells sorts 5-wide rows and takes minutes)
+    cols = [np.broadcast_to(np.asarray(v, np.float64), (n,)) for v in
+            (c.kappa, c.rho, c.heat_capacity, c.tau, c.em)]
+    key = np.zeros(n, np.int64)
+    vals = []
+    for col in cols:
+        u, inv = np.unique(col, return_inverse=True)
+        key = key * len(u) + inv.reshape(-1)
+        vals.append(u)
+    ukey, mat_of = np.unique(key, return_inverse=True)
+    uniq = np.empty((len(ukey), 5))
+    rem = ukey.copy()
+    for j in range(4, -1, -1):
+        uniq[:, j] = vals[j][rem % len(vals[j])]
+        rem //= len(vals[j])
+    if len(uniq) > 99:
+        raise ValueError(f"{len(uniq)} distinct materials: mat.in holds at most 99")
+    os.makedirs(dirname, exist_ok=True)
+    per = "".join(ax for ax, on in zip("xyz", p.periodic) if on)
+    cg = " ".join(t for t, on in zip(("-x", "+x", "-y", "+y", "-z", "+z"), p.cg) if on)
+    kv = {
+        "IVERB": "0", "_WriteToTxt": ".False.", "_Test_Run": ".True.",
+        "_Percentage_Completion": ".False.", "_RunName": "run",
+        "ntime": str(p.ntime if ntime is None else ntime), "write_every": str(write_every),
+        "heattime": str(p.heated_steps), "time_step": repr(p.time_step), "freq": repr(p.freq),
+        "iboundary": "1", "icattaneo": str(p.icattaneo), "isteady": str(p.isteady),
+        "T_System": repr(p.t_system), "T_Bath": repr(p.t_bath_rad),
+        "power_in": repr(p.power_in), "BR": repr(p.br), "T_BathCG": repr(p.t_bath_cg),
+        "CG_dir": f"'{cg}'" if cg else "none", "Periodic": f"'{per}'", "TempDepProp": "0",
+        "_Check_Sparse_Full": ".False.", "_Check_Stability": ".False.",
+        "_InputTempDis": ".False.", "_FullRestart": ".False.", "T_BathCC": ".False.",
+    }
+    for k, v in zip(("T_Bathx1", "T_Bathx2", "T_Bathy1", "T_Bathy2", "T_Bathz1", "T_Bathz2"),
+                    p.t_bath):
+        kv[k] = repr(float(v))
+    for k, v in zip(("kappaBoundx1", "kappaBoundNx", "kappaBoundy1", "kappaBoundNy",
+                     "kappaBoundz1", "kappaBoundNz"), p.kappa_bound):
+        kv[k] = repr(float(v))
+    kv.update({k: str(v) for k, v in param_overrides.items()})
+    with open(os.path.join(dirname, "param.in"), "w") as f:
+        f.writelines(f"    {k} = {v}\n" for k, v in kv.items())
+    with open(os.path.join(dirname, "mat.in"), "w") as f:
+        for i, (k, r, h, t, e) in enumerate(uniq.tolist(), start=1):
+            f.write(f"{i}\nheat_capacity = {h!r}\nh_conv = 0\nkappa = {k!r}\n"
+                    f"kappa3D = {k!r}\nrho = {r!r}\nsound_speed = 0\ntau = {t!r}\n"
+                    f"em = {e!r}\n")
+        f.write("0\n")
+    heat = np.asarray(c.iheater, np.int64).reshape(-1)
+    if heat.min() < 0:
+        raise ValueError("heater modes are >= 0")
+    hspan = int(heat.max()) + 1
+    combos, tok_of = np.unique(mat_of.reshape(-1) * hspan + heat, return_inverse=True)
+    # "m:h" tokens padded to one width (list-directed reads skip the blanks), one row of x per
+    # line, a blank line before each z plane
+    words = [f"{k // hspan + 1}:{k % hspan}" for k in combos.tolist()]
+    w = max(len(t) for t in words) + 1
+    table = np.frombuffer("".join(t.ljust(w) for t in words).encode(), np.uint8).reshape(-1, w)
+    L = (lens[0] * p.nx, lens[1] * p.ny, lens[2] * p.nz)
+    tok_of = tok_of.reshape(p.nz, p.ny * p.nx)
+    with open(os.path.join(dirname, "system.in"), "wb") as f:
+        f.write(f"{p.nx} {p.ny} {p.nz}\n{L[0]!r} {L[1]!r} {L[2]!r}\n".encode())
+        for iz in range(p.nz):
+            rows = table[tok_of[iz]].reshape(p.ny, p.nx * w)
+            rows[:, -1] = ord("\n")
+            f.write(b"\n")
+            f.write(rows.tobytes())
+
+
 def read_inputs(dirname: str) -> System:
     """Reads <dirname>/param.in, mat.in and system.in like read_all_files
     (src/heatflow/mod_inputs.f90:116-193) for file materials, then applies

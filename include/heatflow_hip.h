@@ -214,7 +214,11 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * every cell); bit 4 runs the adjoint step with the flat per-cell kernel instead of the
  * z-marching one (grids with a periodic axis always use the flat kernel); bit 5 starts every
  * adjoint solve from the linear extrapolation 2 lambda^{n+2} - lambda^{n+3} instead of the
- * polynomial through the last six solutions. */
+ * polynomial through the last six solutions; bit 6 runs the FP64 steps of a small grid with the
+ * multi-kernel path instead of the persistent one (FP64 contexts of at most 32768 cells run
+ * whole steps, up to 256 per launch, in one workgroup: no launch per CG iteration and one host
+ * synchronisation per launch; its sums group differently, so the two paths agree to the
+ * tolerance, not bit for bit). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */
@@ -246,7 +250,7 @@ int hf_reset_stats(hf_ctx *ctx);
  * arrays each kernel touches, every array element counted once).  Reading resets the pool. */
 enum {
     HF_K_QDENS = 0, HF_K_RHS64, HF_K_RHS32, HF_K_CG_A64, HF_K_CG_B64, HF_K_CG_A32, HF_K_CG_B32,
-    HF_K_FINAL, HF_K_IR_RES, HF_K_IR_STEP, HF_K_STEP64, HF_K_COUNT
+    HF_K_FINAL, HF_K_IR_RES, HF_K_IR_STEP, HF_K_STEP64, HF_K_PERSIST, HF_K_COUNT
 };
 int hf_set_timing(hf_ctx *ctx, int32_t on);
 int hf_kernel_times(hf_ctx *ctx, double *ms, int64_t *launches, double *bytes);

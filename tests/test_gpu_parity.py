@@ -140,12 +140,18 @@ def test_manual_cases_on_gpu(idx):
 
 
 @pytest.mark.parametrize("gi", range(len(GEOMETRIES)))
-def test_steps_match_oracle(gi):
+@pytest.mark.parametrize("dbg", [0, 64])
+def test_steps_match_oracle(gi, dbg):
+    """Four steps from a random state on every geometry (periodic axes, CG faces, 1-cell axes,
+    heater modes 1-6 with radiation, near-zero conductivities, Fourier with a per-cell source)
+    against the oracle: the persistent small-grid path (debug 0) and the multi-kernel path
+    (debug bit 6)."""
     s = random_system(600 + gi, **GEOMETRIES[gi])
     tp0, tpp0 = random_state(gi, s.params.n)
     rc, otp, otpp, oit = oracle.OracleSystem(s).simulate_steps([1, 2, 3, 4], tp0, tpp0)
     assert rc == 0
     with Simulation(s) as sim:
+        sim.set_debug(dbg)
         sim.set_state(tp0, tpp0)
         its = [sim.simulate(k)[0] for k in (1, 2, 3, 4)]
         tp, tpp = sim.get_state()
@@ -158,9 +164,10 @@ FUSED64_CASES = [dict(GEOMETRIES[i]) for i in (0, 1, 2, 3, 5, 6, 7, 8, 9)] + [
 
 
 def _fp64_runs(s, tp0, tpp0, nsteps):
-    """(T^n, T^{n-1}, stats) of the fused FP64 step and of the unfused one (debug bit 0)."""
+    """(T^n, T^{n-1}, stats) of the fused FP64 step and of the unfused one (debug bit 0), both
+    on the multi-kernel path (debug bit 6)."""
     out = []
-    for dbg in (0, 1):
+    for dbg in (64, 65):
         with Simulation(s) as sim:
             sim.set_debug(dbg)
             sim.set_state(tp0, tpp0)
@@ -581,7 +588,7 @@ def test_pair_kernels_match_scalar(monkeypatch, prec):
     heating, 24^3 and a periodic 8 x 6 x 5 grid)."""
     variants = [{}, {"HF_NO_QUADS": "1"}, {"HF_NO_PAIRS": "1"}]
     if prec == "fp64":
-        variants = variants[1:]   # no FP64 quads
+        variants = variants[1:]   # no FP64 quads (and debug bit 6: not the persistent path)
     for s in (workloads.c3(24)[0], random_system(31, 8, 6, 5, periodic=(True, False, True))):
         out = []
         for env in variants:
@@ -591,6 +598,7 @@ def test_pair_kernels_match_scalar(monkeypatch, prec):
                 else:
                     monkeypatch.delenv(k, raising=False)
             with Simulation(s, precision=prec, tol=1e-14) as sim:
+                sim.set_debug(64)
                 sim.initial_evolve()
                 sim.run(1, 10)
                 out.append(sim.get_state()[0])

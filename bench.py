@@ -39,21 +39,22 @@ def sync():
     torch.cuda.synchronize()
 
 
-def timed_run(sim, steps: int, warmup: int):
-    """warm-up, reset, then `steps` timed steps with kernel timing on; returns (seconds, stats,
-    kernel times, final state)."""
+def timed_run(sim, steps: int, warmup: int, timing: bool = True):
+    """warm-up, reset, then `steps` timed steps (with kernel timing unless `timing` is False:
+    its events cost a few microseconds a launch, which counts on small grids); returns
+    (seconds, stats, kernel times, final state)."""
     sim.initial_evolve()
     if warmup:
         sim.run(1, warmup)
     sim.initial_evolve()
     sim.reset_stats()
-    sim.set_timing(True)
+    sim.set_timing(timing)
     sync()
     t0 = time.perf_counter()
     sim.run(1, steps)
     sync()
     dt = time.perf_counter() - t0
-    kt = sim.kernel_times()
+    kt = sim.kernel_times() if timing else {}
     sim.set_timing(False)
     st = sim.stats()
     tp, _ = sim.get_state()
@@ -99,6 +100,64 @@ def measured_traffic(kernel: str, n: int):
 def rel_errors(a, ref):
     e = np.abs(a - ref) / np.abs(ref)
     return float(e.max()), float(np.sqrt(np.mean(e * e)))
+
+
+def err_summary(a, ref, t_bath: float = 300.0) -> dict:
+    """max / rms error relative to T, and the largest error on the signal T - T_bath relative
+    to that signal's peak (the ~300 K offset hides the error of the heating response)."""
+    mx, rms = rel_errors(a, ref)
+    peak = float(np.abs(ref - t_bath).max())
+    return dict(max=mx, rms=rms, signal=float(np.abs(a - ref).max()) / peak if peak > 0 else None,
+                signal_peak_K=peak)
+
+
+def _set_param(path: str, key: str, value) -> None:
+    with open(path) as f:
+        lines = f.readlines()
+    with open(path, "w") as f:
+        for ln in lines:
+            if ln.split("=", 1)[0].strip() == key:
+                ln = f"    {key} = {value}\n"
+            f.write(ln)
+
+
+def program_rate(s, exe: str, workdir: str, n1: int, n2: int, threads: int = 1,
+                 inputs_ready: bool = False, repeats: int = 1) -> dict:
+    """timesteps/s of a HeatFlow program (the reference, or the drop-in) from the wall-clock
+    difference of two runs of n1 < n2 steps on the same inputs, which removes input parsing,
+    setup and assembly (the fastest of `repeats` runs of each).  Output off (IVERB 0, no text
+    output); stack unlimited (the reference's automatic arrays); `threads` OpenMP threads,
+    bound spread."""
+    import resource
+    import subprocess
+    from heatflow_amd.system import write_inputs
+    inp = os.path.join(workdir, "inputs")
+    if not inputs_ready:
+        write_inputs(s, inp, ntime=n1)
+    os.makedirs(os.path.join(workdir, "outputs"), exist_ok=True)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
+
+    def unlimited_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    times = []
+    for nt in (n1, n2):
+        _set_param(os.path.join(inp, "param.in"), "ntime", nt)
+        best = None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            p = subprocess.run([exe], cwd=workdir, env=env, preexec_fn=unlimited_stack,
+                               capture_output=True, text=True)
+            t = time.perf_counter() - t0
+            if p.returncode != 0:
+                raise RuntimeError(f"{exe} failed ({p.returncode}): {p.stdout[-500:]} "
+                                   f"{p.stderr[-500:]}")
+            best = t if best is None else min(best, t)
+        times.append(best)
+        log(f"[bench] {os.path.basename(exe)} {nt} steps: {times[-1]:.3f} s")
+    per = (times[1] - times[0]) / (n2 - n1)
+    return dict(value=round(1.0 / per, 4) if per > 0 else None, unit="timesteps/s",
+                cores=threads, seconds=[round(t, 3) for t in times], steps=[n1, n2])
 
 
 def _omp_threads() -> int:
@@ -157,6 +216,116 @@ def cpu_baseline(n_full: int, tol: float, sample_n: int, sample_steps: int) -> d
                         f"{dtb:.2f} s, {itb} solver iterations; scaled by cell count to "
                         f"{n_full}^3"),
                 sample_seconds=round(dtb, 3), builds=out)
+
+
+DROPIN_EXE = os.path.join(ROOT, "heatflow_amd", "fortran", "_build", "heatflow_hip")
+
+
+def c1_bench(steps: int, warmup: int, cpu: bool) -> dict:
+    """The 16^3 Cattaneo workload (1 um cells, one centre heater of 1e-4 W, x faces 300 K, y/z
+    insulating, dt 1e-10 s), 100 steps in FP64 at the configuration's CG tolerance 1e-10 and at
+    1e-12 (which keeps the temperatures within 1e-10 of the reference tolerance's), on the
+    persistent small-grid path and on the multi-kernel path (debug bit 6); errors against FP64
+    at 1e-32.  Kernel timing is off (its events would add to the latency measured here).
+    With `cpu`: the reference program (serial and OpenMP) and the Fortran drop-in timed on the
+    same input files by the two-run difference (100 and 1000 steps, fastest of three runs
+    each), at the reference's own tolerance 1e-32."""
+    import shutil
+    import tempfile
+    from heatflow_amd import Simulation, workloads
+    s, _ = workloads.c1(16)
+    out, states = {}, {}
+    for label, dbg, tol in (("persistent", 0, 1e-10), ("persistent", 0, 1e-12),
+                            ("multi_kernel", 64, 1e-10), ("multi_kernel", 64, 1e-12),
+                            ("persistent", 0, 1e-32)):
+        key = f"{label}@{tol:g}"
+        with Simulation(s, tol=tol) as sim:
+            sim.set_debug(dbg)
+            dt, st, _, tp = timed_run(sim, steps, warmup, timing=False)
+        states[key] = tp
+        out[key] = dict(timesteps_per_s=steps / dt, ms_per_step=1000.0 * dt / steps,
+                        cg_iters=int(st.outer_iters), cg_iters_per_s=st.outer_iters / dt)
+        log(f"[bench] c1 {key}: {steps / dt:.1f} steps/s, {st.outer_iters} CG iterations")
+    ref = states["persistent@1e-32"]
+    for key in out:
+        if key != "persistent@1e-32":
+            out[key]["err_vs_fp64_reftol"] = err_summary(states[key], ref)
+    out["persistent_over_multi_kernel"] = {
+        f"{t:g}": out[f"persistent@{t:g}"]["timesteps_per_s"]
+        / out[f"multi_kernel@{t:g}"]["timesteps_per_s"] for t in (1e-10, 1e-12)}
+    if cpu:
+        import oracle
+        progs = [("reference_serial", oracle.REF_EXE, 1),
+                 ("reference_openmp", oracle.REF_OMP_EXE, _host_threads()),
+                 ("dropin_gpu", DROPIN_EXE, 1)]
+        for label, exe, thr in progs:
+            if not os.path.exists(exe):
+                out[label] = None
+                continue
+            d = tempfile.mkdtemp(prefix="hf_c1_")
+            try:
+                out[label] = program_rate(s, exe, d, 100, 1000, threads=thr, repeats=3)
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def _host_threads() -> int:
+    """OpenMP threads for the reference's parallel build: OMP_NUM_THREADS when set (the GPU
+    box sets its CPU share), else the usable cores."""
+    v = os.environ.get("OMP_NUM_THREADS")
+    return int(v) if v and v.isdigit() else len(os.sched_getaffinity(0))
+
+
+def c2_bench(n: int, steps: int, warmup: int) -> dict:
+    """The 256^3 two-phase voxel workload (Bernoulli(0.3) phase B kappa 0.2 / rho 1200 / c_p
+    1500 in phase A kappa 150 / rho 2330 / c_p 700, 32 heaters of 1e-5 W, x faces 300 K,
+    dt 5e-11 s), 50 steps in FP64 at the configuration's CG tolerance 1e-10, at 1e-12 and at
+    1e-14; errors against FP64 at 1e-32; kernel table and roofline of each."""
+    from heatflow_amd import Simulation, workloads
+    s, _ = workloads.c2(n)
+    N = s.params.n
+    out, states = {}, {}
+    for tol in (1e-10, 1e-12, 1e-14, 1e-32):
+        key = f"fp64@{tol:g}"
+        with Simulation(s, tol=tol) as sim:
+            dt, st, kt, tp = timed_run(sim, steps, warmup)
+        states[key] = tp
+        out[key] = dict(timesteps_per_s=steps / dt, ms_per_step=1000.0 * dt / steps,
+                        cg_iters=int(st.outer_iters), cg_iters_per_s=st.outer_iters / dt,
+                        kernels=kernel_summary(kt, N), roofline=roofline(kt))
+        log(f"[bench] c2 {key}: {steps / dt:.2f} steps/s, {st.outer_iters} CG iterations, "
+            f"roofline {out[key]['roofline']}")
+    for key in out:
+        if key != "fp64@1e-32":
+            out[key]["err_vs_fp64_reftol"] = err_summary(states[key], states["fp64@1e-32"])
+    return out
+
+
+def c2_reference(n: int, n1: int, n2: int) -> dict:
+    """The reference program (serial and OpenMP) and the Fortran drop-in on the 256^3 two-phase
+    workload's input files (a 16.8M-token system.in), timed by the two-run difference of n1 and
+    n2 steps at the reference's tolerance 1e-32."""
+    import shutil
+    import tempfile
+    import oracle
+    from heatflow_amd import workloads
+    from heatflow_amd.system import write_inputs
+    s, _ = workloads.c2(n)
+    out = {}
+    d = tempfile.mkdtemp(prefix="hf_c2_")
+    try:
+        t0 = time.perf_counter()
+        write_inputs(s, os.path.join(d, "inputs"), ntime=n1)
+        log(f"[bench] c2 inputs written in {time.perf_counter() - t0:.1f} s")
+        for label, exe, thr in (("dropin_gpu", DROPIN_EXE, 1),
+                                ("reference_openmp", oracle.REF_OMP_EXE, _host_threads()),
+                                ("reference_serial", oracle.REF_EXE, 1)):
+            if os.path.exists(exe):
+                out[label] = program_rate(s, exe, d, n1, n2, threads=thr, inputs_ready=True)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
 
 
 def per_cell_rc_bench(s, steps: int, warmup: int, tol: float) -> dict:
@@ -224,7 +393,7 @@ def grad_bench(n: int, steps: int, ckpt: int, fd_cells: int = 16) -> dict:
     every `ckpt` steps, reverse sweep with recomputation) on the 300 K system.
 
     Checked against central finite differences (relative step 1e-6) of the GPU forward:
-      - kappa on the `fd_cells` cells with the largest |dL/dkappa|;
+      - kappa on `fd_cells` random cells (seed 43) within one cell of a probe;
       - kappa along a seeded direction, kappa (1 + eps u), u ~ U(-1, 1), against
         sum_i g_i kappa_i u_i;
       - 4 heater powers.
@@ -281,19 +450,37 @@ def grad_bench(n: int, steps: int, ckpt: int, fd_cells: int = 16) -> dict:
                     return dict(max_rel_err=float(max(errs)),
                                 median_rel_err=float(np.median(errs)))
 
-                cells = np.argsort(-np.abs(gk))[:fd_cells]
-                e0, e300 = [], []
+                def stats_g(fds, gs):
+                    fds, gs = np.asarray(fds), np.asarray(gs)
+                    gmax = float(np.abs(gs).max())
+                    rel = np.abs(fds - gs) / np.abs(gs)
+                    big = np.abs(gs) >= 1e-3 * gmax
+                    return {"max_abs_err_over_max_g": float(np.abs(fds - gs).max()) / gmax,
+                            "max_rel_err_cells_ge_1e-3_max": float(rel[big].max()),
+                            "cells_ge_1e-3_max": int(big.sum()),
+                            "max_rel_err_all": float(rel.max()),
+                            "median_rel_err_all": float(np.median(rel)),
+                            "g_range": [float(np.abs(gs).min()), gmax]}
+
+                # `fd_cells` cells drawn with seed 43 from those within one cell of a probe
+                # (the support of dL/dkappa)
+                near = workloads.near_cells(n, np.asarray(probes), 1)
+                cells = np.sort(np.random.default_rng(43).choice(near, size=fd_cells,
+                                                                 replace=False))
+                f0, f300 = [], []
                 for c in cells:
                     h = 1e-6 * kap[c]
                     kp, km = kap.copy(), kap.copy()
                     kp[c] += h
                     km[c] -= h
                     pw = m["power"]
-                    e0.append(abs(cd(sim0, targets0, kp, km, pw, pw, h) - gk[c]) / abs(gk[c]))
-                    e300.append(abs(cd(sim, targets, kp, km, pw, pw, h) - gk[c]) / abs(gk[c]))
-                res["fd_kappa_cells"] = {"cells": len(cells), "choice": "largest |dL/dkappa|",
-                                         "rel_step": 1e-6, "bath_origin": stats(e0),
-                                         "at_300K": stats(e300)}
+                    f0.append(cd(sim0, targets0, kp, km, pw, pw, h))
+                    f300.append(cd(sim, targets, kp, km, pw, pw, h))
+                res["fd_kappa_cells"] = {"cells": len(cells),
+                                         "choice": "random (seed 43) within one cell of a probe",
+                                         "rel_step": 1e-6,
+                                         "bath_origin": stats_g(f0, gk[cells]),
+                                         "at_300K": stats_g(f300, gk[cells])}
                 u = np.random.default_rng(44).uniform(-1.0, 1.0, kap.size)
                 dirder = float(np.dot(gk, kap * u))
                 eps = 1e-6
@@ -436,6 +623,12 @@ def main():
                     help="only run the 384^3 accuracy-vs-tolerance sweep and print it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 2 GiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 16^3 leg")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 256^3 two-phase leg")
+    ap.add_argument("--c2-n", type=int, default=256)
+    ap.add_argument("--ref-c2", action="store_true",
+                    help="only time the reference program and the drop-in on the 256^3 "
+                         "two-phase input files (minutes of CPU) and print that")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -450,6 +643,9 @@ def main():
 
     if args.tol_sweep:
         print(json.dumps({"tol_sweep": tol_sweep(args.c5_n, 500)}), flush=True)
+        return
+    if args.ref_c2:
+        print(json.dumps({"c2_reference": c2_reference(args.c2_n, 1, 2)}), flush=True)
         return
 
     if args.copy_probe:
@@ -477,18 +673,23 @@ def main():
     K, W = args.steps, args.warmup
 
     results = {}
-    modes = [("fp32_ir", args.tol), ("fp64", args.tol)]
+    # (precision, tolerance, debug flags, label): debug bit 2 (value 4) turns off the reuse of
+    # the inner step length, so that leg compares the precisions with every step solving
+    modes = [("fp32_ir", args.tol, 0, "fp32_ir"), ("fp64", args.tol, 0, "fp64")]
     if args.only:
-        modes = [(args.only, args.tol)]
-    elif not args.no_ref:
-        modes.append(("fp64", 1e-32))
+        modes = [(args.only, args.tol, 0, args.only)]
+    else:
+        modes.append(("fp32_ir", args.tol, 4, "fp32_ir_noreuse"))
+        if not args.no_ref:
+            modes.append(("fp64", 1e-32, 0, "fp64"))
     states = {}
-    for prec, tol in modes:
-        key = f"{prec}@{tol:g}"
+    for prec, tol, dbg, label in modes:
+        key = f"{label}@{tol:g}"
         t_setup = time.perf_counter()
         sim = Simulation(s, precision=prec, tol=tol)
         try:
             setup = time.perf_counter() - t_setup
+            sim.set_debug(dbg)
             dt, st, kt, tp = timed_run(sim, K, W)
         finally:
             sim.close()
@@ -506,18 +707,19 @@ def main():
     if ref_key in states:
         for key in results:
             if key != ref_key:
-                mx, rms = rel_errors(states[key], states[ref_key])
-                results[key]["err_vs_fp64_reftol"] = dict(max=mx, rms=rms)
+                results[key]["err_vs_fp64_reftol"] = err_summary(states[key], states[ref_key])
     del states
 
     head = results.get(f"fp32_ir@{args.tol:g}") or next(iter(results.values()))
     fp64 = results.get(f"fp64@{args.tol:g}")
     percell = None if args.only else per_cell_rc_bench(s, K, W, args.tol)
     stiff = None if (args.no_stiff or args.only) else stiff_bench(s, args.stiff_steps, args.tol)
-    grad = None if (args.no_grad or args.only) else grad_bench(args.grad_n, 200, 20, fd_cells=12)
+    grad = None if (args.no_grad or args.only) else grad_bench(args.grad_n, 200, 20, fd_cells=16)
     grad5 = None if (args.no_grad or args.no_c5 or args.only) else grad_c5_bench(args.c5_n, 500)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, args.tol, args.cpu_n,
                                                                 args.cpu_steps)
+    c1 = None if (args.no_c1 or args.only) else c1_bench(100, 5, cpu=not args.no_cpu)
+    c2 = None if (args.no_c2 or args.only) else c2_bench(args.c2_n, 50, 2)
     for r in results.values():
         r["roofline"]["traffic"] = measured_traffic(r["roofline"]["kernel"], args.n)
     line = {
@@ -540,6 +742,10 @@ def main():
                    "uniform_rho_cp": True},
         "fp32_ir_over_fp64": (round(head["timesteps_per_s"] / fp64["timesteps_per_s"], 3)
                               if fp64 and head is not fp64 else None),
+        "fp32_ir_noreuse_over_fp64": (
+            round(results[f"fp32_ir_noreuse@{args.tol:g}"]["timesteps_per_s"]
+                  / fp64["timesteps_per_s"], 3)
+            if fp64 and f"fp32_ir_noreuse@{args.tol:g}" in results else None),
         "modes": results,
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
@@ -547,6 +753,8 @@ def main():
         "per_cell_rho_cp": percell,
         "stiff_dt_x1000": stiff,
         "grad_c5_384": grad5,
+        "c1_16": c1,
+        "c2_256": c2,
     }
     print(json.dumps(line), flush=True)
 

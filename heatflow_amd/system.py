@@ -287,6 +287,8 @@ def write_inputs(system: System, dirname: str, ntime: Optional[int] = None,
         "CG_dir": f"'{cg}'" if cg else "none", "Periodic": f"'{per}'", "TempDepProp": "0",
         "_Check_Sparse_Full": ".False.", "_Check_Stability": ".False.",
         "_InputTempDis": ".False.", "_FullRestart": ".False.", "T_BathCC": ".False.",
+        "start_ix": "1", "end_ix": str(p.nx), "start_iy": "1", "end_iy": str(p.ny),
+        "start_iz": "1", "end_iz": str(p.nz), "kappaBound": "0",
     }
     for k, v in zip(("T_Bathx1", "T_Bathx2", "T_Bathy1", "T_Bathy2", "T_Bathz1", "T_Bathz2"),
                     p.t_bath):
@@ -297,12 +299,13 @@ def write_inputs(system: System, dirname: str, ntime: Optional[int] = None,
     kv.update({k: str(v) for k, v in param_overrides.items()})
     with open(os.path.join(dirname, "param.in"), "w") as f:
         f.writelines(f"    {k} = {v}\n" for k, v in kv.items())
+    # each material block ends with a 0 line: read_mat checks and resets its key counts there
+    # (mod_inputs.f90:675-684)
     with open(os.path.join(dirname, "mat.in"), "w") as f:
         for i, (k, r, h, t, e) in enumerate(uniq.tolist(), start=1):
             f.write(f"{i}\nheat_capacity = {h!r}\nh_conv = 0\nkappa = {k!r}\n"
                     f"kappa3D = {k!r}\nrho = {r!r}\nsound_speed = 0\ntau = {t!r}\n"
-                    f"em = {e!r}\n")
-        f.write("0\n")
+                    f"em = {e!r}\n0\n")
     heat = np.asarray(c.iheater, np.int64).reshape(-1)
     if heat.min() < 0:
         raise ValueError("heater modes are >= 0")

@@ -29,6 +29,7 @@ ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libhfref.so")
 REF_OMP_SO = os.path.join(HERE, "_ref", "libhfref_omp.so")   # the same with -fopenmp
 REF_EXE = os.path.join(HERE, "_ref", "heatflow_ref")
+REF_OMP_EXE = os.path.join(HERE, "_ref", "heatflow_ref_omp")   # the same with -fopenmp
 REF_SRC = "/root/reference"
 
 

@@ -174,6 +174,64 @@ def test_adjoint_central_fd(icat):
         sim0.close()
 
 
+def fd_random_cells(n: int, probes, seed: int = 43, count: int = 16) -> np.ndarray:
+    """`count` cells drawn with `seed` from the cells within one cell of a probe (the support of
+    dL/dkappa: farther cells move the probes' temperatures by less than the loss resolves)."""
+    near = workloads.near_cells(n, np.asarray(probes), 1)
+    return np.sort(np.random.default_rng(seed).choice(near, size=count, replace=False))
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("icat", [1, 0])
+def test_adjoint_fd_random_cells_128(icat):
+    """The 128^3 / 200-step gradient workload as configured (checkpoints every 20 steps, 8
+    heaters, 64 probes, Cattaneo and Fourier): dL/dkappa on 16 random cells (seed 43, near the
+    probes) against central differences of the GPU forward at a relative step of 1e-6: the
+    16-cell gradient to 1e-6 relative (max |fd - g| over max |g|; measured 7e-8 Cattaneo,
+    1.7e-7 Fourier), and each cell whose gradient is at least 1e-3 of the largest to 1e-5 of
+    its own (measured up to 1.8e-6); differences taken on the bath-origin system as in
+    test_adjoint_central_fd.  The random cells' gradients span four orders of magnitude
+    (8e-15 .. 2.5e-11): a 1e-6 step in the smallest moves the loss by ~1e-24, near its
+    round-off, where the relative error of the difference itself reaches ~4e-5."""
+    n, steps, ckpt = 128, 200, 20
+    s0, m = workloads.c4(n, icattaneo=icat, t_ref=0.0)
+    probes = m["probes"]
+    kap = s0.cells.kappa.copy()
+    vol = s0.cells.volume
+    cells = fd_random_cells(n, probes)
+    with Simulation(s0, tol=1e-32) as sim:
+        def final_t(kv, pv):
+            sim.set_fields(kv, pv / vol)
+            sim.sparse_Hmatrix()
+            sim.initial_evolve()
+            sim.run(1, steps)
+            return sim.get_state()[0]
+
+        targets = final_t(m["kstar"], m["power"])[probes]
+        final_t(kap, m["power"])
+        sim.initial_evolve()
+        L, gk, gp = sim.grad(1, steps, ckpt, probes, targets)
+        fds = []
+        for c in cells:
+            hk = 1e-6 * kap[c]
+            kp, km = kap.copy(), kap.copy()
+            kp[c] += hk
+            km[c] -= hk
+            lp = _loss_np(final_t(kp, m["power"]), probes, targets)
+            lm = _loss_np(final_t(km, m["power"]), probes, targets)
+            fds.append((lp - lm) / (2 * hk))
+        fd, g = np.array(fds), gk[cells]
+        gmax = np.abs(g).max()
+        rel = np.abs(fd - g) / np.abs(g)
+        big = np.abs(g) >= 1e-3 * gmax
+        print(f"128^3 icattaneo={icat}: FD on {len(cells)} random cells, |g| "
+              f"{np.abs(g).min():.3e}..{gmax:.3e}; max |fd - g| / max |g| "
+              f"{np.abs(fd - g).max() / gmax:.3e}; per cell (|g| >= 1e-3 max, {big.sum()} "
+              f"cells) max {rel[big].max():.3e}; all cells max {rel.max():.3e}", flush=True)
+        assert np.abs(fd - g).max() <= 1e-6 * gmax
+        assert big.sum() >= 8 and rel[big].max() <= 1e-5, rel
+
+
 def test_adjoint_mixed_precision_matches_fp64():
     """FP32-inner / FP64-refined forward and adjoint solves give the FP64 gradient to the
     solver tolerance (layered-slab workload with inclusions, scaled to 24^3, 40 steps).

@@ -2,11 +2,16 @@
 
 Metric: timesteps/s of the FP32-inner / FP64-refined solver on the 512^3 log-normal-conductivity
 Cattaneo workload (20 steps, Gaussian heating, Dirichlet faces), next to the plain FP64 path at
-the same tolerance and at the reference's own tolerance (1e-32), with the max/rms relative
-temperature error of each against the FP64 run at the reference tolerance.  Also reported:
-the same grid at a 1000x time step (about 60 CG iterations per step instead of one); the
-forward+adjoint gradient at 128^3 / 200 steps with central finite differences; the mixed-
-precision forward+adjoint at 384^3 / 500 steps; and the reference Fortran timed on the host.
+the same tolerance and at the reference's own tolerance (1e-32), and the mixed path with the
+reuse of the inner step length off, with the max/rms relative temperature error of each (and
+the error on the heating signal T - 300 K) against the FP64 run at the reference tolerance.
+Also reported: the same grid at a 1000x time step (about 60 CG iterations per step instead of
+one); the forward+adjoint gradient at 128^3 / 200 steps with central finite differences; the
+mixed-precision forward+adjoint at 384^3 / 500 steps; the 16^3 workload on the persistent
+small-grid path and on the multi-kernel path; the 256^3 two-phase workload in FP64; and the
+reference Fortran timed on the host (its modules on a 128^3 sample of the 512^3 workload, and
+its program and the Fortran drop-in on the 16^3 input files; `--ref-c2` times them on the
+256^3 input files).
 
 A "step" is one time step (evolution::simulate, src/heatflow/mod_evolve.f90:45-177) of the
 whole grid.  W untimed warm-up steps run first, then the state is reset and the K timed steps
@@ -305,7 +310,8 @@ def c2_bench(n: int, steps: int, warmup: int) -> dict:
 def c2_reference(n: int, n1: int, n2: int) -> dict:
     """The reference program (serial and OpenMP) and the Fortran drop-in on the 256^3 two-phase
     workload's input files (a 16.8M-token system.in), timed by the two-run difference of n1 and
-    n2 steps at the reference's tolerance 1e-32."""
+    n2 steps at the reference's tolerance 1e-32; the drop-in by the difference of 1 and 51
+    steps, fastest of two runs each (its steps take milliseconds against ~14 s of set-up)."""
     import shutil
     import tempfile
     import oracle
@@ -318,11 +324,13 @@ def c2_reference(n: int, n1: int, n2: int) -> dict:
         t0 = time.perf_counter()
         write_inputs(s, os.path.join(d, "inputs"), ntime=n1)
         log(f"[bench] c2 inputs written in {time.perf_counter() - t0:.1f} s")
-        for label, exe, thr in (("dropin_gpu", DROPIN_EXE, 1),
-                                ("reference_openmp", oracle.REF_OMP_EXE, _host_threads()),
-                                ("reference_serial", oracle.REF_EXE, 1)):
+        for label, exe, thr, a, b, rep in (
+                ("dropin_gpu", DROPIN_EXE, 1, 1, 51, 2),
+                ("reference_openmp", oracle.REF_OMP_EXE, _host_threads(), n1, n2, 1),
+                ("reference_serial", oracle.REF_EXE, 1, n1, n2, 1)):
             if os.path.exists(exe):
-                out[label] = program_rate(s, exe, d, n1, n2, threads=thr, inputs_ready=True)
+                out[label] = program_rate(s, exe, d, a, b, threads=thr, inputs_ready=True,
+                                          repeats=rep)
     finally:
         shutil.rmtree(d, ignore_errors=True)
     return out
@@ -626,6 +634,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 16^3 leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 256^3 two-phase leg")
     ap.add_argument("--c2-n", type=int, default=256)
+    ap.add_argument("--c1-only", action="store_true",
+                    help="only run the 16^3 leg (GPU paths) and print it (profiling)")
     ap.add_argument("--ref-c2", action="store_true",
                     help="only time the reference program and the drop-in on the 256^3 "
                          "two-phase input files (minutes of CPU) and print that")
@@ -646,6 +656,9 @@ def main():
         return
     if args.ref_c2:
         print(json.dumps({"c2_reference": c2_reference(args.c2_n, 1, 2)}), flush=True)
+        return
+    if args.c1_only:
+        print(json.dumps({"c1_16": c1_bench(100, 5, cpu=False)}), flush=True)
         return
 
     if args.copy_probe:

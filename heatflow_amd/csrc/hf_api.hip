@@ -302,6 +302,7 @@ extern "C" int hf_create(hf_ctx **out, const hf_params *p) {
     hf_ctx *c = new hf_ctx();
     c->p = *p;
     hipGetDevice(&c->dev);
+    if (const char *e = getenv("HF_SMALL_MAX_CELLS")) c->small_max = (uint32_t)atol(e);
     {
         // the static scratch of hf_export_coo_noalloc on this device (once per device)
         const hipError_t e = hf::noalloc_setup(c->dev);
@@ -1167,7 +1168,7 @@ static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
 // Steps of an FP64 context on a grid of at most kSmallMaxCells cells run in one persistent
 // workgroup (k_run_small), unless debug bit 0 (keep S) or bit 6 asks for the multi-kernel path.
 static bool small_path(const hf_ctx *c) {
-    return c->p.precision == HF_PREC_FP64 && c->hx != nullptr && c->g.N <= kSmallMaxCells &&
+    return c->p.precision == HF_PREC_FP64 && c->hx != nullptr && c->g.N <= c->small_max &&
            (c->debug & (1 | 64)) == 0;
 }
 

@@ -85,6 +85,7 @@ struct hf_ctx {
     double adj_tol = 0.0;        // tolerance of the adjoint solves (0: p.tol)
     hf_stats stats{};
     // persistent small-grid steps (hf_persist.hip): run summary and per-step heat parameters
+    uint32_t small_max = hf::kSmallMaxCells;   // HF_SMALL_MAX_CELLS overrides (tuning)
     hf::SmallOut *small_out = nullptr;
     hf::HeatParams *small_hp = nullptr;
     std::vector<hf::HeatParams> small_hp_host;

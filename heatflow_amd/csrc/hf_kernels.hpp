@@ -241,7 +241,7 @@ void launch_extrap(const Dims &g, const ExtrapArgs &a, double *out, hipStream_t 
 // ---- FP64 time steps of a small grid in one persistent workgroup (hf_persist.hip) ----------
 
 constexpr int kSmallThreads = 1024;             // the one workgroup
-constexpr uint32_t kSmallMaxCells = 32768;      // grids up to this many cells take the path
+constexpr uint32_t kSmallMaxCells = 8192;       // grids up to this many cells take the path
 constexpr int32_t kSmallMaxSteps = 256;         // steps per launch (per-step heat parameters)
 
 // what a run of k_run_small leaves (written by thread 0 at its end)

@@ -215,7 +215,7 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * z-marching one (grids with a periodic axis always use the flat kernel); bit 5 starts every
  * adjoint solve from the linear extrapolation 2 lambda^{n+2} - lambda^{n+3} instead of the
  * polynomial through the last six solutions; bit 6 runs the FP64 steps of a small grid with the
- * multi-kernel path instead of the persistent one (FP64 contexts of at most 32768 cells run
+ * multi-kernel path instead of the persistent one (FP64 contexts of at most 8192 cells run
  * whole steps, up to 256 per launch, in one workgroup: no launch per CG iteration and one host
  * synchronisation per launch; its sums group differently, so the two paths agree to the
  * tolerance, not bit for bit). */

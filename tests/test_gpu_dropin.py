@@ -158,3 +158,39 @@ def test_dropin_temperature_dependent_properties(tmp_path):
         outs[name] = _outputs(str(tmp_path / name))
     assert np.array_equal(outs["hip1"][0], outs["hip0"][0])
     assert np.abs(outs["hip1"][0] - outs["ref"][0]).max() <= 1e-6
+
+
+def _heterogeneous(which):
+    """Input-file forms of the two-phase workload (32^3, 8 heaters) and of the layered-slab +
+    inclusion workload (24^3: kappa 150 / 1.4 / 30 / 0.2, 8 heaters entered as mode-1 heaters
+    sharing power_in, which gives each the same Qdens as its per-cell power)."""
+    from heatflow_amd import workloads
+    if which == "two_phase":
+        s, _ = workloads.c2(32, nheat=8)
+    else:
+        s, m = workloads.c5(24, nheat=8, nprobe=8)
+        ih = np.zeros(s.params.n, np.int64)
+        ih[m["heaters"]] = 1
+        s.cells.iheater = ih
+        s.params.power_in = float(m["power"][m["heaters"]].sum())
+        s.cells.qdens_extra = None
+    return s
+
+
+@pytest.mark.skipif(not os.path.exists(REFEXE), reason="reference program not built")
+@pytest.mark.parametrize("which", ["two_phase", "layered"])
+def test_dropin_matches_reference_program_heterogeneous(tmp_path, which):
+    """The drop-in and the reference program on the same input files of a heterogeneous grid
+    (several materials in mat.in, several heaters; written by write_inputs), 20 steps at the
+    reference's tolerance: TempDis.dat and TempDisTPD.dat agree to the print resolution."""
+    from heatflow_amd.system import write_inputs
+    s = _heterogeneous(which)
+    outs = {}
+    for name, exe in (("ref", REFEXE), ("hip", DROPIN)):
+        write_inputs(s, str(tmp_path / name / "inputs"), ntime=20)
+        run_program(exe, str(tmp_path / name))
+        outs[name] = _outputs(str(tmp_path / name))
+    assert outs["ref"][0].max() > 300.0 + 1e-6      # the heaters heated
+    for a, b in zip(outs["hip"], outs["ref"]):
+        assert a.shape == b.shape
+        assert np.abs(a - b).max() <= 1e-6

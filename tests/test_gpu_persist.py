@@ -1,4 +1,4 @@
-"""The persistent small-grid path: FP64 steps of grids up to 32768 cells run in one workgroup,
+"""The persistent small-grid path: FP64 steps of grids up to 8192 cells run in one workgroup,
 up to 256 steps per launch, with the CG decisions taken on the device (hf_persist.hip).
 Checked against the oracle, against the multi-kernel path (debug bit 6) and against itself
 one step per call."""

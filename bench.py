@@ -97,8 +97,11 @@ def measured_traffic(kernel: str, n: int):
             continue
         with open(path) as f:
             d = json.load(f)
-        if d.get("kernel") == kernel and d.get("n") == n:
-            return d["bytes_per_launch"]
+        if d.get("n") != n:
+            continue
+        entries = d.get("kernels", {d.get("kernel"): d})
+        if kernel in entries:
+            return entries[kernel]["bytes_per_launch"]
     return None
 
 

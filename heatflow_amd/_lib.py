@@ -19,7 +19,7 @@ HF_PREC_FP64, HF_PREC_FP32_IR = 0, 1
 # kernel ids of hf_kernel_times
 KERNELS = ("k_qdens", "k_rhs_res<fp64>", "k_rhs_res<fp32>", "k_cg_a<fp64>", "k_cg_b<fp64>",
            "k_cg_a<fp32>", "k_cg_b<fp32>", "k_finalize", "k_ir_res", "k_ir_step", "k_step64",
-           "k_run_small")
+           "k_run_small", "k_ir_step2")
 
 # every symbol include/heatflow_hip.h declares
 EXPORTS = (

@@ -194,11 +194,19 @@ static void make_dims(Dims &g, const hf_params &p, int cus) {
     g.zchunk4 = l0.zchunk4;
     g.ztiles4 = l0.ztiles4;
     g.zblk = l0.zblk;
+    {
+        // the two-step pass's tiles, over the pipelined kernels' z-chunks
+        const uint32_t zc = g.zlay[kZPipe].zchunk;
+        g.p2x = (g.nx + kP2X - 1) / kP2X;
+        g.p2y = (g.ny + kP2Y - 1) / kP2Y;
+        g.p2tiles = g.p2x * g.p2y * ((g.nz + zc - 1) / zc);
+        g.p2blk = (g.p2tiles + 7) / 8 * 8;
+    }
 }
 
 // the largest z-march grid over the chunkings (partial sums per block)
 static uint32_t max_zblk(const Dims &g) {
-    return std::max(g.zlay[0].zblk, std::max(g.zlay[1].zblk, g.zlay[2].zblk));
+    return std::max(std::max(g.zlay[0].zblk, std::max(g.zlay[1].zblk, g.zlay[2].zblk)), g.p2blk);
 }
 
 // the z-chunking of the CG kernels in precision T: whole columns for the 16-byte-access
@@ -244,7 +252,7 @@ static bool allocs_base(hf_ctx *c, int *rc) {
         (r = dalloc(c, &c->bvec, N)) || (r = dalloc(c, &c->sv, N)) || (r = dalloc(c, &c->qd, N)) ||
         (r = dalloc(c, &c->T[0], N)) || (r = dalloc(c, &c->T[1], N)) ||
         (r = dalloc(c, &c->sc, 2)) ||
-        (r = dalloc(c, &c->partials, (size_t)4 * std::max(c->g.nblk, max_zblk(c->g))))) {
+        (r = dalloc(c, &c->partials, (size_t)8 * std::max(c->g.nblk, max_zblk(c->g))))) {
         *rc = r;
         return false;
     }
@@ -1020,7 +1028,7 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         };
         pass();
         launch_fin_step(c->g, c->partials, c->sc, c->sci, sweeps_max, c->tiny_guess, reused,
-                        c->st);
+                        (c->debug & 256) != 0, c->st);
         TRY(check_launch(c, "k_ir_step"));
         TRY(sync_scal(c, c->sc, c->h_sc, 2));   // outer and inner scalars
         sweeps += 1;
@@ -1087,6 +1095,98 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
     c->stats.last_err = e;
     if (iter) *iter = inner_total + sweeps;
     if (err) *err = e;
+    return HF_OK;
+}
+
+// Whether the next two steps can run as one two-step pass (k_ir_step2_z): the fused mixed path
+// is carrying a residual into a step that will reuse the inner step length after a one-sweep
+// step, the grid has no periodic axis and the step is transient.  Debug bit 7 turns pairs off.
+static bool pair_ready(const hf_ctx *c) {
+    const hf_params &p = c->p;
+    return p.precision == HF_PREC_FP32_IR && !c->dyn_heat && (c->debug & (1 | 4 | 128)) == 0 &&
+           allow_short(c) && c->carry && c->reuse_ok && c->reuse_pause == 0 &&
+           c->last_sweeps == 1 && p.isteady == 0 && !c->g.per[0] && !c->g.per[1] &&
+           !c->g.per[2] && c->qb_valid;
+}
+
+// Two reusing steps in one pass.  *advanced: 2 when both stand, 1 when only the first does (the
+// next step then starts from the state), 0 when neither does (the caller runs the step alone).
+// A step that fails its test counts as a step-length reuse that missed, as in the single-step
+// path, and pauses reuse; a wrong TINY guess counts as a TINY miss.
+static int step_ir_pair(hf_ctx *c, int64_t *advanced, int64_t *iters) {
+    const hf_params &p = c->p;
+    const double N = (double)c->g.N;
+    *advanced = 0;
+    launch_carry_start(c->sc, c->sci, p.tol, p.itmax, allow_short(c), p.inner_tol, p.inner_itmax,
+                       kInnerSafety, true, c->st);
+    StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo, c->tiny_guess ? 1 : 0, 0,
+                  rc_mask(c)};
+    const float *rin = c->carry_r;
+    float *rout = other_rvec(c, rin);
+    double *x1 = c->T[2], *x2 = c->S;
+    // k_ir_step2: read T^n, T^{n-1}, r^0 (FP32), Qdens + B, diag, 3 couplings (rho c_p, tau' rho
+    // c_p per cell unless uniform); write x'_n, x'_{n+1}, r^0_{n+2} (FP32)
+    long sl = tbeg(c, HF_K_IR_STEP2, N * (8 * 7 + 4 + rc_bytes(c) + 8 + 8 + 4));
+    launch_ir_step2(c->g, sp, c->T[0], c->T[1], rin, c->sci, c->rhoc, c->cvec, c->qb, c->d,
+                    c->cx, c->cy, c->cz, x1, x2, rout, c->sc, c->partials, c->st);
+    tend(c, sl);
+    launch_fin_step2(c->g, c->partials, c->sc, c->sci, c->tiny_guess, (c->debug & 256) != 0,
+                     p.tol, p.inner_tol, kInnerSafety, c->st);
+    TRY(check_launch(c, "k_ir_step2"));
+    TRY(sync_scal(c, c->sc, c->h_sc, 2));
+    const int32_t outcome = c->h_sc->pair_ok[0];
+    c->stats.alpha_reuses += outcome == 2 ? 2 : 1;
+    if (outcome == 0) {
+        // step n runs again alone from the state (its test failed or a NaN guard tripped)
+        c->carry = false;
+        c->stats.reuse_misses += 1;
+        c->reuse_pause = kReusePause << std::min<int64_t>(c->reuse_miss_run, 6);
+        c->reuse_miss_run += 1;
+        c->reuse_ok = false;
+        return HF_OK;
+    }
+    double *old_tp = c->T[0], *old_tpp = c->T[1];
+    if (outcome == 2) {
+        // T^{n+2} = x'_{n+1}, T^{n+1} = x'_n; the old state buffers become the spares
+        c->T[0] = x2;
+        c->T[1] = x1;
+        c->T[2] = old_tpp;
+        c->S = old_tp;
+        const bool t2 = c->h_sc->pair_ok[1] != 0;
+        c->carry = t2 == c->tiny_guess;
+        if (!c->carry) c->stats.tiny_misses += 1;
+        c->tiny_guess = t2;
+        c->carry_r = rout;
+        c->reuse_miss_run = 0;
+        c->reuse_ok = true;
+    } else {
+        // T^{n+1} = x'_n; step n+1 starts from the state
+        c->T[0] = x1;
+        c->T[1] = old_tp;
+        c->T[2] = old_tpp;
+        c->S = x2;
+        const bool t1 = (c->h_sc->flags & FLAG_TINY_NEXT) != 0;
+        if (t1 != c->tiny_guess) {
+            c->stats.tiny_misses += 1;
+            c->reuse_ok = true;
+        } else {
+            // step n+1 missed its test (or a NaN guard tripped: its own run reports it)
+            c->stats.reuse_misses += 1;
+            c->reuse_pause = kReusePause << std::min<int64_t>(c->reuse_miss_run, 6);
+            c->reuse_miss_run += 1;
+            c->reuse_ok = false;
+        }
+        c->carry = false;
+        c->tiny_guess = t1;
+    }
+    c->last_sweeps = 1;
+    const double e =
+        c->h_sc->bnorm2 > 0 ? std::sqrt(c->h_sc->outer_rr / c->h_sc->bnorm2) : 0.0;
+    c->stats.steps += outcome;
+    c->stats.outer_iters += outcome;
+    c->stats.last_err = e;
+    *advanced = outcome;
+    *iters = outcome;
     return HF_OK;
 }
 
@@ -1297,14 +1397,26 @@ extern "C" int hf_run(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t *iters_
         return run_small(c, itime0, nsteps, iters_total, nullptr, nullptr);
     }
     int64_t tot = 0;
-    for (int64_t k = 0; k < nsteps; ++k) {
+    for (int64_t k = 0; k < nsteps;) {
         int64_t it = 0;
+        if (k + 1 < nsteps && c->assembled && c->has_state && !c->setup_flags &&
+            pair_ready(c)) {
+            int64_t adv = 0;
+            int rc = step_ir_pair(c, &adv, &it);
+            if (rc != HF_OK) return rc;
+            if (adv > 0) {
+                k += adv;
+                tot += it;
+                continue;
+            }
+        }
         // NaN in x of step k is seen at the next step's first synchronisation (the flag is
         // sticky within a step) or at the final check
         int rc = step_common(c, itime0 + k, &it, nullptr, k + 1 == nsteps);
         if (rc != HF_OK) return rc;
         if (c->h_sc->flags & FLAG_NAN_X) return flags_to_rc(c, FLAG_NAN_X);
         tot += it;
+        k += 1;
     }
     if (iters_total) *iters_total = tot;
     return HF_OK;

@@ -62,7 +62,14 @@ struct Dims {
     struct ZLayout {
         uint32_t zchunk, ztiles, zchunk2, ztiles2, zchunk4, ztiles4, zblk;
     } zlay[3];
+    // the two-step pass (k_ir_step2_z): tiles of kP2X x kP2Y columns, p2x x p2y a plane, the
+    // pipelined kernels' z-chunks; p2tiles tiles, p2blk blocks (a multiple of 8)
+    uint32_t p2x, p2y, p2tiles, p2blk;
 };
+// 128 x 8 (1024 threads, one block a CU by its LDS): of 16x16, 32x8, 32x16, 64x8, 64x16,
+// 128x8 and 256x4 tiles timed on the 512^3 workload, the fastest -- the wide rows keep the
+// ring's column cells (one cache line each) few
+constexpr uint32_t kP2X = 128, kP2Y = 8;
 
 // z-chunkings (Dims::zlay): kZDefault; kZPipe for the software-pipelined kernels
 // (k_rhs_res_z, k_ir_step_z), which favour fewer, longer chunks; kZColumn, whole columns (512
@@ -117,6 +124,12 @@ struct Scal {
     // linbcg's own stop rule (hf_solve): ||b||^2 unscaled and the last iteration's weighted
     // residual norm^2 (itol 1: ||r||^2, itol 2: ||r / diag H||^2)
     double wb2, wrr;
+    // two reusing steps in one pass (k_ir_step2_z, k_fin_step2): flags of the pass, the sums
+    // (per step: ||r'||^2, r^0 . r', ||b^_next||^2, ||r^0_next||^2), and whether each step met
+    // its tests (pair_ok[0]: step n; pair_ok[1]: step n+1, given step n)
+    int32_t flags2;
+    int32_t pair_ok[2];
+    double pair_v[8];
 };
 
 // The solution y of a CG solve on A^ y = r0, as its consumers read it.  The SpMV pass of
@@ -143,7 +156,10 @@ __device__ __forceinline__ CgY<T> cg_y(const Scal *sc, const T *y, const T *r0) 
 
 enum : int32_t {
     FLAG_NAN_B = 1, FLAG_NAN_Q = 2, FLAG_NAN_QDENS = 4, FLAG_NAN_SCAT = 8, FLAG_NAN_S = 16,
-    FLAG_NAN_X = 32, FLAG_TINY_NEXT = 64, FLAG_BREAKDOWN = 128
+    FLAG_NAN_X = 32, FLAG_TINY_NEXT = 64, FLAG_BREAKDOWN = 128,
+    // the second step of a two-step pass (k_ir_step2_z; Scal::flags2): its x, the TINY test of
+    // the step after it, the NaN guards of that step's S
+    FLAG_NAN_X2 = 256, FLAG_TINY_NEXT2 = 512, FLAG_NAN_SCAT2 = 1024, FLAG_NAN_S2 = 2048
 };
 
 __device__ __forceinline__ uint32_t logical_block(const Dims &g) {

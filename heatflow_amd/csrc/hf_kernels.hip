@@ -613,17 +613,18 @@ __global__ void k_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax,
 // inner->alpha for the next reuse, so the reused step length is never more than one step old.
 __global__ void __launch_bounds__(kBlock)
 k_fin_step(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int64_t sweeps_max,
-           int32_t guess, int32_t reused) {
+           int32_t guess, int32_t reused, int32_t freeze) {
     double v[4];
     sum_partials<4>(partials, nb, v);
     if (threadIdx.x != 0) return;
-    if (reused) {
+    if (reused && !freeze) {
         const double r0r0 = sc->rr_start, den = r0r0 - v[3];
         if (den > 0.0 && isfinite(den)) inner->alpha = inner->alpha * (r0r0 / den);
     }
     const double prev = sc->outer_rr;
-    // a sweep that reused alpha passes only within the inner target (reuse_thresh2)
-    const double thr = reused ? sc->reuse_thresh2 : sc->thresh2;
+    // a sweep that reused alpha passes only within the inner target (reuse_thresh2), and within
+    // the outer tolerance like any sweep
+    const double thr = reused ? fmin(sc->reuse_thresh2, sc->thresh2) : sc->thresh2;
     sc->outer_rr = v[0];
     sc->iter += 1;
     const bool stalled = sc->iter > 1 && v[0] > 0.25 * prev;
@@ -632,6 +633,68 @@ k_fin_step(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int64_t s
     sc->next_bnorm2 = v[1];
     sc->next_rr[guess ? 1 : 0] = v[2];
     sc->next_rr[guess ? 0 : 1] = __builtin_nan("");
+}
+
+// After a two-step pass (k_ir_step2_z): the sums of both steps, their tests and the outcome in
+// sc->pair_ok[0]: 2 both steps stand, 1 only step n (step n+1's test failed, its TINY outcome
+// differed from the guess, or a NaN guard of step n+1 tripped), 0 neither (step n's test
+// failed or a NaN guard of step n tripped: the host runs step n again, alone).  Step n meets
+// the test of a sweep that reused alpha (k_fin_step: half the inner target and the outer
+// tolerance); step n+1 the same test with its targets formed from the pass's own sums, as
+// k_carry_start forms them from the carried ones.  The step length of the last step that stands
+// is left in inner->alpha for the next reuse (unless `freeze`).  pair_ok[1]: the TINY outcome of
+// the step after the pair (carry the pass's residual if it equals the guess).
+__global__ void __launch_bounds__(kBlock)
+k_fin_step2(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int32_t guess,
+            int32_t freeze, double tol, double inner_tol, double safety) {
+    double v[8];
+    sum_partials<8>(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 8; ++k) sc->pair_v[k] = v[k];
+    const int32_t nan0 = (sc->flags & FLAG_NAN_X) | (sc->flags_next & (FLAG_NAN_SCAT | FLAG_NAN_S));
+    const int32_t nan1 = sc->flags2 & (FLAG_NAN_X2 | FLAG_NAN_SCAT2 | FLAG_NAN_S2);
+    const bool t1 = (sc->flags & FLAG_TINY_NEXT) != 0;
+    const bool t2 = (sc->flags2 & FLAG_TINY_NEXT2) != 0;
+    // step n: ||r'||^2 = v0, ||b^_{n+1}||^2 = v1, ||r^0_{n+1}||^2 = v2, r^0_n . r' = v3
+    const bool ok0 = !nan0 && v[0] <= fmin(sc->reuse_thresh2, sc->thresh2);
+    // step n+1: v4 .. v7 likewise, its targets from v1, v2
+    const double thresh2_1 = tol * tol * v[1];
+    const double t_rel = inner_tol * inner_tol * v[2];
+    const double t_abs = safety * safety * tol * tol * v[1];
+    const double reuse2_1 = 0.25 * (t_rel > t_abs ? t_rel : t_abs);
+    const bool ok1 = !nan1 && (t1 == (guess != 0)) && v[4] <= fmin(reuse2_1, thresh2_1);
+    const int32_t outcome = ok0 ? (ok1 ? 2 : 1) : 0;
+    sc->pair_ok[0] = outcome;
+    sc->pair_ok[1] = t2 ? 1 : 0;
+    if (!freeze) {
+        double r0r0 = 0.0, den = 0.0;
+        if (outcome == 2) {
+            r0r0 = v[2];
+            den = r0r0 - v[7];
+        } else if (outcome == 1) {
+            r0r0 = sc->rr_start;
+            den = r0r0 - v[3];
+        }
+        if (outcome > 0 && den > 0.0 && isfinite(den)) inner->alpha = inner->alpha * (r0r0 / den);
+    }
+    if (outcome == 2) {
+        sc->bnorm2 = v[1];
+        sc->outer_rr = v[4];
+        sc->next_bnorm2 = v[5];
+        sc->next_rr[guess ? 1 : 0] = v[6];
+        sc->next_rr[guess ? 0 : 1] = __builtin_nan("");
+        // the TINY outcome of the step after the pair, for its step_begin
+        sc->flags = (sc->flags & ~FLAG_TINY_NEXT) | (t2 ? FLAG_TINY_NEXT : 0);
+        sc->flags_next = 0;
+    } else if (outcome == 1) {
+        sc->outer_rr = v[0];
+        sc->flags_next = 0;
+    } else {
+        // nothing stands: step n runs again from the state with its own TINY outcome
+        sc->flags = (sc->flags & FLAG_NAN_B) | (sc->tiny ? FLAG_TINY_NEXT : 0);
+        sc->flags_next = 0;
+    }
+    sc->flags2 = 0;
 }
 
 // After the fused FP64 step: the next step's norms ||b^_next||^2, ||r^0_next||^2 (TINY case
@@ -800,10 +863,18 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
     HF_FIN(k_fin_ir, g.zblk, st, partials, sc, sweeps_max);
 }
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, Scal *inner,
-                     int64_t sweeps_max, bool tiny_guess, bool reused, hipStream_t st) {
+                     int64_t sweeps_max, bool tiny_guess, bool reused, bool freeze,
+                     hipStream_t st) {
     // after k_ir_step_z (pipelined z-chunking)
     HF_FIN(k_fin_step, g.zlay[kZPipe].zblk, st, partials, sc, inner, sweeps_max,
-           (int32_t)tiny_guess, (int32_t)reused);
+           (int32_t)tiny_guess, (int32_t)reused, (int32_t)freeze);
+}
+void launch_fin_step2(const Dims &g, const double *partials, Scal *sc, Scal *inner,
+                      bool tiny_guess, bool freeze, double tol, double inner_tol, double safety,
+                      hipStream_t st) {
+    // after k_ir_step2_z (its own tiles)
+    HF_FIN(k_fin_step2, g.p2blk, st, partials, sc, inner, (int32_t)tiny_guess,
+           (int32_t)freeze, tol, inner_tol, safety);
 }
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st) {

@@ -211,7 +211,18 @@ void launch_fin_ir(const Dims &g, const double *partials, Scal *sc, int64_t swee
 // inner solve's alpha (launch_carry_start), passes only within the inner target, and updates
 // inner->alpha to this step's own step length
 void launch_fin_step(const Dims &g, const double *partials, Scal *sc, Scal *inner,
-                     int64_t sweeps_max, bool tiny_guess, bool reused, hipStream_t st);
+                     int64_t sweeps_max, bool tiny_guess, bool reused, bool freeze,
+                     hipStream_t st);
+// two reusing steps of the mixed path in one pass, and its finishing kernel (hf_stencil.hip,
+// hf_kernels.hip)
+void launch_ir_step2(const Dims &g, const StepParams &sp, const double *tp, const double *tpp,
+                     const float *r0, const Scal *inner, const double *rhoc, const double *cvec,
+                     const double *qb, const double *d, const double *cx, const double *cy,
+                     const double *cz, double *x1out, double *x2out, float *rnext, Scal *sc,
+                     double *partials, hipStream_t st);
+void launch_fin_step2(const Dims &g, const double *partials, Scal *sc, Scal *inner,
+                      bool tiny_guess, bool freeze, double tol, double inner_tol, double safety,
+                      hipStream_t st);
 // after launch_step64: the next step's norms for the TINY case `tiny_guess`
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st);

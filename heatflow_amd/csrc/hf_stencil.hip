@@ -1331,6 +1331,347 @@ k_cg_b1_zw(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *_
 
 // ---- launchers -----------------------------------------------------------------------------
 
+// ---- two reusing steps in one z-march (k_ir_step2_z) ---------------------------------------
+//
+// In the carried regime where the inner step length is reused, a mixed step is one stencil pass
+// of per-cell arithmetic: x' = x0 + s (alpha r^0), its FP64 residual, and the next step's r^0.
+// Two such steps are two passes over the same operator, Qdens + B and state.  k_ir_step2_z does
+// both in one z-march, so those arrays are read once for two steps (80 instead of 144
+// algorithmic bytes a cell):
+//   step n     (as k_ir_step_z): x'_n, rho_n = -s (S_n - H x'_n), r^0_{n+1} from x1_n
+//   step n+1   x'_{n+1} = x1_n (+TINY) + s (alpha r^0_{n+1}), rho_{n+1}, r^0_{n+2} from x1_{n+1}
+// with the same alpha for both steps and the TINY outcome of both next steps guessed as the
+// last one seen.  Every per-cell operation, its order and its rounding (r^0 is rounded to FP32
+// as the single-step pass stores it) are those of two k_ir_step_z passes, so a pair whose tests
+// pass gives the bits of two single steps that reuse the same alpha.  The residual sums run over
+// the same cells grouped by this kernel's tiles: they agree with the single passes' to rounding,
+// as does the step length formed from them for the next pair.
+//
+// Step n+1 at a cell needs r^0_{n+1} of its neighbours, which needs x1_n two cells away: the
+// (x'_n, x1_n) plane ("a") is formed on the tile and a two-cell ring, r^0_{n+1} and
+// (x'_{n+1}, x1_{n+1}) ("b") on the tile and a one-cell ring, and step n+1 runs one plane
+// behind step n.  Trip q of the march forms a(q), then (barrier) step n on plane q-1 for the
+// ring positions (with its residual test on the tile only) and step n+1 on plane q-2 for the
+// tile; three a-planes and two b-planes rotate in LDS.  Each thread takes its own tile cell as
+// its first position in every stage, so what one stage hands the next for that cell stays in
+// its registers.  Non-periodic grids only.
+//
+// Outputs: x'_n -> x1out, x'_{n+1} -> x2out, r^0_{n+2} (FP32) -> rnext; partials 0-3 of step n,
+// 4-7 of step n+1, each ||rho||^2, ||b^_next||^2, ||r^0_next||^2, r^0 . rho; flags of step n
+// (NAN_X, TINY_NEXT, NAN_SCAT, NAN_S of step n+1's S) in Scal::flags, of step n+1 in flags2.
+constexpr int kAW = (int)kP2X + 4, kAH = (int)kP2Y + 4, kAN = kAW * kAH;
+constexpr int kBW = (int)kP2X + 2, kBH = (int)kP2Y + 2, kBN = kBW * kBH;
+constexpr int kTT = (int)(kP2X * kP2Y);
+static_assert(kAN - kTT <= kTT && kBN - kTT <= kTT, "at most two positions a thread");
+
+struct Step2Lds {
+    D2 a[3][kAN];
+    D2 b[2][kBN];
+};
+
+// the k-th position (k >= 0) of a region of W x H around the tile (ring width R): k < T is tile
+// cell (k % TX, k / TX) at (k % TX + R, k / TX + R); then the ring, row by row for its full
+// rows (the R top and R bottom rows), then the R left and R right columns of the tile rows
+template <int R>
+__device__ __forceinline__ void region_pos(int k, int &px, int &py) {
+    constexpr int TX = (int)kP2X, TY = (int)kP2Y, W = TX + 2 * R;
+    if (k < kTT) {
+        px = k % TX + R;
+        py = k / TX + R;
+        return;
+    }
+    k -= kTT;
+    if (k < 2 * R * W) {
+        const int row = k / W;
+        px = k - row * W;
+        py = row < R ? row : TY + row;   // rows 0..R-1 and TY+R..TY+2R-1
+        return;
+    }
+    k -= 2 * R * W;
+    const int col = k / TY;              // columns 0..R-1, then TX+R..TX+2R-1
+    py = k - col * TY + R;
+    px = col < R ? col : TX + col;
+}
+
+template <bool Cattaneo, bool UniformRc>
+__global__ void __launch_bounds__(kP2X * kP2Y) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
+             const double *__restrict__ tpp, const float *__restrict__ r0, const Scal *inner,
+             const double *__restrict__ rhoc, const double *__restrict__ cvec,
+             const double *__restrict__ qb, const double *__restrict__ d,
+             const double *__restrict__ cx, const double *__restrict__ cy,
+             const double *__restrict__ cz, double *__restrict__ x1out,
+             double *__restrict__ x2out, float *__restrict__ rnext, Scal *sc,
+             double *partials) {
+    __shared__ Step2Lds L;
+    const bool tiny = sc->tiny != 0;           // step n's TINY outcome (known)
+    const bool guess = sp_.tiny_guess != 0;    // steps n+1 and n+2, guessed
+    const float ysc = (float)inner->alpha;     // the reused step length (y = alpha r^0)
+    const double rc_u = UniformRc ? rhoc[0] : 0.0;
+    const double cv_u = UniformRc && Cattaneo ? cvec[0] : 0.0;
+    const double it = sp_.inverse_time;
+    // this block's tile (XCD-aware as the other z-marching kernels) and z-chunk
+    const uint32_t lt = logical_tile(g.p2tiles);
+    const uint32_t per_plane = g.p2x * g.p2y;
+    const uint32_t zc = lt / per_plane, rem = lt - zc * per_plane;
+    const uint32_t tby = rem / g.p2x, tbx = rem - tby * g.p2x;
+    const uint32_t zchunk = g.zlay[kZPipe].zchunk;
+    const uint32_t z0 = zc * zchunk, z1 = z0 + zchunk < g.nz ? z0 + zchunk : g.nz;
+    const bool zvalid = lt < g.p2tiles && z0 < g.nz;
+    const int t = (int)threadIdx.x;
+    const uint32_t lx = (uint32_t)t % kP2X, ly = (uint32_t)t / kP2X;
+    const uint32_t ox = tbx * kP2X + lx, oy = tby * kP2Y + ly;
+    const bool active = zvalid && ox < g.nx && oy < g.ny;
+    const uint32_t ocol = ox + g.nx * oy;
+    const int64_t gx0 = (int64_t)tbx * kP2X, gy0 = (int64_t)tby * kP2Y;
+    const int nq_a = t < kAN - kTT ? 2 : 1;   // a-positions of this thread
+    const int nq_b = t < kBN - kTT ? 2 : 1;   // b-positions
+    auto rcv = [&](uint32_t J) { return UniformRc ? rc_u : rhoc[J & sp_.rc_mask]; };
+    auto cvv = [&](uint32_t J) {
+        return Cattaneo ? (UniformRc ? cv_u : cvec[J & sp_.rc_mask]) : 0.0;
+    };
+    // grid column of a region position (R: ring width), or false outside the grid
+    auto cell_of = [&](int px, int py, int R, uint32_t &col) -> bool {
+        const int64_t x = gx0 + px - R, y = gy0 + py - R;
+        if (!zvalid || x < 0 || y < 0 || x >= (int64_t)g.nx || y >= (int64_t)g.ny) return false;
+        col = (uint32_t)x + g.nx * (uint32_t)y;
+        return true;
+    };
+    auto mod3 = [](int v) { return ((v % 3) + 3) % 3; };
+    auto mod2 = [](int v) { return v & 1; };
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t fl = 0, fl2 = 0;
+    // own-cell values handed from stage to stage (plane noted)
+    double own_tp = 0.0, own_tpp = 0.0;   // a-stage of plane q -> step n on plane q
+    float own_r = 0.0f;
+    double hS1 = 0.0, hxn = 0.0;          // step n on plane p -> step n+1 on plane p: S_{n+1}, x'_n
+    float hfn = 0.0f;                     // r^0_{n+1} (FP32)
+    D2 b_m = 0.0, b_0 = 0.0, b_p = 0.0;   // own b of planes p-1, p, p+1 (step n+1 on plane p)
+    const int zq0 = (int)z0 - 2, zq1 = (int)z1 + 1;
+    for (int q = zq0; q <= zq1; ++q) {
+        // ---- a(q): x'_n and x1_n on the tile and a two-cell ring
+        D2 *A = L.a[mod3(q)];
+        const bool zin = q >= 0 && q < (int)g.nz;
+        double n_tp = 0.0, n_tpp = 0.0;
+        float n_r = 0.0f;
+        for (int k = 0; k < nq_a; ++k) {
+            int px, py;
+            region_pos<2>(t + k * kTT, px, py);
+            uint32_t col;
+            D2 v = 0.0;
+            if (zin && cell_of(px, py, 2, col)) {
+                const uint32_t J = col + g.nxny * (uint32_t)q;
+                const double a_tp = tp[J], a_tpp = tpp[J], a_d = d[J];
+                const float a_r = r0[J];
+                double base = a_tp + (a_tp - a_tpp);   // mod_evolve.f90:140
+                if (tiny) base = base + kTiny;         // :141
+                const double xn = base + (1.0 / sqrt(-a_d)) * (double)(ysc * a_r);
+                v = D2(xn, xn + (xn - a_tp));
+                if (k == 0) {
+                    n_tp = a_tp;
+                    n_tpp = a_tpp;
+                    n_r = a_r;
+                }
+            }
+            A[py * kAW + px] = v;
+        }
+        __syncthreads();   // a(q) staged
+        // ---- step n on plane p = q - 1 (ring positions; its test on the tile only)
+        const int p = q - 1;
+        const bool step_n = p >= (int)z0 - 1 && p <= (int)z1 && p >= 0 && p < (int)g.nz;
+        // the tile's own planes: its residual test, sums and flags (the planes z0 - 1 and z1
+        // are the neighbouring chunks')
+        const bool own_plane = p >= (int)z0 && p < (int)z1;
+        const D2 *Am = L.a[mod3(p - 1)], *A0 = L.a[mod3(p)], *Ap = L.a[mod3(p + 1)];
+        D2 *B = L.b[mod2(p)];
+        D2 own_b = 0.0;
+        double own_S1 = 0.0, own_xn = 0.0;
+        float own_fn = 0.0f;
+        for (int k = 0; k < nq_b; ++k) {
+            int px, py;
+            region_pos<1>(t + k * kTT, px, py);
+            uint32_t col;
+            D2 v = 0.0;
+            if (step_n && cell_of(px, py, 1, col)) {
+                const uint32_t J = col + g.nxny * (uint32_t)p;
+                const uint32_t x = (uint32_t)(gx0 + px - 1), y = (uint32_t)(gy0 + py - 1);
+                const bool xm_in = x > 0, xp_in = x + 1 < g.nx;
+                const bool ym_in = y > 0, yp_in = y + 1 < g.ny;
+                const bool zm_in = p > 0, zp_in = p + 1 < (int)g.nz;
+                const int ai = (py + 1) * kAW + (px + 1);   // this position in an a-plane
+                const D2 xi = A0[ai];
+                const double di = d[J], cxi = cx[J], cyi = cy[J], czi = cz[J];
+                const double cxm = cx[xm_in ? J - 1 : J], cym = cy[ym_in ? J - g.nx : J];
+                const double czm = zm_in ? cz[J - g.nxny] : 0.0;
+                const double qbi = qb[J];
+                const double tpi = k == 0 ? own_tp : tp[J];
+                const double rci = rcv(J), cvi = cvv(J);
+                // H x (pick: x' or x1) in the reference's serial order
+                auto hsum = [&](auto pick) {
+                    double h = 0.0;
+                    h = h + czm * (zm_in ? pick(Am[ai]) : 0.0);
+                    h = h + cym * (ym_in ? pick(A0[ai - kAW]) : 0.0);
+                    h = h + cxm * (xm_in ? pick(A0[ai - 1]) : 0.0);
+                    h = h + di * pick(xi);
+                    h = h + cxi * (xp_in ? pick(A0[ai + 1]) : 0.0);
+                    h = h + cyi * (yp_in ? pick(A0[ai + kAW]) : 0.0);
+                    h = h + czi * (zp_in ? pick(Ap[ai]) : 0.0);
+                    return h;
+                };
+                const double s = 1.0 / sqrt(-di);
+                // S_{n+1} from (x'_n, T^n) (mod_evolve.f90:112-119, S_catS)
+                double S1 = -(it * xi.a * rci) - qbi;
+                if (Cattaneo) {
+                    const double scat = (tpi - 2.0 * xi.a) * cvi;
+                    if (k == 0 && own_plane && isnan(scat)) fl |= FLAG_NAN_SCAT;
+                    S1 = S1 + scat;
+                }
+                double ra = -s * (S1 - hsum([](const D2 &w) { return w.b; }));
+                if (guess) {
+                    double h1s = di;
+                    h1s = h1s + czm;
+                    h1s = h1s + (ym_in ? cym : 0.0);
+                    h1s = h1s + (xm_in ? cxm : 0.0);
+                    h1s = h1s + cxi;
+                    h1s = h1s + cyi;
+                    h1s = h1s + czi;
+                    ra = ra + kTiny * s * h1s;
+                }
+                const float fn = (float)ra;
+                // step n+1's x'_{n+1} = x1_n (+TINY) + s (alpha r^0_{n+1}), x1_{n+1}
+                double base = xi.a + (xi.a - tpi);
+                if (guess) base = base + kTiny;
+                const double xn1 = base + s * (double)(ysc * fn);
+                v = D2(xn1, xn1 + (xn1 - xi.a));
+                if (k == 0 && own_plane) {
+                    // the tile cell: step n's residual test and sums
+                    double Sn = -(it * tpi * rci) - qbi;
+                    if (Cattaneo) Sn = Sn + (own_tpp - 2.0 * tpi) * cvi;
+                    if (isnan(S1)) fl |= FLAG_NAN_S;
+                    const float rf = (float)(-s * (Sn - hsum([](const D2 &w) { return w.a; })));
+                    const double bh = s * S1;
+                    acc[0] += (double)rf * (double)rf;
+                    acc[1] += bh * bh;
+                    acc[2] += (double)fn * (double)fn;
+                    acc[3] += (double)own_r * (double)rf;
+                    if (isnan(xi.a)) fl |= FLAG_NAN_X;
+                    if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;
+                }
+                if (k == 0) {
+                    own_b = v;
+                    own_S1 = S1;
+                    own_xn = xi.a;
+                    own_fn = fn;
+                }
+            }
+            B[py * kBW + px] = v;
+        }
+        own_tp = n_tp;
+        own_tpp = n_tpp;
+        own_r = n_r;
+        // ---- step n+1 on plane p2 = q - 2 (tile cells): b(p2) in LDS (the previous trip), the
+        // own b of planes p2 - 1 and p2 + 1 in registers
+        const int p2 = q - 2;
+        b_m = b_0;
+        b_0 = b_p;
+        b_p = own_b;
+        if (active && p2 >= (int)z0 && p2 < (int)z1) {
+            const D2 *B0 = L.b[mod2(p2)];
+            const uint32_t J = ocol + g.nxny * (uint32_t)p2;
+            const bool xm_in = ox > 0, xp_in = ox + 1 < g.nx;
+            const bool ym_in = oy > 0, yp_in = oy + 1 < g.ny;
+            const bool zm_in = p2 > 0, zp_in = p2 + 1 < (int)g.nz;
+            const int bi = ((int)ly + 1) * kBW + ((int)lx + 1);
+            const D2 xi = B0[bi];
+            const double di = d[J], cxi = cx[J], cyi = cy[J], czi = cz[J];
+            const double cxm = cx[xm_in ? J - 1 : J], cym = cy[ym_in ? J - g.nx : J];
+            const double czm = zm_in ? cz[J - g.nxny] : 0.0;
+            const double qbi = qb[J];
+            const double rci = rcv(J), cvi = cvv(J);
+            auto hsum = [&](auto pick) {
+                double h = 0.0;
+                h = h + czm * (zm_in ? pick(b_m) : 0.0);
+                h = h + cym * (ym_in ? pick(B0[bi - kBW]) : 0.0);
+                h = h + cxm * (xm_in ? pick(B0[bi - 1]) : 0.0);
+                h = h + di * pick(xi);
+                h = h + cxi * (xp_in ? pick(B0[bi + 1]) : 0.0);
+                h = h + cyi * (yp_in ? pick(B0[bi + kBW]) : 0.0);
+                h = h + czi * (zp_in ? pick(b_p) : 0.0);
+                return h;
+            };
+            const double s = 1.0 / sqrt(-di);
+            // S_{n+2} from (x'_{n+1}, x'_n)
+            double S2 = -(it * xi.a * rci) - qbi;
+            if (Cattaneo) {
+                const double scat = (hxn - 2.0 * xi.a) * cvi;
+                if (isnan(scat)) fl2 |= FLAG_NAN_SCAT2;
+                S2 = S2 + scat;
+            }
+            if (isnan(S2)) fl2 |= FLAG_NAN_S2;
+            const float rf = (float)(-s * (hS1 - hsum([](const D2 &w) { return w.a; })));
+            double ra = -s * (S2 - hsum([](const D2 &w) { return w.b; }));
+            if (guess) {
+                double h1s = di;
+                h1s = h1s + czm;
+                h1s = h1s + (ym_in ? cym : 0.0);
+                h1s = h1s + (xm_in ? cxm : 0.0);
+                h1s = h1s + cxi;
+                h1s = h1s + cyi;
+                h1s = h1s + czi;
+                ra = ra + kTiny * s * h1s;
+            }
+            const float fn = (float)ra;
+            stnt(&x1out[J], hxn);
+            stnt(&x2out[J], xi.a);
+            stnt(&rnext[J], fn);
+            const double bh = s * S2;
+            acc[4] += (double)rf * (double)rf;
+            acc[5] += bh * bh;
+            acc[6] += (double)fn * (double)fn;
+            acc[7] += (double)hfn * (double)rf;
+            if (isnan(xi.a)) fl2 |= FLAG_NAN_X2;
+            if (xi.b - xi.a < kTiny) fl2 |= FLAG_TINY_NEXT2;
+        }
+        hS1 = own_S1;
+        hxn = own_xn;
+        hfn = own_fn;
+        __syncthreads();   // a and b planes of this trip were read before the next writes
+    }
+    block_flags(fl & (FLAG_NAN_X | FLAG_TINY_NEXT), &sc->flags);
+    block_flags(fl & (FLAG_NAN_SCAT | FLAG_NAN_S), &sc->flags_next);
+    block_flags(fl2, &sc->flags2);
+    // partials: components 0-3 (step n), then 4-7 (step n+1)
+    block_partials<8>(acc, partials);
+}
+
+void launch_ir_step2(const Dims &g0, const StepParams &sp, const double *tp, const double *tpp,
+                     const float *r0, const Scal *inner, const double *rhoc, const double *cvec,
+                     const double *qb, const double *d, const double *cx, const double *cy,
+                     const double *cz, double *x1out, double *x2out, float *rnext, Scal *sc,
+                     double *partials, hipStream_t st) {
+    const Dims g = zdims(g0, kZPipe);
+    const bool uni = sp.rc_mask == 0;
+    if (sp.icattaneo == 1) {
+        if (uni)
+            hipLaunchKernelGGL((k_ir_step2_z<true, true>), dim3(g.p2blk), dim3(kP2X * kP2Y), 0, st,
+                               g, sp, tp, tpp, r0, inner, rhoc, cvec, qb, d, cx, cy, cz, x1out,
+                               x2out, rnext, sc, partials);
+        else
+            hipLaunchKernelGGL((k_ir_step2_z<true, false>), dim3(g.p2blk), dim3(kP2X * kP2Y), 0,
+                               st, g, sp, tp, tpp, r0, inner, rhoc, cvec, qb, d, cx, cy, cz,
+                               x1out, x2out, rnext, sc, partials);
+    } else {
+        if (uni)
+            hipLaunchKernelGGL((k_ir_step2_z<false, true>), dim3(g.p2blk), dim3(kP2X * kP2Y), 0,
+                               st, g, sp, tp, tpp, r0, inner, rhoc, cvec, qb, d, cx, cy, cz,
+                               x1out, x2out, rnext, sc, partials);
+        else
+            hipLaunchKernelGGL((k_ir_step2_z<false, false>), dim3(g.p2blk), dim3(kP2X * kP2Y), 0,
+                               st, g, sp, tp, tpp, r0, inner, rhoc, cvec, qb, d, cx, cy, cz,
+                               x1out, x2out, rnext, sc, partials);
+    }
+}
+
 #define HF_ZLAUNCH(kern, g, st, ...) \
     hipLaunchKernelGGL(kern, dim3((g).zblk), dim3(kZTX * kZTY), 0, st, __VA_ARGS__)
 

@@ -218,7 +218,11 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * multi-kernel path instead of the persistent one (FP64 contexts of at most 8192 cells run
  * whole steps, up to 256 per launch, in one workgroup: no launch per CG iteration and one host
  * synchronisation per launch; its sums group differently, so the two paths agree to the
- * tolerance, not bit for bit). */
+ * tolerance, not bit for bit); bit 7 turns off the two-step pass of mixed mode (where two
+ * consecutive steps would both reuse the inner step length, one z-march does both, reading the
+ * operator, Qdens + B and the state once; a step whose test misses is redone alone); bit 8
+ * keeps the reused step length fixed instead of updating it after each reusing step (then
+ * the two-step pass and single steps give the same bits). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */
@@ -250,7 +254,7 @@ int hf_reset_stats(hf_ctx *ctx);
  * arrays each kernel touches, every array element counted once).  Reading resets the pool. */
 enum {
     HF_K_QDENS = 0, HF_K_RHS64, HF_K_RHS32, HF_K_CG_A64, HF_K_CG_B64, HF_K_CG_A32, HF_K_CG_B32,
-    HF_K_FINAL, HF_K_IR_RES, HF_K_IR_STEP, HF_K_STEP64, HF_K_PERSIST, HF_K_COUNT
+    HF_K_FINAL, HF_K_IR_RES, HF_K_IR_STEP, HF_K_STEP64, HF_K_PERSIST, HF_K_IR_STEP2, HF_K_COUNT
 };
 int hf_set_timing(hf_ctx *ctx, int32_t on);
 int hf_kernel_times(hf_ctx *ctx, double *ms, int64_t *launches, double *bytes);

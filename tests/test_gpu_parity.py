@@ -536,6 +536,84 @@ def test_fp32_ir_alpha_reuse_miss():
     assert relerr(t32, t64) <= 1e-10
 
 
+def _mixed_run(s, nsteps, debug, tol=1e-12):
+    with Simulation(s, precision="fp32_ir", tol=tol) as sim:
+        sim.set_debug(debug)
+        sim.initial_evolve()
+        sim.set_timing(True)
+        sim.run(1, nsteps)
+        kt = sim.kernel_times()
+        st = sim.stats()
+        stats = (st.steps, st.outer_iters, st.inner_iters, st.first_iter_stops, st.tiny_misses,
+                 st.residual_reruns, st.alpha_reuses, st.reuse_misses)
+        return sim.get_state(), stats, kt
+
+
+@pytest.mark.parametrize("n", [24, 40])
+def test_fp32_ir_pairs_bit_identical_with_frozen_alpha(n):
+    """Two reusing steps in one pass (k_ir_step2_z) do the per-cell operations of two
+    single-step passes in the same order and rounding; only the block sums are grouped by other
+    tiles.  With the reused step length frozen (debug bit 8) and the same decisions, the two
+    paths give identical bits and statistics: 40 steps of the log-normal workload at tol 1e-11,
+    pairs on and off (debug bit 7); at least 15 pairs ran."""
+    s, _ = workloads.c3(n)
+    (ta, tpa), sa, ka = _mixed_run(s, 40, 256, tol=1e-11)
+    (tb, tpb), sb, kb = _mixed_run(s, 40, 256 | 128, tol=1e-11)
+    print(f"pairs {n}^3 frozen: launches {ka.get('k_ir_step2', (0, 0, 0))[1]}, stats {sa}",
+          flush=True)
+    assert ka.get("k_ir_step2", (0.0, 0, 0.0))[1] >= 15 and "k_ir_step2" not in kb
+    assert sa == sb
+    assert np.array_equal(bits(ta), bits(tb)) and np.array_equal(bits(tpa), bits(tpb))
+
+
+@pytest.mark.parametrize("case", ["gaussian", "uniform_insulated", "fourier"])
+def test_fp32_ir_pairs_match_oracle(case):
+    """Pairs with the step length updated after each pair (the default) against the oracle and
+    the single-step path: 40 steps of the log-normal workload at 24^3 and tol 1e-11 with
+    Gaussian heating, with uniform heating of an insulated grid, and Fourier (no tau term).  At
+    this tolerance (pairs run only where one inner iteration meets it) 40 steps of either path
+    are about 3e-10 from the oracle; the pairs add no error of their own."""
+    s, _ = workloads.c3(24)
+    p = s.params
+    if case == "uniform_insulated":
+        p.kappa_bound = (0.0,) * 6
+        s.cells.qdens_extra = np.full(p.n, 1.0e12)
+    if case == "fourier":
+        p.icattaneo = 0
+    t0 = np.full(p.n, p.t_system)
+    rc, otp, otpp, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 41)), t0, t0)
+    assert rc == 0
+    (ta, tpa), sa, ka = _mixed_run(s, 40, 0, tol=1e-11)
+    (tb, tpb), sb, kb = _mixed_run(s, 40, 128, tol=1e-11)
+    print(f"pairs {case}: launches {ka.get('k_ir_step2', (0, 0, 0))[1]}, stats {sa} / {sb}, "
+          f"err {relerr(ta, otp):.3g} / {relerr(tb, otp):.3g}", flush=True)
+    assert ka.get("k_ir_step2", (0.0, 0, 0.0))[1] >= 1
+    ea, eb = relerr(ta, otp), relerr(tb, otp)
+    assert ea <= 1e-9 and relerr(tpa, otpp) <= 1e-9
+    assert ea <= 1.2 * eb + 1e-12
+    assert relerr(ta, tb) <= 1e-10
+
+
+def test_fp32_ir_pairs_with_misses():
+    """The 48^3 workload at tol 1e-12, where one-iteration residuals sit near the inner target:
+    a pair whose step misses its test falls back to single steps and pauses reuse.  The 24
+    steps with pairs on and off (debug bit 7) end as far from FP64 at the reference tolerance
+    as each other, within 2e-10 of it (1.17e-10 and 1.11e-10 measured)."""
+    s, _ = workloads.c3(48)
+    with Simulation(s, tol=1e-32) as ref:
+        ref.initial_evolve()
+        ref.run(1, 24)
+        t64, _ = ref.get_state()
+    (ta, _), sa, ka = _mixed_run(s, 24, 0)
+    (tb, _), sb, _ = _mixed_run(s, 24, 128)
+    ea, eb = relerr(ta, t64), relerr(tb, t64)
+    print(f"pairs 48^3: launches {ka.get('k_ir_step2', (0, 0, 0))[1]}, stats {sa} / {sb}, "
+          f"err {ea:.3g} / {eb:.3g}", flush=True)
+    assert sa[0] == 24 and sb[0] == 24
+    assert ka.get("k_ir_step2", (0.0, 0, 0.0))[1] >= 1
+    assert ea <= 2e-10 and ea <= 1.2 * eb + 1e-12
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
 def test_uniform_heat_capacity_reads_are_exact(prec):
     """Where rho c_p and tau' rho c_p are the same in every cell (found at upload), the step

@@ -1366,7 +1366,8 @@ static_assert(kAN - kTT <= kTT && kBN - kTT <= kTT, "at most two positions a thr
 
 struct Step2Lds {
     D2 a[3][kAN];
-    D2 b[2][kBN];
+    D2 b[kBN];
+    double acc[8][kTT];   // each thread's residual sums (its own slots)
 };
 
 // the k-th position (k >= 0) of a region of W x H around the tile (ring width R): k < T is tile
@@ -1424,88 +1425,150 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     const bool active = zvalid && ox < g.nx && oy < g.ny;
     const uint32_t ocol = ox + g.nx * oy;
     const int64_t gx0 = (int64_t)tbx * kP2X, gy0 = (int64_t)tby * kP2Y;
-    const int nq_a = t < kAN - kTT ? 2 : 1;   // a-positions of this thread
-    const int nq_b = t < kBN - kTT ? 2 : 1;   // b-positions
-    auto rcv = [&](uint32_t J) { return UniformRc ? rc_u : rhoc[J & sp_.rc_mask]; };
-    auto cvv = [&](uint32_t J) {
-        return Cattaneo ? (UniformRc ? cv_u : cvec[J & sp_.rc_mask]) : 0.0;
-    };
-    // grid column of a region position (R: ring width), or false outside the grid
-    auto cell_of = [&](int px, int py, int R, uint32_t &col) -> bool {
+    const bool two_a = t < kAN - kTT;   // a second a-position (on the ring)
+    const bool two_b = t < kBN - kTT;   // a second b-position
+    const uint32_t P = g.nxny;
+    // grid column of a region position (R: ring width), or kNoCol outside the grid
+    constexpr uint32_t kNoCol = 0xffffffffu;
+    auto col_of = [&](int px, int py, int R) -> uint32_t {
         const int64_t x = gx0 + px - R, y = gy0 + py - R;
-        if (!zvalid || x < 0 || y < 0 || x >= (int64_t)g.nx || y >= (int64_t)g.ny) return false;
-        col = (uint32_t)x + g.nx * (uint32_t)y;
-        return true;
+        if (!zvalid || x < 0 || y < 0 || x >= (int64_t)g.nx || y >= (int64_t)g.ny) return kNoCol;
+        return (uint32_t)x + g.nx * (uint32_t)y;
     };
+    // the thread's a-positions (k = 0: its tile cell) and b-positions: region coordinates
+    // (x | y << 16) and grid columns
+    uint32_t apos[2], bpos[2], acol[2], bcol[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        int px, py;
+        region_pos<2>(k == 0 || two_a ? t + k * kTT : t, px, py);
+        apos[k] = (uint32_t)px | (uint32_t)py << 16;
+        acol[k] = k == 0 || two_a ? col_of(px, py, 2) : kNoCol;
+        region_pos<1>(k == 0 || two_b ? t + k * kTT : t, px, py);
+        bpos[k] = (uint32_t)px | (uint32_t)py << 16;
+        bcol[k] = k == 0 || two_b ? col_of(px, py, 1) : kNoCol;
+    }
+    auto pos_x = [](uint32_t v) { return (int)(v & 0xffffu); };
+    auto pos_y = [](uint32_t v) { return (int)(v >> 16); };
     auto mod3 = [](int v) { return ((v % 3) + 3) % 3; };
-    auto mod2 = [](int v) { return v & 1; };
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) L.acc[k][t] = 0.0;
     int32_t fl = 0, fl2 = 0;
-    // own-cell values handed from stage to stage (plane noted)
-    double own_tp = 0.0, own_tpp = 0.0;   // a-stage of plane q -> step n on plane q
+
+    // Loads.  Every thread issues the same loads on every trip, without branches, so that the
+    // compiler's wait counts know how many are in flight; a position with nothing to load
+    // reads cell 0.  a(q)'s loads are issued on the trip before (before its last barrier),
+    // step n's loads at the top of the trip that uses them (before a(q) is formed).
+    struct ALd {
+        double tp, tpp, d;
+        float r;
+    };
+    struct BLd {
+        double d, cx, cy, cz, cxm, cym, czm, qb, tp, rc, cv;
+    };
+    auto a_load = [&](int k, int q) {
+        const bool ok = acol[k] != kNoCol && q >= 0 && q < (int)g.nz;
+        uint32_t J = ok ? acol[k] + P * (uint32_t)q : 0u;
+        asm volatile("" : "+v"(J));
+        ALd v;
+        v.tp = tp[J];
+        v.tpp = tpp[J];
+        v.d = d[J];
+        v.r = r0[J];
+        return v;
+    };
+    // step n's operator row, Qdens + B and T^n at b-position k of plane p (the tile cell takes
+    // d and T^n from its a-loads of the trip before)
+    auto b_load = [&](int k, int p) {
+        const bool ok = bcol[k] != kNoCol && p >= 0 && p < (int)g.nz;
+        const uint32_t x = (uint32_t)(gx0 + pos_x(bpos[k]) - 1);
+        const uint32_t y = (uint32_t)(gy0 + pos_y(bpos[k]) - 1);
+        uint32_t J = ok ? bcol[k] + P * (uint32_t)p : 0u;
+        uint32_t Jxm = ok && x > 0 ? J - 1 : J;
+        uint32_t Jym = ok && y > 0 ? J - g.nx : J;
+        uint32_t Jzm = ok && p > 0 ? J - P : J;
+        asm volatile("" : "+v"(J), "+v"(Jxm), "+v"(Jym), "+v"(Jzm));
+        BLd v;
+        v.cx = cx[J];
+        v.cy = cy[J];
+        v.cz = cz[J];
+        v.cxm = cx[Jxm];
+        v.cym = cy[Jym];
+        v.czm = cz[Jzm];
+        v.qb = qb[J];
+        if (k == 1) {
+            v.d = d[J];
+            v.tp = tp[J];
+        }
+        if (!UniformRc) {
+            v.rc = rhoc[J & sp_.rc_mask];
+            v.cv = Cattaneo ? cvec[J & sp_.rc_mask] : 0.0;
+        }
+        return v;
+    };
+
+    // own-cell values handed from stage to stage
+    double own_tp = 0.0, own_tpp = 0.0, own_d = 0.0;   // a(q) -> step n on plane q
     float own_r = 0.0f;
-    double hS1 = 0.0, hxn = 0.0;          // step n on plane p -> step n+1 on plane p: S_{n+1}, x'_n
-    float hfn = 0.0f;                     // r^0_{n+1} (FP32)
-    D2 b_m = 0.0, b_0 = 0.0, b_p = 0.0;   // own b of planes p-1, p, p+1 (step n+1 on plane p)
+    D2 b_prev = 0.0;   // own b of the plane before the one step n ran on last
+    // step n+1 at the own cell of plane p, formed on the trip step n ran on p, with the z+ term
+    // (b of plane p + 1) added on the next trip: H x'_{n+1} and H x1_{n+1} without it, the
+    // tiny-guess row sum without it, the z+ coupling, s, S_{n+1}, S_{n+2}, r^0_{n+1}
+    double w_ha = 0.0, w_hb = 0.0, w_h1 = 0.0, w_cz = 0.0, w_s = 0.0, w_S1 = 0.0, w_S2 = 0.0;
+    float w_fn = 0.0f;
     const int zq0 = (int)z0 - 2, zq1 = (int)z1 + 1;
+    ALd al0 = a_load(0, zq0), al1 = a_load(1, zq0);
     for (int q = zq0; q <= zq1; ++q) {
+        const int p = q - 1;
+        // step n's loads (plane p)
+        const BLd bl0 = b_load(0, p), bl1 = b_load(1, p);
         // ---- a(q): x'_n and x1_n on the tile and a two-cell ring
         D2 *A = L.a[mod3(q)];
         const bool zin = q >= 0 && q < (int)g.nz;
-        double n_tp = 0.0, n_tpp = 0.0;
-        float n_r = 0.0f;
-        for (int k = 0; k < nq_a; ++k) {
-            int px, py;
-            region_pos<2>(t + k * kTT, px, py);
-            uint32_t col;
-            D2 v = 0.0;
-            if (zin && cell_of(px, py, 2, col)) {
-                const uint32_t J = col + g.nxny * (uint32_t)q;
-                const double a_tp = tp[J], a_tpp = tpp[J], a_d = d[J];
-                const float a_r = r0[J];
-                double base = a_tp + (a_tp - a_tpp);   // mod_evolve.f90:140
+        auto stage_a = [&](const ALd &v, int k) {
+            D2 o = 0.0;
+            if (zin && acol[k] != kNoCol) {
+                double base = v.tp + (v.tp - v.tpp);   // mod_evolve.f90:140
                 if (tiny) base = base + kTiny;         // :141
-                const double xn = base + (1.0 / sqrt(-a_d)) * (double)(ysc * a_r);
-                v = D2(xn, xn + (xn - a_tp));
-                if (k == 0) {
-                    n_tp = a_tp;
-                    n_tpp = a_tpp;
-                    n_r = a_r;
-                }
+                const double xn = base + (1.0 / sqrt(-v.d)) * (double)(ysc * v.r);
+                o = D2(xn, xn + (xn - v.tp));
             }
-            A[py * kAW + px] = v;
-        }
+            A[pos_y(apos[k]) * kAW + pos_x(apos[k])] = o;
+        };
+        stage_a(al0, 0);
+        if (two_a) stage_a(al1, 1);
         __syncthreads();   // a(q) staged
         // ---- step n on plane p = q - 1 (ring positions; its test on the tile only)
-        const int p = q - 1;
         const bool step_n = p >= (int)z0 - 1 && p <= (int)z1 && p >= 0 && p < (int)g.nz;
         // the tile's own planes: its residual test, sums and flags (the planes z0 - 1 and z1
         // are the neighbouring chunks')
         const bool own_plane = p >= (int)z0 && p < (int)z1;
         const D2 *Am = L.a[mod3(p - 1)], *A0 = L.a[mod3(p)], *Ap = L.a[mod3(p + 1)];
-        D2 *B = L.b[mod2(p)];
-        D2 own_b = 0.0;
-        double own_S1 = 0.0, own_xn = 0.0;
-        float own_fn = 0.0f;
-        for (int k = 0; k < nq_b; ++k) {
-            int px, py;
-            region_pos<1>(t + k * kTT, px, py);
-            uint32_t col;
+        // the own cell's operator row, S_{n+1}, x'_n, r^0_{n+1} and b: step n+1 on plane p
+        D2 o_b = 0.0;
+        double o_d = 0.0, o_cxi = 0.0, o_cyi = 0.0, o_czi = 0.0, o_cxm = 0.0, o_cym = 0.0;
+        double o_czm = 0.0, o_s = 0.0, o_S1 = 0.0, o_S2 = 0.0, o_h1 = 0.0, o_xn = 0.0;
+        float o_fn = 0.0f;
+        // position k of this thread (k = 0: its tile cell)
+        auto stage_b = [&](auto kc, const BLd &ld) {
+            constexpr int k = decltype(kc)::value;
+            const int px = pos_x(bpos[k]), py = pos_y(bpos[k]);
             D2 v = 0.0;
-            if (step_n && cell_of(px, py, 1, col)) {
-                const uint32_t J = col + g.nxny * (uint32_t)p;
+            if (step_n && bcol[k] != kNoCol) {
                 const uint32_t x = (uint32_t)(gx0 + px - 1), y = (uint32_t)(gy0 + py - 1);
                 const bool xm_in = x > 0, xp_in = x + 1 < g.nx;
                 const bool ym_in = y > 0, yp_in = y + 1 < g.ny;
                 const bool zm_in = p > 0, zp_in = p + 1 < (int)g.nz;
                 const int ai = (py + 1) * kAW + (px + 1);   // this position in an a-plane
                 const D2 xi = A0[ai];
-                const double di = d[J], cxi = cx[J], cyi = cy[J], czi = cz[J];
-                const double cxm = cx[xm_in ? J - 1 : J], cym = cy[ym_in ? J - g.nx : J];
-                const double czm = zm_in ? cz[J - g.nxny] : 0.0;
-                const double qbi = qb[J];
-                const double tpi = k == 0 ? own_tp : tp[J];
-                const double rci = rcv(J), cvi = cvv(J);
+                const double di = k == 0 ? own_d : ld.d;
+                const double cxi = ld.cx, cyi = ld.cy, czi = ld.cz;
+                const double cxm = ld.cxm, cym = ld.cym;
+                const double czm = zm_in ? ld.czm : 0.0;
+                const double qbi = ld.qb;
+                const double tpi = k == 0 ? own_tp : ld.tp;
+                const double rci = UniformRc ? rc_u : ld.rc;
+                const double cvi = Cattaneo ? (UniformRc ? cv_u : ld.cv) : 0.0;
                 // H x (pick: x' or x1) in the reference's serial order
                 auto hsum = [&](auto pick) {
                     double h = 0.0;
@@ -1550,97 +1613,118 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     if (isnan(S1)) fl |= FLAG_NAN_S;
                     const float rf = (float)(-s * (Sn - hsum([](const D2 &w) { return w.a; })));
                     const double bh = s * S1;
-                    acc[0] += (double)rf * (double)rf;
-                    acc[1] += bh * bh;
-                    acc[2] += (double)fn * (double)fn;
-                    acc[3] += (double)own_r * (double)rf;
+                    L.acc[0][t] += (double)rf * (double)rf;
+                    L.acc[1][t] += bh * bh;
+                    L.acc[2][t] += (double)fn * (double)fn;
+                    L.acc[3][t] += (double)own_r * (double)rf;
                     if (isnan(xi.a)) fl |= FLAG_NAN_X;
                     if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;
+                    // step n+1's S_{n+2} from (x'_{n+1}, x'_n), and its tiny-guess row sum but
+                    // for the z+ coupling
+                    double S2 = -(it * v.a * rci) - qbi;
+                    if (Cattaneo) {
+                        const double scat = (xi.a - 2.0 * v.a) * cvi;
+                        if (isnan(scat)) fl2 |= FLAG_NAN_SCAT2;
+                        S2 = S2 + scat;
+                    }
+                    if (isnan(S2)) fl2 |= FLAG_NAN_S2;
+                    o_S2 = S2;
+                    if (guess) {
+                        double h1s = di;
+                        h1s = h1s + czm;
+                        h1s = h1s + (ym_in ? cym : 0.0);
+                        h1s = h1s + (xm_in ? cxm : 0.0);
+                        h1s = h1s + cxi;
+                        h1s = h1s + cyi;
+                        o_h1 = h1s;
+                    }
                 }
                 if (k == 0) {
-                    own_b = v;
-                    own_S1 = S1;
-                    own_xn = xi.a;
-                    own_fn = fn;
+                    o_b = v;
+                    o_d = di;
+                    o_cxi = cxi;
+                    o_cyi = cyi;
+                    o_czi = czi;
+                    o_cxm = cxm;
+                    o_cym = cym;
+                    o_czm = czm;
+                    o_s = s;
+                    o_S1 = S1;
+                    o_xn = xi.a;
+                    o_fn = fn;
                 }
             }
-            B[py * kBW + px] = v;
-        }
-        own_tp = n_tp;
-        own_tpp = n_tpp;
-        own_r = n_r;
-        // ---- step n+1 on plane p2 = q - 2 (tile cells): b(p2) in LDS (the previous trip), the
-        // own b of planes p2 - 1 and p2 + 1 in registers
+            L.b[py * kBW + px] = v;
+        };
+        if (two_b) stage_b(std::integral_constant<int, 1>{}, bl1);
+        stage_b(std::integral_constant<int, 0>{}, bl0);
+        // ---- step n+1 on plane p2 = q - 2 (tile cells): add the z+ term, b(q - 1) of the own
+        // cell, to what the previous trip formed, and finish
         const int p2 = q - 2;
-        b_m = b_0;
-        b_0 = b_p;
-        b_p = own_b;
         if (active && p2 >= (int)z0 && p2 < (int)z1) {
-            const D2 *B0 = L.b[mod2(p2)];
-            const uint32_t J = ocol + g.nxny * (uint32_t)p2;
+            const bool zp_in = p2 + 1 < (int)g.nz;
+            const double ha = w_ha + w_cz * (zp_in ? o_b.a : 0.0);
+            const double hb = w_hb + w_cz * (zp_in ? o_b.b : 0.0);
+            const float rf = (float)(-w_s * (w_S1 - ha));
+            double ra = -w_s * (w_S2 - hb);
+            if (guess) ra = ra + kTiny * w_s * (w_h1 + w_cz);
+            const float fn = (float)ra;
+            stnt(&rnext[ocol + P * (uint32_t)p2], fn);
+            L.acc[4][t] += (double)rf * (double)rf;
+            L.acc[6][t] += (double)fn * (double)fn;
+            L.acc[7][t] += (double)w_fn * (double)rf;
+        }
+        own_tp = al0.tp;
+        own_tpp = al0.tpp;
+        own_d = al0.d;
+        own_r = al0.r;
+        // a(q + 1)'s loads
+        al0 = a_load(0, q + 1);
+        al1 = a_load(1, q + 1);
+        __syncthreads();   // b(p) staged; a(q - 2) no longer read
+        // ---- step n+1 on plane p (tile cells) but for its z+ term: b(p) in LDS, the own b of
+        // plane p - 1 in a register
+        if (active && own_plane) {
             const bool xm_in = ox > 0, xp_in = ox + 1 < g.nx;
             const bool ym_in = oy > 0, yp_in = oy + 1 < g.ny;
-            const bool zm_in = p2 > 0, zp_in = p2 + 1 < (int)g.nz;
+            const bool zm_in = p > 0;
             const int bi = ((int)ly + 1) * kBW + ((int)lx + 1);
-            const D2 xi = B0[bi];
-            const double di = d[J], cxi = cx[J], cyi = cy[J], czi = cz[J];
-            const double cxm = cx[xm_in ? J - 1 : J], cym = cy[ym_in ? J - g.nx : J];
-            const double czm = zm_in ? cz[J - g.nxny] : 0.0;
-            const double qbi = qb[J];
-            const double rci = rcv(J), cvi = cvv(J);
-            auto hsum = [&](auto pick) {
+            const D2 xi = o_b;
+            auto hsum6 = [&](auto pick) {
                 double h = 0.0;
-                h = h + czm * (zm_in ? pick(b_m) : 0.0);
-                h = h + cym * (ym_in ? pick(B0[bi - kBW]) : 0.0);
-                h = h + cxm * (xm_in ? pick(B0[bi - 1]) : 0.0);
-                h = h + di * pick(xi);
-                h = h + cxi * (xp_in ? pick(B0[bi + 1]) : 0.0);
-                h = h + cyi * (yp_in ? pick(B0[bi + kBW]) : 0.0);
-                h = h + czi * (zp_in ? pick(b_p) : 0.0);
+                h = h + o_czm * (zm_in ? pick(b_prev) : 0.0);
+                h = h + o_cym * (ym_in ? pick(L.b[bi - kBW]) : 0.0);
+                h = h + o_cxm * (xm_in ? pick(L.b[bi - 1]) : 0.0);
+                h = h + o_d * pick(xi);
+                h = h + o_cxi * (xp_in ? pick(L.b[bi + 1]) : 0.0);
+                h = h + o_cyi * (yp_in ? pick(L.b[bi + kBW]) : 0.0);
                 return h;
             };
-            const double s = 1.0 / sqrt(-di);
-            // S_{n+2} from (x'_{n+1}, x'_n)
-            double S2 = -(it * xi.a * rci) - qbi;
-            if (Cattaneo) {
-                const double scat = (hxn - 2.0 * xi.a) * cvi;
-                if (isnan(scat)) fl2 |= FLAG_NAN_SCAT2;
-                S2 = S2 + scat;
-            }
-            if (isnan(S2)) fl2 |= FLAG_NAN_S2;
-            const float rf = (float)(-s * (hS1 - hsum([](const D2 &w) { return w.a; })));
-            double ra = -s * (S2 - hsum([](const D2 &w) { return w.b; }));
-            if (guess) {
-                double h1s = di;
-                h1s = h1s + czm;
-                h1s = h1s + (ym_in ? cym : 0.0);
-                h1s = h1s + (xm_in ? cxm : 0.0);
-                h1s = h1s + cxi;
-                h1s = h1s + cyi;
-                h1s = h1s + czi;
-                ra = ra + kTiny * s * h1s;
-            }
-            const float fn = (float)ra;
-            stnt(&x1out[J], hxn);
+            w_ha = hsum6([](const D2 &w) { return w.a; });
+            w_hb = hsum6([](const D2 &w) { return w.b; });
+            w_h1 = o_h1;
+            w_cz = o_czi;
+            w_s = o_s;
+            w_S1 = o_S1;
+            w_S2 = o_S2;
+            w_fn = o_fn;
+            const uint32_t J = ocol + P * (uint32_t)p;
+            stnt(&x1out[J], o_xn);
             stnt(&x2out[J], xi.a);
-            stnt(&rnext[J], fn);
-            const double bh = s * S2;
-            acc[4] += (double)rf * (double)rf;
-            acc[5] += bh * bh;
-            acc[6] += (double)fn * (double)fn;
-            acc[7] += (double)hfn * (double)rf;
+            const double bh = o_s * o_S2;
+            L.acc[5][t] += bh * bh;
             if (isnan(xi.a)) fl2 |= FLAG_NAN_X2;
             if (xi.b - xi.a < kTiny) fl2 |= FLAG_TINY_NEXT2;
         }
-        hS1 = own_S1;
-        hxn = own_xn;
-        hfn = own_fn;
-        __syncthreads();   // a and b planes of this trip were read before the next writes
+        b_prev = o_b;
     }
     block_flags(fl & (FLAG_NAN_X | FLAG_TINY_NEXT), &sc->flags);
     block_flags(fl & (FLAG_NAN_SCAT | FLAG_NAN_S), &sc->flags_next);
     block_flags(fl2, &sc->flags2);
     // partials: components 0-3 (step n), then 4-7 (step n+1)
+    double acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = L.acc[k][t];
     block_partials<8>(acc, partials);
 }
 

@@ -1100,13 +1100,14 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
 
 // Whether the next two steps can run as one two-step pass (k_ir_step2_z): the fused mixed path
 // is carrying a residual into a step that will reuse the inner step length after a one-sweep
-// step, the grid has no periodic axis and the step is transient.  Debug bit 7 turns pairs off.
+// step, the grid has no periodic axis, the step is transient, and the grid has fewer than 2^29
+// cells (the pass addresses its arrays with 32-bit byte offsets).  Debug bit 7 turns pairs off.
 static bool pair_ready(const hf_ctx *c) {
     const hf_params &p = c->p;
     return p.precision == HF_PREC_FP32_IR && !c->dyn_heat && (c->debug & (1 | 4 | 128)) == 0 &&
            allow_short(c) && c->carry && c->reuse_ok && c->reuse_pause == 0 &&
            c->last_sweeps == 1 && p.isteady == 0 && !c->g.per[0] && !c->g.per[1] &&
-           !c->g.per[2] && c->qb_valid;
+           !c->g.per[2] && c->qb_valid && (uint64_t)c->g.N < (1ull << 29);
 }
 
 // Two reusing steps in one pass.  *advanced: 2 when both stand, 1 when only the first does (the

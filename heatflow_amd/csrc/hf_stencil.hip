@@ -1394,6 +1394,18 @@ __device__ __forceinline__ void region_pos(int k, int &px, int &py) {
     px = col < R ? col : TX + col;
 }
 
+// Element at a 32-bit byte offset from p (the address form with the base in scalar registers
+// and one 32-bit offset a lane, shared by the arrays of one element size); the caller
+// guarantees the offset fits (grids of fewer than 2^29 cells).
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *p, uint32_t byte_off) {
+    return *(const T *)((const char *)p + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T *p, uint32_t byte_off, T v) {
+    __builtin_nontemporal_store(v, (T *)((char *)p + byte_off));
+}
+
 template <bool Cattaneo, bool UniformRc>
 __global__ void __launch_bounds__(kP2X * kP2Y) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
@@ -1470,11 +1482,12 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         const bool ok = acol[k] != kNoCol && q >= 0 && q < (int)g.nz;
         uint32_t J = ok ? acol[k] + P * (uint32_t)q : 0u;
         asm volatile("" : "+v"(J));
+        const uint32_t o8 = J * 8u;
         ALd v;
-        v.tp = tp[J];
-        v.tpp = tpp[J];
-        v.d = d[J];
-        v.r = r0[J];
+        v.tp = ld_at(tp, o8);
+        v.tpp = ld_at(tpp, o8);
+        v.d = ld_at(d, o8);
+        v.r = ld_at(r0, J * 4u);
         return v;
     };
     // step n's operator row, Qdens + B and T^n at b-position k of plane p (the tile cell takes
@@ -1483,22 +1496,23 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         const bool ok = bcol[k] != kNoCol && p >= 0 && p < (int)g.nz;
         const uint32_t x = (uint32_t)(gx0 + pos_x(bpos[k]) - 1);
         const uint32_t y = (uint32_t)(gy0 + pos_y(bpos[k]) - 1);
-        uint32_t J = ok ? bcol[k] + P * (uint32_t)p : 0u;
-        uint32_t Jxm = ok && x > 0 ? J - 1 : J;
-        uint32_t Jym = ok && y > 0 ? J - g.nx : J;
-        uint32_t Jzm = ok && p > 0 ? J - P : J;
-        asm volatile("" : "+v"(J), "+v"(Jxm), "+v"(Jym), "+v"(Jzm));
+        const uint32_t J = ok ? bcol[k] + P * (uint32_t)p : 0u;
+        uint32_t o8 = J * 8u;
+        uint32_t oxm = ok && x > 0 ? o8 - 8u : o8;
+        uint32_t oym = ok && y > 0 ? o8 - 8u * g.nx : o8;
+        uint32_t ozm = ok && p > 0 ? o8 - 8u * P : o8;
+        asm volatile("" : "+v"(o8), "+v"(oxm), "+v"(oym), "+v"(ozm));
         BLd v;
-        v.cx = cx[J];
-        v.cy = cy[J];
-        v.cz = cz[J];
-        v.cxm = cx[Jxm];
-        v.cym = cy[Jym];
-        v.czm = cz[Jzm];
-        v.qb = qb[J];
+        v.cx = ld_at(cx, o8);
+        v.cy = ld_at(cy, o8);
+        v.cz = ld_at(cz, o8);
+        v.cxm = ld_at(cx, oxm);
+        v.cym = ld_at(cy, oym);
+        v.czm = ld_at(cz, ozm);
+        v.qb = ld_at(qb, o8);
         if (k == 1) {
-            v.d = d[J];
-            v.tp = tp[J];
+            v.d = ld_at(d, o8);
+            v.tp = ld_at(tp, o8);
         }
         if (!UniformRc) {
             v.rc = rhoc[J & sp_.rc_mask];
@@ -1556,9 +1570,7 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             D2 v = 0.0;
             if (step_n && bcol[k] != kNoCol) {
                 const uint32_t x = (uint32_t)(gx0 + px - 1), y = (uint32_t)(gy0 + py - 1);
-                const bool xm_in = x > 0, xp_in = x + 1 < g.nx;
-                const bool ym_in = y > 0, yp_in = y + 1 < g.ny;
-                const bool zm_in = p > 0, zp_in = p + 1 < (int)g.nz;
+                const bool xm_in = x > 0, ym_in = y > 0, zm_in = p > 0;
                 const int ai = (py + 1) * kAW + (px + 1);   // this position in an a-plane
                 const D2 xi = A0[ai];
                 const double di = k == 0 ? own_d : ld.d;
@@ -1569,16 +1581,17 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 const double tpi = k == 0 ? own_tp : ld.tp;
                 const double rci = UniformRc ? rc_u : ld.rc;
                 const double cvi = Cattaneo ? (UniformRc ? cv_u : ld.cv) : 0.0;
-                // H x (pick: x' or x1) in the reference's serial order
+                // H x (pick: x' or x1) in the reference's serial order; an a-plane holds 0 at
+                // the positions outside the grid, so an absent neighbour adds coefficient x 0
                 auto hsum = [&](auto pick) {
                     double h = 0.0;
-                    h = h + czm * (zm_in ? pick(Am[ai]) : 0.0);
-                    h = h + cym * (ym_in ? pick(A0[ai - kAW]) : 0.0);
-                    h = h + cxm * (xm_in ? pick(A0[ai - 1]) : 0.0);
+                    h = h + czm * pick(Am[ai]);
+                    h = h + cym * pick(A0[ai - kAW]);
+                    h = h + cxm * pick(A0[ai - 1]);
                     h = h + di * pick(xi);
-                    h = h + cxi * (xp_in ? pick(A0[ai + 1]) : 0.0);
-                    h = h + cyi * (yp_in ? pick(A0[ai + kAW]) : 0.0);
-                    h = h + czi * (zp_in ? pick(Ap[ai]) : 0.0);
+                    h = h + cxi * pick(A0[ai + 1]);
+                    h = h + cyi * pick(A0[ai + kAW]);
+                    h = h + czi * pick(Ap[ai]);
                     return h;
                 };
                 const double s = 1.0 / sqrt(-di);
@@ -1662,14 +1675,14 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         // cell, to what the previous trip formed, and finish
         const int p2 = q - 2;
         if (active && p2 >= (int)z0 && p2 < (int)z1) {
-            const bool zp_in = p2 + 1 < (int)g.nz;
-            const double ha = w_ha + w_cz * (zp_in ? o_b.a : 0.0);
-            const double hb = w_hb + w_cz * (zp_in ? o_b.b : 0.0);
+            // (o_b is 0 when plane p2 + 1 is outside the grid)
+            const double ha = w_ha + w_cz * o_b.a;
+            const double hb = w_hb + w_cz * o_b.b;
             const float rf = (float)(-w_s * (w_S1 - ha));
             double ra = -w_s * (w_S2 - hb);
             if (guess) ra = ra + kTiny * w_s * (w_h1 + w_cz);
             const float fn = (float)ra;
-            stnt(&rnext[ocol + P * (uint32_t)p2], fn);
+            st_at(rnext, (ocol + P * (uint32_t)p2) * 4u, fn);
             L.acc[4][t] += (double)rf * (double)rf;
             L.acc[6][t] += (double)fn * (double)fn;
             L.acc[7][t] += (double)w_fn * (double)rf;
@@ -1685,19 +1698,17 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         // ---- step n+1 on plane p (tile cells) but for its z+ term: b(p) in LDS, the own b of
         // plane p - 1 in a register
         if (active && own_plane) {
-            const bool xm_in = ox > 0, xp_in = ox + 1 < g.nx;
-            const bool ym_in = oy > 0, yp_in = oy + 1 < g.ny;
-            const bool zm_in = p > 0;
             const int bi = ((int)ly + 1) * kBW + ((int)lx + 1);
             const D2 xi = o_b;
+            // (the b-plane holds 0 at the positions outside the grid, and b_prev is 0 on plane 0)
             auto hsum6 = [&](auto pick) {
                 double h = 0.0;
-                h = h + o_czm * (zm_in ? pick(b_prev) : 0.0);
-                h = h + o_cym * (ym_in ? pick(L.b[bi - kBW]) : 0.0);
-                h = h + o_cxm * (xm_in ? pick(L.b[bi - 1]) : 0.0);
+                h = h + o_czm * pick(b_prev);
+                h = h + o_cym * pick(L.b[bi - kBW]);
+                h = h + o_cxm * pick(L.b[bi - 1]);
                 h = h + o_d * pick(xi);
-                h = h + o_cxi * (xp_in ? pick(L.b[bi + 1]) : 0.0);
-                h = h + o_cyi * (yp_in ? pick(L.b[bi + kBW]) : 0.0);
+                h = h + o_cxi * pick(L.b[bi + 1]);
+                h = h + o_cyi * pick(L.b[bi + kBW]);
                 return h;
             };
             w_ha = hsum6([](const D2 &w) { return w.a; });
@@ -1708,9 +1719,9 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             w_S1 = o_S1;
             w_S2 = o_S2;
             w_fn = o_fn;
-            const uint32_t J = ocol + P * (uint32_t)p;
-            stnt(&x1out[J], o_xn);
-            stnt(&x2out[J], xi.a);
+            const uint32_t o8 = (ocol + P * (uint32_t)p) * 8u;
+            st_at(x1out, o8, o_xn);
+            st_at(x2out, o8, xi.a);
             const double bh = o_s * o_S2;
             L.acc[5][t] += bh * bh;
             if (isnan(xi.a)) fl2 |= FLAG_NAN_X2;

@@ -1469,8 +1469,8 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
 
     // Loads.  Every thread issues the same loads on every trip, without branches, so that the
     // compiler's wait counts know how many are in flight; a position with nothing to load
-    // reads cell 0.  a(q)'s loads are issued on the trip before (before its last barrier),
-    // step n's loads at the top of the trip that uses them (before a(q) is formed).
+    // reads cell 0.  A trip's loads (a(q), then step n's) are issued on the trip before, ahead
+    // of its last barrier.
     struct ALd {
         double tp, tpp, d;
         float r;
@@ -1532,10 +1532,10 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     float w_fn = 0.0f;
     const int zq0 = (int)z0 - 2, zq1 = (int)z1 + 1;
     ALd al0 = a_load(0, zq0), al1 = a_load(1, zq0);
+    BLd bl0 = b_load(0, zq0 - 1);
     for (int q = zq0; q <= zq1; ++q) {
         const int p = q - 1;
-        // step n's loads (plane p)
-        const BLd bl0 = b_load(0, p), bl1 = b_load(1, p);
+        const BLd bl1 = b_load(1, p);   // the ring position's
         // ---- a(q): x'_n and x1_n on the tile and a two-cell ring
         D2 *A = L.a[mod3(q)];
         const bool zin = q >= 0 && q < (int)g.nz;
@@ -1691,9 +1691,10 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         own_tpp = al0.tpp;
         own_d = al0.d;
         own_r = al0.r;
-        // a(q + 1)'s loads
+        // the next trip's loads: a(q + 1), then step n on plane q
         al0 = a_load(0, q + 1);
         al1 = a_load(1, q + 1);
+        bl0 = b_load(0, q);
         __syncthreads();   // b(p) staged; a(q - 2) no longer read
         // ---- step n+1 on plane p (tile cells) but for its z+ term: b(p) in LDS, the own b of
         // plane p - 1 in a register

@@ -960,13 +960,22 @@ static float *other_rvec(hf_ctx *c, const float *a, const float *b = nullptr) {
 // further miss in a row.  A reusing sweep also yields its own step's alpha (k_fin_step), which
 // the next step reuses: the lag is one step, so the small residual component along r^0 that a
 // lagged alpha leaves does not build up over long runs of reuse.
-static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
+// A carried step whose inner solve ends on its first iteration (y = alpha r^0, as a reused step
+// length gives it) runs as the first step of a two-step pass with the next step, which reuses
+// that alpha, when `pair_adv` is given (the caller has a next step to give): *pair_adv is then the
+// number of steps taken (1 or 2), or 0 when the step ran alone.
+static int pair_pass(hf_ctx *c, bool solved, int64_t *advanced, int64_t *iters);
+static bool pair_grid_ok(const hf_ctx *c);
+static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err,
+                         int64_t *pair_adv = nullptr) {
     (void)itime;
     const hf_params &p = c->p;
     const double N = (double)c->g.N;
     RhsParams rp{1.0 / p.time_step, p.tol, p.itmax, p.isteady, p.icattaneo, rc_mask(c)};
     const float *rin;
     bool inner_started = false, reuse = false;
+    if (pair_adv) *pair_adv = 0;
+    const bool carried = c->carry;
     if (c->carry) {
         reuse = c->reuse_ok && c->reuse_pause == 0 && allow_short(c) && (c->debug & 4) == 0;
         // step, carried scalars and the first inner solve's scalars (or the reused alpha) in
@@ -1009,6 +1018,25 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
         const bool reused = reuse && sweeps == 0;
         if (!reused) TRY(inner_solve(c, rin, &batch, slots, !inner_started));
         inner_started = false;
+        if (pair_adv && carried && !reused && sweeps == 0 && c->reuse_pause == 0 &&
+            pair_grid_ok(c)) {
+            TRY(sync_scal(c, c->sci, c->h_sci, 1));   // did the solve end on its first iteration?
+            if (c->h_sci->shortcut && c->h_sci->done) {
+                int64_t adv = 0, it2 = 0;
+                TRY(pair_pass(c, true, &adv, &it2));
+                if (adv > 0) {
+                    const int64_t in_it = c->h_sci->iter;
+                    drop_idle_slots(c, slots, in_it, true);
+                    c->stats.first_iter_stops += 1;
+                    c->stats.inner_iters += in_it;
+                    c->pred_inner = std::max<int64_t>(1, in_it + 1);
+                    *pair_adv = adv;
+                    if (iter) *iter = in_it + it2;
+                    if (err) *err = c->stats.last_err;
+                    return HF_OK;
+                }
+            }
+        }
         double *xo = xbuf[nb];
         // the pass reads y (or the right-hand side, for y = alpha r) and writes two residual
         // vectors: neither may be the right-hand side
@@ -1102,24 +1130,31 @@ static int step_ir_fused(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
 // is carrying a residual into a step that will reuse the inner step length after a one-sweep
 // step, the grid has no periodic axis, the step is transient, and the grid has fewer than 2^29
 // cells (the pass addresses its arrays with 32-bit byte offsets).  Debug bit 7 turns pairs off.
-static bool pair_ready(const hf_ctx *c) {
+static bool pair_grid_ok(const hf_ctx *c) {
     const hf_params &p = c->p;
     return p.precision == HF_PREC_FP32_IR && !c->dyn_heat && (c->debug & (1 | 4 | 128)) == 0 &&
-           allow_short(c) && c->carry && c->reuse_ok && c->reuse_pause == 0 &&
-           c->last_sweeps == 1 && p.isteady == 0 && !c->g.per[0] && !c->g.per[1] &&
-           !c->g.per[2] && c->qb_valid && (uint64_t)c->g.N < (1ull << 29);
+           allow_short(c) && c->last_sweeps == 1 && p.isteady == 0 && !c->g.per[0] &&
+           !c->g.per[1] && !c->g.per[2] && c->qb_valid && (uint64_t)c->g.N < (1ull << 29);
+}
+static bool pair_ready(const hf_ctx *c) {
+    return pair_grid_ok(c) && c->carry && c->reuse_ok && c->reuse_pause == 0;
 }
 
-// Two reusing steps in one pass.  *advanced: 2 when both stand, 1 when only the first does (the
-// next step then starts from the state), 0 when neither does (the caller runs the step alone).
-// A step that fails its test counts as a step-length reuse that missed, as in the single-step
-// path, and pauses reuse; a wrong TINY guess counts as a TINY miss.
-static int step_ir_pair(hf_ctx *c, int64_t *advanced, int64_t *iters) {
+// Two steps in one pass, the second reusing the first's inner step length.  The first reuses the
+// last step's (`solved` false), or took its own from an inner solve that ended on its first
+// iteration, already queued with the carried residual as its right-hand side (`solved`).
+// *advanced: 2 when both stand, 1 when only the first does (the next step then starts from the
+// state), 0 when neither does (the caller runs the step alone; with `solved`, the caller goes
+// on with its single-step pass from the same solve, and nothing here has changed).  A step that
+// fails its test after reusing alpha counts as a reuse that missed, as in the single-step path,
+// and pauses reuse; a wrong TINY guess counts as a TINY miss.
+static int pair_pass(hf_ctx *c, bool solved, int64_t *advanced, int64_t *iters) {
     const hf_params &p = c->p;
     const double N = (double)c->g.N;
     *advanced = 0;
-    launch_carry_start(c->sc, c->sci, p.tol, p.itmax, allow_short(c), p.inner_tol, p.inner_itmax,
-                       kInnerSafety, true, c->st);
+    if (!solved)
+        launch_carry_start(c->sc, c->sci, p.tol, p.itmax, allow_short(c), p.inner_tol,
+                           p.inner_itmax, kInnerSafety, true, c->st);
     StepParams sp{1.0 / p.time_step, p.isteady, p.icattaneo, c->tiny_guess ? 1 : 0, 0,
                   rc_mask(c)};
     const float *rin = c->carry_r;
@@ -1132,11 +1167,17 @@ static int step_ir_pair(hf_ctx *c, int64_t *advanced, int64_t *iters) {
                     c->cx, c->cy, c->cz, x1, x2, rout, c->sc, c->partials, c->st);
     tend(c, sl);
     launch_fin_step2(c->g, c->partials, c->sc, c->sci, c->tiny_guess, (c->debug & 256) != 0,
-                     p.tol, p.inner_tol, kInnerSafety, c->st);
+                     solved, p.tol, p.inner_tol, kInnerSafety, c->st);
     TRY(check_launch(c, "k_ir_step2"));
     TRY(sync_scal(c, c->sc, c->h_sc, 2));
     const int32_t outcome = c->h_sc->pair_ok[0];
-    c->stats.alpha_reuses += outcome == 2 ? 2 : 1;
+    if (solved && outcome == 0) return HF_OK;
+    // steps that took y = alpha r^0 with a reused alpha, as the single-step path counts them:
+    // step n unless `solved`, and step n+1 unless step n's TINY outcome differed from the guess
+    // (step n+1 then starts from the state)
+    const bool t1_guessed = ((c->h_sc->flags & FLAG_TINY_NEXT) != 0) == c->tiny_guess;
+    c->stats.alpha_reuses +=
+        (solved ? 0 : 1) + (outcome == 2 || (outcome == 1 && t1_guessed) ? 1 : 0);
     if (outcome == 0) {
         // step n runs again alone from the state (its test failed or a NaN guard tripped)
         c->carry = false;
@@ -1191,8 +1232,13 @@ static int step_ir_pair(hf_ctx *c, int64_t *advanced, int64_t *iters) {
     return HF_OK;
 }
 
-static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err) {
-    if (!c->dyn_heat && !(c->debug & 1)) return step_ir_fused(c, itime, iter, err);
+static int step_ir_pair(hf_ctx *c, int64_t *advanced, int64_t *iters) {
+    return pair_pass(c, false, advanced, iters);
+}
+
+static int step_ir(hf_ctx *c, int64_t itime, int64_t *iter, double *err,
+                  int64_t *pair_adv = nullptr) {
+    if (!c->dyn_heat && !(c->debug & 1)) return step_ir_fused(c, itime, iter, err, pair_adv);
     c->carry = false;
     const hf_params &p = c->p;
     launch_step_begin(c->sc, c->st);
@@ -1369,12 +1415,14 @@ static int run_small(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t *iters_t
     return HF_OK;
 }
 
-static int step_common(hf_ctx *c, int64_t itime, int64_t *iter, double *err, bool final_sync) {
+static int step_common(hf_ctx *c, int64_t itime, int64_t *iter, double *err, bool final_sync,
+                       int64_t *pair_adv = nullptr) {
+    if (pair_adv) *pair_adv = 0;
     if (!c->assembled) return set_err(c, HF_ERR_STATE, "hf_step: not assembled");
     if (!c->has_state) return set_err(c, HF_ERR_STATE, "hf_step: no state (hf_set_state)");
     if (c->setup_flags) return flags_to_rc(c, c->setup_flags);
     if (small_path(c)) return run_small(c, itime, 1, nullptr, iter, err);
-    int rc = c->p.precision == HF_PREC_FP32_IR ? step_ir(c, itime, iter, err)
+    int rc = c->p.precision == HF_PREC_FP32_IR ? step_ir(c, itime, iter, err, pair_adv)
                                                : step_fp64(c, itime, iter, err);
     if (rc != HF_OK) return rc;
     if (final_sync) {
@@ -1412,12 +1460,15 @@ extern "C" int hf_run(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t *iters_
             }
         }
         // NaN in x of step k is seen at the next step's first synchronisation (the flag is
-        // sticky within a step) or at the final check
-        int rc = step_common(c, itime0 + k, &it, nullptr, k + 1 == nsteps);
+        // sticky within a step) or at the final check.  A step whose inner solve ends on its
+        // first iteration may take the next step with it (pair_adv 2).
+        int64_t pair_adv = 0;
+        int rc = step_common(c, itime0 + k, &it, nullptr, k + 1 == nsteps,
+                             k + 1 < nsteps ? &pair_adv : nullptr);
         if (rc != HF_OK) return rc;
         if (c->h_sc->flags & FLAG_NAN_X) return flags_to_rc(c, FLAG_NAN_X);
         tot += it;
-        k += 1;
+        k += pair_adv > 0 ? pair_adv : 1;
     }
     if (iters_total) *iters_total = tot;
     return HF_OK;

@@ -638,15 +638,17 @@ k_fin_step(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int64_t s
 // After a two-step pass (k_ir_step2_z): the sums of both steps, their tests and the outcome in
 // sc->pair_ok[0]: 2 both steps stand, 1 only step n (step n+1's test failed, its TINY outcome
 // differed from the guess, or a NaN guard of step n+1 tripped), 0 neither (step n's test
-// failed or a NaN guard of step n tripped: the host runs step n again, alone).  Step n meets
+// failed or a NaN guard of step n tripped: the host runs step n again, alone, or, when step n
+// took alpha from its own solve (`solved`), goes on with its single-step pass).  Step n meets
 // the test of a sweep that reused alpha (k_fin_step: half the inner target and the outer
-// tolerance); step n+1 the same test with its targets formed from the pass's own sums, as
-// k_carry_start forms them from the carried ones.  The step length of the last step that stands
-// is left in inner->alpha for the next reuse (unless `freeze`).  pair_ok[1]: the TINY outcome of
-// the step after the pair (carry the pass's residual if it equals the guess).
+// tolerance), or the outer tolerance alone when `solved`; step n+1 the reuse test with its
+// targets formed from the pass's own sums, as k_carry_start forms them from the carried ones.
+// The step length of the last step that stands and reused alpha is left in inner->alpha for
+// the next reuse (unless `freeze`).  pair_ok[1]: the TINY outcome of the step after the pair
+// (carry the pass's residual if it equals the guess).
 __global__ void __launch_bounds__(kBlock)
 k_fin_step2(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int32_t guess,
-            int32_t freeze, double tol, double inner_tol, double safety) {
+            int32_t freeze, int32_t solved, double tol, double inner_tol, double safety) {
     double v[8];
     sum_partials<8>(partials, nb, v);
     if (threadIdx.x != 0) return;
@@ -655,8 +657,9 @@ k_fin_step2(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int32_t 
     const int32_t nan1 = sc->flags2 & (FLAG_NAN_X2 | FLAG_NAN_SCAT2 | FLAG_NAN_S2);
     const bool t1 = (sc->flags & FLAG_TINY_NEXT) != 0;
     const bool t2 = (sc->flags2 & FLAG_TINY_NEXT2) != 0;
-    // step n: ||r'||^2 = v0, ||b^_{n+1}||^2 = v1, ||r^0_{n+1}||^2 = v2, r^0_n . r' = v3
-    const bool ok0 = !nan0 && v[0] <= fmin(sc->reuse_thresh2, sc->thresh2);
+    // step n: ||r'||^2 = v0, ||b^_{n+1}||^2 = v1, ||r^0_{n+1}||^2 = v2, r^0_n . r' = v3; a
+    // step whose alpha came from its own solve meets the outer tolerance only (as k_fin_step)
+    const bool ok0 = !nan0 && v[0] <= (solved ? sc->thresh2 : fmin(sc->reuse_thresh2, sc->thresh2));
     // step n+1: v4 .. v7 likewise, its targets from v1, v2
     const double thresh2_1 = tol * tol * v[1];
     const double t_rel = inner_tol * inner_tol * v[2];
@@ -671,7 +674,7 @@ k_fin_step2(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int32_t 
         if (outcome == 2) {
             r0r0 = v[2];
             den = r0r0 - v[7];
-        } else if (outcome == 1) {
+        } else if (outcome == 1 && !solved) {
             r0r0 = sc->rr_start;
             den = r0r0 - v[3];
         }
@@ -688,6 +691,11 @@ k_fin_step2(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int32_t 
         sc->flags_next = 0;
     } else if (outcome == 1) {
         sc->outer_rr = v[0];
+        sc->flags_next = 0;
+    } else if (solved) {
+        // nothing stands: step n goes on from its solve with the single-step pass, which sets
+        // the flags again
+        sc->flags &= ~(FLAG_NAN_X | FLAG_TINY_NEXT);
         sc->flags_next = 0;
     } else {
         // nothing stands: step n runs again from the state with its own TINY outcome
@@ -870,11 +878,11 @@ void launch_fin_step(const Dims &g, const double *partials, Scal *sc, Scal *inne
            (int32_t)tiny_guess, (int32_t)reused, (int32_t)freeze);
 }
 void launch_fin_step2(const Dims &g, const double *partials, Scal *sc, Scal *inner,
-                      bool tiny_guess, bool freeze, double tol, double inner_tol, double safety,
-                      hipStream_t st) {
+                      bool tiny_guess, bool freeze, bool solved, double tol, double inner_tol,
+                      double safety, hipStream_t st) {
     // after k_ir_step2_z (its own tiles)
     HF_FIN(k_fin_step2, g.p2blk, st, partials, sc, inner, (int32_t)tiny_guess,
-           (int32_t)freeze, tol, inner_tol, safety);
+           (int32_t)freeze, (int32_t)solved, tol, inner_tol, safety);
 }
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st) {

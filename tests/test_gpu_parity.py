@@ -566,6 +566,24 @@ def test_fp32_ir_pairs_bit_identical_with_frozen_alpha(n):
     assert np.array_equal(bits(ta), bits(tb)) and np.array_equal(bits(tpa), bits(tpb))
 
 
+def test_fp32_ir_pair_after_first_iteration_stop():
+    """A carried step whose inner solve ends on its first iteration (y = alpha r^0, with that
+    solve's alpha) runs as the first step of a two-step pass whose second step reuses the same
+    alpha.  In the log-normal workload at 24^3 and tol 1e-11 the second step is the first whose
+    solve stops there, and every later step reuses alpha: 39 steps run as one single step and
+    19 pairs (the first of them starting from a solve), where pairs of reusing steps alone
+    would leave steps 1, 2 and 39 single.  With alpha frozen (debug bit 8) the result and the
+    statistics are those of the same 39 steps without pairs (debug bit 7), bit for bit."""
+    s, _ = workloads.c3(24)
+    (ta, tpa), sa, ka = _mixed_run(s, 39, 256, tol=1e-11)
+    (tb, tpb), sb, kb = _mixed_run(s, 39, 256 | 128, tol=1e-11)
+    n2, n1 = ka.get("k_ir_step2", (0.0, 0, 0.0))[1], ka.get("k_ir_step", (0.0, 0, 0.0))[1]
+    print(f"solve then pair: pairs {n2}, single passes {n1}, stats {sa} / {sb}", flush=True)
+    assert sa == sb and sa[3] >= 1
+    assert (n2, n1) == (19, 1)
+    assert np.array_equal(bits(ta), bits(tb)) and np.array_equal(bits(tpa), bits(tpb))
+
+
 @pytest.mark.parametrize("case", ["gaussian", "uniform_insulated", "fourier"])
 def test_fp32_ir_pairs_match_oracle(case):
     """Pairs with the step length updated after each pair (the default) against the oracle and

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -336,12 +338,15 @@ extern "C" int hf_create(hf_ctx **out, const hf_params *p) {
     }
     int rc = HF_OK;
     if (!allocs_base(c, &rc) || (rc = alloc_mode(c)) != HF_OK ||
-        hipHostMalloc((void **)&c->h_sc, 2 * sizeof(Scal), 0) != hipSuccess) {
+        hipHostMalloc((void **)&c->h_sc, 2 * sizeof(Scal) + 64, hipHostMallocCoherent) !=
+            hipSuccess) {
         std::string m = c->err.empty() ? "hf_create: allocation failed" : c->err;
         hf_destroy(c);
         return set_err(nullptr, rc ? rc : HF_ERR_ALLOC, m);
     }
     c->h_sci = c->h_sc + 1;
+    c->h_seq = (int64_t *)(c->h_sc + 2);
+    *(volatile int64_t *)c->h_seq = 0;
     hipMemsetAsync(c->sc, 0, 2 * sizeof(Scal), c->st);
     if (hipStreamSynchronize(c->st) != hipSuccess) {
         hf_destroy(c);
@@ -547,9 +552,27 @@ static HeatParams heat_params(const hf_ctx *c, int64_t itime) {
 }
 
 // reads back `count` consecutive scalar blocks
+// Brings `count` solver-scalar records to the host once the stream's work before them is done.
+// The device copies them into the pinned mirror and then raises a sequence number (k_publish),
+// which the host watches for up to 2 ms before it falls back to a blocking synchronisation of
+// the stream (which also reports a failed kernel): watching the word saves the blocking wait's
+// wake-up latency, tens of microseconds a step.
+static_assert(sizeof(Scal) % 8 == 0, "k_publish copies 8-byte words");
 static int sync_scal(hf_ctx *c, Scal *dev, Scal *host, int count = 1) {
-    HIPCHK(c, hipMemcpyAsync(host, dev, count * sizeof(Scal), hipMemcpyDeviceToHost, c->st));
+    const int64_t want = ++c->seq;
+    launch_publish(dev, host, count * sizeof(Scal), c->h_seq, want, c->st);
+    TRY(check_launch(c, "k_publish"));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t k = 1;; ++k) {
+        if (*(volatile int64_t *)c->h_seq == want) {
+            std::atomic_thread_fence(std::memory_order_acquire);
+            return HF_OK;
+        }
+        if ((k & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2))
+            break;
+    }
     HIPCHK(c, hipStreamSynchronize(c->st));
+    std::atomic_thread_fence(std::memory_order_acquire);
     return HF_OK;
 }
 

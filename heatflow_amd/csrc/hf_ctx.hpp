@@ -70,8 +70,10 @@ struct hf_ctx {
     // reductions / scalars
     Scal *sc = nullptr, *sci = nullptr;
     double *partials = nullptr;
-    Scal *h_sc = nullptr;        // pinned mirror
+    Scal *h_sc = nullptr;        // pinned mirror (coherent: k_publish writes it)
     Scal *h_sci = nullptr;
+    int64_t *h_seq = nullptr;    // pinned: the sequence number of the last k_publish
+    int64_t seq = 0;
 
     bool uploaded = false, assembled = false, has_state = false;
     bool any_heater = false, dyn_heat = false;

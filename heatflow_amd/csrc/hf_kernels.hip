@@ -889,6 +889,22 @@ void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_
     // after k_step64_z (pipelined z-chunking)
     HF_FIN(k_fin_next, g.zlay[kZPipe].zblk, st, partials, sc, (int32_t)tiny_guess);
 }
+// Copies `words` 8-byte words from src (device) to dst (pinned host memory), then, after a
+// system-scope fence, writes `seq` to *seq_host: the host sees the copy complete once it sees
+// the sequence number (one thread: a few hundred bytes).
+__global__ void k_publish(const int64_t *src, int64_t *dst, int32_t words, int64_t *seq_host,
+                          int64_t seq) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int32_t k = 0; k < words; ++k) dst[k] = src[k];
+    __threadfence_system();
+    *(volatile int64_t *)seq_host = seq;
+}
+void launch_publish(const void *src, void *dst, size_t bytes, int64_t *seq_host, int64_t seq,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, (const int64_t *)src,
+                       (int64_t *)dst, (int32_t)(bytes / 8), seq_host, seq);
+}
+
 void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
                         double inner_tol, int64_t inner_itmax, double safety, bool reuse,
                         hipStream_t st) {

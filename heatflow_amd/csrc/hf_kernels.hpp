@@ -231,6 +231,9 @@ void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_
 // null), the first inner solve's scalars (as launch_inner_begin(sc, inner, ..., outer_tol =
 // tol)), in one launch.  reuse (mixed): no first inner solve; the first sweep takes
 // y = alpha r^0 with the last solve's alpha and must meet the inner target
+// copy of `bytes` (a multiple of 8) device bytes into pinned host memory, then *seq_host = seq
+void launch_publish(const void *src, void *dst, size_t bytes, int64_t *seq_host, int64_t seq,
+                    hipStream_t st);
 void launch_carry_start(Scal *sc, Scal *inner, double tol, int64_t itmax, bool allow_short,
                         double inner_tol, int64_t inner_itmax, double safety, bool reuse,
                         hipStream_t st);

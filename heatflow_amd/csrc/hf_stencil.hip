@@ -1367,7 +1367,6 @@ static_assert(kAN - kTT <= kTT && kBN - kTT <= kTT, "at most two positions a thr
 struct Step2Lds {
     D2 a[3][kAN];
     D2 b[kBN];
-    double acc[8][kTT];   // each thread's residual sums (its own slots)
 };
 
 // the k-th position (k >= 0) of a region of W x H around the tile (ring width R): k < T is tile
@@ -1404,6 +1403,66 @@ __device__ __forceinline__ T ld_at(const T *p, uint32_t byte_off) {
 template <typename T>
 __device__ __forceinline__ void st_at(T *p, uint32_t byte_off, T v) {
     __builtin_nontemporal_store(v, (T *)((char *)p + byte_off));
+}
+
+// ---- cross-lane sums (gfx950 DPP and permlane swaps) ----------------------------------------
+
+// A 64-bit value of another lane under DPP control Ctrl (both halves moved alike; all lanes of
+// the wave active).
+template <int Ctrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, Ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), Ctrl, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32));
+}
+__device__ __forceinline__ double f64_of(uint32_t lo, uint32_t hi) {
+    return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
+}
+// a's upper 32 lanes <-> b's lower 32 lanes
+__device__ __forceinline__ void swap32_f64(double &a, double &b) {
+    const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+    const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)ua, (uint32_t)ub, false, false);
+    const auto hi =
+        __builtin_amdgcn_permlane32_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+    a = f64_of(lo[0], hi[0]);
+    b = f64_of(lo[1], hi[1]);
+}
+// a's odd 16-lane rows <-> b's even rows
+__device__ __forceinline__ void swap16_f64(double &a, double &b) {
+    const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+    const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)ua, (uint32_t)ub, false, false);
+    const auto hi =
+        __builtin_amdgcn_permlane16_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+    a = f64_of(lo[0], hi[0]);
+    b = f64_of(lo[1], hi[1]);
+}
+// Eight values a lane summed over the wave's 64 lanes: lane L gets the sum of component L >> 3.
+// Each exchange halves the components a lane carries (32-lane swap, 16-lane swap, a rotation by
+// 8 within rows), then three more sum the 8 lanes of a group; the order is fixed, so the sums
+// are deterministic.  All lanes must be active.
+__device__ __forceinline__ double wave_sum8(const double (&v)[8]) {
+    double w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double a = v[k], b = v[k + 4];
+        swap32_f64(a, b);
+        w[k] = a + b;   // lanes 0-31: component k; 32-63: component k + 4
+    }
+    double u[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        double a = w[k], b = w[k + 2];
+        swap16_f64(a, b);
+        u[k] = a + b;   // rows 0-3 (16 lanes each): components (0, 1), (2, 3), (4, 5), (6, 7)
+    }
+    const bool hi8 = (threadIdx.x & 8) != 0;
+    const double keep = hi8 ? u[1] : u[0], send = hi8 ? u[0] : u[1];
+    double x = keep + dpp_f64<0x128>(send);   // row_ror:8 -> component L >> 3
+    x = x + dpp_f64<0x141>(x);                // row_half_mirror
+    x = x + dpp_f64<0x4E>(x);                 // quad_perm [2, 3, 0, 1]
+    x = x + dpp_f64<0xB1>(x);                 // quad_perm [1, 0, 3, 2]
+    return x;
 }
 
 template <bool Cattaneo, bool UniformRc>
@@ -1463,8 +1522,9 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     auto pos_x = [](uint32_t v) { return (int)(v & 0xffffu); };
     auto pos_y = [](uint32_t v) { return (int)(v >> 16); };
     auto mod3 = [](int v) { return ((v % 3) + 3) % 3; };
-#pragma unroll
-    for (int k = 0; k < 8; ++k) L.acc[k][t] = 0.0;
+    // the residual sums: each trip's eight per-cell terms are summed over the wave at once
+    // (wave_sum8), and lane L accumulates component L >> 3
+    double acc = 0.0;
     int32_t fl = 0, fl2 = 0;
 
     // Loads.  Every thread issues the same loads on every trip, without branches, so that the
@@ -1558,6 +1618,8 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         // are the neighbouring chunks')
         const bool own_plane = p >= (int)z0 && p < (int)z1;
         const D2 *Am = L.a[mod3(p - 1)], *A0 = L.a[mod3(p)], *Ap = L.a[mod3(p + 1)];
+        // this trip's terms of the residual sums: step n on plane p (0-3), step n+1 on p - 1
+        double tv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         // the own cell's operator row, S_{n+1}, x'_n, r^0_{n+1} and b: step n+1 on plane p
         D2 o_b = 0.0;
         double o_d = 0.0, o_cxi = 0.0, o_cyi = 0.0, o_czi = 0.0, o_cxm = 0.0, o_cym = 0.0;
@@ -1626,10 +1688,10 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     if (isnan(S1)) fl |= FLAG_NAN_S;
                     const float rf = (float)(-s * (Sn - hsum([](const D2 &w) { return w.a; })));
                     const double bh = s * S1;
-                    L.acc[0][t] += (double)rf * (double)rf;
-                    L.acc[1][t] += bh * bh;
-                    L.acc[2][t] += (double)fn * (double)fn;
-                    L.acc[3][t] += (double)own_r * (double)rf;
+                    tv[0] = (double)rf * (double)rf;
+                    tv[1] = bh * bh;
+                    tv[2] = (double)fn * (double)fn;
+                    tv[3] = (double)own_r * (double)rf;
                     if (isnan(xi.a)) fl |= FLAG_NAN_X;
                     if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;
                     // step n+1's S_{n+2} from (x'_{n+1}, x'_n), and its tiny-guess row sum but
@@ -1683,10 +1745,13 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             if (guess) ra = ra + kTiny * w_s * (w_h1 + w_cz);
             const float fn = (float)ra;
             st_at(rnext, (ocol + P * (uint32_t)p2) * 4u, fn);
-            L.acc[4][t] += (double)rf * (double)rf;
-            L.acc[6][t] += (double)fn * (double)fn;
-            L.acc[7][t] += (double)w_fn * (double)rf;
+            const double bh = w_s * w_S2;
+            tv[4] = (double)rf * (double)rf;
+            tv[5] = bh * bh;
+            tv[6] = (double)fn * (double)fn;
+            tv[7] = (double)w_fn * (double)rf;
         }
+        acc += wave_sum8(tv);
         own_tp = al0.tp;
         own_tpp = al0.tpp;
         own_d = al0.d;
@@ -1723,8 +1788,6 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             const uint32_t o8 = (ocol + P * (uint32_t)p) * 8u;
             st_at(x1out, o8, o_xn);
             st_at(x2out, o8, xi.a);
-            const double bh = o_s * o_S2;
-            L.acc[5][t] += bh * bh;
             if (isnan(xi.a)) fl2 |= FLAG_NAN_X2;
             if (xi.b - xi.a < kTiny) fl2 |= FLAG_TINY_NEXT2;
         }
@@ -1733,11 +1796,19 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     block_flags(fl & (FLAG_NAN_X | FLAG_TINY_NEXT), &sc->flags);
     block_flags(fl & (FLAG_NAN_SCAT | FLAG_NAN_S), &sc->flags_next);
     block_flags(fl2, &sc->flags2);
-    // partials: components 0-3 (step n), then 4-7 (step n+1)
-    double acc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = L.acc[k][t];
-    block_partials<8>(acc, partials);
+    // partials: components 0-3 (step n), then 4-7 (step n+1); the waves' sums in wave order,
+    // staged in the a-planes (no longer read)
+    static_assert(kTT / 64 * 8 <= 2 * kAN, "wave sums fit an a-plane");
+    double *red = &L.a[0][0].a;
+    const int lane = t & 63, wv = t >> 6;
+    __syncthreads();
+    if ((lane & 7) == 0) red[(lane >> 3) * (kTT / 64) + wv] = acc;
+    __syncthreads();
+    if (t < 8) {
+        double sum = 0.0;
+        for (int w = 0; w < kTT / 64; ++w) sum += red[t * (kTT / 64) + w];
+        partials[(size_t)t * gridDim.x + blockIdx.x] = sum;
+    }
 }
 
 void launch_ir_step2(const Dims &g0, const StepParams &sp, const double *tp, const double *tpp,

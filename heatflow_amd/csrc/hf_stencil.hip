@@ -1367,6 +1367,9 @@ static_assert(kAN - kTT <= kTT && kBN - kTT <= kTT, "at most two positions a thr
 struct Step2Lds {
     D2 a[3][kAN];
     D2 b[kBN];
+    // the Jacobi scale, diagonal and T^n of plane q's b-region cells, formed by the operand
+    // stage on trip q and read by step n on trip q + 1 (two planes rotate)
+    double sdt[2][3][kBN];
 };
 
 // the k-th position (k >= 0) of a region of W x H around the tile (ring width R): k < T is tile
@@ -1570,10 +1573,6 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         v.cym = ld_at(cy, oym);
         v.czm = ld_at(cz, ozm);
         v.qb = ld_at(qb, o8);
-        if (k == 1) {
-            v.d = ld_at(d, o8);
-            v.tp = ld_at(tp, o8);
-        }
         if (!UniformRc) {
             v.rc = rhoc[J & sp_.rc_mask];
             v.cv = Cattaneo ? cvec[J & sp_.rc_mask] : 0.0;
@@ -1582,7 +1581,7 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     };
 
     // own-cell values handed from stage to stage
-    double own_tp = 0.0, own_tpp = 0.0, own_d = 0.0;   // a(q) -> step n on plane q
+    double own_tpp = 0.0;   // a(q) -> step n on plane q
     float own_r = 0.0f;
     D2 b_prev = 0.0;   // own b of the plane before the one step n ran on last
     // step n+1 at the own cell of plane p, formed on the trip step n ran on p, with the z+ term
@@ -1599,15 +1598,25 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         // ---- a(q): x'_n and x1_n on the tile and a two-cell ring
         D2 *A = L.a[mod3(q)];
         const bool zin = q >= 0 && q < (int)g.nz;
+        double(*SD)[kBN] = L.sdt[q & 1];
         auto stage_a = [&](const ALd &v, int k) {
             D2 o = 0.0;
+            const int px = pos_x(apos[k]), py = pos_y(apos[k]);
             if (zin && acol[k] != kNoCol) {
+                const double sa = 1.0 / sqrt(-v.d);
                 double base = v.tp + (v.tp - v.tpp);   // mod_evolve.f90:140
                 if (tiny) base = base + kTiny;         // :141
-                const double xn = base + (1.0 / sqrt(-v.d)) * (double)(ysc * v.r);
+                const double xn = base + sa * (double)(ysc * v.r);
                 o = D2(xn, xn + (xn - v.tp));
+                // inside the b-region (all but the outer ring): for step n on the next trip
+                if (px >= 1 && px <= kBW && py >= 1 && py <= kBH) {
+                    const int bi = (py - 1) * kBW + (px - 1);
+                    SD[0][bi] = sa;
+                    SD[1][bi] = v.d;
+                    SD[2][bi] = v.tp;
+                }
             }
-            A[pos_y(apos[k]) * kAW + pos_x(apos[k])] = o;
+            A[py * kAW + px] = o;
         };
         stage_a(al0, 0);
         if (two_a) stage_a(al1, 1);
@@ -1635,12 +1644,14 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 const bool xm_in = x > 0, ym_in = y > 0, zm_in = p > 0;
                 const int ai = (py + 1) * kAW + (px + 1);   // this position in an a-plane
                 const D2 xi = A0[ai];
-                const double di = k == 0 ? own_d : ld.d;
+                const int sb = py * kBW + px;   // this position in the b-region
+                const double(*SDp)[kBN] = L.sdt[p & 1];
+                const double di = SDp[1][sb];
                 const double cxi = ld.cx, cyi = ld.cy, czi = ld.cz;
                 const double cxm = ld.cxm, cym = ld.cym;
                 const double czm = zm_in ? ld.czm : 0.0;
                 const double qbi = ld.qb;
-                const double tpi = k == 0 ? own_tp : ld.tp;
+                const double tpi = SDp[2][sb];
                 const double rci = UniformRc ? rc_u : ld.rc;
                 const double cvi = Cattaneo ? (UniformRc ? cv_u : ld.cv) : 0.0;
                 // H x (pick: x' or x1) in the reference's serial order; an a-plane holds 0 at
@@ -1656,7 +1667,7 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     h = h + czi * pick(Ap[ai]);
                     return h;
                 };
-                const double s = 1.0 / sqrt(-di);
+                const double s = SDp[0][sb];
                 // S_{n+1} from (x'_n, T^n) (mod_evolve.f90:112-119, S_catS)
                 double S1 = -(it * xi.a * rci) - qbi;
                 if (Cattaneo) {
@@ -1752,9 +1763,7 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             tv[7] = (double)w_fn * (double)rf;
         }
         acc += wave_sum8(tv);
-        own_tp = al0.tp;
         own_tpp = al0.tpp;
-        own_d = al0.d;
         own_r = al0.r;
         // the next trip's loads: a(q + 1), then step n on plane q
         al0 = a_load(0, q + 1);

@@ -83,7 +83,10 @@ struct hf_ctx {
     bool uniform_len = false;    // each axis has one cell length (the adjoint's precondition)
     double heated_volume = 0.0;
     int32_t setup_flags = 0;
-    int64_t pred_iters = 4, pred_inner = 8, pred_adj = 8;
+    // CG iterations queued before the first read-back of a solve (then the last solve's count
+    // + 1, doubling while a batch falls short): launches queued past convergence return at once
+    // but still cost a launch each
+    int64_t pred_iters = 4, pred_inner = 4, pred_adj = 8;
     double adj_tol = 0.0;        // tolerance of the adjoint solves (0: p.tol)
     hf_stats stats{};
     // persistent small-grid steps (hf_persist.hip): run summary and per-step heat parameters

@@ -197,11 +197,28 @@ static void make_dims(Dims &g, const hf_params &p, int cus) {
     g.ztiles4 = l0.ztiles4;
     g.zblk = l0.zblk;
     {
-        // the two-step pass's tiles, over the pipelined kernels' z-chunks
-        const uint32_t zc = g.zlay[kZPipe].zchunk;
+        // the two-step pass: one block a CU (its LDS), and a z-march of zc planes takes zc + 4
+        // trips (the pipeline's fill and drain), so the chunk count c minimises
+        // ceil(tiles(c) / CUs) * (ceil(nz / c) + 4): whole columns when a plane has about as many
+        // tiles as there are CUs (512^3 on 256 CUs: 256 columns, one round)
         g.p2x = (g.nx + kP2X - 1) / kP2X;
         g.p2y = (g.ny + kP2Y - 1) / kP2Y;
-        g.p2tiles = g.p2x * g.p2y * ((g.nz + zc - 1) / zc);
+        const uint64_t per_plane = (uint64_t)g.p2x * g.p2y;
+        uint32_t best_c = 1;
+        uint64_t best = ~0ull;
+        for (uint32_t c = 1; c <= std::max(1u, g.nz / 16u); ++c) {
+            const uint32_t zc = (g.nz + c - 1) / c;
+            if ((g.nz + zc - 1) / zc != c) continue;   // c chunks of zc planes, none empty
+            const uint64_t rounds = (per_plane * c + cus - 1) / cus;
+            const uint64_t cost = rounds * (zc + 4);
+            if (cost < best) {
+                best = cost;
+                best_c = c;
+            }
+        }
+        if (const char *e = getenv("HF_P2_CHUNKS")) best_c = std::max(1, atoi(e));
+        g.p2zc = (g.nz + best_c - 1) / best_c;
+        g.p2tiles = (uint32_t)(per_plane * ((g.nz + g.p2zc - 1) / g.p2zc));
         g.p2blk = (g.p2tiles + 7) / 8 * 8;
     }
 }

@@ -62,9 +62,9 @@ struct Dims {
     struct ZLayout {
         uint32_t zchunk, ztiles, zchunk2, ztiles2, zchunk4, ztiles4, zblk;
     } zlay[3];
-    // the two-step pass (k_ir_step2_z): tiles of kP2X x kP2Y columns, p2x x p2y a plane, the
-    // pipelined kernels' z-chunks; p2tiles tiles, p2blk blocks (a multiple of 8)
-    uint32_t p2x, p2y, p2tiles, p2blk;
+    // the two-step pass (k_ir_step2_z): tiles of kP2X x kP2Y columns, p2x x p2y a plane, z-chunks
+    // of p2zc planes; p2tiles tiles, p2blk blocks (a multiple of 8)
+    uint32_t p2x, p2y, p2zc, p2tiles, p2blk;
 };
 // 128 x 8 (1024 threads, one block a CU by its LDS): of 16x16, 32x8, 32x16, 64x8, 64x16,
 // 128x8 and 256x4 tiles timed on the 512^3 workload, the fastest -- the wide rows keep the

@@ -1490,7 +1490,7 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     const uint32_t per_plane = g.p2x * g.p2y;
     const uint32_t zc = lt / per_plane, rem = lt - zc * per_plane;
     const uint32_t tby = rem / g.p2x, tbx = rem - tby * g.p2x;
-    const uint32_t zchunk = g.zlay[kZPipe].zchunk;
+    const uint32_t zchunk = g.p2zc;
     const uint32_t z0 = zc * zchunk, z1 = z0 + zchunk < g.nz ? z0 + zchunk : g.nz;
     const bool zvalid = lt < g.p2tiles && z0 < g.nz;
     const int t = (int)threadIdx.x;

@@ -1207,7 +1207,7 @@ static int pair_pass(hf_ctx *c, bool solved, int64_t *advanced, int64_t *iters) 
                     c->cx, c->cy, c->cz, x1, x2, rout, c->sc, c->partials, c->st);
     tend(c, sl);
     launch_fin_step2(c->g, c->partials, c->sc, c->sci, c->tiny_guess, (c->debug & 256) != 0,
-                     solved, p.tol, p.inner_tol, kInnerSafety, c->st);
+                     solved, (c->debug & 512) != 0, p.tol, p.inner_tol, kInnerSafety, c->st);
     TRY(check_launch(c, "k_ir_step2"));
     TRY(sync_scal(c, c->sc, c->h_sc, 2));
     const int32_t outcome = c->h_sc->pair_ok[0];

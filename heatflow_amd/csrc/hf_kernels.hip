@@ -645,10 +645,12 @@ k_fin_step(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int64_t s
 // targets formed from the pass's own sums, as k_carry_start forms them from the carried ones.
 // The step length of the last step that stands and reused alpha is left in inner->alpha for
 // the next reuse (unless `freeze`).  pair_ok[1]: the TINY outcome of the step after the pair
-// (carry the pass's residual if it equals the guess).
+// (carry the pass's residual if it equals the guess).  `reject0` (a test aid) fails step n's
+// test, so that every pair falls back.
 __global__ void __launch_bounds__(kBlock)
 k_fin_step2(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int32_t guess,
-            int32_t freeze, int32_t solved, double tol, double inner_tol, double safety) {
+            int32_t freeze, int32_t solved, int32_t reject0, double tol, double inner_tol,
+            double safety) {
     double v[8];
     sum_partials<8>(partials, nb, v);
     if (threadIdx.x != 0) return;
@@ -659,7 +661,8 @@ k_fin_step2(uint32_t nb, const double *partials, Scal *sc, Scal *inner, int32_t 
     const bool t2 = (sc->flags2 & FLAG_TINY_NEXT2) != 0;
     // step n: ||r'||^2 = v0, ||b^_{n+1}||^2 = v1, ||r^0_{n+1}||^2 = v2, r^0_n . r' = v3; a
     // step whose alpha came from its own solve meets the outer tolerance only (as k_fin_step)
-    const bool ok0 = !nan0 && v[0] <= (solved ? sc->thresh2 : fmin(sc->reuse_thresh2, sc->thresh2));
+    const bool ok0 = !reject0 && !nan0 &&
+                     v[0] <= (solved ? sc->thresh2 : fmin(sc->reuse_thresh2, sc->thresh2));
     // step n+1: v4 .. v7 likewise, its targets from v1, v2
     const double thresh2_1 = tol * tol * v[1];
     const double t_rel = inner_tol * inner_tol * v[2];
@@ -878,11 +881,11 @@ void launch_fin_step(const Dims &g, const double *partials, Scal *sc, Scal *inne
            (int32_t)tiny_guess, (int32_t)reused, (int32_t)freeze);
 }
 void launch_fin_step2(const Dims &g, const double *partials, Scal *sc, Scal *inner,
-                      bool tiny_guess, bool freeze, bool solved, double tol, double inner_tol,
-                      double safety, hipStream_t st) {
+                      bool tiny_guess, bool freeze, bool solved, bool reject0, double tol,
+                      double inner_tol, double safety, hipStream_t st) {
     // after k_ir_step2_z (its own tiles)
     HF_FIN(k_fin_step2, g.p2blk, st, partials, sc, inner, (int32_t)tiny_guess,
-           (int32_t)freeze, (int32_t)solved, tol, inner_tol, safety);
+           (int32_t)freeze, (int32_t)solved, (int32_t)reject0, tol, inner_tol, safety);
 }
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st) {

@@ -221,8 +221,8 @@ void launch_ir_step2(const Dims &g, const StepParams &sp, const double *tp, cons
                      const double *cz, double *x1out, double *x2out, float *rnext, Scal *sc,
                      double *partials, hipStream_t st);
 void launch_fin_step2(const Dims &g, const double *partials, Scal *sc, Scal *inner,
-                      bool tiny_guess, bool freeze, bool solved, double tol, double inner_tol,
-                      double safety, hipStream_t st);
+                      bool tiny_guess, bool freeze, bool solved, bool reject0, double tol,
+                      double inner_tol, double safety, hipStream_t st);
 // after launch_step64: the next step's norms for the TINY case `tiny_guess`
 void launch_fin_next(const Dims &g, const double *partials, Scal *sc, bool tiny_guess,
                      hipStream_t st);

@@ -218,11 +218,13 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * multi-kernel path instead of the persistent one (FP64 contexts of at most 8192 cells run
  * whole steps, up to 256 per launch, in one workgroup: no launch per CG iteration and one host
  * synchronisation per launch; its sums group differently, so the two paths agree to the
- * tolerance, not bit for bit); bit 7 turns off the two-step pass of mixed mode (where two
- * consecutive steps would both reuse the inner step length, one z-march does both, reading the
- * operator, Qdens + B and the state once; a step whose test misses is redone alone); bit 8
- * keeps the reused step length fixed instead of updating it after each reusing step (then
- * the two-step pass and single steps give the same bits). */
+ * tolerance, not bit for bit); bit 7 turns off the two-step pass of mixed mode (where a step
+ * would reuse the inner step length, or its own inner solve ends on its first iteration, one
+ * z-march does it and the next step, reading the operator, Qdens + B and the state once; a step
+ * whose test misses is redone alone); bit 8 keeps the reused step length fixed instead of
+ * updating it after each reusing step (then the two-step pass and single steps give the same
+ * bits); bit 9 fails the first step's test in every two-step pass, so that each pair falls back
+ * (a test aid). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */

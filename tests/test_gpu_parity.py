@@ -632,6 +632,32 @@ def test_fp32_ir_pairs_with_misses():
     assert ea <= 2e-10 and ea <= 1.2 * eb + 1e-12
 
 
+def test_fp32_ir_pair_fallbacks():
+    """Both ways out of a two-step pass whose first step misses its test: a pair started from a
+    first-iteration solve leaves the step to its single pass, and a pair of reusing steps
+    reruns step n alone from the state and pauses reuse.  Debug bit 9 fails that test in every
+    pair, so that each pair falls back: 40 steps of the log-normal workload at 24^3 and tol
+    1e-11 take both fallbacks (pair launches beyond the reuse misses are solve-started pairs)
+    and end as close to the oracle as the same run without pairs (debug bit 7)."""
+    s, _ = workloads.c3(24)
+    p = s.params
+    t0 = np.full(p.n, p.t_system)
+    rc, otp, otpp, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 41)), t0, t0)
+    assert rc == 0
+    (ta, tpa), sa, ka = _mixed_run(s, 40, 512, tol=1e-11)
+    (tb, tpb), sb, kb = _mixed_run(s, 40, 128, tol=1e-11)
+    n2 = ka.get("k_ir_step2", (0.0, 0, 0.0))[1]
+    ea, eb = relerr(ta, otp), relerr(tb, otp)
+    print(f"pair fallbacks: pair launches {n2}, stats {sa} / {sb}, err {ea:.3g} / {eb:.3g}",
+          flush=True)
+    assert sa[0] == 40 and sb[0] == 40
+    assert sa[7] >= 1 and sa[6] == sa[7]   # every reusing pair missed and was redone
+    assert n2 > sa[7]                      # and some pairs started from a solve
+    assert ea <= 1e-9 and relerr(tpa, otpp) <= 1e-9
+    assert ea <= 1.2 * eb + 1e-12
+    assert relerr(ta, tb) <= 1e-10
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
 def test_uniform_heat_capacity_reads_are_exact(prec):
     """Where rho c_p and tau' rho c_p are the same in every cell (found at upload), the step

@@ -108,6 +108,18 @@ struct D2 {
     __device__ constexpr D2(double x, double y) : a(x), b(y) {}
 };
 
+// 1/sqrt(v), within about an ulp: the hardware estimate y (v_rsq_f64, relative error near
+// 2^-23) refined by one third-order Newton (Householder) step, e = 1 - v y^2,
+// y' = y + y e (1/2 + 3/8 e), which leaves an error of order e^3 before the last roundings.
+// Where e is NaN (v zero, infinite, negative or NaN) the estimate stands: inf, 0 or NaN, as
+// 1/sqrt gives.  Nine instructions, against about thirty for a rounded sqrt and a divide.
+__device__ __forceinline__ double inv_sqrt(double v) {
+    const double y = __builtin_amdgcn_rsq(v);
+    const double e = __builtin_fma(-(v * y), y, 1.0);
+    const double r = __builtin_fma(y * e, __builtin_fma(0.375, e, 0.5), y);
+    return e == e ? r : y;
+}
+
 // One component of the in-plane neighbours of a staged pair.
 template <int K>
 struct InPlaneOf {
@@ -660,11 +672,13 @@ k_ir_res_z(Dims g, IrParams ip, const double *__restrict__ x, const float *__res
 }
 
 // operand inputs of the fused step: T^n, T^{n-1} (or the refined x), the diagonal (s is
-// formed from it: reading s as well would cost 8 bytes a cell, the division and square root
-// are hidden under the pipelined loads) and the inner solution (y, or r0 when scaled by alpha:
-// CgY).  The cell's own record of plane z also gives it T^n, T^{n-1} and the diagonal.
+// formed from it, reading s as well would cost 8 bytes a cell) and the inner solution (y, or
+// r0 when scaled by alpha: CgY).  The cell's own record of plane z also gives it T^n, T^{n-1}
+// and the diagonal.  s = inv_sqrt(-d) may differ from k_scale's 1/sqrt(-d) in the last bit:
+// it scales the update x0 + s y and the residual the pass forms, both of which the refinement
+// measures in FP64 (the two-step pass forms it the same way, so both give the same bits).
 struct StepRaw {
-    double tp, b, d;   // d: the diagonal, s = 1/sqrt(-d) as k_scale forms it
+    double tp, b, d;   // d: the diagonal
     float y;
 };
 
@@ -734,7 +748,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         } else {
             base = r.b;
         }
-        const double xn = base + (1.0 / sqrt(-r.d)) * (double)(ysc * r.y);
+        const double xn = base + inv_sqrt(-r.d) * (double)(ysc * r.y);
         return D2(xn, xn + (xn - r.tp));
     };
     // form for the thread's own cells, also handing back s (same operations)
@@ -746,7 +760,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         } else {
             base = r.b;
         }
-        si = 1.0 / sqrt(-r.d);
+        si = inv_sqrt(-r.d);
         const double xn = base + si * (double)(ysc * r.y);
         return D2(xn, xn + (xn - r.tp));
     };
@@ -1364,13 +1378,18 @@ constexpr int kBW = (int)kP2X + 2, kBH = (int)kP2Y + 2, kBN = kBW * kBH;
 constexpr int kTT = (int)(kP2X * kP2Y);
 static_assert(kAN - kTT <= kTT && kBN - kTT <= kTT, "at most two positions a thread");
 
+// A plane of (x', x1) pairs is held as two arrays, x' then x1, a multiple of 64 doubles apart:
+// a lane reading one half of its neighbours' pairs reads consecutive doubles (no bank
+// conflicts), and one ds_read2st64_b64 reads both halves of a pair.
+constexpr int kANp = (kAN + 63) / 64 * 64, kBNp = (kBN + 63) / 64 * 64;
 struct Step2Lds {
-    D2 a[3][kAN];
-    D2 b[kBN];
+    double a[3][2][kANp];
+    double b[2][kBNp];
     // the Jacobi scale, diagonal and T^n of plane q's b-region cells, formed by the operand
     // stage on trip q and read by step n on trip q + 1 (two planes rotate)
-    double sdt[2][3][kBN];
+    double sdt[2][3][kBNp];
 };
+static_assert(sizeof(Step2Lds) <= 160 * 1024, "the two-step pass's LDS");
 
 // the k-th position (k >= 0) of a region of W x H around the tile (ring width R): k < T is tile
 // cell (k % TX, k / TX) at (k % TX + R, k / TX + R); then the ring, row by row for its full
@@ -1499,8 +1518,12 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     const bool active = zvalid && ox < g.nx && oy < g.ny;
     const uint32_t ocol = ox + g.nx * oy;
     const int64_t gx0 = (int64_t)tbx * kP2X, gy0 = (int64_t)tby * kP2Y;
-    const bool two_a = t < kAN - kTT;   // a second a-position (on the ring)
-    const bool two_b = t < kBN - kTT;   // a second b-position
+    // a second a-position (on the ring) for the first threads; a second b-position for the
+    // threads of the waves after theirs, so that no wave runs two ring stages
+    constexpr int kBOff = (kAN - kTT + 63) / 64 * 64;
+    static_assert(kBOff + (kBN - kTT) <= kTT, "the b-ring's threads");
+    const bool two_a = t < kAN - kTT;
+    const bool two_b = t >= kBOff && t < kBOff + (kBN - kTT);
     const uint32_t P = g.nxny;
     // grid column of a region position (R: ring width), or kNoCol outside the grid
     constexpr uint32_t kNoCol = 0xffffffffu;
@@ -1518,7 +1541,7 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         region_pos<2>(k == 0 || two_a ? t + k * kTT : t, px, py);
         apos[k] = (uint32_t)px | (uint32_t)py << 16;
         acol[k] = k == 0 || two_a ? col_of(px, py, 2) : kNoCol;
-        region_pos<1>(k == 0 || two_b ? t + k * kTT : t, px, py);
+        region_pos<1>(k == 1 && two_b ? kTT + (t - kBOff) : t, px, py);
         bpos[k] = (uint32_t)px | (uint32_t)py << 16;
         bcol[k] = k == 0 || two_b ? col_of(px, py, 1) : kNoCol;
     }
@@ -1596,14 +1619,14 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         const int p = q - 1;
         const BLd bl1 = b_load(1, p);   // the ring position's
         // ---- a(q): x'_n and x1_n on the tile and a two-cell ring
-        D2 *A = L.a[mod3(q)];
+        double *A = L.a[mod3(q)][0];
         const bool zin = q >= 0 && q < (int)g.nz;
-        double(*SD)[kBN] = L.sdt[q & 1];
+        double(*SD)[kBNp] = L.sdt[q & 1];
         auto stage_a = [&](const ALd &v, int k) {
             D2 o = 0.0;
             const int px = pos_x(apos[k]), py = pos_y(apos[k]);
             if (zin && acol[k] != kNoCol) {
-                const double sa = 1.0 / sqrt(-v.d);
+                const double sa = inv_sqrt(-v.d);
                 double base = v.tp + (v.tp - v.tpp);   // mod_evolve.f90:140
                 if (tiny) base = base + kTiny;         // :141
                 const double xn = base + sa * (double)(ysc * v.r);
@@ -1616,7 +1639,8 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     SD[2][bi] = v.tp;
                 }
             }
-            A[py * kAW + px] = o;
+            A[py * kAW + px] = o.a;
+            A[kANp + py * kAW + px] = o.b;
         };
         stage_a(al0, 0);
         if (two_a) stage_a(al1, 1);
@@ -1626,7 +1650,9 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         // the tile's own planes: its residual test, sums and flags (the planes z0 - 1 and z1
         // are the neighbouring chunks')
         const bool own_plane = p >= (int)z0 && p < (int)z1;
-        const D2 *Am = L.a[mod3(p - 1)], *A0 = L.a[mod3(p)], *Ap = L.a[mod3(p + 1)];
+        const double *Am = L.a[mod3(p - 1)][0], *A0 = L.a[mod3(p)][0], *Ap = L.a[mod3(p + 1)][0];
+        // the pair at position i of an a-plane
+        auto ga = [](const double *P, int i) { return D2(P[i], P[kANp + i]); };
         // this trip's terms of the residual sums: step n on plane p (0-3), step n+1 on p - 1
         double tv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         // the own cell's operator row, S_{n+1}, x'_n, r^0_{n+1} and b: step n+1 on plane p
@@ -1643,9 +1669,9 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 const uint32_t x = (uint32_t)(gx0 + px - 1), y = (uint32_t)(gy0 + py - 1);
                 const bool xm_in = x > 0, ym_in = y > 0, zm_in = p > 0;
                 const int ai = (py + 1) * kAW + (px + 1);   // this position in an a-plane
-                const D2 xi = A0[ai];
+                const D2 xi = ga(A0, ai);
                 const int sb = py * kBW + px;   // this position in the b-region
-                const double(*SDp)[kBN] = L.sdt[p & 1];
+                const double(*SDp)[kBNp] = L.sdt[p & 1];
                 const double di = SDp[1][sb];
                 const double cxi = ld.cx, cyi = ld.cy, czi = ld.cz;
                 const double cxm = ld.cxm, cym = ld.cym;
@@ -1658,13 +1684,13 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 // the positions outside the grid, so an absent neighbour adds coefficient x 0
                 auto hsum = [&](auto pick) {
                     double h = 0.0;
-                    h = h + czm * pick(Am[ai]);
-                    h = h + cym * pick(A0[ai - kAW]);
-                    h = h + cxm * pick(A0[ai - 1]);
+                    h = h + czm * pick(ga(Am, ai));
+                    h = h + cym * pick(ga(A0, ai - kAW));
+                    h = h + cxm * pick(ga(A0, ai - 1));
                     h = h + di * pick(xi);
-                    h = h + cxi * pick(A0[ai + 1]);
-                    h = h + cyi * pick(A0[ai + kAW]);
-                    h = h + czi * pick(Ap[ai]);
+                    h = h + cxi * pick(ga(A0, ai + 1));
+                    h = h + cyi * pick(ga(A0, ai + kAW));
+                    h = h + czi * pick(ga(Ap, ai));
                     return h;
                 };
                 const double s = SDp[0][sb];
@@ -1740,7 +1766,8 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     o_fn = fn;
                 }
             }
-            L.b[py * kBW + px] = v;
+            L.b[0][py * kBW + px] = v.a;
+            L.b[1][py * kBW + px] = v.b;
         };
         if (two_b) stage_b(std::integral_constant<int, 1>{}, bl1);
         stage_b(std::integral_constant<int, 0>{}, bl0);
@@ -1775,15 +1802,16 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         if (active && own_plane) {
             const int bi = ((int)ly + 1) * kBW + ((int)lx + 1);
             const D2 xi = o_b;
+            auto gb = [&](int i) { return D2(L.b[0][i], L.b[1][i]); };
             // (the b-plane holds 0 at the positions outside the grid, and b_prev is 0 on plane 0)
             auto hsum6 = [&](auto pick) {
                 double h = 0.0;
                 h = h + o_czm * pick(b_prev);
-                h = h + o_cym * pick(L.b[bi - kBW]);
-                h = h + o_cxm * pick(L.b[bi - 1]);
+                h = h + o_cym * pick(gb(bi - kBW));
+                h = h + o_cxm * pick(gb(bi - 1));
                 h = h + o_d * pick(xi);
-                h = h + o_cxi * pick(L.b[bi + 1]);
-                h = h + o_cyi * pick(L.b[bi + kBW]);
+                h = h + o_cxi * pick(gb(bi + 1));
+                h = h + o_cyi * pick(gb(bi + kBW));
                 return h;
             };
             w_ha = hsum6([](const D2 &w) { return w.a; });
@@ -1807,8 +1835,8 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
     block_flags(fl2, &sc->flags2);
     // partials: components 0-3 (step n), then 4-7 (step n+1); the waves' sums in wave order,
     // staged in the a-planes (no longer read)
-    static_assert(kTT / 64 * 8 <= 2 * kAN, "wave sums fit an a-plane");
-    double *red = &L.a[0][0].a;
+    static_assert(kTT / 64 * 8 <= 2 * kANp, "wave sums fit an a-plane");
+    double *red = L.a[0][0];
     const int lane = t & 63, wv = t >> 6;
     __syncthreads();
     if ((lane & 7) == 0) red[(lane >> 3) * (kTT / 64) + wv] = acc;

@@ -748,7 +748,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         } else {
             base = r.b;
         }
-        const double xn = base + inv_sqrt(-r.d) * (double)(ysc * r.y);
+        const double xn = __builtin_fma(inv_sqrt(-r.d), (double)(ysc * r.y), base);
         return D2(xn, xn + (xn - r.tp));
     };
     // form for the thread's own cells, also handing back s (same operations)
@@ -761,7 +761,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             base = r.b;
         }
         si = inv_sqrt(-r.d);
-        const double xn = base + si * (double)(ysc * r.y);
+        const double xn = __builtin_fma(si, (double)(ysc * r.y), base);
         return D2(xn, xn + (xn - r.tp));
     };
     auto X = [&](uint32_t J) -> D2 { return form(load(J)); };
@@ -780,26 +780,28 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         v.tpp = FromState ? 0.0 : ldnt(&tpp[I]);
         return v;
     };
-    // H v of the cell in the reference's serial order (as hx_sum), K picks x' or x1
+    // H v of the cell in the reference's serial order (as hx_sum), K picks x' or x1; each term
+    // is one fused multiply-add (the mixed step measures its residual in FP64: the rounding of
+    // the products is not the oracle's, unlike the FP64 path's sums)
     auto hsum = [&](uint32_t I, uint32_t z, const StepCell &cv, double di, double cz_m,
                     const D2 &xi, const D2 &xz_m, const D2 &xz_p, const D2 *nbc,
                     auto pick) -> double {
         double h = 0.0;
-        h = h + cz_m * (z > 0 ? pick(xz_m) : 0.0);
-        h = h + cv.cym * (ym_in ? pick(nbc[-(int)(kZTX + 2)]) : 0.0);
-        h = h + cv.cxm * (xm_in ? pick(nbc[-1]) : 0.0);
-        h = h + di * pick(xi);
-        h = h + cv.cx * (xp_in ? pick(nbc[1]) : 0.0);
-        h = h + cv.cy * (yp_in ? pick(nbc[(int)(kZTX + 2)]) : 0.0);
-        h = h + cv.cz * (z + 1 < g.nz ? pick(xz_p) : 0.0);
+        h = __builtin_fma(cz_m, z > 0 ? pick(xz_m) : 0.0, h);
+        h = __builtin_fma(cv.cym, ym_in ? pick(nbc[-(int)(kZTX + 2)]) : 0.0, h);
+        h = __builtin_fma(cv.cxm, xm_in ? pick(nbc[-1]) : 0.0, h);
+        h = __builtin_fma(di, pick(xi), h);
+        h = __builtin_fma(cv.cx, xp_in ? pick(nbc[1]) : 0.0, h);
+        h = __builtin_fma(cv.cy, yp_in ? pick(nbc[(int)(kZTX + 2)]) : 0.0, h);
+        h = __builtin_fma(cv.cz, z + 1 < g.nz ? pick(xz_p) : 0.0, h);
         if (Wrap) {
             const bool hzm = z > 0 || g.per[2], hzp = z + 1 < g.nz || g.per[2];
-            if (c.hxm && c.wxm) h = h + cx[I + c.oxm] * pick(X(I + c.oxm));
-            if (c.hxp && c.wxp) h = h + cv.cx * pick(X(I + c.oxp));
-            if (c.hym && c.wym) h = h + cy[I + c.oym] * pick(X(I + c.oym));
-            if (c.hyp && c.wyp) h = h + cv.cy * pick(X(I + c.oyp));
-            if (hzm && z == 0) h = h + cz_m * pick(xz_m);
-            if (hzp && z + 1 == g.nz) h = h + cv.cz * pick(xz_p);
+            if (c.hxm && c.wxm) h = __builtin_fma(cx[I + c.oxm], pick(X(I + c.oxm)), h);
+            if (c.hxp && c.wxp) h = __builtin_fma(cv.cx, pick(X(I + c.oxp)), h);
+            if (c.hym && c.wym) h = __builtin_fma(cy[I + c.oym], pick(X(I + c.oym)), h);
+            if (c.hyp && c.wyp) h = __builtin_fma(cv.cy, pick(X(I + c.oyp)), h);
+            if (hzm && z == 0) h = __builtin_fma(cz_m, pick(xz_m), h);
+            if (hzp && z + 1 == g.nz) h = __builtin_fma(cv.cz, pick(xz_p), h);
         }
         return h;
     };
@@ -877,9 +879,11 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             if (Transient) {
                 const double rci = UniformRc ? rc_u : cur.rc;
                 const double cvi = UniformRc ? cv_u : cur.cv;
-                Sn = -(sp_.inverse_time * tpi * rci) - cur.qb;
-                Sn1 = -(sp_.inverse_time * xi.a * rci) - cur.qb;
+                Sn = __builtin_fma(-(sp_.inverse_time * tpi), rci, -cur.qb);
+                Sn1 = __builtin_fma(-(sp_.inverse_time * xi.a), rci, -cur.qb);
                 if (Cattaneo) {
+                    // (a product and a sum, as the next step's scat below and in the
+                    // two-step pass)
                     Sn = Sn + (tppi - 2.0 * tpi) * cvi;
                     const double scat = (tpi - 2.0 * xi.a) * cvi;
                     if (isnan(scat)) fl |= FLAG_NAN_SCAT;
@@ -906,7 +910,7 @@ k_ir_step_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 h1s = h1s + cur.cx;
                 h1s = h1s + cur.cy;
                 h1s = h1s + cur.cz;
-                ra = ra + kTiny * s * h1s;
+                ra = __builtin_fma(kTiny * s, h1s, ra);
             }
             const float fn = (float)ra;
             stnt(&xout[I], xi.a);
@@ -1629,7 +1633,7 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 const double sa = inv_sqrt(-v.d);
                 double base = v.tp + (v.tp - v.tpp);   // mod_evolve.f90:140
                 if (tiny) base = base + kTiny;         // :141
-                const double xn = base + sa * (double)(ysc * v.r);
+                const double xn = __builtin_fma(sa, (double)(ysc * v.r), base);
                 o = D2(xn, xn + (xn - v.tp));
                 // inside the b-region (all but the outer ring): for step n on the next trip
                 if (px >= 1 && px <= kBW && py >= 1 && py <= kBH) {
@@ -1684,18 +1688,18 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 // the positions outside the grid, so an absent neighbour adds coefficient x 0
                 auto hsum = [&](auto pick) {
                     double h = 0.0;
-                    h = h + czm * pick(ga(Am, ai));
-                    h = h + cym * pick(ga(A0, ai - kAW));
-                    h = h + cxm * pick(ga(A0, ai - 1));
-                    h = h + di * pick(xi);
-                    h = h + cxi * pick(ga(A0, ai + 1));
-                    h = h + cyi * pick(ga(A0, ai + kAW));
-                    h = h + czi * pick(ga(Ap, ai));
+                    h = __builtin_fma(czm, pick(ga(Am, ai)), h);
+                    h = __builtin_fma(cym, pick(ga(A0, ai - kAW)), h);
+                    h = __builtin_fma(cxm, pick(ga(A0, ai - 1)), h);
+                    h = __builtin_fma(di, pick(xi), h);
+                    h = __builtin_fma(cxi, pick(ga(A0, ai + 1)), h);
+                    h = __builtin_fma(cyi, pick(ga(A0, ai + kAW)), h);
+                    h = __builtin_fma(czi, pick(ga(Ap, ai)), h);
                     return h;
                 };
                 const double s = SDp[0][sb];
                 // S_{n+1} from (x'_n, T^n) (mod_evolve.f90:112-119, S_catS)
-                double S1 = -(it * xi.a * rci) - qbi;
+                double S1 = __builtin_fma(-(it * xi.a), rci, -qbi);
                 if (Cattaneo) {
                     const double scat = (tpi - 2.0 * xi.a) * cvi;
                     if (k == 0 && own_plane && isnan(scat)) fl |= FLAG_NAN_SCAT;
@@ -1710,17 +1714,17 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     h1s = h1s + cxi;
                     h1s = h1s + cyi;
                     h1s = h1s + czi;
-                    ra = ra + kTiny * s * h1s;
+                    ra = __builtin_fma(kTiny * s, h1s, ra);
                 }
                 const float fn = (float)ra;
                 // step n+1's x'_{n+1} = x1_n (+TINY) + s (alpha r^0_{n+1}), x1_{n+1}
                 double base = xi.a + (xi.a - tpi);
                 if (guess) base = base + kTiny;
-                const double xn1 = base + s * (double)(ysc * fn);
+                const double xn1 = __builtin_fma(s, (double)(ysc * fn), base);
                 v = D2(xn1, xn1 + (xn1 - xi.a));
                 if (k == 0 && own_plane) {
                     // the tile cell: step n's residual test and sums
-                    double Sn = -(it * tpi * rci) - qbi;
+                    double Sn = __builtin_fma(-(it * tpi), rci, -qbi);
                     if (Cattaneo) Sn = Sn + (own_tpp - 2.0 * tpi) * cvi;
                     if (isnan(S1)) fl |= FLAG_NAN_S;
                     const float rf = (float)(-s * (Sn - hsum([](const D2 &w) { return w.a; })));
@@ -1733,7 +1737,7 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;
                     // step n+1's S_{n+2} from (x'_{n+1}, x'_n), and its tiny-guess row sum but
                     // for the z+ coupling
-                    double S2 = -(it * v.a * rci) - qbi;
+                    double S2 = __builtin_fma(-(it * v.a), rci, -qbi);
                     if (Cattaneo) {
                         const double scat = (xi.a - 2.0 * v.a) * cvi;
                         if (isnan(scat)) fl2 |= FLAG_NAN_SCAT2;
@@ -1776,11 +1780,11 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
         const int p2 = q - 2;
         if (active && p2 >= (int)z0 && p2 < (int)z1) {
             // (o_b is 0 when plane p2 + 1 is outside the grid)
-            const double ha = w_ha + w_cz * o_b.a;
-            const double hb = w_hb + w_cz * o_b.b;
+            const double ha = __builtin_fma(w_cz, o_b.a, w_ha);
+            const double hb = __builtin_fma(w_cz, o_b.b, w_hb);
             const float rf = (float)(-w_s * (w_S1 - ha));
             double ra = -w_s * (w_S2 - hb);
-            if (guess) ra = ra + kTiny * w_s * (w_h1 + w_cz);
+            if (guess) ra = __builtin_fma(kTiny * w_s, w_h1 + w_cz, ra);
             const float fn = (float)ra;
             st_at(rnext, (ocol + P * (uint32_t)p2) * 4u, fn);
             const double bh = w_s * w_S2;
@@ -1806,12 +1810,12 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             // (the b-plane holds 0 at the positions outside the grid, and b_prev is 0 on plane 0)
             auto hsum6 = [&](auto pick) {
                 double h = 0.0;
-                h = h + o_czm * pick(b_prev);
-                h = h + o_cym * pick(gb(bi - kBW));
-                h = h + o_cxm * pick(gb(bi - 1));
-                h = h + o_d * pick(xi);
-                h = h + o_cxi * pick(gb(bi + 1));
-                h = h + o_cyi * pick(gb(bi + kBW));
+                h = __builtin_fma(o_czm, pick(b_prev), h);
+                h = __builtin_fma(o_cym, pick(gb(bi - kBW)), h);
+                h = __builtin_fma(o_cxm, pick(gb(bi - 1)), h);
+                h = __builtin_fma(o_d, pick(xi), h);
+                h = __builtin_fma(o_cxi, pick(gb(bi + 1)), h);
+                h = __builtin_fma(o_cyi, pick(gb(bi + kBW)), h);
                 return h;
             };
             w_ha = hsum6([](const D2 &w) { return w.a; });

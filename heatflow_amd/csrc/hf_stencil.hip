@@ -1375,8 +1375,11 @@ k_cg_b1_zw(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *_
 // its registers.  Non-periodic grids only.
 //
 // Outputs: x'_n -> x1out, x'_{n+1} -> x2out, r^0_{n+2} (FP32) -> rnext; partials 0-3 of step n,
-// 4-7 of step n+1, each ||rho||^2, ||b^_next||^2, ||r^0_next||^2, r^0 . rho; flags of step n
-// (NAN_X, TINY_NEXT, NAN_SCAT, NAN_S of step n+1's S) in Scal::flags, of step n+1 in flags2.
+// 4-7 of step n+1, each ||rho||^2, ||b^_next||^2, ||r^0_next||^2, r^0 . rho; flags: TINY_NEXT
+// of step n in Scal::flags and a NaN anywhere in step n (x'_n, scat, S_{n+1}: tested on
+// S_{n+1}, which any of them makes NaN) as NAN_S in flags_next; those of step n+1 (TINY_NEXT2,
+// NAN_S2) in flags2.  The pair needs only whether each step met a NaN: it then falls back to
+// single steps, whose flags name the quantity.
 constexpr int kAW = (int)kP2X + 4, kAH = (int)kP2Y + 4, kAN = kAW * kAH;
 constexpr int kBW = (int)kP2X + 2, kBH = (int)kP2Y + 2, kBN = kBW * kBH;
 constexpr int kTT = (int)(kP2X * kP2Y);
@@ -1702,7 +1705,6 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                 double S1 = __builtin_fma(-(it * xi.a), rci, -qbi);
                 if (Cattaneo) {
                     const double scat = (tpi - 2.0 * xi.a) * cvi;
-                    if (k == 0 && own_plane && isnan(scat)) fl |= FLAG_NAN_SCAT;
                     S1 = S1 + scat;
                 }
                 double ra = -s * (S1 - hsum([](const D2 &w) { return w.b; }));
@@ -1726,6 +1728,8 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     // the tile cell: step n's residual test and sums
                     double Sn = __builtin_fma(-(it * tpi), rci, -qbi);
                     if (Cattaneo) Sn = Sn + (own_tpp - 2.0 * tpi) * cvi;
+                    // a NaN in x'_n or in its scat makes S_{n+1} NaN: one test flags step n
+                    // (the pair then falls back to single steps, which tell them apart)
                     if (isnan(S1)) fl |= FLAG_NAN_S;
                     const float rf = (float)(-s * (Sn - hsum([](const D2 &w) { return w.a; })));
                     const double bh = s * S1;
@@ -1733,17 +1737,15 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
                     tv[1] = bh * bh;
                     tv[2] = (double)fn * (double)fn;
                     tv[3] = (double)own_r * (double)rf;
-                    if (isnan(xi.a)) fl |= FLAG_NAN_X;
                     if (xi.b - xi.a < kTiny) fl |= FLAG_TINY_NEXT;
                     // step n+1's S_{n+2} from (x'_{n+1}, x'_n), and its tiny-guess row sum but
                     // for the z+ coupling
                     double S2 = __builtin_fma(-(it * v.a), rci, -qbi);
                     if (Cattaneo) {
                         const double scat = (xi.a - 2.0 * v.a) * cvi;
-                        if (isnan(scat)) fl2 |= FLAG_NAN_SCAT2;
                         S2 = S2 + scat;
                     }
-                    if (isnan(S2)) fl2 |= FLAG_NAN_S2;
+                    if (isnan(S2)) fl2 |= FLAG_NAN_S2;   // x'_{n+1}, its scat or S_{n+2}
                     o_S2 = S2;
                     if (guess) {
                         double h1s = di;
@@ -1829,7 +1831,6 @@ k_ir_step2_z(Dims g, StepParams sp_, const double *__restrict__ tp,
             const uint32_t o8 = (ocol + P * (uint32_t)p) * 8u;
             st_at(x1out, o8, o_xn);
             st_at(x2out, o8, xi.a);
-            if (isnan(xi.a)) fl2 |= FLAG_NAN_X2;
             if (xi.b - xi.a < kTiny) fl2 |= FLAG_TINY_NEXT2;
         }
         b_prev = o_b;

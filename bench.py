@@ -621,7 +621,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 1e-32 runs")
-    ap.add_argument("--only", choices=["fp32_ir", "fp64"], default=None,
+    ap.add_argument("--only", choices=["fp32_ir", "fp64", "fp32_ir_noreuse"], default=None,
                     help="time one mode only (profiling)")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=20)
@@ -692,7 +692,9 @@ def main():
     # (precision, tolerance, debug flags, label): debug bit 2 (value 4) turns off the reuse of
     # the inner step length, so that leg compares the precisions with every step solving
     modes = [("fp32_ir", args.tol, 0, "fp32_ir"), ("fp64", args.tol, 0, "fp64")]
-    if args.only:
+    if args.only == "fp32_ir_noreuse":
+        modes = [("fp32_ir", args.tol, 4, args.only)]
+    elif args.only:
         modes = [(args.only, args.tol, 0, args.only)]
     else:
         modes.append(("fp32_ir", args.tol, 4, "fp32_ir_noreuse"))
@@ -749,7 +751,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-inner/f64-refined" if (args.only or "fp32_ir") == "fp32_ir" else "f64",
+        "dtype": "f64" if args.only == "fp64" else "f32-inner/f64-refined",
         "data": "synthetic",
         "config": {"workload": f"c3: {args.n}^3 Cattaneo, log-normal kappa exp(N(ln50,1)) seed 3, "
                                f"Gaussian heating sigma=32 cells peak 1e12 W/m^3, Dirichlet 300 K "

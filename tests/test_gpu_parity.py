@@ -612,6 +612,45 @@ def test_fp32_ir_pairs_match_oracle(case):
     assert relerr(ta, tb) <= 1e-10
 
 
+def _lognormal_box(nx, ny, nz, seed=3, sigma=4.0):
+    """The log-normal workload on an nx x ny x nz grid: kappa = exp(N(ln 50, 1)) per cell, 0.5 um
+    cells, Dirichlet 300 K on six faces, a Gaussian heating of sigma cells at the centre."""
+    p = Params(nx=nx, ny=ny, nz=nz, time_step=5.0e-11, icattaneo=1, isteady=0,
+               kappa_bound=(50.0,) * 6, t_bath=(300.0,) * 6, t_system=300.0, ntime=40)
+    kappa = np.exp(np.random.default_rng(seed).normal(np.log(50.0), 1.0, p.n))
+    g = [np.exp(-((np.arange(m) - (m - 1) / 2.0) ** 2) / (2.0 * sigma * sigma))
+         for m in (nx, ny, nz)]
+    q = 1.0e12 * g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    return System.uniform(p, (nx * 0.5e-6, ny * 0.5e-6, nz * 0.5e-6), kappa=kappa,
+                          rho=workloads.RHO_A, heat_capacity=workloads.CP_A, tau=1.0e-12,
+                          qdens_extra=q.reshape(-1))
+
+
+def test_fp32_ir_pairs_across_tiles():
+    """The two-step pass on a grid of several of its 128 x 8 tiles a plane: 300 x 20 x 10 cells
+    (3 x 3 tiles, the last ones 44 and 4 cells wide), so ring cells lie on tile edges inside the
+    grid and on partial tiles at its far edges.  40 steps of the log-normal workload at tol
+    1e-11 with pairs on and off (debug bit 7) end as close to the oracle as each other; with the
+    step length frozen (debug bit 8) they give the same bits."""
+    s = _lognormal_box(300, 20, 10)
+    t0 = np.full(s.params.n, s.params.t_system)
+    rc, otp, otpp, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 41)), t0, t0)
+    assert rc == 0
+    (ta, tpa), sa, ka = _mixed_run(s, 40, 0, tol=1e-11)
+    (tb, tpb), sb, _ = _mixed_run(s, 40, 128, tol=1e-11)
+    (tc, tpc), sc_, kc = _mixed_run(s, 40, 256, tol=1e-11)
+    (td, tpd), sd, _ = _mixed_run(s, 40, 256 | 128, tol=1e-11)
+    n2, n2c = ka.get("k_ir_step2", (0.0, 0, 0.0))[1], kc.get("k_ir_step2", (0.0, 0, 0.0))[1]
+    ea, eb = relerr(ta, otp), relerr(tb, otp)
+    print(f"pairs 300x20x10: launches {n2} / {n2c} (frozen), stats {sa} / {sb}, "
+          f"err {ea:.3g} / {eb:.3g}", flush=True)
+    assert n2 >= 10 and n2c >= 10
+    assert ea <= 1e-9 and relerr(tpa, otpp) <= 1e-9
+    assert ea <= 1.2 * eb + 1e-12 and relerr(ta, tb) <= 1e-10
+    assert sc_ == sd
+    assert np.array_equal(bits(tc), bits(td)) and np.array_equal(bits(tpc), bits(tpd))
+
+
 def test_fp32_ir_pairs_with_misses():
     """The 48^3 workload at tol 1e-12, where one-iteration residuals sit near the inner target:
     a pair whose step misses its test falls back to single steps and pauses reuse.  The 24

@@ -1351,8 +1351,10 @@ k_cg_b1_zw(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *_
 
 // ---- two reusing steps in one z-march (k_ir_step2_z) ---------------------------------------
 //
-// In the carried regime where the inner step length is reused, a mixed step is one stencil pass
-// of per-cell arithmetic: x' = x0 + s (alpha r^0), its FP64 residual, and the next step's r^0.
+// In the carried regime where the inner step length is reused (or a step's own inner solve ends
+// on its first iteration, and the next step reuses that alpha), a mixed step is one stencil
+// pass of per-cell arithmetic: x' = x0 + s (alpha r^0), its FP64 residual, and the next step's
+// r^0.
 // Two such steps are two passes over the same operator, Qdens + B and state.  k_ir_step2_z does
 // both in one z-march, so those arrays are read once for two steps (80 instead of 144
 // algorithmic bytes a cell):
@@ -1368,11 +1370,14 @@ k_cg_b1_zw(Dims g, const T *__restrict__ r, const T *__restrict__ hx, const T *_
 // Step n+1 at a cell needs r^0_{n+1} of its neighbours, which needs x1_n two cells away: the
 // (x'_n, x1_n) plane ("a") is formed on the tile and a two-cell ring, r^0_{n+1} and
 // (x'_{n+1}, x1_{n+1}) ("b") on the tile and a one-cell ring, and step n+1 runs one plane
-// behind step n.  Trip q of the march forms a(q), then (barrier) step n on plane q-1 for the
-// ring positions (with its residual test on the tile only) and step n+1 on plane q-2 for the
-// tile; three a-planes and two b-planes rotate in LDS.  Each thread takes its own tile cell as
-// its first position in every stage, so what one stage hands the next for that cell stays in
-// its registers.  Non-periodic grids only.
+// behind step n.  Trip q of the march forms a(q); then (barrier) step n on plane q-1 for the
+// tile and the one-cell ring (its residual test on the tile only) and step n+1's z+ term and
+// residual on plane q-2; then (barrier) step n+1's other terms on plane q-1.  Three a-planes
+// rotate in LDS beside one b-plane (the own cell's b of the plane below stays in registers) and
+// two planes of the b-region's (s, d, T^n).  Each thread takes its own tile cell as its first
+// position in every stage, so what one stage hands the next for that cell stays in its
+// registers; the a-ring's cells are second positions of the first waves, the b-ring's of the
+// waves after them.  Non-periodic grids only.
 //
 // Outputs: x'_n -> x1out, x'_{n+1} -> x2out, r^0_{n+2} (FP32) -> rnext; partials 0-3 of step n,
 // 4-7 of step n+1, each ||rho||^2, ||b^_next||^2, ||r^0_next||^2, r^0 . rho; flags: TINY_NEXT

@@ -518,8 +518,9 @@ def grad_bench(n: int, steps: int, ckpt: int, fd_cells: int = 16) -> dict:
 def grad_c5_bench(n: int, steps: int) -> dict:
     """The 384^3 layered workload: forward + adjoint with sqrt(steps) checkpoints, FP32-inner /
     FP64-refined and plain FP64 solves both stopping at a scaled residual of 1e-13 (and the
-    FP32-IR run once more with the adjoint solves at 1e-10), each against the FP64 gradient at
-    the reference tolerance 1e-32; device memory held by each context."""
+    FP32-IR run once more with the adjoint solves at 1e-10, and once more with every state kept
+    when they fit in device memory: checkpoint spacing 0, nothing recomputed), each against the
+    FP64 gradient at the reference tolerance 1e-32; device memory held by each context."""
     import torch
     from heatflow_amd import Simulation, workloads
     s, m = workloads.c5(n)
@@ -527,9 +528,12 @@ def grad_c5_bench(n: int, steps: int) -> dict:
     targets = np.full(len(m["probes"]), 300.0)
     res = {}
     grads = {}
-    for prec, tol, atol in (("fp64", 1e-32, 0.0), ("fp32_ir", 1e-13, 0.0), ("fp64", 1e-13, 0.0),
-                            ("fp32_ir", 1e-13, 1e-10)):
-        key = f"{prec}@{tol:g}" + (f"/adjoint@{atol:g}" if atol else "")
+    every_bytes = (steps + 1) * s.params.n * 8
+    for prec, tol, atol, k in (("fp64", 1e-32, 0.0, ckpt), ("fp32_ir", 1e-13, 0.0, ckpt),
+                               ("fp64", 1e-13, 0.0, ckpt), ("fp32_ir", 1e-13, 1e-10, ckpt),
+                               ("fp32_ir", 1e-13, 1e-10, 0)):
+        key = f"{prec}@{tol:g}" + (f"/adjoint@{atol:g}" if atol else "") + \
+            ("/every_state" if k == 0 else "")
         torch.cuda.synchronize()
         free0, total = torch.cuda.mem_get_info()
         sim = Simulation(s, precision=prec, tol=tol)
@@ -538,7 +542,7 @@ def grad_c5_bench(n: int, steps: int) -> dict:
             sim.initial_evolve()
             sync()
             t0 = time.perf_counter()
-            loss, gk, gp = sim.grad(1, steps, ckpt, m["probes"], targets)
+            loss, gk, gp = sim.grad(1, steps, k, m["probes"], targets)
             sync()
             secs = time.perf_counter() - t0
             free1, _ = torch.cuda.mem_get_info()
@@ -546,9 +550,13 @@ def grad_c5_bench(n: int, steps: int) -> dict:
             sim.close()
         grads[key] = (loss, gk, gp[m["heaters"]])
         res[key] = dict(seconds=round(secs, 3), loss=loss, device_bytes=int(free0 - free1))
+        if k == 0:
+            # spacing 0 keeps every state (about (steps + 1) N doubles) when they fit
+            res[key]["kept_every_state"] = bool(free0 - free1 >= every_bytes)
         log(f"[bench] grad c5 {n}^3 {key}: {res[key]}")
     L0, gk0, gp0 = grads["fp64@1e-32"]
-    for key in ("fp32_ir@1e-13", "fp64@1e-13", "fp32_ir@1e-13/adjoint@1e-10"):
+    for key in ("fp32_ir@1e-13", "fp64@1e-13", "fp32_ir@1e-13/adjoint@1e-10",
+                "fp32_ir@1e-13/adjoint@1e-10/every_state"):
         L, gk, gp = grads[key]
         res[key]["err_vs_fp64_reftol"] = dict(
             loss=abs(L - L0) / abs(L0),

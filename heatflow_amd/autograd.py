@@ -10,7 +10,8 @@ density P / volume (the superset per-cell input of the C-ABI), both float64 tens
 the library's device context, in the context's precision mode (FP64, or FP32-inner / FP64-refined)
 and tolerance, and keeps (T^n, T^{n-1}) checkpoints every `ckpt_every` steps; backward re-runs
 each segment from its checkpoint and sweeps the adjoint backwards (hf_forward_checkpointed /
-hf_adjoint in include/heatflow_hip.h).
+hf_adjoint in include/heatflow_hip.h).  ckpt_every = 1 keeps every state instead (nothing is
+re-run); 0 picks 1 when the states fit in free device memory, else ceil(sqrt(nsteps)).
 
 The checkpoints live in the context, so one `Simulation` holds one pending graph: backward of a
 graph whose forward was followed by another forward on the same context raises RuntimeError

@@ -18,7 +18,9 @@
 // The BR switch is treated as locally constant (it is discontinuous at k_I == k_J).
 //
 // Memory: (T^n, T^{n-1}) checkpoints every K steps plus the K states of the segment being
-// reversed, recomputed from its checkpoint.
+// reversed, recomputed from its checkpoint; for K = 1 every state once (nsteps + 1 of them) and
+// nothing recomputed.  K = 0 takes K = 1 when that fits in the device memory free at the call,
+// else K = ceil(sqrt(nsteps)).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,8 +33,10 @@ namespace hf {
 
 struct AdjointState {
     size_t N = 0;
-    int64_t nseg_cap = 0, k_cap = 0;
-    double *ckpt = nullptr;   // nseg x 2 x N
+    size_t ckpt_cap = 0;      // doubles allocated for ckpt
+    int64_t k_cap = 0;
+    // nseg x (T^n, T^{n-1}), T^n at ckpt + 2 N s; for K = 1, T^0 .. T^nsteps at ckpt + N n
+    double *ckpt = nullptr;
     double *seg = nullptr;    // max(K - 1, 1) x N: the recomputed states of one segment
     double *g_hi = nullptr, *g_lo = nullptr, *g_new = nullptr;
     // the last adjoint solutions, newest first (hist[0] is lambda^{n+1}), and the next guess
@@ -192,24 +196,43 @@ static int validate(hf_ctx *c) {
 }
 
 // Forward run of nsteps from the current state, keeping (T^n, T^{n-1}) every ckpt_every steps.
+// The checkpoint spacing for ckpt_every = 0: every state (K = 1) when nsteps + 1 states fit in
+// the device memory free now, counting what this context's checkpoints already hold and
+// leaving max(2 GiB, 5% of the device) to spare; else ceil(sqrt(nsteps)).
+static int64_t auto_spacing(hf_ctx *c, int64_t nsteps) {
+    const int64_t ks = std::max<int64_t>(1, (int64_t)std::ceil(std::sqrt((double)nsteps)));
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+        (void)hipGetLastError();
+        return ks;
+    }
+    const AdjointState *A = c->adj;
+    const size_t held = A ? ((A->ckpt ? A->ckpt_cap : 0) + (A->seg ? A->k_cap * A->N : 0)) * 8 : 0;
+    const size_t need = (size_t)(nsteps + 1) * c->g.N * 8;
+    const size_t spare = std::max<size_t>((size_t)2 << 30, totalb / 20);
+    return need + spare <= freeb + held ? 1 : ks;
+}
+
 int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_every) {
     ATRY(validate(c));
-    if (nsteps < 1 || ckpt_every < 1)
+    if (nsteps < 1 || ckpt_every < 0)
         return ctx_set_err(c, HF_ERR_ARG, "hf_forward_checkpointed: bad step counts");
     const size_t N = c->g.N, B8 = N * 8;
+    const int64_t K = ckpt_every > 0 ? ckpt_every : auto_spacing(c, nsteps);
     if (!c->adj) c->adj = new AdjointState();
     AdjointState &A = *c->adj;
     // the checkpoints are rewritten below: valid again only once the run has completed
     A.valid = false;
-    const int64_t K = ckpt_every;
     const int64_t nseg = (nsteps + K - 1) / K;
+    const bool every = K == 1;   // every state once, nothing recomputed
+    const size_t ck_need = every ? (size_t)(nsteps + 1) * N : (size_t)2 * N * nseg;
     // a segment of K steps is recomputed up to its second-to-last state (the last one is the
     // next checkpoint's): K - 1 states
     const int64_t kseg = std::max<int64_t>(K - 1, 1);
-    if (A.N != N || A.nseg_cap < nseg || A.k_cap < kseg) {
+    if (A.N != N || A.ckpt_cap < ck_need || A.k_cap < kseg) {
         hipFree(A.ckpt); hipFree(A.seg);
         A.ckpt = A.seg = nullptr;
-        A.nseg_cap = std::max(nseg, A.N == N ? A.nseg_cap : 0);
+        A.ckpt_cap = ck_need;
         A.k_cap = std::max(kseg, A.N == N ? A.k_cap : 0);
     }
     if (A.N != N) {
@@ -223,15 +246,23 @@ int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_ever
         A.g_hi = A.g_lo = A.g_new = A.lam_next = A.gk = A.gp = A.fin = nullptr;
         A.N = N;
     }
-    ATRY(ensure(c, &A.ckpt, (size_t)2 * N * A.nseg_cap));
+    ATRY(ensure(c, &A.ckpt, A.ckpt_cap));
     ATRY(ensure(c, &A.seg, (size_t)N * A.k_cap));
     ATRY(ensure(c, &A.fin, (size_t)2 * N));
     for (int64_t s = 0; s < nseg; ++s) {
-        AHIP(c, hipMemcpyAsync(A.ckpt + (size_t)2 * N * s, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
-        AHIP(c, hipMemcpyAsync(A.ckpt + (size_t)2 * N * s + N, c->T[1], B8, hipMemcpyDeviceToDevice, c->st));
+        if (every) {
+            AHIP(c, hipMemcpyAsync(A.ckpt + N * s, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
+        } else {
+            AHIP(c, hipMemcpyAsync(A.ckpt + (size_t)2 * N * s, c->T[0], B8,
+                                   hipMemcpyDeviceToDevice, c->st));
+            AHIP(c, hipMemcpyAsync(A.ckpt + (size_t)2 * N * s + N, c->T[1], B8,
+                                   hipMemcpyDeviceToDevice, c->st));
+        }
         const int64_t e = std::min(nsteps, (s + 1) * K);
         for (int64_t n = s * K; n < e; ++n) ATRY(ctx_step(c, itime0 + n, n + 1 == nsteps));
     }
+    if (every)
+        AHIP(c, hipMemcpyAsync(A.ckpt + N * nseg, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
     AHIP(c, hipMemcpyAsync(A.fin, c->T[0], B8, hipMemcpyDeviceToDevice, c->st));
     AHIP(c, hipMemcpyAsync(A.fin + N, c->T[1], B8, hipMemcpyDeviceToDevice, c->st));
     AHIP(c, hipStreamSynchronize(c->st));
@@ -291,8 +322,11 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t b = s * K, e = std::min(nsteps, (s + 1) * K);
         // T^e, the state after the segment's last step, is the next checkpoint's (or the final
-        // state): the recomputation stops one step short (none at all for ckpt_every = 1)
-        const double *Te = s + 1 < nseg ? A.ckpt + (size_t)2 * N * (s + 1) : A.fin;
+        // state): the recomputation stops one step short (none at all for ckpt_every = 1,
+        // which keeps every state)
+        const double *Te = K == 1                 ? A.ckpt + N * (s + 1)
+                           : s + 1 < nseg         ? A.ckpt + (size_t)2 * N * (s + 1)
+                                                  : A.fin;
         if (e - 1 > b)
             ATRY(restore_state(c, A.ckpt + (size_t)2 * N * s, A.ckpt + (size_t)2 * N * s + N));
         for (int64_t n = b; n < e - 1; ++n) {
@@ -332,7 +366,9 @@ int adjoint_backward(hf_ctx *c, const double *seed, double *dl_dkappa, double *d
             c->stats.adjoint_iters += it;
             const double *Tn1 = n == e - 1 ? Te : A.seg + (size_t)N * (n - b);
             // T^n, the state step n started from (the radiative term's derivative)
-            q.tn = n == b ? A.ckpt + (size_t)2 * N * s : A.seg + (size_t)N * (n - 1 - b);
+            q.tn = K == 1   ? A.ckpt + N * s
+                   : n == b ? A.ckpt + (size_t)2 * N * s
+                            : A.seg + (size_t)N * (n - 1 - b);
             if (zmarch_adj)
                 launch_adj_step_z(c->g, q, Tn1, A.hist[0], c->kappa, c->lx, c->ly, c->lz, c->rhoc,
                                   c->cvec, c->vol, A.g_lo, A.g_new, A.gk, A.gp, c->st);

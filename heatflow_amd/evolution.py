@@ -245,7 +245,9 @@ class Simulation:
 
     def grad(self, itime0: int, nsteps: int, ckpt_every: int, probes, targets, weights=None):
         """Forward nsteps with checkpoints, then the reverse-time adjoint.  Returns
-        (loss, dL/dkappa, dL/dP) where P_I enters as Qdens_I += P_I / volume_I."""
+        (loss, dL/dkappa, dL/dP) where P_I enters as Qdens_I += P_I / volume_I.  ckpt_every: the
+        checkpoint spacing; 1 keeps every state (nothing recomputed), 0 picks 1 when that fits
+        in free device memory, else ceil(sqrt(nsteps))."""
         pi = np.ascontiguousarray(probes, dtype=np.int64)
         tg = _f64(targets)
         w = None if weights is None else _f64(weights)

@@ -268,7 +268,10 @@ int hf_set_solver(hf_ctx *ctx, double tol, int64_t itmax, int32_t precision, dou
 /* ---- adjoint (no reference counterpart: reverse-time gradients) ---------------------------
  * hf_forward_checkpointed runs nsteps time steps (itime0 .. itime0+nsteps-1) from the current
  * state, keeping (T^n, T^{n-1}) on the device every `ckpt_every` steps; the context ends at the
- * final state.  hf_adjoint then runs the reverse sweep (each segment recomputed from its
+ * final state.  ckpt_every = 1 keeps every state once instead (nsteps + 1 states of N doubles)
+ * and nothing is recomputed; ckpt_every = 0 takes 1 when those states fit in the device memory
+ * free at the call (leaving max(2 GiB, 5% of the device) spare), else ceil(sqrt(nsteps)).
+ * hf_adjoint then runs the reverse sweep (each segment recomputed from its
  * checkpoint) seeded with dL/dT^{final} (length N), and returns the gradients with respect to
  * the per-cell conductivity kappa (dl_dkappa, length N) and to a per-cell power P_I that enters
  * as Qdens_I += P_I / volume_I (dl_dpower, length N).  The adjoint solves use the context's

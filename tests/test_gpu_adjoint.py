@@ -368,15 +368,16 @@ def test_adjoint_guess_orders_agree(prec):
 @pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
 def test_adjoint_checkpoint_spacing_is_exact(prec):
     """The reverse pass recomputes each segment from its checkpoint except the last step, whose
-    state is the next checkpoint (or the final state); with checkpoints every step nothing is
-    recomputed.  FP64 steps give the same bits from a restored state as from the carried
-    residual, so loss and gradients are bit-identical for every spacing (1, 4, 5 with a partial
-    last segment, 13 = all steps in one segment); mixed-mode steps agree to the tolerance
-    (1e-13 here; gradients within 1e-9 of max |g|)."""
+    state is the next checkpoint (or the final state); with checkpoints every step (spacing 1)
+    every state is kept once and nothing is recomputed, and spacing 0 picks that when it fits in
+    device memory (here it does).  FP64 steps give the same bits from a restored state as from
+    the carried residual, so loss and gradients are bit-identical for every spacing (1, 0, 4, 5
+    with a partial last segment, 13 = all steps in one segment); mixed-mode steps agree to the
+    tolerance (1e-13 here; gradients within 1e-9 of max |g|)."""
     s, m = workloads.c4(16, icattaneo=1, nprobe=16)
     targets = np.full(len(m["probes"]), 300.0)
     out = []
-    for k in (1, 4, 5, 13):
+    for k in (1, 0, 4, 5, 13):
         with Simulation(s, precision=prec, tol=1e-13) as sim:
             sim.initial_evolve()
             out.append(sim.grad(1, 13, k, m["probes"], targets))
@@ -455,3 +456,9 @@ def test_adjoint_refuses_mixed_uses():
         with pytest.raises(HeatFlowError) as ei:
             sim.forward_checkpointed(1, 4, 2)
         assert ei.value.code == 10
+    with Simulation(s, tol=1e-32) as sim:   # a negative spacing or no steps
+        sim.initial_evolve()
+        for nsteps, k in ((4, -1), (0, 2)):
+            with pytest.raises(HeatFlowError) as ei:
+                sim.forward_checkpointed(1, nsteps, k)
+            assert ei.value.code == 10

@@ -1688,8 +1688,7 @@ int ctx_solve_dev(hf_ctx *c, const double *b, double *x, double tol, int64_t itm
             TRY(check_launch(c, "k_ir_res(solve)"));
             std::swap(xa, xb);
             rin = rout;
-            HIPCHK(c, hipMemcpyAsync(c->h_sci, c->sci, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
-            TRY(sync_scal(c, c->sc, c->h_sc));
+            TRY(sync_scal(c, c->sc, c->h_sc, 2));   // outer and inner scalars
             inner_total += c->h_sci->iter;
             c->pred_adj = c->h_sci->done ? std::max<int64_t>(1, c->h_sci->iter + 1) : batch * 2;
             if (c->h_sc->done) break;

@@ -218,23 +218,41 @@ int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_ever
     if (nsteps < 1 || ckpt_every < 0)
         return ctx_set_err(c, HF_ERR_ARG, "hf_forward_checkpointed: bad step counts");
     const size_t N = c->g.N, B8 = N * 8;
-    const int64_t K = ckpt_every > 0 ? ckpt_every : auto_spacing(c, nsteps);
+    int64_t K = ckpt_every > 0 ? ckpt_every : auto_spacing(c, nsteps);
     if (!c->adj) c->adj = new AdjointState();
     AdjointState &A = *c->adj;
     // the checkpoints are rewritten below: valid again only once the run has completed
     A.valid = false;
+    // the checkpoint buffer for spacing K; spacing 0 that chose every state falls back to
+    // ceil(sqrt(nsteps)) if the allocation fails after all (memory taken since the estimate)
+    for (;;) {
+        const size_t need =
+            K == 1 ? (size_t)(nsteps + 1) * N : (size_t)2 * N * ((nsteps + K - 1) / K);
+        // a segment of K steps is recomputed up to its second-to-last state (the last one is
+        // the next checkpoint's): K - 1 states
+        const int64_t kseg = std::max<int64_t>(K - 1, 1);
+        if (A.N != N || A.ckpt_cap < need || A.k_cap < kseg) {
+            hipFree(A.ckpt); hipFree(A.seg);
+            A.ckpt = A.seg = nullptr;
+            A.ckpt_cap = need;
+            A.k_cap = std::max(kseg, A.N == N ? A.k_cap : 0);
+        }
+        // (debug bit 10, a test aid: the every-state allocation of spacing 0 fails)
+        const bool auto_every = ckpt_every == 0 && K == 1;
+        const int rc = auto_every && (c->debug & 1024)
+                           ? ctx_set_err(c, HF_ERR_ALLOC, "hf_grad: device allocation failed")
+                           : ensure(c, &A.ckpt, A.ckpt_cap);
+        const int64_t ks = std::max<int64_t>(1, (int64_t)std::ceil(std::sqrt((double)nsteps)));
+        if (rc != HF_OK && auto_every && ks > 1) {
+            A.ckpt_cap = 0;
+            K = ks;
+            continue;
+        }
+        ATRY(rc);
+        break;
+    }
     const int64_t nseg = (nsteps + K - 1) / K;
     const bool every = K == 1;   // every state once, nothing recomputed
-    const size_t ck_need = every ? (size_t)(nsteps + 1) * N : (size_t)2 * N * nseg;
-    // a segment of K steps is recomputed up to its second-to-last state (the last one is the
-    // next checkpoint's): K - 1 states
-    const int64_t kseg = std::max<int64_t>(K - 1, 1);
-    if (A.N != N || A.ckpt_cap < ck_need || A.k_cap < kseg) {
-        hipFree(A.ckpt); hipFree(A.seg);
-        A.ckpt = A.seg = nullptr;
-        A.ckpt_cap = ck_need;
-        A.k_cap = std::max(kseg, A.N == N ? A.k_cap : 0);
-    }
     if (A.N != N) {
         hipFree(A.g_hi); hipFree(A.g_lo); hipFree(A.g_new);
         for (double *&h : A.hist) {
@@ -246,7 +264,6 @@ int adjoint_forward(hf_ctx *c, int64_t itime0, int64_t nsteps, int64_t ckpt_ever
         A.g_hi = A.g_lo = A.g_new = A.lam_next = A.gk = A.gp = A.fin = nullptr;
         A.N = N;
     }
-    ATRY(ensure(c, &A.ckpt, A.ckpt_cap));
     ATRY(ensure(c, &A.seg, (size_t)N * A.k_cap));
     ATRY(ensure(c, &A.fin, (size_t)2 * N));
     for (int64_t s = 0; s < nseg; ++s) {

@@ -224,7 +224,8 @@ int hf_get_vector(hf_ctx *ctx, int32_t which, double *out);
  * whose test misses is redone alone); bit 8 keeps the reused step length fixed instead of
  * updating it after each reusing step (then the two-step pass and single steps give the same
  * bits); bit 9 fails the first step's test in every two-step pass, so that each pair falls back
- * (a test aid). */
+ * (a test aid); bit 10 makes the every-state checkpoint allocation of spacing 0 fail, so that it
+ * falls back to ceil(sqrt(nsteps)) checkpoints (a test aid). */
 int hf_set_debug(hf_ctx *ctx, int32_t flags);
 
 /* Timing / introspection for the benchmark: achieved counters since the last reset. */

@@ -393,6 +393,30 @@ def test_adjoint_checkpoint_spacing_is_exact(prec):
             assert np.max(np.abs(pa - pb)) <= 1e-9 * np.max(np.abs(pb))
 
 
+def test_adjoint_every_state_fallback():
+    """Checkpoint spacing 0 keeps every state when they fit in device memory (nothing
+    recomputed: 13 steps counted for 13); when that allocation fails after all (forced by debug
+    bit 10) it falls back to ceil(sqrt(13)) = 4, whose reverse sweep recomputes 9 steps.  FP64
+    recomputation is exact, so all three give the same bits as spacing 1."""
+    s, m = workloads.c4(16, icattaneo=1, nprobe=16)
+    targets = np.full(len(m["probes"]), 300.0)
+    out, steps = [], []
+    for k, dbg in ((1, 0), (0, 0), (0, 1024), (4, 0)):
+        with Simulation(s, precision="fp64", tol=1e-13) as sim:
+            sim.set_debug(dbg)
+            sim.initial_evolve()
+            sim.reset_stats()
+            out.append(sim.grad(1, 13, k, m["probes"], targets))
+            steps.append(sim.stats().steps)
+    print(f"steps counted: {steps}", flush=True)
+    assert steps == [13, 13, 22, 22]
+    la, ka, pa = out[0]
+    for lb, kb, pb in out[1:]:
+        assert la == lb
+        assert np.array_equal(ka.view(np.int64), kb.view(np.int64))
+        assert np.array_equal(pa.view(np.int64), pb.view(np.int64))
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32_ir"])
 @pytest.mark.parametrize("icat", [0, 1])
 def test_adjoint_radiative_source_matches_dense(icat, prec):

@@ -612,10 +612,10 @@ def test_fp32_ir_pairs_match_oracle(case):
     assert relerr(ta, tb) <= 1e-10
 
 
-def _lognormal_box(nx, ny, nz, seed=3, sigma=4.0):
+def _lognormal_box(nx, ny, nz, seed=3, sigma=4.0, dt=5.0e-11):
     """The log-normal workload on an nx x ny x nz grid: kappa = exp(N(ln 50, 1)) per cell, 0.5 um
     cells, Dirichlet 300 K on six faces, a Gaussian heating of sigma cells at the centre."""
-    p = Params(nx=nx, ny=ny, nz=nz, time_step=5.0e-11, icattaneo=1, isteady=0,
+    p = Params(nx=nx, ny=ny, nz=nz, time_step=dt, icattaneo=1, isteady=0,
                kappa_bound=(50.0,) * 6, t_bath=(300.0,) * 6, t_system=300.0, ntime=40)
     kappa = np.exp(np.random.default_rng(seed).normal(np.log(50.0), 1.0, p.n))
     g = [np.exp(-((np.arange(m) - (m - 1) / 2.0) ** 2) / (2.0 * sigma * sigma))
@@ -649,6 +649,32 @@ def test_fp32_ir_pairs_across_tiles():
     assert ea <= 1.2 * eb + 1e-12 and relerr(ta, tb) <= 1e-10
     assert sc_ == sd
     assert np.array_equal(bits(tc), bits(td)) and np.array_equal(bits(tpc), bits(tpd))
+
+
+# a shorter time step for the thin grids: on two planes every cell touches a Dirichlet face, and
+# at 5e-11 s one inner iteration rarely meets the reuse test there (reuse then pauses and few
+# pairs run); at 1e-11 s the step is mass-dominated enough for every shape to pair
+DT_THIN = 1.0e-11
+
+
+@pytest.mark.parametrize("shape", [(40, 30, 2), (20, 5, 30), (130, 9, 3)])
+def test_fp32_ir_pairs_thin_grids(shape):
+    """The two-step pass where its z-march and tiles meet the grid's edges: two planes only (the
+    pipeline's fill and drain trips cover the whole grid), tiles with rows outside the grid
+    (ny = 5 < 8), and one cell past a full tile in x and in y (130 x 9).  Pairs run, the steps
+    meet the oracle, and with the step length frozen (debug bit 8) pairs give the bits of single
+    steps (debug bit 7)."""
+    s = _lognormal_box(*shape, sigma=2.0, dt=DT_THIN)
+    t0 = np.full(s.params.n, s.params.t_system)
+    rc, otp, _, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 41)), t0, t0)
+    assert rc == 0
+    (ta, _), sa, ka = _mixed_run(s, 40, 256, tol=1e-11)
+    (tb, _), sb, _ = _mixed_run(s, 40, 256 | 128, tol=1e-11)
+    n2 = ka.get("k_ir_step2", (0.0, 0, 0.0))[1]
+    print(f"pairs {shape}: launches {n2}, stats {sa}, err {relerr(ta, otp):.3g}", flush=True)
+    assert n2 >= 10
+    assert relerr(ta, otp) <= 1e-9
+    assert sa == sb and np.array_equal(bits(ta), bits(tb))
 
 
 def test_fp32_ir_pairs_with_misses():

@@ -160,6 +160,7 @@ contains
     type(hf_params_t) :: p
     integer(c_int) :: rc
     call hf_params_default(p)       ! tol = 1e-32, itmax = 50000 as mod_evolve.f90:142-144
+    call solver_from_environment(p)
     p%nx = nx; p%ny = ny; p%nz = nz
     p%time_step = time_step
     p%icattaneo = int(icattaneo, c_int32_t)
@@ -180,6 +181,37 @@ contains
     if (rc /= 0) call fail(rc)
     call upload_cells()
   end subroutine device_setup
+
+  !! Optional solver controls from the environment; unset, the reference's stand (FP64, tol
+  !! 1e-32, hard-coded in mod_evolve.f90:142-144).  HF_PRECISION = fp64 | fp32_ir (FP32 inner CG
+  !! inside FP64 refinement), HF_TOL = the scaled-residual tolerance.  Read once, at set-up.
+  subroutine solver_from_environment(p)
+    type(hf_params_t), intent(inout) :: p
+    character(len=64) :: v
+    integer :: n, st
+    real(c_double) :: t
+    call get_environment_variable("HF_PRECISION", v, length=n, status=st)
+    if (st == 0 .and. n > 0) then
+       select case (trim(v))
+       case ("fp64")
+          p%precision = 0_c_int32_t
+       case ("fp32_ir")
+          p%precision = 1_c_int32_t
+       case default
+          write(0,*) "HF_PRECISION must be fp64 or fp32_ir, not ", trim(v)
+          stop 1
+       end select
+    end if
+    call get_environment_variable("HF_TOL", v, length=n, status=st)
+    if (st == 0 .and. n > 0) then
+       read(v, *, iostat=st) t
+       if (st /= 0 .or. .not. (t > 0.0_c_double)) then
+          write(0,*) "HF_TOL must be a positive number, not ", trim(v)
+          stop 1
+       end if
+       p%tol = t
+    end if
+  end subroutine solver_from_environment
 
   !! grid(:,:,:) (mod_constructions.f90:25-31) as structure-of-arrays, x fastest
   subroutine upload_cells()

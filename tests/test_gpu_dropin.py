@@ -43,20 +43,41 @@ def write_param(src_lines, path, overrides):
         f.write("\n".join(out) + "\n")
 
 
-def run_program(exe, workdir):
+def run_program(exe, workdir, env=None):
     os.makedirs(os.path.join(workdir, "outputs"), exist_ok=True)
-    p = subprocess.run([exe], cwd=workdir, capture_output=True, text=True, timeout=240)
+    p = subprocess.run([exe], cwd=workdir, capture_output=True, text=True, timeout=240,
+                       env=None if env is None else {**os.environ, **env})
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     return read_tempdis(os.path.join(workdir, "outputs", "TempDis.dat"))
 
 
-def test_dropin_golden_tempdis(tmp_path):
-    wd = tmp_path / "run"
+def _golden_run_dir(wd):
     (wd / "inputs").mkdir(parents=True)
     shutil.copy(os.path.join(GOLDEN, "mat.in"), wd / "inputs")
     shutil.copy(os.path.join(GOLDEN, "system.in"), wd / "inputs")
     with open(os.path.join(GOLDEN, "param.in")) as f:
         write_param(f.readlines(), wd / "inputs" / "param.in", QUIET)
+
+
+def test_dropin_mixed_precision_from_environment(tmp_path):
+    """HF_PRECISION=fp32_ir with HF_TOL=1e-12 runs the drop-in on the mixed solver (the reference
+    hard-codes FP64 at 1e-32): the golden case still ends within TempDis.dat's 5e-7 K.  A value
+    it does not know stops the program with status 1 before any step."""
+    wd = tmp_path / "mixed"
+    _golden_run_dir(wd)
+    gold = read_tempdis(os.path.join(GOLDEN, "TempDis.dat"))
+    T = run_program(DROPIN, str(wd), env={"HF_PRECISION": "fp32_ir", "HF_TOL": "1e-12"})
+    assert T.shape == gold.shape
+    assert np.abs(T - gold).max() <= 5.0e-7 + 1e-12
+    for bad in ({"HF_PRECISION": "fp16"}, {"HF_TOL": "-1"}):
+        p = subprocess.run([DROPIN], cwd=str(wd), capture_output=True, text=True, timeout=120,
+                           env={**os.environ, **bad})
+        assert p.returncode == 1 and "HF_" in p.stdout + p.stderr, (bad, p.returncode)
+
+
+def test_dropin_golden_tempdis(tmp_path):
+    wd = tmp_path / "run"
+    _golden_run_dir(wd)
     T = run_program(DROPIN, str(wd))
     gold = read_tempdis(os.path.join(GOLDEN, "TempDis.dat"))
     assert T.shape == gold.shape

@@ -677,6 +677,29 @@ def test_fp32_ir_pairs_thin_grids(shape):
     assert sa == sb and np.array_equal(bits(ta), bits(tb))
 
 
+@pytest.mark.parametrize("icat", [1, 0])
+def test_fp32_ir_pairs_per_cell_heat_capacity(icat):
+    """The two-step pass's per-cell variants (rho c_p and tau' rho c_p read for every cell
+    rather than once: rho varied by up to 0.1% per cell), Cattaneo and Fourier: 40 steps at
+    24^3 and tol 1e-11 pair, meet the oracle, and with the step length frozen (debug bit 8)
+    give the bits of single steps (debug bit 7)."""
+    s, _ = workloads.c3(24)
+    s.params.icattaneo = icat
+    s.cells.rho = s.cells.rho * (1.0 + 1e-3 * np.random.default_rng(11).random(s.params.n))
+    t0 = np.full(s.params.n, s.params.t_system)
+    rc, otp, _, _ = oracle.OracleSystem(s).simulate_steps(list(range(1, 41)), t0, t0)
+    assert rc == 0
+    (ta, tpa), sa, ka = _mixed_run(s, 40, 256, tol=1e-11)
+    (tb, tpb), sb, _ = _mixed_run(s, 40, 256 | 128, tol=1e-11)
+    n2 = ka.get("k_ir_step2", (0.0, 0, 0.0))[1]
+    print(f"pairs per-cell rho_c icat {icat}: launches {n2}, stats {sa}, "
+          f"err {relerr(ta, otp):.3g}", flush=True)
+    assert n2 >= 10
+    assert relerr(ta, otp) <= 1e-9
+    assert sa == sb
+    assert np.array_equal(bits(ta), bits(tb)) and np.array_equal(bits(tpa), bits(tpb))
+
+
 def test_fp32_ir_pairs_with_misses():
     """The 48^3 workload at tol 1e-12, where one-iteration residuals sit near the inner target:
     a pair whose step misses its test falls back to single steps and pauses reuse.  The 24
